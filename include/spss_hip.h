@@ -1,0 +1,171 @@
+/*
+ * spss_hip.h -- C ABI of the MI355X (gfx950) right-hand-side hot path for
+ * self-propelled particles with Kuramoto-type phase coupling.
+ *
+ * The library (libspss_hip.so) keeps one particle system per handle with its
+ * state resident in HBM and runs hand-written HIP kernels.  Signatures use
+ * plain C types only: pointers, sizes and POD structs.  Every entry point
+ * returns SPSS_OK or an error code; spss_last_error() gives the message of the
+ * last failure on the calling thread.
+ *
+ * Host pointers are borrowed for the duration of a call.  Device pointers
+ * (the *_device variants) are read/written on the handle's stream.  One
+ * handle is bound to one HIP stream: calls on a handle are stream-ordered and
+ * not thread-safe; different handles may be used from different threads.
+ *
+ * State layout everywhere is the reference's: array-of-structures FP64 with
+ * s values per particle
+ *   model 4: (x, y, phi, omega)          DynamicalSystems/ParticleSystemFirstOrderPtr.hpp:12-114
+ *   model 6: (x, y, vx, vy, phi, omega)  DynamicalSystems/ParticleSystemSecondOrderPtr.hpp:12-116
+ *   model 3: (x, y, phi)                 first-order (m = 0) Kuramoto; no reference counterpart
+ */
+#ifndef SPSS_HIP_H
+#define SPSS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPSS_ABI_VERSION 1
+
+/* status codes */
+#define SPSS_OK 0
+#define SPSS_ERR_INVALID 1     /* bad argument */
+#define SPSS_ERR_DEVICE 2      /* a HIP runtime call failed */
+#define SPSS_ERR_UNSUPPORTED 3 /* combination the reference does not define */
+#define SPSS_ERR_CAPACITY 4    /* a fixed-size buffer would overflow */
+
+/* neighbour strategies; AUTO is the reference dispatch on rho
+ * (ParticleSystemFirstOrderPtr.cpp:131-153, ParticleSystemSecondOrderPtr.cpp:125-138) */
+#define SPSS_METHOD_AUTO 0
+#define SPSS_METHOD_GLOBAL_FIELD 1      /* EvaluateInteractionsWithGlobalPolarOrderField, :155-223 */
+#define SPSS_METHOD_ALL_PAIRS 2         /* EvaluateInteractionsWithAllPairs, :225-316 */
+#define SPSS_METHOD_LINKED_LIST 3       /* EvaluateInteractionsWithLinkedList, :318-468 */
+#define SPSS_METHOD_VERLET 4            /* EvaluateInteractionsWithVerletNeighborList, :470-614 */
+#define SPSS_METHOD_LINKED_LIST_VERLET 5 /* EvaluateInteractionsWithLinkedListAndVerletNeighborList, :616-815 */
+
+/* arithmetic of the pair sums */
+#define SPSS_PRECISION_FP64 0  /* FP64, the reference's operation order */
+#define SPSS_PRECISION_FP32C 1 /* FP32 pair math on cell-relative offsets, two-sum compensated
+                                  coupling sums, FP64 stage combination (cell-list methods) */
+
+/* Verlet semantics */
+#define SPSS_VERLET_COMPAT 0 /* reference: r_max = rho + 0.05, candidates from the 9 cells */
+#define SPSS_VERLET_EXACT 1  /* r_max = rho + verlet_skin, candidates from cells >= r_max */
+
+/* steppers */
+#define SPSS_STEPPER_RK4 0   /* RungeKutta4Stepper[Ptr]::DoStep, Steppers/RungeKutta4StepperPtr.cpp:68-115 */
+#define SPSS_STEPPER_EULER 1 /* StochasticEulerStepperPtr.hpp:40-73 (s=4), StochasticEulerStepper.hpp:33-51 (s=6) */
+#define SPSS_STEPPER_SRK 2   /* StochasticRungeKuttaStepperPtr.hpp:40-76 (s=4 only) */
+
+/* where the normal draws of a stochastic step come from */
+#define SPSS_NOISE_NONE 0   /* all draws are 0 */
+#define SPSS_NOISE_HOST 1   /* host array of the draws for this step (noise injection) */
+#define SPSS_NOISE_DEVICE 2 /* device array of the draws for this step */
+#define SPSS_NOISE_PHILOX 3 /* counter-based Philox4x32-10 + Box-Muller on the device,
+                               key = seed, counter = (particle, draw block, step) */
+
+typedef struct spss_system spss_system;
+
+/* Constructor arguments of ParticleSystemFirstOrderPtr (ParticleSystemFirstOrderPtr.hpp:16-24),
+ * ParticleSystemSecondOrderPtr (ParticleSystemSecondOrderPtr.hpp:16-26) and
+ * PeriodicBoundaryConditions (PeriodicBoundaryConditions.hpp:15-19), plus build options. */
+typedef struct spss_params {
+  int model;       /* 4, 6 or 3 (= number of state variables s) */
+  int n;           /* number of particles */
+  double v0;       /* self-propulsion speed v_0 */
+  double xi;       /* model 4: phase friction xi; model 6: xi_phi */
+  double sigma;    /* coupling strength */
+  double rho;      /* interaction radius */
+  double alpha;    /* phase lag */
+  double D_phi;    /* phase noise intensity */
+  double xi_r;     /* model 6: translational friction */
+  double D_r;      /* model 6: translational noise */
+  double x_size;   /* periodic box (the reference's cell grid assumes 1.0) */
+  double y_size;
+  double verlet_skin;  /* exact Verlet mode: r_max - rho */
+  int verlet_mode;     /* SPSS_VERLET_* */
+  int natfreq_local;   /* 1: natural frequencies enter the local branches too (model 4) */
+  int precision;       /* SPSS_PRECISION_* */
+  int cell_subdivision; /* FP32C cell list: cells of side 1/(k*nc) with a (2k+1)^2 stencil; 0 -> 1 */
+  uint64_t seed;       /* Philox key */
+} spss_params;
+
+/* fill with the reference defaults of ParticleSystemFirstOrderPtr (model 4, L = 1) */
+int spss_params_default(spss_params *p);
+
+/* ParticleSystemFirstOrderPtr::ParticleSystemFirstOrderPtr / ~ParticleSystemFirstOrderPtr
+ * (ParticleSystemFirstOrderPtr.cpp:10-124); the device owns the state and scratch. */
+int spss_create(const spss_params *p, spss_system **out);
+int spss_destroy(spss_system *sys);
+const char *spss_last_error(void);
+int spss_abi_version(void);
+
+/* stream binding; the stream is a hipStream_t passed as void* (NULL = a stream of the library's own) */
+int spss_set_stream(spss_system *sys, void *hip_stream);
+int spss_synchronize(spss_system *sys);
+
+/* device-resident master state (SimulationEngineFirstOrderPtr.cpp:71 `new Real[s*N]`) */
+int spss_set_state(spss_system *sys, const double *host_state);
+int spss_get_state(spss_system *sys, double *host_state);
+int spss_set_state_device(spss_system *sys, const double *device_state);
+double *spss_state_device(spss_system *sys);
+
+/* ParticleSystemFirstOrderPtr::SetNaturalFrequencies (ParticleSystemFirstOrderPtr.cpp:126-129) */
+int spss_set_natural_frequencies(spss_system *sys, const double *host_w, int n);
+/* the public flag is_higher_order_sde_integration_ (ParticleSystemFirstOrderPtr.hpp:72) */
+int spss_set_higher_order(spss_system *sys, int on);
+/* GetComplimentaryAlignmentForce (ParticleSystemFirstOrderPtr.cpp:863-866) */
+int spss_get_complementary_force(spss_system *sys, double *host_out);
+
+/* EvaluateRhs (ParticleSystemFirstOrderPtr.cpp:131-153) and the EvaluateInteractionsWith*
+ * methods it dispatches to, on host buffers of s*n doubles.  k_next may alias k_prev. */
+int spss_evaluate_rhs(spss_system *sys, int method, const double *state, const double *k_prev,
+                      double *k_next, double k_coef, double dt);
+/* the same on device buffers of s*n doubles */
+int spss_evaluate_rhs_device(spss_system *sys, int method, const double *d_state, const double *d_k_prev,
+                             double *d_k_next, double k_coef, double dt);
+
+/* DoStep of the three steppers on the device-resident state.  `noise` holds the draws
+ * of this step for SPSS_NOISE_HOST/DEVICE: model 4 Euler n, model 6 Euler 3n (slots 2,3,5),
+ * model 3 Euler n, SRK 2n (U1, U2 interleaved). */
+int spss_rk4_step(spss_system *sys, int method, double dt);
+int spss_euler_step(spss_system *sys, int method, double dt, int noise_mode, const double *noise);
+int spss_srk_step(spss_system *sys, int method, double dt, int noise_mode, const double *noise);
+/* `nsteps` steps of one stepper (the time loop of SimulationEngineFirstOrderPtr::RunSimulation,
+ * SimulationEngineFirstOrderPtr.cpp:170-181, without observers).  Noise: NONE or PHILOX. */
+int spss_run(spss_system *sys, int stepper, int method, long nsteps, double dt, int noise_mode);
+/* number of steps taken so far (the Philox step counter) */
+int spss_get_step(spss_system *sys, long *step);
+int spss_set_step(spss_system *sys, long step);
+
+/* CalculateMaxDisplacement (ParticleSystemFirstOrderPtr.cpp:836-861) on a host total-displacement
+ * vector, and the Verlet bookkeeping it updates */
+int spss_calculate_max_displacement(spss_system *sys, const double *host_total_displacement);
+int spss_verlet_status(spss_system *sys, int *should_update, double *accumulated_displacement,
+                       int *uses_verlet_list);
+/* the current Verlet lists: counts[n], then up to `cap` entries in list order */
+int spss_get_verlet_lists(spss_system *sys, int *host_counts, int *host_flat, long cap, long *total);
+
+/* neighbour set of one stage state (host, s*n doubles) found by `method`, as (i, j) pairs sorted
+ * by (i, j): at most `cap` pairs into host_pairs[2*cap]; *count = number found */
+int spss_neighbour_pairs(spss_system *sys, int method, const double *host_stage, int32_t *host_pairs,
+                         long cap, long *count);
+
+/* SaveSummaryStatistics' complex order parameter (BinaryObserverFirstOrderPtr.cpp:181-200),
+ * accumulated in FP64 on the device: (1/N) sum exp(i * state[s*k + slot]) */
+int spss_order_parameter(spss_system *sys, int slot, double *re, double *im);
+/* ApplyPeriodicBoundaryConditions(Real*, long) on the device state (PeriodicBoundaryConditions.cpp:197-205) */
+int spss_apply_periodic_boundary_conditions(spss_system *sys);
+
+/* AveragingObserver (Observers/AveragingObserver.cpp:66-105): running FP64 sum of the state */
+int spss_averaging_reset(spss_system *sys);
+int spss_averaging_accumulate(spss_system *sys);
+int spss_averaging_get(spss_system *sys, double *host_sum, long *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPSS_HIP_H */

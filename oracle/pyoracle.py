@@ -174,27 +174,30 @@ class Oracle:
         self.lib.orc_calculate_max_displacement(self.h, np.ascontiguousarray(total, dtype=np.float64))
 
     def pairs_linked_list(self, stage, cap=None):
-        cap = cap or self.n * 256
+        cap = cap or self.n * 64
         out = np.zeros(2 * cap, dtype=np.int32)
         cnt = self.lib.orc_pairs_linked_list(self.h, np.ascontiguousarray(stage), out, cap)
-        assert cnt <= cap
+        if cnt > cap:
+            return self.pairs_linked_list(stage, cap=int(cnt))
         return out[: 2 * cnt].reshape(-1, 2)
 
     def verlet_lists(self, cap=None):
-        cap = cap or self.n * 2048
+        cap = cap or self.n * 256
         oi = np.zeros(cap, dtype=np.int32)
         oj = np.zeros(cap, dtype=np.int32)
         cnt = self.lib.orc_verlet_lists(self.h, oi, oj, cap)
-        assert cnt <= cap
+        if cnt > cap:
+            return self.verlet_lists(cap=int(cnt))
         return oi[:cnt].copy(), oj[:cnt].copy()
 
 
 def pairs_bruteforce(stage, n, s, rho, xs=1.0, ys=1.0, cap=None):
     lib = load_library()
-    cap = cap or n * 256
+    cap = cap or n * 64
     out = np.zeros(2 * cap, dtype=np.int32)
     cnt = lib.orc_pairs_bruteforce(np.ascontiguousarray(stage), n, s, rho, xs, ys, out, cap)
-    assert cnt <= cap
+    if cnt > cap:
+        return pairs_bruteforce(stage, n, s, rho, xs, ys, cap=int(cnt))
     return out[: 2 * cnt].reshape(-1, 2)
 
 

@@ -1,0 +1,116 @@
+"""ctypes declarations of the C ABI in include/spss_hip.h (libspss_hip.so).
+
+Loading fails loudly: there is no CPU fallback behind this package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspss_hip.so")
+
+SPSS_OK = 0
+METHOD_AUTO, METHOD_GLOBAL_FIELD, METHOD_ALL_PAIRS, METHOD_LINKED_LIST, METHOD_VERLET, \
+    METHOD_LINKED_LIST_VERLET = range(6)
+PRECISION_FP64, PRECISION_FP32C = 0, 1
+VERLET_COMPAT, VERLET_EXACT = 0, 1
+STEPPER_RK4, STEPPER_EULER, STEPPER_SRK = 0, 1, 2
+NOISE_NONE, NOISE_HOST, NOISE_DEVICE, NOISE_PHILOX = 0, 1, 2, 3
+
+# every entry point include/spss_hip.h declares (checked by tests/test_capi_symbols.py)
+EXPORTS = [
+    "spss_params_default", "spss_create", "spss_destroy", "spss_last_error", "spss_abi_version",
+    "spss_set_stream", "spss_synchronize", "spss_set_state", "spss_get_state", "spss_set_state_device",
+    "spss_state_device", "spss_set_natural_frequencies", "spss_set_higher_order",
+    "spss_get_complementary_force", "spss_evaluate_rhs", "spss_evaluate_rhs_device", "spss_rk4_step",
+    "spss_euler_step", "spss_srk_step", "spss_run", "spss_get_step", "spss_set_step",
+    "spss_calculate_max_displacement", "spss_verlet_status", "spss_get_verlet_lists",
+    "spss_neighbour_pairs", "spss_order_parameter", "spss_apply_periodic_boundary_conditions",
+    "spss_averaging_reset", "spss_averaging_accumulate", "spss_averaging_get",
+]
+
+
+class SpssParams(C.Structure):
+    _fields_ = [
+        ("model", C.c_int), ("n", C.c_int),
+        ("v0", C.c_double), ("xi", C.c_double), ("sigma", C.c_double), ("rho", C.c_double),
+        ("alpha", C.c_double), ("D_phi", C.c_double), ("xi_r", C.c_double), ("D_r", C.c_double),
+        ("x_size", C.c_double), ("y_size", C.c_double), ("verlet_skin", C.c_double),
+        ("verlet_mode", C.c_int), ("natfreq_local", C.c_int), ("precision", C.c_int),
+        ("cell_subdivision", C.c_int), ("seed", C.c_uint64),
+    ]
+
+
+class SpssError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load(path=None):
+    """Load libspss_hip.so; raises ImportError when it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+    lib = C.CDLL(path)
+    vp, i, d, l = C.c_void_p, C.c_int, C.c_double, C.c_long
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    sig = {
+        "spss_params_default": (i, [C.POINTER(SpssParams)]),
+        "spss_create": (i, [C.POINTER(SpssParams), C.POINTER(vp)]),
+        "spss_destroy": (i, [vp]),
+        "spss_last_error": (C.c_char_p, []),
+        "spss_abi_version": (i, []),
+        "spss_set_stream": (i, [vp, vp]),
+        "spss_synchronize": (i, [vp]),
+        "spss_set_state": (i, [vp, vp]),
+        "spss_get_state": (i, [vp, vp]),
+        "spss_set_state_device": (i, [vp, vp]),
+        "spss_state_device": (vp, [vp]),
+        "spss_set_natural_frequencies": (i, [vp, vp, i]),
+        "spss_set_higher_order": (i, [vp, i]),
+        "spss_get_complementary_force": (i, [vp, vp]),
+        "spss_evaluate_rhs": (i, [vp, i, vp, vp, vp, d, d]),
+        "spss_evaluate_rhs_device": (i, [vp, i, vp, vp, vp, d, d]),
+        "spss_rk4_step": (i, [vp, i, d]),
+        "spss_euler_step": (i, [vp, i, d, i, vp]),
+        "spss_srk_step": (i, [vp, i, d, i, vp]),
+        "spss_run": (i, [vp, i, i, l, d, i]),
+        "spss_get_step": (i, [vp, C.POINTER(l)]),
+        "spss_set_step": (i, [vp, l]),
+        "spss_calculate_max_displacement": (i, [vp, vp]),
+        "spss_verlet_status": (i, [vp, ip, dp, ip]),
+        "spss_get_verlet_lists": (i, [vp, vp, vp, l, C.POINTER(l)]),
+        "spss_neighbour_pairs": (i, [vp, i, vp, vp, l, C.POINTER(l)]),
+        "spss_order_parameter": (i, [vp, i, dp, dp]),
+        "spss_apply_periodic_boundary_conditions": (i, [vp]),
+        "spss_averaging_reset": (i, [vp]),
+        "spss_averaging_accumulate": (i, [vp]),
+        "spss_averaging_get": (i, [vp, vp, C.POINTER(l)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def check(rc):
+    if rc != SPSS_OK:
+        raise SpssError(f"spss error {rc}: {load().spss_last_error().decode()}")
+    return rc
+
+
+def ptr(a):
+    """data pointer of a C-contiguous numpy array (None passes NULL)"""
+    if a is None:
+        return None
+    assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"], "need a C-contiguous array"
+    return a.ctypes.data

@@ -1,0 +1,725 @@
+// spss_capi.hip -- the C ABI of libspss_hip.so (declared in include/spss_hip.h).
+//
+// A handle owns the device-resident particle system: the FP64 master state,
+// the stage state, the stepper k-buffers, cell and Verlet structures, and the
+// Verlet bookkeeping of the reference (should_update_lists_, the accumulated
+// displacement).  Kernels run on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spss_hip.h"
+#include "spss_fast.hpp"
+#include "spss_launch.hpp"
+
+using namespace spss;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+}  // namespace
+
+#define HIPCHECK(call)                                                                           \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return fail(SPSS_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+#define TRY(call)                 \
+  do {                            \
+    int r_ = (call);              \
+    if (r_ != SPSS_OK) return r_; \
+  } while (0)
+
+struct spss_system {
+  spss_params p{};
+  int s = 4, n = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  DevParams P{};
+  int ncx = 1, ncy = 1;    // linked-list grid, int(1.0/rho) (ParticleSystemFirstOrderPtr.cpp:37-38)
+  int ncvx = 1, ncvy = 1;  // grid the Verlet candidates come from
+  double r_min = 0.0, r_max = 0.0;
+  // device buffers
+  double *state = nullptr, *stage = nullptr, *k[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *deriv = nullptr, *compl_f = nullptr, *nat = nullptr, *noise = nullptr;
+  double *partial = nullptr, *scal = nullptr, *acc = nullptr;
+  double *io[3] = {nullptr, nullptr, nullptr};
+  CellBuffers cells, vcells;
+  int *vcount = nullptr, *vstart = nullptr, *vlist = nullptr;
+  long vcap = 0;
+  int *pcount = nullptr, *pstart = nullptr, *plist = nullptr;
+  long pcap = 0;
+  // host bookkeeping mirroring the reference members
+  bool should_update = true;      // should_update_lists_
+  double acc_disp = 0.0;          // accumulated_displacement_
+  bool uses_verlet = false;       // uses_verlet_list_
+  bool higher_order = false;      // is_higher_order_sde_integration_
+  bool have_nat = false;
+  long step = 0;
+  long avg_count = 0;
+  FastPath *fast = nullptr;
+};
+
+namespace {
+
+int dalloc(void **p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(SPSS_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return SPSS_OK;
+}
+
+template <class T>
+int dalloc_n(T **p, size_t count) {
+  return dalloc((void **)p, count * sizeof(T));
+}
+
+int ensure_cells(spss_system *h, CellBuffers &cb, int ncx, int ncy) {
+  const int cells = ncx * ncy;
+  if (cb.key && cb.cells >= cells) return SPSS_OK;
+  hipFree(cb.key);
+  hipFree(cb.slot);
+  hipFree(cb.count);
+  hipFree(cb.start);
+  hipFree(cb.tmp);
+  hipFree(cb.sorted);
+  cb = CellBuffers{};
+  TRY(dalloc_n(&cb.key, h->n));
+  TRY(dalloc_n(&cb.slot, h->n));
+  TRY(dalloc_n(&cb.count, cells));
+  TRY(dalloc_n(&cb.start, cells + 1));
+  TRY(dalloc_n(&cb.tmp, h->n));
+  TRY(dalloc_n(&cb.sorted, h->n));
+  HIPCHECK(hipMemsetAsync(cb.count, 0, sizeof(int) * cells, h->stream));
+  cb.cells = cells;
+  return SPSS_OK;
+}
+
+// EvaluateRhs's dispatch on rho (ParticleSystemFirstOrderPtr.cpp:131-153; model 6:
+// ParticleSystemSecondOrderPtr.cpp:125-138)
+int resolve_method(const spss_system *h, int method) {
+  if (method != SPSS_METHOD_AUTO) return method;
+  const double rho = h->p.rho;
+  if (h->s == 6) return rho <= 0.25 ? SPSS_METHOD_LINKED_LIST : SPSS_METHOD_ALL_PAIRS;
+  if (rho >= M_SQRT1_2) return SPSS_METHOD_GLOBAL_FIELD;
+  if (rho <= 0.25) return SPSS_METHOD_LINKED_LIST_VERLET;
+  if (rho <= 0.5) return SPSS_METHOD_VERLET;
+  return SPSS_METHOD_ALL_PAIRS;
+}
+
+int check_method(const spss_system *h, int m) {
+  if (m < SPSS_METHOD_GLOBAL_FIELD || m > SPSS_METHOD_LINKED_LIST_VERLET)
+    return fail(SPSS_ERR_INVALID, "unknown method");
+  if (h->s == 6 && (m == SPSS_METHOD_GLOBAL_FIELD || m == SPSS_METHOD_VERLET || m == SPSS_METHOD_LINKED_LIST_VERLET))
+    return fail(SPSS_ERR_UNSUPPORTED,
+                "model 6 has no global-field or Verlet branch (ParticleSystemSecondOrderPtr.cpp:393-400 is a stub)");
+  if (h->p.precision == SPSS_PRECISION_FP32C && m != SPSS_METHOD_LINKED_LIST)
+    return fail(SPSS_ERR_UNSUPPORTED, "FP32C precision is implemented for the linked-list method");
+  if (h->p.precision == SPSS_PRECISION_FP32C && h->s == 6)
+    return fail(SPSS_ERR_UNSUPPORTED, "FP32C precision is implemented for models 3 and 4");
+  return SPSS_OK;
+}
+
+// Verlet list (re)build into vstart/vlist; two passes with the list size read back in between
+int build_verlet(spss_system *h, int m) {
+  hipStream_t st = h->stream;
+  const double rmax2 = h->r_max * h->r_max;
+  if (!h->vcount) {
+    TRY(dalloc_n(&h->vcount, h->n + 1));
+    TRY(dalloc_n(&h->vstart, h->n + 1));
+  }
+  if (m == SPSS_METHOD_VERLET) {
+    launch_verlet_count_all(st, h->P, h->stage, rmax2, h->vcount);
+  } else {
+    TRY(ensure_cells(h, h->vcells, h->ncvx, h->ncvy));
+    launch_cell_build(st, h->P, h->stage, h->ncvx, h->ncvy, h->vcells);
+    launch_verlet_count_cells(st, h->P, h->stage, h->ncvx, h->ncvy, h->vcells, rmax2, h->vcount);
+  }
+  launch_scan(st, h->vcount, h->vstart, h->n);
+  int total = 0;
+  HIPCHECK(hipMemcpyAsync(&total, h->vstart + h->n, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  if (total < 0) return fail(SPSS_ERR_CAPACITY, "Verlet list size overflows int32");
+  if ((long)total > h->vcap) {
+    hipFree(h->vlist);
+    h->vlist = nullptr;
+    h->vcap = std::max<long>((long)total + total / 4, 1024);
+    TRY(dalloc_n(&h->vlist, (size_t)h->vcap));
+  }
+  if (m == SPSS_METHOD_VERLET)
+    launch_verlet_fill_all(st, h->P, h->stage, rmax2, h->vstart, h->vlist);
+  else
+    launch_verlet_fill_cells(st, h->P, h->stage, h->ncvx, h->ncvy, h->vcells, rmax2, h->vstart, h->vlist);
+  return SPSS_OK;
+}
+
+int rhs_device(spss_system *h, int method, const double *d_state, const double *d_kprev, double *d_knext,
+               double k_coef, double dt) {
+  const int m = resolve_method(h, method);
+  TRY(check_method(h, m));
+  hipStream_t st = h->stream;
+  h->P.higher_order = h->higher_order ? 1 : 0;
+  const double *nat = h->have_nat ? h->nat : nullptr;
+  double *compl_out = h->higher_order ? h->compl_f : nullptr;
+  launch_stage(st, (long)h->s * h->n, d_state, d_kprev, k_coef * dt, h->stage);
+  switch (m) {
+    case SPSS_METHOD_GLOBAL_FIELD:
+      launch_global(st, h->P, h->stage, h->partial, h->scal, nat, d_knext, compl_out);
+      break;
+    case SPSS_METHOD_ALL_PAIRS:
+      launch_all_pairs(st, h->P, h->stage, nat, d_knext, compl_out);
+      break;
+    case SPSS_METHOD_LINKED_LIST:
+      if (h->p.precision == SPSS_PRECISION_FP32C) {
+        TRY(fast_rhs(h->fast, st, h->P, h->stage, nat, d_knext, compl_out));
+      } else {
+        TRY(ensure_cells(h, h->cells, h->ncx, h->ncy));
+        launch_cell_build(st, h->P, h->stage, h->ncx, h->ncy, h->cells);
+        launch_linked_list(st, h->P, h->stage, h->ncx, h->ncy, h->cells, nat, d_knext, compl_out);
+      }
+      break;
+    case SPSS_METHOD_VERLET:
+    case SPSS_METHOD_LINKED_LIST_VERLET:
+      if (h->should_update) {
+        TRY(build_verlet(h, m));
+        h->should_update = false;
+      }
+      launch_verlet_pairs(st, h->P, h->stage, h->vstart, h->vlist, nat, d_knext, compl_out);
+      h->uses_verlet = true;
+      break;
+    default:
+      return fail(SPSS_ERR_INVALID, "unknown method");
+  }
+  HIPCHECK(hipGetLastError());
+  return SPSS_OK;
+}
+
+// CalculateMaxDisplacement's bookkeeping, ParticleSystemFirstOrderPtr.cpp:851-860.  Exact mode
+// adds the step's own displacement as a margin for the RK stages of the next step.
+void update_verlet_flag(spss_system *h, double max2) {
+  const double stepd = std::sqrt(max2);
+  h->acc_disp += stepd;
+  bool flag = false;
+  if (h->p.verlet_mode == SPSS_VERLET_EXACT) {
+    if (h->acc_disp + stepd > 0.5 * (h->r_max - h->r_min)) {
+      h->acc_disp = 0.0;
+      flag = true;
+    }
+  } else if (h->acc_disp > 0.5 * (h->r_max - h->r_min)) {
+    h->acc_disp = 0.0;
+    flag = true;
+  }
+  h->should_update = flag;
+}
+
+int finish_step_displacement(spss_system *h) {
+  if (!h->uses_verlet || h->s == 6) return SPSS_OK;
+  const int nb = (h->n + kBlock - 1) / kBlock;
+  launch_max_final(h->stream, nb, h->partial, h->scal + 8);
+  double max2 = 0.0;
+  HIPCHECK(hipMemcpyAsync(&max2, h->scal + 8, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  update_verlet_flag(h, max2);
+  return SPSS_OK;
+}
+
+int noise_source(spss_system *h, int mode, const double *noise, size_t count, int *kmode, const double **kptr) {
+  *kptr = nullptr;
+  switch (mode) {
+    case SPSS_NOISE_NONE: *kmode = 0; return SPSS_OK;
+    case SPSS_NOISE_HOST:
+      if (!noise) return fail(SPSS_ERR_INVALID, "noise array is NULL");
+      HIPCHECK(hipMemcpyAsync(h->noise, noise, count * sizeof(double), hipMemcpyHostToDevice, h->stream));
+      *kmode = 1;
+      *kptr = h->noise;
+      return SPSS_OK;
+    case SPSS_NOISE_DEVICE:
+      if (!noise) return fail(SPSS_ERR_INVALID, "noise array is NULL");
+      *kmode = 1;
+      *kptr = noise;
+      return SPSS_OK;
+    case SPSS_NOISE_PHILOX: *kmode = 3; return SPSS_OK;
+    default: return fail(SPSS_ERR_INVALID, "unknown noise mode");
+  }
+}
+
+int rk4_step(spss_system *h, int method, double dt) {
+  TRY(rhs_device(h, method, h->state, h->k[0], h->k[0], 0.0, dt));
+  TRY(rhs_device(h, method, h->state, h->k[0], h->k[1], 0.5, dt));
+  TRY(rhs_device(h, method, h->state, h->k[1], h->k[2], 0.5, dt));
+  TRY(rhs_device(h, method, h->state, h->k[2], h->k[3], 1.0, dt));
+  const bool track = h->uses_verlet && h->s != 6;
+  launch_rk4_combine(h->stream, h->s, h->n, h->state, h->k[0], h->k[1], h->k[2], h->k[3], dt,
+                     track ? h->partial : nullptr);
+  HIPCHECK(hipGetLastError());
+  ++h->step;
+  return finish_step_displacement(h);
+}
+
+int euler_step(spss_system *h, int method, double dt, int noise_mode, const double *noise) {
+  const size_t m = (size_t)h->s * h->n;
+  launch_fill_double(h->stream, (long)m, h->deriv, 0.0);  // std::fill(derivative, 0), :43
+  TRY(rhs_device(h, method, h->state, h->deriv, h->deriv, 0.0, dt));
+  int kmode;
+  const double *kptr;
+  TRY(noise_source(h, noise_mode, noise, (size_t)(h->s == 6 ? 3 : 1) * h->n, &kmode, &kptr));
+  const bool track = h->uses_verlet && h->s != 6;
+  launch_euler(h->stream, h->s, h->n, h->state, h->deriv, dt, std::sqrt(dt), std::sqrt(2.0 * h->p.D_phi),
+               std::sqrt(2.0 * h->p.D_r), kmode, kptr, h->p.seed, (uint64_t)h->step,
+               track ? h->partial : nullptr);
+  HIPCHECK(hipGetLastError());
+  ++h->step;
+  return finish_step_displacement(h);
+}
+
+int srk_step(spss_system *h, int method, double dt, int noise_mode, const double *noise) {
+  if (h->s != 4) return fail(SPSS_ERR_UNSUPPORTED, "the stochastic Runge-Kutta stepper is defined for model 4");
+  const size_t m = (size_t)h->s * h->n;
+  launch_fill_double(h->stream, (long)m, h->deriv, 0.0);
+  h->higher_order = true;  // StochasticRungeKuttaStepperPtr.hpp:44
+  TRY(rhs_device(h, method, h->state, h->deriv, h->deriv, 0.0, dt));
+  int kmode;
+  const double *kptr;
+  TRY(noise_source(h, noise_mode, noise, (size_t)2 * h->n, &kmode, &kptr));
+  const bool track = h->uses_verlet;
+  launch_srk(h->stream, h->n, h->state, h->deriv, h->compl_f, dt, std::sqrt(dt), 0.5 * std::pow(dt, 1.5),
+             1.0 / std::sqrt(3.0), std::sqrt(2.0 * h->p.D_phi), h->p.xi, kmode, kptr, h->p.seed, (uint64_t)h->step,
+             track ? h->partial : nullptr);
+  HIPCHECK(hipGetLastError());
+  ++h->step;
+  return finish_step_displacement(h);
+}
+
+}  // namespace
+
+extern "C" {
+
+int spss_abi_version(void) { return SPSS_ABI_VERSION; }
+
+const char *spss_last_error(void) { return g_err.c_str(); }
+
+int spss_params_default(spss_params *p) {
+  if (!p) return fail(SPSS_ERR_INVALID, "params is NULL");
+  std::memset(p, 0, sizeof(*p));
+  p->model = 4;
+  p->n = 1000;
+  p->v0 = 1.0;  // SimulationEngineFirstOrderPtr.cpp:145-150
+  p->xi = 0.1;
+  p->sigma = 1.0;
+  p->rho = 0.8;
+  p->alpha = 0.3;
+  p->D_phi = 0.01;
+  p->xi_r = 1.0;
+  p->x_size = 1.0;
+  p->y_size = 1.0;
+  p->verlet_skin = 0.05;
+  p->verlet_mode = SPSS_VERLET_COMPAT;
+  p->precision = SPSS_PRECISION_FP64;
+  p->cell_subdivision = 1;
+  p->seed = 0x5EED0000ull;
+  return SPSS_OK;
+}
+
+int spss_create(const spss_params *p, spss_system **out) {
+  if (!p || !out) return fail(SPSS_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (p->model != 3 && p->model != 4 && p->model != 6) return fail(SPSS_ERR_INVALID, "model must be 3, 4 or 6");
+  if (p->n <= 0) return fail(SPSS_ERR_INVALID, "n must be positive");
+  if (!(p->rho > 0.0)) return fail(SPSS_ERR_INVALID, "rho must be positive");
+  if (!(p->x_size > 0.0) || !(p->y_size > 0.0)) return fail(SPSS_ERR_INVALID, "box sizes must be positive");
+  if ((long)p->n * p->model > (1L << 31) - 1) return fail(SPSS_ERR_CAPACITY, "s*n exceeds int32 indexing");
+  spss_system *h = new spss_system();
+  h->p = *p;
+  h->s = p->model;
+  h->n = p->n;
+  if (h->p.cell_subdivision <= 0) h->p.cell_subdivision = 1;
+  DevParams &P = h->P;
+  P.n = h->n;
+  P.s = h->s;
+  P.v0 = p->v0;
+  P.xi = p->xi;
+  P.sigma = p->sigma;
+  P.rho2 = p->rho * p->rho;
+  P.alpha = p->alpha;
+  P.cos_a = std::cos(p->alpha);
+  P.sin_a = std::sin(p->alpha);
+  P.xi_r = p->xi_r;
+  P.xs = p->x_size;
+  P.ys = p->y_size;
+  P.xrs = 1.0 / p->x_size;
+  P.yrs = 1.0 / p->y_size;
+  P.natfreq_local = p->natfreq_local;
+  h->ncx = h->ncy = std::max(1, (int)(1.0 / p->rho));
+  h->r_min = p->rho;
+  if (p->verlet_mode == SPSS_VERLET_EXACT) {
+    if (!(p->verlet_skin > 0.0)) {
+      delete h;
+      return fail(SPSS_ERR_INVALID, "exact Verlet mode needs a positive skin");
+    }
+    h->r_max = p->rho + p->verlet_skin;
+    h->ncvx = h->ncvy = std::max(1, (int)(1.0 / h->r_max));
+  } else {
+    h->r_max = (h->s == 6) ? 2 * p->rho : p->rho + 0.05;  // ParticleSystemFirstOrderPtr.cpp:97-98
+    h->ncvx = h->ncx;
+    h->ncvy = h->ncy;
+  }
+  int rc = SPSS_OK;
+  auto step = [&](int r) {
+    if (rc == SPSS_OK) rc = r;
+  };
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(SPSS_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  h->own_stream = true;
+  const size_t m = (size_t)h->s * h->n;
+  const size_t nb = (size_t)(h->n + kBlock - 1) / kBlock;
+  step(dalloc_n(&h->state, m));
+  step(dalloc_n(&h->stage, m));
+  for (int q = 0; q < 4; ++q) step(dalloc_n(&h->k[q], m));
+  step(dalloc_n(&h->deriv, m));
+  step(dalloc_n(&h->compl_f, h->n));
+  step(dalloc_n(&h->nat, h->n));
+  step(dalloc_n(&h->noise, (size_t)3 * h->n));
+  step(dalloc_n(&h->partial, nb * 4 + 64));
+  step(dalloc_n(&h->scal, 64));
+  if (rc != SPSS_OK) {
+    spss_destroy(h);
+    return rc;
+  }
+  // the stepper k-buffers start at zero (RungeKutta4StepperPtr.cpp:48-51) and are never cleared
+  for (int q = 0; q < 4; ++q) launch_fill_double(h->stream, (long)m, h->k[q], 0.0);
+  launch_fill_double(h->stream, (long)m, h->state, 0.0);
+  launch_fill_double(h->stream, (long)m, h->deriv, 0.0);
+  launch_fill_double(h->stream, h->n, h->compl_f, 0.0);
+  launch_fill_double(h->stream, h->n, h->nat, 0.0);
+  if (h->p.precision == SPSS_PRECISION_FP32C) {
+    if (h->s == 6) {
+      spss_destroy(h);
+      return fail(SPSS_ERR_UNSUPPORTED, "FP32C precision is implemented for models 3 and 4");
+    }
+    rc = fast_create(&h->fast, h->stream, h->P, h->p.rho, h->ncx, h->p.cell_subdivision);
+    if (rc != SPSS_OK) {
+      spss_destroy(h);
+      return fail(rc, fast_last_error());
+    }
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    spss_destroy(h);
+    return fail(SPSS_ERR_DEVICE, "initialisation failed");
+  }
+  *out = h;
+  return SPSS_OK;
+}
+
+int spss_destroy(spss_system *h) {
+  if (!h) return SPSS_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  fast_destroy(h->fast);
+  void *bufs[] = {h->state, h->stage, h->k[0], h->k[1], h->k[2], h->k[3], h->deriv, h->compl_f, h->nat,
+                  h->noise, h->partial, h->scal, h->acc, h->io[0], h->io[1], h->io[2], h->vcount, h->vstart,
+                  h->vlist, h->pcount, h->pstart, h->plist};
+  for (void *b : bufs) hipFree(b);
+  for (CellBuffers *cb : {&h->cells, &h->vcells}) {
+    hipFree(cb->key);
+    hipFree(cb->slot);
+    hipFree(cb->count);
+    hipFree(cb->start);
+    hipFree(cb->tmp);
+    hipFree(cb->sorted);
+  }
+  if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return SPSS_OK;
+}
+
+int spss_set_stream(spss_system *h, void *stream) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  if (h->own_stream) hipStreamDestroy(h->stream);
+  if (stream) {
+    h->stream = (hipStream_t)stream;
+    h->own_stream = false;
+  } else {
+    HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->own_stream = true;
+  }
+  if (h->fast) fast_set_stream(h->fast, h->stream);
+  return SPSS_OK;
+}
+
+int spss_synchronize(spss_system *h) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return SPSS_OK;
+}
+
+int spss_set_state(spss_system *h, const double *host_state) {
+  if (!h || !host_state) return fail(SPSS_ERR_INVALID, "NULL argument");
+  HIPCHECK(hipMemcpyAsync(h->state, host_state, sizeof(double) * h->s * h->n, hipMemcpyHostToDevice, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return SPSS_OK;
+}
+
+int spss_get_state(spss_system *h, double *host_state) {
+  if (!h || !host_state) return fail(SPSS_ERR_INVALID, "NULL argument");
+  HIPCHECK(hipMemcpyAsync(host_state, h->state, sizeof(double) * h->s * h->n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return SPSS_OK;
+}
+
+int spss_set_state_device(spss_system *h, const double *d) {
+  if (!h || !d) return fail(SPSS_ERR_INVALID, "NULL argument");
+  HIPCHECK(hipMemcpyAsync(h->state, d, sizeof(double) * h->s * h->n, hipMemcpyDeviceToDevice, h->stream));
+  return SPSS_OK;
+}
+
+double *spss_state_device(spss_system *h) { return h ? h->state : nullptr; }
+
+int spss_set_natural_frequencies(spss_system *h, const double *w, int n) {
+  if (!h || !w) return fail(SPSS_ERR_INVALID, "NULL argument");
+  if (n != h->n) return fail(SPSS_ERR_INVALID, "natural frequencies must have n entries");
+  HIPCHECK(hipMemcpyAsync(h->nat, w, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  h->have_nat = true;
+  return SPSS_OK;
+}
+
+int spss_set_higher_order(spss_system *h, int on) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (on && h->s != 4) return fail(SPSS_ERR_UNSUPPORTED, "the complementary force is defined for model 4");
+  h->higher_order = on != 0;
+  return SPSS_OK;
+}
+
+int spss_get_complementary_force(spss_system *h, double *out) {
+  if (!h || !out) return fail(SPSS_ERR_INVALID, "NULL argument");
+  HIPCHECK(hipMemcpyAsync(out, h->compl_f, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return SPSS_OK;
+}
+
+int spss_evaluate_rhs_device(spss_system *h, int method, const double *d_state, const double *d_kprev,
+                             double *d_knext, double k_coef, double dt) {
+  if (!h || !d_state || !d_kprev || !d_knext) return fail(SPSS_ERR_INVALID, "NULL argument");
+  return rhs_device(h, method, d_state, d_kprev, d_knext, k_coef, dt);
+}
+
+int spss_evaluate_rhs(spss_system *h, int method, const double *state, const double *k_prev, double *k_next,
+                      double k_coef, double dt) {
+  if (!h || !state || !k_prev || !k_next) return fail(SPSS_ERR_INVALID, "NULL argument");
+  const size_t m = (size_t)h->s * h->n;
+  for (int q = 0; q < 3; ++q)
+    if (!h->io[q]) TRY(dalloc_n(&h->io[q], m));
+  hipStream_t st = h->stream;
+  HIPCHECK(hipMemcpyAsync(h->io[0], state, m * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(h->io[1], k_prev, m * sizeof(double), hipMemcpyHostToDevice, st));
+  // k_next entries the RHS leaves untouched (k[3] of isolated particles) keep their values
+  HIPCHECK(hipMemcpyAsync(h->io[2], k_next, m * sizeof(double), hipMemcpyHostToDevice, st));
+  TRY(rhs_device(h, method, h->io[0], h->io[1], h->io[2], k_coef, dt));
+  HIPCHECK(hipMemcpyAsync(k_next, h->io[2], m * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return SPSS_OK;
+}
+
+int spss_rk4_step(spss_system *h, int method, double dt) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  return rk4_step(h, method, dt);
+}
+
+int spss_euler_step(spss_system *h, int method, double dt, int noise_mode, const double *noise) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  return euler_step(h, method, dt, noise_mode, noise);
+}
+
+int spss_srk_step(spss_system *h, int method, double dt, int noise_mode, const double *noise) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  return srk_step(h, method, dt, noise_mode, noise);
+}
+
+int spss_run(spss_system *h, int stepper, int method, long nsteps, double dt, int noise_mode) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (noise_mode != SPSS_NOISE_NONE && noise_mode != SPSS_NOISE_PHILOX)
+    return fail(SPSS_ERR_INVALID, "spss_run draws noise from NONE or PHILOX only");
+  for (long k = 0; k < nsteps; ++k) {
+    switch (stepper) {
+      case SPSS_STEPPER_RK4: TRY(rk4_step(h, method, dt)); break;
+      case SPSS_STEPPER_EULER: TRY(euler_step(h, method, dt, noise_mode, nullptr)); break;
+      case SPSS_STEPPER_SRK: TRY(srk_step(h, method, dt, noise_mode, nullptr)); break;
+      default: return fail(SPSS_ERR_INVALID, "unknown stepper");
+    }
+  }
+  return SPSS_OK;
+}
+
+int spss_get_step(spss_system *h, long *step) {
+  if (!h || !step) return fail(SPSS_ERR_INVALID, "NULL argument");
+  *step = h->step;
+  return SPSS_OK;
+}
+
+int spss_set_step(spss_system *h, long step) {
+  if (!h || step < 0) return fail(SPSS_ERR_INVALID, "bad argument");
+  h->step = step;
+  return SPSS_OK;
+}
+
+int spss_calculate_max_displacement(spss_system *h, const double *tot) {
+  if (!h || !tot) return fail(SPSS_ERR_INVALID, "NULL argument");
+  const size_t m = (size_t)h->s * h->n;
+  if (!h->io[0]) TRY(dalloc_n(&h->io[0], m));
+  HIPCHECK(hipMemcpyAsync(h->io[0], tot, m * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  launch_disp_max(h->stream, h->n, h->s, h->io[0], h->partial);
+  const int nb = (h->n + kBlock - 1) / kBlock;
+  launch_max_final(h->stream, nb, h->partial, h->scal + 8);
+  double max2 = 0.0;
+  HIPCHECK(hipMemcpyAsync(&max2, h->scal + 8, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  update_verlet_flag(h, max2);
+  return SPSS_OK;
+}
+
+int spss_verlet_status(spss_system *h, int *should_update, double *acc, int *uses) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (should_update) *should_update = h->should_update ? 1 : 0;
+  if (acc) *acc = h->acc_disp;
+  if (uses) *uses = h->uses_verlet ? 1 : 0;
+  return SPSS_OK;
+}
+
+int spss_get_verlet_lists(spss_system *h, int *counts, int *flat, long cap, long *total) {
+  if (!h || !counts || !total) return fail(SPSS_ERR_INVALID, "NULL argument");
+  if (!h->vstart) {
+    std::fill(counts, counts + h->n, 0);
+    *total = 0;
+    return SPSS_OK;
+  }
+  std::vector<int> start(h->n + 1);
+  HIPCHECK(hipMemcpyAsync(start.data(), h->vstart, sizeof(int) * (h->n + 1), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  for (int i = 0; i < h->n; ++i) counts[i] = start[i + 1] - start[i];
+  *total = start[h->n];
+  if (flat && cap > 0) {
+    const long cnt = std::min<long>(cap, start[h->n]);
+    HIPCHECK(hipMemcpyAsync(flat, h->vlist, sizeof(int) * cnt, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+  }
+  return SPSS_OK;
+}
+
+int spss_neighbour_pairs(spss_system *h, int method, const double *host_stage, int32_t *pairs, long cap,
+                         long *count) {
+  if (!h || !host_stage || !count) return fail(SPSS_ERR_INVALID, "NULL argument");
+  const int m = resolve_method(h, method);
+  if (h->s == 6) return fail(SPSS_ERR_UNSUPPORTED, "pair export is implemented for models 3 and 4");
+  const size_t sn = (size_t)h->s * h->n;
+  hipStream_t st = h->stream;
+  HIPCHECK(hipMemcpyAsync(h->stage, host_stage, sn * sizeof(double), hipMemcpyHostToDevice, st));
+  if (!h->pcount) {
+    TRY(dalloc_n(&h->pcount, h->n + 1));
+    TRY(dalloc_n(&h->pstart, h->n + 1));
+  }
+  const double rho2 = h->P.rho2;
+  int total = 0;
+  if (m == SPSS_METHOD_LINKED_LIST && h->p.precision == SPSS_PRECISION_FP32C) {
+    TRY(fast_pairs_count(h->fast, st, h->P, h->stage, h->pcount));
+  } else if (m == SPSS_METHOD_LINKED_LIST) {
+    TRY(ensure_cells(h, h->cells, h->ncx, h->ncy));
+    launch_cell_build(st, h->P, h->stage, h->ncx, h->ncy, h->cells);
+    launch_verlet_count_cells(st, h->P, h->stage, h->ncx, h->ncy, h->cells, rho2, h->pcount);
+  } else if (m == SPSS_METHOD_ALL_PAIRS) {
+    launch_verlet_count_all(st, h->P, h->stage, rho2, h->pcount);
+  } else {
+    return fail(SPSS_ERR_UNSUPPORTED, "pair export is implemented for the all-pairs and linked-list methods");
+  }
+  launch_scan(st, h->pcount, h->pstart, h->n);
+  HIPCHECK(hipMemcpyAsync(&total, h->pstart + h->n, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  if ((long)total > h->pcap) {
+    hipFree(h->plist);
+    h->plist = nullptr;
+    h->pcap = std::max<long>(total, 1024);
+    TRY(dalloc_n(&h->plist, (size_t)h->pcap));
+  }
+  if (m == SPSS_METHOD_LINKED_LIST && h->p.precision == SPSS_PRECISION_FP32C)
+    TRY(fast_pairs_fill(h->fast, st, h->P, h->stage, h->pstart, h->plist));
+  else if (m == SPSS_METHOD_LINKED_LIST)
+    launch_verlet_fill_cells(st, h->P, h->stage, h->ncx, h->ncy, h->cells, rho2, h->pstart, h->plist);
+  else
+    launch_verlet_fill_all(st, h->P, h->stage, rho2, h->pstart, h->plist);
+  std::vector<int> start(h->n + 1), js(total > 0 ? total : 1);
+  HIPCHECK(hipMemcpyAsync(start.data(), h->pstart, sizeof(int) * (h->n + 1), hipMemcpyDeviceToHost, st));
+  if (total > 0) HIPCHECK(hipMemcpyAsync(js.data(), h->plist, sizeof(int) * total, hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  *count = total;
+  if (pairs) {
+    long w = 0;
+    for (int i = 0; i < h->n && w < cap; ++i) {
+      std::sort(js.begin() + start[i], js.begin() + start[i + 1]);
+      for (int u = start[i]; u < start[i + 1] && w < cap; ++u, ++w) {
+        pairs[2 * w] = i;
+        pairs[2 * w + 1] = js[u];
+      }
+    }
+  }
+  return SPSS_OK;
+}
+
+int spss_order_parameter(spss_system *h, int slot, double *re, double *im) {
+  if (!h || !re || !im) return fail(SPSS_ERR_INVALID, "NULL argument");
+  if (slot < 0 || slot >= h->s) return fail(SPSS_ERR_INVALID, "slot out of range");
+  launch_order_parameter(h->stream, h->n, h->s, slot, h->state, h->partial, h->scal + 16);
+  double v[2];
+  HIPCHECK(hipMemcpyAsync(v, h->scal + 16, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  *re = v[0];
+  *im = v[1];
+  return SPSS_OK;
+}
+
+int spss_apply_periodic_boundary_conditions(spss_system *h) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  launch_apply_pbc(h->stream, h->n, h->s, h->state, h->P.xs, h->P.ys, h->P.xrs, h->P.yrs);
+  HIPCHECK(hipGetLastError());
+  return SPSS_OK;
+}
+
+int spss_averaging_reset(spss_system *h) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  const size_t m = (size_t)h->s * h->n;
+  if (!h->acc) TRY(dalloc_n(&h->acc, m));
+  launch_fill_double(h->stream, (long)m, h->acc, 0.0);
+  h->avg_count = 0;
+  return SPSS_OK;
+}
+
+int spss_averaging_accumulate(spss_system *h) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (!h->acc) TRY(spss_averaging_reset(h));
+  launch_accumulate(h->stream, (long)h->s * h->n, h->state, h->acc);
+  ++h->avg_count;
+  return SPSS_OK;
+}
+
+int spss_averaging_get(spss_system *h, double *sum, long *count) {
+  if (!h || !sum || !count) return fail(SPSS_ERR_INVALID, "NULL argument");
+  if (!h->acc) TRY(spss_averaging_reset(h));
+  HIPCHECK(hipMemcpyAsync(sum, h->acc, sizeof(double) * h->s * h->n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  *count = h->avg_count;
+  return SPSS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,24 @@
+// spss_fast.hpp -- the FP32-compensated cell-list path (SPSS_PRECISION_FP32C).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "spss_device.hpp"
+
+namespace spss {
+
+struct FastPath;
+
+int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv);
+void fast_destroy(FastPath *f);
+void fast_set_stream(FastPath *f, hipStream_t st);
+const char *fast_last_error();
+// RHS of one stage state (FP64 AoS) with FP32 pair math; writes k_next (and the complementary force)
+int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const double *nat,
+             double *k_next, double *compl_out);
+// neighbour set the fast path uses for a stage state: per-particle counts, then lists
+int fast_pairs_count(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, int *count);
+int fast_pairs_fill(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const int *start,
+                    int *list);
+
+}  // namespace spss

@@ -1,0 +1,57 @@
+// spss_launch.hpp -- host-side launchers of the HIP kernels (one per kernel family).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "spss_device.hpp"
+
+namespace spss {
+
+// device buffers of one cell list (stable counting sort over cell keys)
+struct CellBuffers {
+  int *key = nullptr;     // [n] cell key of each particle
+  int *slot = nullptr;    // [n] arrival slot from the histogram atomics
+  int *count = nullptr;   // [cells] histogram, left zeroed by the scan
+  int *start = nullptr;   // [cells + 1] exclusive scan of count
+  int *tmp = nullptr;     // [n] unstable bucket fill
+  int *sorted = nullptr;  // [n] particle indices, cell-major, ascending index inside a cell
+  int cells = 0;
+};
+
+void launch_stage(hipStream_t st, long m, const double *state, const double *kprev, double kc_dt, double *stage);
+void launch_all_pairs(hipStream_t st, const DevParams &P, const double *stage, const double *nat, double *k_next,
+                      double *compl_out);
+void launch_cell_build(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy, CellBuffers &cb);
+void launch_linked_list(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy,
+                        const CellBuffers &cb, const double *nat, double *k_next, double *compl_out);
+void launch_scan(hipStream_t st, int *count, int *start, int m);
+void launch_verlet_count_all(hipStream_t st, const DevParams &P, const double *stage, double rmax2, int *vcount);
+void launch_verlet_fill_all(hipStream_t st, const DevParams &P, const double *stage, double rmax2,
+                            const int *vstart, int *vlist);
+void launch_verlet_count_cells(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy,
+                               const CellBuffers &cb, double rmax2, int *vcount);
+void launch_verlet_fill_cells(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy,
+                              const CellBuffers &cb, double rmax2, const int *vstart, int *vlist);
+void launch_verlet_pairs(hipStream_t st, const DevParams &P, const double *stage, const int *vstart,
+                         const int *vlist, const double *nat, double *k_next, double *compl_out);
+void launch_global(hipStream_t st, const DevParams &P, const double *stage, double *partial, double *fields,
+                   const double *nat, double *k_next, double *compl_out);
+void launch_rk4_combine(hipStream_t st, int s, int n, double *state, const double *k1, const double *k2,
+                        const double *k3, const double *k4, double dt, double *max_partial);
+void launch_euler(hipStream_t st, int s, int n, double *state, const double *f, double dt, double sqrt_dt,
+                  double int_phi, double int_r, int noise_mode, const double *noise, uint64_t seed, uint64_t step,
+                  double *max_partial);
+void launch_srk(hipStream_t st, int n, double *state, const double *f, const double *C, double dt, double sqrt_dt,
+                double c15, double is3, double sq2d, double friction, int noise_mode, const double *noise,
+                uint64_t seed, uint64_t step, double *max_partial);
+void launch_max_final(hipStream_t st, int nb, const double *partial, double *out);
+void launch_disp_max(hipStream_t st, int n, int s, const double *tot, double *partial);
+void launch_order_parameter(hipStream_t st, int n, int s, int slot, const double *state, double *partial,
+                            double *out);
+void launch_apply_pbc(hipStream_t st, int n, int s, double *state, double xs, double ys, double xrs, double yrs);
+void launch_accumulate(hipStream_t st, long m, const double *state, double *acc);
+void launch_fill_double(hipStream_t st, long m, double *p, double v);
+void launch_fill_int(hipStream_t st, long m, int *p, int v);
+
+}  // namespace spss

@@ -1,0 +1,157 @@
+"""Python view of one device-resident particle system (the C ABI behind it).
+
+Method names follow the reference classes: EvaluateRhs, the three steppers'
+DoStep, SetNaturalFrequencies, CalculateMaxDisplacement, ... in snake case.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as A
+
+
+class ParticleSystem:
+    def __init__(self, model=4, n=0, v0=0.01, xi=1.0, sigma=1.0, rho=0.1, alpha=0.0, D_phi=0.0, xi_r=1.0,
+                 D_r=0.0, x_size=1.0, y_size=1.0, verlet_skin=0.05, verlet_mode=A.VERLET_COMPAT,
+                 natfreq_local=0, precision=A.PRECISION_FP64, cell_subdivision=1, seed=0x5EED0000):
+        self.lib = A.load()
+        p = A.SpssParams()
+        A.check(self.lib.spss_params_default(C.byref(p)))
+        for k, v in dict(model=model, n=n, v0=v0, xi=xi, sigma=sigma, rho=rho, alpha=alpha, D_phi=D_phi,
+                         xi_r=xi_r, D_r=D_r, x_size=x_size, y_size=y_size, verlet_skin=verlet_skin,
+                         verlet_mode=verlet_mode, natfreq_local=natfreq_local, precision=precision,
+                         cell_subdivision=cell_subdivision, seed=seed).items():
+            setattr(p, k, v)
+        self.params = p
+        self.s, self.n = model, n
+        h = C.c_void_p()
+        A.check(self.lib.spss_create(C.byref(p), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.spss_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state
+    def set_state(self, state):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        assert st.size == self.s * self.n
+        A.check(self.lib.spss_set_state(self.h, A.ptr(st)))
+
+    def get_state(self, out=None):
+        out = np.empty(self.s * self.n) if out is None else out
+        A.check(self.lib.spss_get_state(self.h, A.ptr(out)))
+        return out
+
+    def state_device_ptr(self):
+        return self.lib.spss_state_device(self.h)
+
+    def set_natural_frequencies(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        A.check(self.lib.spss_set_natural_frequencies(self.h, A.ptr(w), int(w.size)))
+
+    def set_higher_order(self, on):
+        A.check(self.lib.spss_set_higher_order(self.h, int(bool(on))))
+
+    def complementary_force(self):
+        out = np.empty(self.n)
+        A.check(self.lib.spss_get_complementary_force(self.h, A.ptr(out)))
+        return out
+
+    # ---- RHS and steppers
+    def evaluate_rhs(self, method, state, k_prev, k_next, k_coef, dt):
+        A.check(self.lib.spss_evaluate_rhs(self.h, method, A.ptr(state), A.ptr(k_prev), A.ptr(k_next),
+                                           k_coef, dt))
+
+    def evaluate_rhs_device(self, method, d_state, d_kprev, d_knext, k_coef, dt):
+        A.check(self.lib.spss_evaluate_rhs_device(self.h, method, d_state, d_kprev, d_knext, k_coef, dt))
+
+    def rk4_step(self, method, dt):
+        A.check(self.lib.spss_rk4_step(self.h, method, dt))
+
+    def euler_step(self, method, dt, noise_mode=A.NOISE_NONE, noise=None):
+        nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
+        A.check(self.lib.spss_euler_step(self.h, method, dt, noise_mode, A.ptr(nz)))
+
+    def srk_step(self, method, dt, noise_mode=A.NOISE_NONE, noise=None):
+        nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
+        A.check(self.lib.spss_srk_step(self.h, method, dt, noise_mode, A.ptr(nz)))
+
+    def run(self, stepper, method, nsteps, dt, noise_mode=A.NOISE_NONE):
+        A.check(self.lib.spss_run(self.h, stepper, method, int(nsteps), dt, noise_mode))
+
+    @property
+    def step(self):
+        v = C.c_long()
+        A.check(self.lib.spss_get_step(self.h, C.byref(v)))
+        return v.value
+
+    @step.setter
+    def step(self, v):
+        A.check(self.lib.spss_set_step(self.h, int(v)))
+
+    def synchronize(self):
+        A.check(self.lib.spss_synchronize(self.h))
+
+    # ---- Verlet bookkeeping and neighbour sets
+    def calculate_max_displacement(self, total):
+        t = np.ascontiguousarray(total, dtype=np.float64)
+        A.check(self.lib.spss_calculate_max_displacement(self.h, A.ptr(t)))
+
+    def verlet_status(self):
+        su, uv = C.c_int(), C.c_int()
+        acc = C.c_double()
+        A.check(self.lib.spss_verlet_status(self.h, C.byref(su), C.byref(acc), C.byref(uv)))
+        return bool(su.value), acc.value, bool(uv.value)
+
+    def verlet_lists(self):
+        counts = np.zeros(self.n, dtype=np.int32)
+        total = C.c_long()
+        A.check(self.lib.spss_get_verlet_lists(self.h, A.ptr(counts), None, 0, C.byref(total)))
+        flat = np.zeros(max(total.value, 1), dtype=np.int32)
+        A.check(self.lib.spss_get_verlet_lists(self.h, A.ptr(counts), A.ptr(flat), total.value, C.byref(total)))
+        out, pos = [], 0
+        for c in counts:
+            out.append(flat[pos:pos + c].copy())
+            pos += c
+        return out
+
+    def neighbour_pairs(self, method, stage):
+        st = np.ascontiguousarray(stage, dtype=np.float64)
+        cnt = C.c_long()
+        A.check(self.lib.spss_neighbour_pairs(self.h, method, A.ptr(st), None, 0, C.byref(cnt)))
+        out = np.zeros(2 * max(cnt.value, 1), dtype=np.int32)
+        A.check(self.lib.spss_neighbour_pairs(self.h, method, A.ptr(st), A.ptr(out), cnt.value, C.byref(cnt)))
+        return out[: 2 * cnt.value].reshape(-1, 2)
+
+    # ---- observers
+    def order_parameter(self, slot=None):
+        slot = (4 if self.s == 6 else 2) if slot is None else slot
+        re, im = C.c_double(), C.c_double()
+        A.check(self.lib.spss_order_parameter(self.h, slot, C.byref(re), C.byref(im)))
+        return re.value, im.value
+
+    def apply_periodic_boundary_conditions(self):
+        A.check(self.lib.spss_apply_periodic_boundary_conditions(self.h))
+
+    def averaging_reset(self):
+        A.check(self.lib.spss_averaging_reset(self.h))
+
+    def averaging_accumulate(self):
+        A.check(self.lib.spss_averaging_accumulate(self.h))
+
+    def averaging_get(self):
+        out = np.empty(self.s * self.n)
+        cnt = C.c_long()
+        A.check(self.lib.spss_averaging_get(self.h, A.ptr(out), C.byref(cnt)))
+        return out, cnt.value

@@ -1,0 +1,43 @@
+"""CPU checks of the C ABI library: it loads without a GPU and exports every
+entry point that include/*.h declares (no compute calls are made here)."""
+import ctypes
+import glob
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(spss_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from spsslangevinintegration_amd import _capi
+    lib = _capi.load()
+    names = declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding declares the same set
+    assert set(_capi.EXPORTS) <= names
+
+
+def test_abi_version_and_error_string_without_gpu():
+    from spsslangevinintegration_amd import _capi
+    lib = _capi.load()
+    assert lib.spss_abi_version() == 1
+    # argument validation happens before any device call
+    p = _capi.SpssParams()
+    assert lib.spss_params_default(ctypes.byref(p)) == 0
+    assert p.model == 4 and p.rho == 0.8 and p.xi == 0.1
+    p.n = 0
+    h = ctypes.c_void_p()
+    assert lib.spss_create(ctypes.byref(p), ctypes.byref(h)) == _capi.SPSS_OK + 1
+    assert b"n must be positive" in lib.spss_last_error()
