@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Headline benchmark: particle-steps/s of the N = 1,048,576 cell-list stochastic step.
+
+Workload (the large second-order configuration, one GPU): N = 1,048,576 particles
+uniform in the unit box, rho = 0.004 (250 x 250 cells, ~52.7 neighbours), inertial
+phase model with m = 0.5, sigma = 1.5, D = 0.02 mapped onto the reference's
+variables (xi = 1/m = 2, sigma/m = 3, D/m^2 = 0.08, natural frequencies
+omega_i/m with omega_i ~ Lorentzian(0, 0.05) clipped at +-10), phi_dot(0) = 0,
+v0 = 0.005, dt = 0.002, cell list rebuilt every step, stochastic Euler with
+Philox noise (seed 0x5EED0003), FP32-compensated pair sums, FP64 state.
+
+Prints one JSON line.  `value` is whole-job particle-steps/s with the state
+resident in HBM.  `roofline` is the fused pair+update kernel timed with HIP
+events on its stream over the timed steps; `cpu_baseline` is the reference's
+own serial linked-list RHS (compiled from its sources into oracle/_ref) or,
+without that build, the oracle restatement, on a bounded sample.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s, N=1M cell list; batched vs plain at ≤1e-4 order-param err"
+HBM_PEAK = 8.0e12
+# algorithmic HBM bytes per particle of the fused pair+update kernel: FP64 state read and
+# write (2 x 32) + FP32 snapshot read (12) + particle index read (4)
+PAIR_KERNEL_BYTES = 80.0
+# algorithmic bytes per particle-step of the whole step (state 64, snapshot w+r 24,
+# key and sorted index w+r 16, cell table w+r 8 * 62500 / N)
+STEP_BYTES = 64.0 + 24.0 + 16.0 + 8.0 * 62500.0 / 1048576.0
+
+
+def headline_system(spss, n=1 << 20, precision=None, subdiv=1, seed=0x5EED0003):
+    m, sigma, D = 0.5, 1.5, 0.02
+    kw = dict(model=4, n=n, v0=0.005, xi=1.0 / m, sigma=sigma / m, rho=0.004, alpha=0.0, D_phi=D / m ** 2,
+              natfreq_local=1, seed=seed, cell_subdivision=subdiv,
+              precision=spss.PRECISION_FP32C if precision is None else precision)
+    rng = np.random.default_rng(seed)
+    st = np.zeros((n, 4))
+    st[:, 0] = rng.random(n)
+    st[:, 1] = rng.random(n)
+    st[:, 2] = rng.random(n) * 2.0 * np.pi
+    w = np.clip(0.05 * np.tan(np.pi * (rng.random(n) - 0.5)), -10.0, 10.0)  # Lorentzian(0, 0.05)
+    g = spss.ParticleSystem(**kw)
+    g.set_natural_frequencies(w / m)
+    g.set_state(st.reshape(-1))
+    return g, kw, st.reshape(-1), w / m
+
+
+def time_steps(g, spss, stepper, steps, warmup, dt=0.002):
+    g.run(stepper, spss.METHOD_LINKED_LIST, warmup, dt, spss.NOISE_PHILOX)
+    g.synchronize()
+    g.kernel_timer_start()
+    t0 = time.perf_counter()
+    g.run(stepper, spss.METHOD_LINKED_LIST, steps, dt, spss.NOISE_PHILOX)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    kms, launches = g.kernel_timer_stop()
+    return wall, kms / max(launches, 1)
+
+
+def cpu_baseline(n=1 << 20, steps=3, warmup=1):
+    """The reference's serial linked-list RHS + Euler update on the host (1 core)."""
+    m, sigma, D = 0.5, 1.5, 0.02
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_serial")
+    rng = np.random.default_rng(1)
+    st = np.zeros((n, 4))
+    st[:, 0], st[:, 1], st[:, 2] = rng.random(n), rng.random(n), rng.random(n) * 2 * np.pi
+    sample = f"N={n} rho=0.004 cell list + Euler, {steps} timed steps after {warmup}"
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as td:
+            cfg, inp = os.path.join(td, "c.txt"), os.path.join(td, "in.bin")
+            with open(cfg, "w") as f:
+                f.write(f"model=4\nn={n}\nv0=0.005\nxi={1 / m!r}\nsigma={sigma / m!r}\nrho=0.004\nalpha=0.0\n"
+                        f"D_phi={D / m ** 2!r}\ndt=0.002\nsteps={steps}\nwarmup={warmup}\nstepper=bench\n")
+            st.reshape(-1).tofile(inp)
+            out = subprocess.run([ref, cfg, inp, os.path.join(td, "o.bin")], capture_output=True, text=True,
+                                 check=True, timeout=600).stdout
+        res = json.loads(out.strip().splitlines()[-1])
+        return {"value": res["particle_steps_per_s"], "unit": "particle-steps/s", "cores": 1, "kind": "reference",
+                "sample": sample + " (reference sources, serial class family, g++ -O2)"}
+    import oracle
+    o = oracle.Oracle(model=4, n=n, v0=0.005, xi=1 / m, sigma=sigma / m, rho=0.004, D_phi=D / m ** 2)
+    s = st.reshape(-1).copy()
+    z = np.zeros(n)
+    for _ in range(warmup):
+        o.euler_step(3, s, 0.002, z)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.euler_step(3, s, 0.002, z)
+    sec = time.perf_counter() - t0
+    return {"value": n * steps / sec, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": sample + " (oracle restatement)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--subdiv", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see the sharding design in "
+              "DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import spsslangevinintegration_amd as spss
+
+    g, kw, st0, natf = headline_system(spss, subdiv=args.subdiv)
+    n = kw["n"]
+    print(f"[bench] headline n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+    wall, pair_ms = time_steps(g, spss, spss.STEPPER_EULER, args.steps, args.warmup)
+    value = n * args.steps / wall
+    ms_step = wall / args.steps * 1e3
+    # neighbour count of the final state (pair-interactions/s = sum_i n_i * steps / s)
+    pairs = C_pairs(g, spss)
+    achieved = PAIR_KERNEL_BYTES * n / (pair_ms * 1e-3)
+    # secondary: the SRK stepper (complementary force, 2 draws per particle)
+    g2, _, _, _ = headline_system(spss, subdiv=args.subdiv)
+    srk_steps = max(args.steps // 5, 10)
+    wall2, _ = time_steps(g2, spss, spss.STEPPER_SRK, srk_steps, 10)
+    g2.close()
+    print(f"[bench] {ms_step:.4f} ms/step, pair kernel {pair_ms * 1e3:.1f} us", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(steps=args.cpu_steps)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "N=1,048,576 uniform in the unit box, rho=0.004 cell list rebuilt every step, "
+                               "stochastic Euler dt=0.002, m=0.5 sigma=1.5 D=0.02, Lorentzian(0,0.05) omega, "
+                               "FP32-compensated pair sums, FP64 state",
+                   "n": n, "rho": 0.004, "cell_subdivision": args.subdiv, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None},
+        "cpu_baseline": cpu,
+        "pair_kernel_us": pair_ms * 1e3,
+        "step_roofline_frac": STEP_BYTES * value / HBM_PEAK,
+        "pair_interactions_per_s": pairs / n * value if pairs else None,
+        "srk_particle_steps_per_s": n * srk_steps / wall2,
+    }
+    g.close()
+    print(json.dumps(line), flush=True)
+
+
+def C_pairs(g, spss):
+    """sum_i n_i of the current state (count only, no list copy)"""
+    import ctypes
+    from spsslangevinintegration_amd import _capi
+    st = g.get_state()
+    cnt = ctypes.c_long()
+    _capi.check(g.lib.spss_neighbour_pairs(g.h, spss.METHOD_LINKED_LIST, _capi.ptr(st), None, 0,
+                                           ctypes.byref(cnt)))
+    return cnt.value
+
+
+if __name__ == "__main__":
+    main()

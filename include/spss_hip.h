@@ -111,6 +111,8 @@ int spss_synchronize(spss_system *sys);
 int spss_set_state(spss_system *sys, const double *host_state);
 int spss_get_state(spss_system *sys, double *host_state);
 int spss_set_state_device(spss_system *sys, const double *device_state);
+/* current device state buffer; FP32C stochastic steps alternate between two buffers, so the
+ * pointer is valid until the next step */
 double *spss_state_device(spss_system *sys);
 
 /* ParticleSystemFirstOrderPtr::SetNaturalFrequencies (ParticleSystemFirstOrderPtr.cpp:126-129) */
@@ -150,7 +152,8 @@ int spss_verlet_status(spss_system *sys, int *should_update, double *accumulated
 int spss_get_verlet_lists(spss_system *sys, int *host_counts, int *host_flat, long cap, long *total);
 
 /* neighbour set of one stage state (host, s*n doubles) found by `method`, as (i, j) pairs sorted
- * by (i, j): at most `cap` pairs into host_pairs[2*cap]; *count = number found */
+ * by (i, j): at most `cap` pairs into host_pairs[2*cap]; *count = number found.  With
+ * host_pairs == NULL only the count is computed. */
 int spss_neighbour_pairs(spss_system *sys, int method, const double *host_stage, int32_t *host_pairs,
                          long cap, long *count);
 
@@ -159,6 +162,12 @@ int spss_neighbour_pairs(spss_system *sys, int method, const double *host_stage,
 int spss_order_parameter(spss_system *sys, int slot, double *re, double *im);
 /* ApplyPeriodicBoundaryConditions(Real*, long) on the device state (PeriodicBoundaryConditions.cpp:197-205) */
 int spss_apply_periodic_boundary_conditions(spss_system *sys);
+
+/* Instrumentation with no reference counterpart (the reference only prints wall-clock
+ * seconds per output frame, BinaryObserverFirstOrderPtr.cpp:158-160): HIP events on the
+ * handle's stream around every launch of the RHS pair kernel between start and stop. */
+int spss_kernel_timer_start(spss_system *sys);
+int spss_kernel_timer_stop(spss_system *sys, double *total_ms, long *launches);
 
 /* AveragingObserver (Observers/AveragingObserver.cpp:66-105): running FP64 sum of the state */
 int spss_averaging_reset(spss_system *sys);

@@ -28,6 +28,7 @@ EXPORTS = [
     "spss_calculate_max_displacement", "spss_verlet_status", "spss_get_verlet_lists",
     "spss_neighbour_pairs", "spss_order_parameter", "spss_apply_periodic_boundary_conditions",
     "spss_averaging_reset", "spss_averaging_accumulate", "spss_averaging_get",
+    "spss_kernel_timer_start", "spss_kernel_timer_stop",
 ]
 
 
@@ -93,6 +94,8 @@ def load(path=None):
         "spss_averaging_reset": (i, [vp]),
         "spss_averaging_accumulate": (i, [vp]),
         "spss_averaging_get": (i, [vp, vp, C.POINTER(l)]),
+        "spss_kernel_timer_start": (i, [vp]),
+        "spss_kernel_timer_stop": (i, [vp, C.POINTER(d), C.POINTER(l)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

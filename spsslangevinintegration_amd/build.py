@@ -1,6 +1,6 @@
 """Build libspss_hip.so (the HIP kernels + C ABI) for gfx950 with hipcc.
 
-    python -m spsslangevinintegration_amd.build        # or __graft_entry__.build()
+    python spsslangevinintegration_amd/build.py        # or __graft_entry__.build()
 
 Every .hip file under csrc/ is compiled to an object in _build/ (in parallel)
 and linked into spsslangevinintegration_amd/libspss_hip.so.  All device code is

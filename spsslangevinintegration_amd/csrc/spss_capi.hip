@@ -51,7 +51,7 @@ struct spss_system {
   int ncvx = 1, ncvy = 1;  // grid the Verlet candidates come from
   double r_min = 0.0, r_max = 0.0;
   // device buffers
-  double *state = nullptr, *stage = nullptr, *k[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *state = nullptr, *state_alt = nullptr, *stage = nullptr, *k[4] = {nullptr, nullptr, nullptr, nullptr};
   double *deriv = nullptr, *compl_f = nullptr, *nat = nullptr, *noise = nullptr;
   double *partial = nullptr, *scal = nullptr, *acc = nullptr;
   double *io[3] = {nullptr, nullptr, nullptr};
@@ -69,6 +69,7 @@ struct spss_system {
   long step = 0;
   long avg_count = 0;
   FastPath *fast = nullptr;
+  KernelTimer *timer = nullptr;  // around the pair kernel of the RHS
 };
 
 namespace {
@@ -186,7 +187,9 @@ int rhs_device(spss_system *h, int method, const double *d_state, const double *
       } else {
         TRY(ensure_cells(h, h->cells, h->ncx, h->ncy));
         launch_cell_build(st, h->P, h->stage, h->ncx, h->ncy, h->cells);
+        if (h->timer) h->timer->begin(st);
         launch_linked_list(st, h->P, h->stage, h->ncx, h->ncy, h->cells, nat, d_knext, compl_out);
+        if (h->timer) h->timer->end(st);
       }
       break;
     case SPSS_METHOD_VERLET:
@@ -267,7 +270,29 @@ int rk4_step(spss_system *h, int method, double dt) {
   return finish_step_displacement(h);
 }
 
+// FP32C linked list: the fused step of the fast path, double-buffered state
+int fast_fused_step(spss_system *h, int srk, double dt, int noise_mode, const double *noise) {
+  int kmode;
+  const double *kptr;
+  TRY(noise_source(h, noise_mode, noise, (size_t)(srk ? 2 : 1) * h->n, &kmode, &kptr));
+  if (!h->state_alt) TRY(dalloc_n(&h->state_alt, (size_t)h->s * h->n));
+  h->P.higher_order = srk ? 1 : 0;
+  int rc = fast_step(h->fast, h->stream, h->P, srk, h->state, h->state_alt, h->have_nat ? h->nat : nullptr,
+                     srk ? h->compl_f : nullptr, dt, h->p.D_phi, h->p.xi, kmode, kptr, h->p.seed,
+                     (uint64_t)h->step);
+  if (rc != SPSS_OK) return fail(rc, fast_last_error());
+  std::swap(h->state, h->state_alt);
+  ++h->step;
+  return SPSS_OK;
+}
+
+bool use_fused(const spss_system *h, int method) {
+  return h->p.precision == SPSS_PRECISION_FP32C && resolve_method(h, method) == SPSS_METHOD_LINKED_LIST &&
+         h->s != 6;
+}
+
 int euler_step(spss_system *h, int method, double dt, int noise_mode, const double *noise) {
+  if (use_fused(h, method)) return fast_fused_step(h, 0, dt, noise_mode, noise);
   const size_t m = (size_t)h->s * h->n;
   launch_fill_double(h->stream, (long)m, h->deriv, 0.0);  // std::fill(derivative, 0), :43
   TRY(rhs_device(h, method, h->state, h->deriv, h->deriv, 0.0, dt));
@@ -285,6 +310,10 @@ int euler_step(spss_system *h, int method, double dt, int noise_mode, const doub
 
 int srk_step(spss_system *h, int method, double dt, int noise_mode, const double *noise) {
   if (h->s != 4) return fail(SPSS_ERR_UNSUPPORTED, "the stochastic Runge-Kutta stepper is defined for model 4");
+  if (use_fused(h, method)) {
+    h->higher_order = true;  // StochasticRungeKuttaStepperPtr.hpp:44
+    return fast_fused_step(h, 1, dt, noise_mode, noise);
+  }
   const size_t m = (size_t)h->s * h->n;
   launch_fill_double(h->stream, (long)m, h->deriv, 0.0);
   h->higher_order = true;  // StochasticRungeKuttaStepperPtr.hpp:44
@@ -428,7 +457,8 @@ int spss_destroy(spss_system *h) {
   if (!h) return SPSS_OK;
   if (h->stream) hipStreamSynchronize(h->stream);
   fast_destroy(h->fast);
-  void *bufs[] = {h->state, h->stage, h->k[0], h->k[1], h->k[2], h->k[3], h->deriv, h->compl_f, h->nat,
+  delete h->timer;
+  void *bufs[] = {h->state, h->state_alt, h->stage, h->k[0], h->k[1], h->k[2], h->k[3], h->deriv, h->compl_f, h->nat,
                   h->noise, h->partial, h->scal, h->acc, h->io[0], h->io[1], h->io[2], h->vcount, h->vstart,
                   h->vlist, h->pcount, h->pstart, h->plist};
   for (void *b : bufs) hipFree(b);
@@ -647,6 +677,10 @@ int spss_neighbour_pairs(spss_system *h, int method, const double *host_stage, i
   launch_scan(st, h->pcount, h->pstart, h->n);
   HIPCHECK(hipMemcpyAsync(&total, h->pstart + h->n, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
+  if (!pairs) {  // count only
+    *count = total;
+    return SPSS_OK;
+  }
   if ((long)total > h->pcap) {
     hipFree(h->plist);
     h->plist = nullptr;
@@ -693,6 +727,29 @@ int spss_apply_periodic_boundary_conditions(spss_system *h) {
   if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
   launch_apply_pbc(h->stream, h->n, h->s, h->state, h->P.xs, h->P.ys, h->P.xrs, h->P.yrs);
   HIPCHECK(hipGetLastError());
+  return SPSS_OK;
+}
+
+int spss_kernel_timer_start(spss_system *h) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (!h->timer) h->timer = new KernelTimer();
+  h->timer->on = true;
+  h->timer->used = 0;
+  if (h->fast) fast_set_timer(h->fast, h->timer);
+  return SPSS_OK;
+}
+
+int spss_kernel_timer_stop(spss_system *h, double *total_ms, long *launches) {
+  if (!h || !total_ms || !launches) return fail(SPSS_ERR_INVALID, "NULL argument");
+  *total_ms = 0.0;
+  *launches = 0;
+  if (!h->timer) return SPSS_OK;
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  *total_ms = h->timer->total_ms();
+  *launches = h->timer->used;
+  h->timer->on = false;
+  h->timer->used = 0;
+  if (h->fast) fast_set_timer(h->fast, nullptr);
   return SPSS_OK;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "spss_device.hpp"
+#include "spss_launch.hpp"
 
 namespace spss {
 
@@ -12,10 +13,16 @@ struct FastPath;
 int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv);
 void fast_destroy(FastPath *f);
 void fast_set_stream(FastPath *f, hipStream_t st);
+void fast_set_timer(FastPath *f, KernelTimer *t);
 const char *fast_last_error();
 // RHS of one stage state (FP64 AoS) with FP32 pair math; writes k_next (and the complementary force)
 int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const double *nat,
              double *k_next, double *compl_out);
+// one fused stochastic step (srk = 0: Euler, 1: SRK) from state_in to state_out (distinct buffers):
+// cell build, pair sums and the update in the pair kernel; noise_mode 0 none, 1 device array, 3 Philox
+int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state_in, double *state_out,
+              const double *nat, double *compl_out, double dt, double D_phi, double friction, int noise_mode,
+              const double *noise, uint64_t seed, uint64_t step);
 // neighbour set the fast path uses for a stage state: per-particle counts, then lists
 int fast_pairs_count(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, int *count);
 int fast_pairs_fill(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const int *start,

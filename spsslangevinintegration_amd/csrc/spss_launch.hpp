@@ -8,6 +8,42 @@
 
 namespace spss {
 
+// HIP-event timer around one kernel family, enabled on request (bench instrumentation):
+// records an event pair per launch on the launching stream, sums the durations on read.
+struct KernelTimer {
+  bool on = false;
+  int used = 0;
+  hipEvent_t ev[2 * 4096];
+  bool created = false;
+  void begin(hipStream_t st) {
+    if (!on || used >= 4096) return;
+    if (!created) {
+      for (auto &e : ev) hipEventCreate(&e);
+      created = true;
+    }
+    hipEventRecord(ev[2 * used], st);
+  }
+  void end(hipStream_t st) {
+    if (!on || used >= 4096) return;
+    hipEventRecord(ev[2 * used + 1], st);
+    ++used;
+  }
+  double total_ms() {
+    double t = 0.0;
+    for (int k = 0; k < used; ++k) {
+      float ms = 0.f;
+      hipEventSynchronize(ev[2 * k + 1]);
+      hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
+      t += ms;
+    }
+    return t;
+  }
+  ~KernelTimer() {
+    if (created)
+      for (auto &e : ev) hipEventDestroy(e);
+  }
+};
+
 // device buffers of one cell list (stable counting sort over cell keys)
 struct CellBuffers {
   int *key = nullptr;     // [n] cell key of each particle

@@ -134,6 +134,15 @@ class ParticleSystem:
         A.check(self.lib.spss_neighbour_pairs(self.h, method, A.ptr(st), A.ptr(out), cnt.value, C.byref(cnt)))
         return out[: 2 * cnt.value].reshape(-1, 2)
 
+    # ---- instrumentation
+    def kernel_timer_start(self):
+        A.check(self.lib.spss_kernel_timer_start(self.h))
+
+    def kernel_timer_stop(self):
+        ms, cnt = C.c_double(), C.c_long()
+        A.check(self.lib.spss_kernel_timer_stop(self.h, C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
     # ---- observers
     def order_parameter(self, slot=None):
         slot = (4 if self.s == 6 else 2) if slot is None else slot
