@@ -116,7 +116,10 @@ def main():
         sys.exit(2)
 
     import spsslangevinintegration_amd as spss
+    from spsslangevinintegration_amd import _capi
 
+    # attainable HBM bandwidth of the build's own streaming copy kernel (1 GiB, read + write)
+    copy_gbps = _capi.copy_bandwidth(1 << 30, 10)
     g, kw, st0, natf = headline_system(spss, subdiv=args.subdiv)
     n = kw["n"]
     print(f"[bench] headline n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
@@ -157,6 +160,7 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": None},
         "cpu_baseline": cpu,
         "pair_kernel_us": pair_ms * 1e3,
+        "hbm_copy_gbps": copy_gbps,
         "step_roofline_frac": STEP_BYTES * value / HBM_PEAK,
         "pair_interactions_per_s": pairs / n * value if pairs else None,
         "srk_particle_steps_per_s": n * srk_steps / wall2,

@@ -168,6 +168,9 @@ int spss_apply_periodic_boundary_conditions(spss_system *sys);
  * handle's stream around every launch of the RHS pair kernel between start and stop. */
 int spss_kernel_timer_start(spss_system *sys);
 int spss_kernel_timer_stop(spss_system *sys, double *total_ms, long *launches);
+/* attainable HBM bandwidth of a plain 16-byte-vector copy of `bytes` bytes, repeated, in GB/s
+ * (read + write bytes / time); no reference counterpart */
+int spss_device_copy_bandwidth(long bytes, int repeats, double *gbps);
 
 /* AveragingObserver (Observers/AveragingObserver.cpp:66-105): running FP64 sum of the state */
 int spss_averaging_reset(spss_system *sys);

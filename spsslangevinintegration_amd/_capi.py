@@ -28,7 +28,7 @@ EXPORTS = [
     "spss_calculate_max_displacement", "spss_verlet_status", "spss_get_verlet_lists",
     "spss_neighbour_pairs", "spss_order_parameter", "spss_apply_periodic_boundary_conditions",
     "spss_averaging_reset", "spss_averaging_accumulate", "spss_averaging_get",
-    "spss_kernel_timer_start", "spss_kernel_timer_stop",
+    "spss_kernel_timer_start", "spss_kernel_timer_stop", "spss_device_copy_bandwidth",
 ]
 
 
@@ -96,6 +96,7 @@ def load(path=None):
         "spss_averaging_get": (i, [vp, vp, C.POINTER(l)]),
         "spss_kernel_timer_start": (i, [vp]),
         "spss_kernel_timer_stop": (i, [vp, C.POINTER(d), C.POINTER(l)]),
+        "spss_device_copy_bandwidth": (i, [l, i, C.POINTER(d)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -117,3 +118,10 @@ def ptr(a):
         return None
     assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"], "need a C-contiguous array"
     return a.ctypes.data
+
+
+def copy_bandwidth(nbytes=1 << 30, repeats=20):
+    """attainable HBM GB/s of the library's plain copy kernel"""
+    v = C.c_double()
+    check(load().spss_device_copy_bandwidth(int(nbytes), int(repeats), C.byref(v)))
+    return v.value

@@ -51,7 +51,7 @@ struct spss_system {
   int ncvx = 1, ncvy = 1;  // grid the Verlet candidates come from
   double r_min = 0.0, r_max = 0.0;
   // device buffers
-  double *state = nullptr, *state_alt = nullptr, *stage = nullptr, *k[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *state = nullptr, *stage = nullptr, *k[4] = {nullptr, nullptr, nullptr, nullptr};
   double *deriv = nullptr, *compl_f = nullptr, *nat = nullptr, *noise = nullptr;
   double *partial = nullptr, *scal = nullptr, *acc = nullptr;
   double *io[3] = {nullptr, nullptr, nullptr};
@@ -69,10 +69,30 @@ struct spss_system {
   long step = 0;
   long avg_count = 0;
   FastPath *fast = nullptr;
+  // the fused FP32C steps advance a resident copy of the state kept in cell order; `state`
+  // is then behind it until the next sync_master
+  bool master_stale = false;
   KernelTimer *timer = nullptr;  // around the pair kernel of the RHS
 };
 
 namespace {
+
+// brings `state` (original order) up to date with the resident copy of the fused steps
+int sync_master(spss_system *h) {
+  if (!h->master_stale) return SPSS_OK;
+  int rc = fast_resident_sync(h->fast, h->stream, h->state);
+  if (rc != SPSS_OK) return fail(rc, fast_last_error());
+  h->master_stale = false;
+  return SPSS_OK;
+}
+
+// before `state` changes by any other means than a fused step: the resident copy is dropped
+int master_write(spss_system *h, bool keep_values) {
+  if (keep_values) TRY(sync_master(h));
+  h->master_stale = false;
+  if (h->fast) fast_resident_drop(h->fast);
+  return SPSS_OK;
+}
 
 int dalloc(void **p, size_t bytes) {
   if (bytes == 0) bytes = 8;
@@ -258,6 +278,7 @@ int noise_source(spss_system *h, int mode, const double *noise, size_t count, in
 }
 
 int rk4_step(spss_system *h, int method, double dt) {
+  TRY(master_write(h, true));
   TRY(rhs_device(h, method, h->state, h->k[0], h->k[0], 0.0, dt));
   TRY(rhs_device(h, method, h->state, h->k[0], h->k[1], 0.5, dt));
   TRY(rhs_device(h, method, h->state, h->k[1], h->k[2], 0.5, dt));
@@ -270,18 +291,17 @@ int rk4_step(spss_system *h, int method, double dt) {
   return finish_step_displacement(h);
 }
 
-// FP32C linked list: the fused step of the fast path, double-buffered state
+// FP32C linked list: the fused step of the fast path on its resident copy of the state
 int fast_fused_step(spss_system *h, int srk, double dt, int noise_mode, const double *noise) {
   int kmode;
   const double *kptr;
   TRY(noise_source(h, noise_mode, noise, (size_t)(srk ? 2 : 1) * h->n, &kmode, &kptr));
-  if (!h->state_alt) TRY(dalloc_n(&h->state_alt, (size_t)h->s * h->n));
   h->P.higher_order = srk ? 1 : 0;
-  int rc = fast_step(h->fast, h->stream, h->P, srk, h->state, h->state_alt, h->have_nat ? h->nat : nullptr,
+  int rc = fast_step(h->fast, h->stream, h->P, srk, h->state, h->have_nat ? h->nat : nullptr,
                      srk ? h->compl_f : nullptr, dt, h->p.D_phi, h->p.xi, kmode, kptr, h->p.seed,
                      (uint64_t)h->step);
   if (rc != SPSS_OK) return fail(rc, fast_last_error());
-  std::swap(h->state, h->state_alt);
+  h->master_stale = true;
   ++h->step;
   return SPSS_OK;
 }
@@ -293,6 +313,7 @@ bool use_fused(const spss_system *h, int method) {
 
 int euler_step(spss_system *h, int method, double dt, int noise_mode, const double *noise) {
   if (use_fused(h, method)) return fast_fused_step(h, 0, dt, noise_mode, noise);
+  TRY(master_write(h, true));
   const size_t m = (size_t)h->s * h->n;
   launch_fill_double(h->stream, (long)m, h->deriv, 0.0);  // std::fill(derivative, 0), :43
   TRY(rhs_device(h, method, h->state, h->deriv, h->deriv, 0.0, dt));
@@ -314,6 +335,7 @@ int srk_step(spss_system *h, int method, double dt, int noise_mode, const double
     h->higher_order = true;  // StochasticRungeKuttaStepperPtr.hpp:44
     return fast_fused_step(h, 1, dt, noise_mode, noise);
   }
+  TRY(master_write(h, true));
   const size_t m = (size_t)h->s * h->n;
   launch_fill_double(h->stream, (long)m, h->deriv, 0.0);
   h->higher_order = true;  // StochasticRungeKuttaStepperPtr.hpp:44
@@ -458,7 +480,7 @@ int spss_destroy(spss_system *h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   fast_destroy(h->fast);
   delete h->timer;
-  void *bufs[] = {h->state, h->state_alt, h->stage, h->k[0], h->k[1], h->k[2], h->k[3], h->deriv, h->compl_f, h->nat,
+  void *bufs[] = {h->state, h->stage, h->k[0], h->k[1], h->k[2], h->k[3], h->deriv, h->compl_f, h->nat,
                   h->noise, h->partial, h->scal, h->acc, h->io[0], h->io[1], h->io[2], h->vcount, h->vstart,
                   h->vlist, h->pcount, h->pstart, h->plist};
   for (void *b : bufs) hipFree(b);
@@ -498,6 +520,7 @@ int spss_synchronize(spss_system *h) {
 
 int spss_set_state(spss_system *h, const double *host_state) {
   if (!h || !host_state) return fail(SPSS_ERR_INVALID, "NULL argument");
+  TRY(master_write(h, false));
   HIPCHECK(hipMemcpyAsync(h->state, host_state, sizeof(double) * h->s * h->n, hipMemcpyHostToDevice, h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   return SPSS_OK;
@@ -505,6 +528,7 @@ int spss_set_state(spss_system *h, const double *host_state) {
 
 int spss_get_state(spss_system *h, double *host_state) {
   if (!h || !host_state) return fail(SPSS_ERR_INVALID, "NULL argument");
+  TRY(sync_master(h));
   HIPCHECK(hipMemcpyAsync(host_state, h->state, sizeof(double) * h->s * h->n, hipMemcpyDeviceToHost, h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   return SPSS_OK;
@@ -512,15 +536,21 @@ int spss_get_state(spss_system *h, double *host_state) {
 
 int spss_set_state_device(spss_system *h, const double *d) {
   if (!h || !d) return fail(SPSS_ERR_INVALID, "NULL argument");
+  TRY(master_write(h, false));
   HIPCHECK(hipMemcpyAsync(h->state, d, sizeof(double) * h->s * h->n, hipMemcpyDeviceToDevice, h->stream));
   return SPSS_OK;
 }
 
-double *spss_state_device(spss_system *h) { return h ? h->state : nullptr; }
+// the caller may read and write the returned buffer: the resident copy is synced and dropped
+double *spss_state_device(spss_system *h) {
+  if (!h || master_write(h, true) != SPSS_OK) return nullptr;
+  return h->state;
+}
 
 int spss_set_natural_frequencies(spss_system *h, const double *w, int n) {
   if (!h || !w) return fail(SPSS_ERR_INVALID, "NULL argument");
   if (n != h->n) return fail(SPSS_ERR_INVALID, "natural frequencies must have n entries");
+  TRY(master_write(h, true));
   HIPCHECK(hipMemcpyAsync(h->nat, w, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   h->have_nat = true;
@@ -714,6 +744,7 @@ int spss_neighbour_pairs(spss_system *h, int method, const double *host_stage, i
 int spss_order_parameter(spss_system *h, int slot, double *re, double *im) {
   if (!h || !re || !im) return fail(SPSS_ERR_INVALID, "NULL argument");
   if (slot < 0 || slot >= h->s) return fail(SPSS_ERR_INVALID, "slot out of range");
+  TRY(sync_master(h));
   launch_order_parameter(h->stream, h->n, h->s, slot, h->state, h->partial, h->scal + 16);
   double v[2];
   HIPCHECK(hipMemcpyAsync(v, h->scal + 16, sizeof(v), hipMemcpyDeviceToHost, h->stream));
@@ -725,8 +756,40 @@ int spss_order_parameter(spss_system *h, int slot, double *re, double *im) {
 
 int spss_apply_periodic_boundary_conditions(spss_system *h) {
   if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
-  launch_apply_pbc(h->stream, h->n, h->s, h->state, h->P.xs, h->P.ys, h->P.xrs, h->P.yrs);
+  // elementwise, so it applies to the resident copy in cell order as well (cells are
+  // assigned from wrapped positions and do not change)
+  if (double *r = fast_resident_state(h->fast))
+    launch_apply_pbc(h->stream, h->n, h->s, r, h->P.xs, h->P.ys, h->P.xrs, h->P.yrs);
+  if (!h->master_stale) launch_apply_pbc(h->stream, h->n, h->s, h->state, h->P.xs, h->P.ys, h->P.xrs, h->P.yrs);
   HIPCHECK(hipGetLastError());
+  return SPSS_OK;
+}
+
+int spss_device_copy_bandwidth(long bytes, int repeats, double *gbps) {
+  if (bytes <= 0 || repeats <= 0 || !gbps) return fail(SPSS_ERR_INVALID, "bad argument");
+  bytes = (bytes / 16) * 16;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st;
+  HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  HIPCHECK(hipMalloc(&a, bytes));
+  HIPCHECK(hipMalloc(&b, bytes));
+  HIPCHECK(hipMemsetAsync(a, 1, bytes, st));
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  launch_copy16(st, bytes / 16, a, b);  // warm-up
+  HIPCHECK(hipEventRecord(e0, st));
+  for (int r = 0; r < repeats; ++r) launch_copy16(st, bytes / 16, a, b);
+  HIPCHECK(hipEventRecord(e1, st));
+  HIPCHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+  *gbps = 2.0 * (double)bytes * repeats / (ms * 1e-3) / 1e9;  // read + write
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  hipStreamDestroy(st);
   return SPSS_OK;
 }
 
@@ -765,6 +828,7 @@ int spss_averaging_reset(spss_system *h) {
 int spss_averaging_accumulate(spss_system *h) {
   if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
   if (!h->acc) TRY(spss_averaging_reset(h));
+  TRY(sync_master(h));
   launch_accumulate(h->stream, (long)h->s * h->n, h->state, h->acc);
   ++h->avg_count;
   return SPSS_OK;

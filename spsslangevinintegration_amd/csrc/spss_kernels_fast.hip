@@ -6,20 +6,23 @@
 //   fk_scatter   unstable bucket fill
 //   fk_rank      stable position inside the cell (ascending particle index) -> sorted order,
 //                and the snapshot: FP32 offsets of the particle inside its cell, phase in
-//                revolutions reduced to [0,1), the cell column; omega for the SRK force
+//                revolutions reduced to [0,1); omega for the SRK force
 //   fk_pairs     one workgroup per tile of W cells of one row: the (2k+1) rows x (W+2k)
-//                cells around it are staged into LDS with x re-based to the tile
-//                (FP64 rounding, once per staged particle); per lane, phase 1 tests the
-//                candidates of its (2k+1) contiguous row ranges in FP32 and compacts the
-//                in-range ones into an LDS list, phase 2 evaluates the coupling term for
-//                the list and accumulates it with a two-sum (FP32 value + FP32 error).
-//                Pairs whose FP32 distance lies in a guard band around rho^2 are re-tested
-//                with the reference's FP64 ClassC formula, so the neighbour sets equal
-//                the FP64 ones exactly.
+//                cells around it are staged into LDS with x re-based to the tile (FP64
+//                rounding, once per staged particle); each lane walks the (2k+1) contiguous
+//                row ranges of its particle in FP32 and accumulates the coupling terms with
+//                a two-sum (FP32 value + FP32 error).  Pairs whose FP32 distance lies in a
+//                guard band around rho^2 are re-tested with the reference's FP64 ClassC
+//                formula, so the neighbour sets equal the FP64 ones exactly.
 //   self terms (v0 cos phi, v0 sin phi, omega, -xi omega + sigma A / n) in FP64.
+//
+// Fused stochastic steps keep the state resident in cell order (see spss_fast_kernels.hpp):
+//   fk_pairs (MODE 3/4) -> fr_cell_counts -> scan of the arrivals -> fr_far_bucket -> fr_scatter,
+// five launches per step and no host synchronisation.
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <utility>
 
 #include "../../include/spss_hip.h"
 #include "spss_fast.hpp"
@@ -36,15 +39,29 @@ thread_local std::string g_fast_err;
 struct FastPath {
   int n = 0, s = 4;
   double rho = 0.0, rho2 = 0.0;
-  int nc = 1, k = 1, ncf = 1;
+  int nc = 1, k = 1, ncf = 1, cells = 1, nsb = 1;
   double H = 1.0;
   int W = 8, chunks = 1, ntiles = 1;
   float r2lo = 0.f, r2hi = 0.f;
   float alpha_rev = 0.f;
+  // rebuilt cell list (RHS, pair export, entry into resident mode)
   int *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr, *sorted = nullptr;
   int *blocksum = nullptr;
   float4 *snap = nullptr;
   float *omega_f = nullptr;
+  // resident copy of the fused steps (double-buffered where a step permutes)
+  bool resident = false;
+  bool resident_nat = false;   // rnat holds natural frequencies
+  bool romega_valid = false;   // romega holds the FP32 omega of the resident order (SRK)
+  int cur = 0;
+  double *rstate[2] = {nullptr, nullptr};
+  int *rid[2] = {nullptr, nullptr};
+  double *rnat[2] = {nullptr, nullptr};
+  int *rstart[2] = {nullptr, nullptr};
+  float4 *rsnap = nullptr;
+  float *romega = nullptr;
+  double *tmp_state = nullptr;
+  int *mv = nullptr, *cnt9 = nullptr, *farcnt = nullptr;
   hipStream_t st = nullptr;
   KernelTimer *timer = nullptr;
 };
@@ -65,6 +82,11 @@ static int ffail(const std::string &m) {
     if (e_ != hipSuccess) return ffail(std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+template <class T>
+static hipError_t falloc(T **p, size_t count) {
+  return hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
+}
+
 int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv) {
   *out = nullptr;
   if (subdiv < 1 || subdiv > 3) {
@@ -80,6 +102,10 @@ int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, 
                  "more than once (use the FP64 precision there)";
     return SPSS_ERR_UNSUPPORTED;
   }
+  if ((long)nc * subdiv * nc * subdiv >= (1L << 27)) {
+    g_fast_err = "the FP32C cell list supports at most 2^27 cells (int(1/rho) * cell_subdivision < 11585)";
+    return SPSS_ERR_UNSUPPORTED;
+  }
   FastPath *f = new FastPath();
   f->n = P.n;
   f->s = P.s;
@@ -89,6 +115,8 @@ int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, 
   f->k = subdiv;
   f->ncf = nc * subdiv;
   f->H = 1.0 / (double)f->ncf;
+  f->cells = f->ncf * f->ncf;
+  f->nsb = (f->cells + 256 * kScanItems - 1) / (256 * kScanItems);
   // tile width: ~200 particles per tile at the mean density, at most kMaxCols - 2k
   const double per_cell = (double)P.n / ((double)f->ncf * f->ncf);
   int W = (int)std::lround(200.0 / std::max(per_cell, 1e-9));
@@ -109,25 +137,49 @@ int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, 
   f->r2hi = (float)(f->rho2 * (1.0 + g));
   f->alpha_rev = (float)(P.alpha / (2.0 * M_PI));
   f->st = st;
-  const int cells = f->ncf * f->ncf;
-  const int nsb = (cells + 256 * kScanItems - 1) / (256 * kScanItems);
-  FCHK(hipMalloc(&f->key, sizeof(int) * P.n));
-  FCHK(hipMalloc(&f->slot, sizeof(int) * P.n));
-  FCHK(hipMalloc(&f->count, sizeof(int) * cells));
-  FCHK(hipMalloc(&f->start, sizeof(int) * (cells + 1)));
-  FCHK(hipMalloc(&f->tmp, sizeof(int) * P.n));
-  FCHK(hipMalloc(&f->sorted, sizeof(int) * P.n));
-  FCHK(hipMalloc(&f->blocksum, sizeof(int) * (nsb + 1)));
-  FCHK(hipMalloc(&f->snap, sizeof(float4) * P.n));
-  FCHK(hipMalloc(&f->omega_f, sizeof(float) * P.n));
+  const int cells = f->cells;
+  const size_t n = (size_t)P.n;
+  hipError_t e = hipSuccess;
+  auto A = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  A(falloc(&f->key, n));
+  A(falloc(&f->slot, n));
+  A(falloc(&f->count, cells));
+  A(falloc(&f->start, cells + 1));
+  A(falloc(&f->tmp, n));
+  A(falloc(&f->sorted, n));
+  A(falloc(&f->blocksum, f->nsb + 1));
+  A(falloc(&f->snap, n));
+  A(falloc(&f->omega_f, n));
+  for (int b = 0; b < 2; ++b) {
+    A(falloc(&f->rstate[b], n * P.s));
+    A(falloc(&f->rid[b], n));
+    A(falloc(&f->rnat[b], n));
+    A(falloc(&f->rstart[b], cells + 1));
+  }
+  A(falloc(&f->rsnap, n));
+  A(falloc(&f->romega, n));
+  A(falloc(&f->tmp_state, n * P.s));
+  A(falloc(&f->mv, n));
+  A(falloc(&f->cnt9, (size_t)cells * 9));
+  A(falloc(&f->farcnt, cells));
+  if (e != hipSuccess) {
+    fast_destroy(f);
+    return ffail(std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
   launch_fill_int(st, cells, f->count, 0);
+  launch_fill_int(st, cells, f->farcnt, 0);
   *out = f;
   return SPSS_OK;
 }
 
 void fast_destroy(FastPath *f) {
   if (!f) return;
-  void *b[] = {f->key, f->slot, f->count, f->start, f->tmp, f->sorted, f->blocksum, f->snap, f->omega_f};
+  void *b[] = {f->key,     f->slot,      f->count,     f->start,     f->tmp,       f->sorted,    f->blocksum,
+               f->snap,    f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],    f->rid[1],    f->rnat[0],
+               f->rnat[1], f->rstart[0], f->rstart[1], f->rsnap,     f->romega,    f->tmp_state, f->mv,
+               f->cnt9,    f->farcnt};
   for (void *p : b) hipFree(p);
   delete f;
 }
@@ -142,6 +194,8 @@ void fast_set_timer(FastPath *f, KernelTimer *t) {
 
 // the XCD-aware tile order needs a grid that is a multiple of the 8 XCDs
 static int tile_grid(int ntiles) { return ((ntiles + 7) / 8) * 8; }
+
+static unsigned nblocks(long n) { return (unsigned)((n + 255) / 256); }
 
 static FastArgs args_of(const FastPath *f) {
   FastArgs a;
@@ -158,55 +212,119 @@ static FastArgs args_of(const FastPath *f) {
   return a;
 }
 
-static void build(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, bool omega) {
+// sorted order, cell starts and snapshot of a stage state (original order)
+static void build(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, int *start, float4 *snap,
+                  float *omega) {
   const int n = f->n;
-  const unsigned nb = (unsigned)((n + 255) / 256);
-  const int cells = f->ncf * f->ncf;
-  const int nsb = (cells + 256 * kScanItems - 1) / (256 * kScanItems);
+  const unsigned nb = nblocks(n);
   if (P.s == 4)
     fk_keys<4><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->key, f->slot, f->count);
   else
     fk_keys<3><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->key, f->slot, f->count);
-  fk_scan_local<<<nsb, 256, 0, st>>>(f->count, cells, f->start, f->blocksum);
-  fk_scan_add<<<nsb, 256, 0, st>>>(f->count, cells, f->start, f->blocksum, nsb);
-  fk_scatter<<<nb, 256, 0, st>>>(n, f->key, f->slot, f->start, f->tmp);
+  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->ncf, f->cells, start, f->blocksum);
+  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->count, f->cells, start, f->blocksum, f->nsb);
+  fk_scatter<<<nb, 256, 0, st>>>(n, f->key, f->slot, start, f->tmp);
   if (P.s == 4)
-    fk_rank<4><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->H, f->key, f->start, f->tmp, f->sorted, f->snap,
-                                   omega ? f->omega_f : nullptr);
+    fk_rank<4><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->H, f->key, start, f->tmp, f->sorted, snap, omega);
   else
-    fk_rank<3><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->H, f->key, f->start, f->tmp, f->sorted, f->snap, nullptr);
+    fk_rank<3><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->H, f->key, start, f->tmp, f->sorted, snap, nullptr);
+}
+
+static PairBufs rebuilt_bufs(const FastPath *f, const double *stage) {
+  PairBufs B{};
+  B.stage = stage;
+  B.sorted = f->sorted;
+  B.start = f->start;
+  B.snap = f->snap;
+  B.omega_f = f->omega_f;
+  return B;
+}
+
+template <int S, bool SRK, int MODE>
+static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, const PairBufs &B,
+                         const StepArgs &U) {
+  fk_pairs<S, SRK, MODE><<<tile_grid(f->ntiles), kFB, 0, st>>>(P, args_of(f), B, U);
 }
 
 int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const double *nat,
              double *k_next, double *compl_out) {
   const bool srk = P.higher_order && compl_out && P.s == 4;
-  build(f, st, P, stage, srk);
-  const FastArgs a = args_of(f);
-  const unsigned grid = (unsigned)tile_grid(a.ntiles);
+  build(f, st, P, stage, f->start, f->snap, srk ? f->omega_f : nullptr);
+  PairBufs B = rebuilt_bufs(f, stage);
+  B.nat = nat;
+  B.k_next = k_next;
+  B.compl_out = srk ? compl_out : nullptr;
   if (f->timer) f->timer->begin(st);
-  if (P.s == 4) {
-    if (srk)
-      fk_pairs<4, true, 0><<<grid, kFB, 0, st>>>(P, a, stage, f->start, f->snap, f->omega_f, nat, k_next,
-                                                 compl_out, nullptr, nullptr, nullptr, StepArgs{});
-    else
-      fk_pairs<4, false, 0><<<grid, kFB, 0, st>>>(P, a, stage, f->start, f->snap, nullptr, nat, k_next,
-                                                  nullptr, nullptr, nullptr, nullptr, StepArgs{});
-  } else {
-    fk_pairs<3, false, 0><<<grid, kFB, 0, st>>>(P, a, stage, f->start, f->snap, nullptr, nat, k_next,
-                                                nullptr, nullptr, nullptr, nullptr, StepArgs{});
-  }
+  if (P.s == 4 && srk)
+    launch_pairs<4, true, 0>(f, st, P, B, StepArgs{});
+  else if (P.s == 4)
+    launch_pairs<4, false, 0>(f, st, P, B, StepArgs{});
+  else
+    launch_pairs<3, false, 0>(f, st, P, B, StepArgs{});
   if (f->timer) f->timer->end(st);
   FCHK(hipGetLastError());
   return SPSS_OK;
 }
 
-int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state_in, double *state_out,
-              const double *nat, double *compl_out, double dt, double D_phi, double friction, int noise_mode,
-              const double *noise, uint64_t seed, uint64_t step) {
-  build(f, st, P, state_in, srk != 0);
-  const FastArgs a = args_of(f);
+bool fast_is_resident(const FastPath *f) { return f && f->resident; }
+
+double *fast_resident_state(FastPath *f) { return (f && f->resident) ? f->rstate[f->cur] : nullptr; }
+
+void fast_resident_drop(FastPath *f) {
+  if (f) f->resident = false;
+}
+
+int fast_resident_sync(FastPath *f, hipStream_t st, double *state) {
+  if (!f->resident) return SPSS_OK;
+  const unsigned nb = nblocks(f->n);
+  if (f->s == 4)
+    fr_unpermute<4><<<nb, 256, 0, st>>>(f->n, f->rstate[f->cur], f->rid[f->cur], state);
+  else
+    fr_unpermute<3><<<nb, 256, 0, st>>>(f->n, f->rstate[f->cur], f->rid[f->cur], state);
+  FCHK(hipGetLastError());
+  return SPSS_OK;
+}
+
+// entry into resident mode: cell order of `state`, then the gather into the resident arrays
+static void resident_enter(FastPath *f, hipStream_t st, const DevParams &P, const double *state,
+                           const double *nat, bool srk) {
+  const int c = f->cur;
+  build(f, st, P, state, f->rstart[c], f->rsnap, (srk && P.s == 4) ? f->romega : nullptr);
+  const unsigned nb = nblocks(f->n);
+  if (P.s == 4)
+    fr_gather<4><<<nb, 256, 0, st>>>(f->n, state, f->sorted, nat, f->rstate[c], f->rid[c], f->rnat[c]);
+  else
+    fr_gather<3><<<nb, 256, 0, st>>>(f->n, state, f->sorted, nat, f->rstate[c], f->rid[c], f->rnat[c]);
+  f->resident = true;
+  f->resident_nat = nat != nullptr;
+}
+
+template <int S>
+static void resident_resort(FastPath *f, hipStream_t st, const DevParams &P, bool srk) {
+  const int c = f->cur, o = 1 - c;
+  const int n = f->n;
+  const unsigned nb = nblocks(n);
+  fr_cell_counts<<<nblocks(f->cells), 256, 0, st>>>(f->cells, f->rstart[c], f->mv, f->cnt9);
+  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->farcnt, f->cnt9, f->ncf, f->cells, f->rstart[o], f->blocksum);
+  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->farcnt, f->cells, f->rstart[o], f->blocksum, f->nsb);
+  fr_far_bucket<S><<<nb, 256, 0, st>>>(P, n, f->ncf, f->mv, f->tmp_state, f->rstart[o], f->cnt9, f->slot, f->tmp);
+  fr_scatter<S><<<nb, 256, 0, st>>>(P, n, f->ncf, f->H, f->rstart[c], f->rstart[o], f->mv, f->cnt9, f->tmp,
+                                    f->tmp_state, f->rid[c], f->resident_nat ? f->rnat[c] : nullptr,
+                                    f->rstate[o], f->rid[o], f->rnat[o], f->rsnap,
+                                    (srk && S == 4) ? f->romega : nullptr);
+  f->cur = o;
+}
+
+int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat,
+              double *compl_out, double dt, double D_phi, double friction, int noise_mode, const double *noise,
+              uint64_t seed, uint64_t step) {
+  const bool do_srk = srk && P.s == 4;
+  // the SRK step needs the FP32 omega of every particle, which an Euler-entered copy lacks
+  if (f->resident && (nat != nullptr) != f->resident_nat) f->resident = false;
+  if (f->resident && do_srk && !f->romega_valid) f->resident = false;
+  if (!f->resident) resident_enter(f, st, P, state, nat, do_srk);
+  f->romega_valid = do_srk;
   StepArgs U{};
-  U.state_out = state_out;
   U.dt = dt;
   U.sqrt_dt = std::sqrt(dt);
   U.int_phi = std::sqrt(2.0 * D_phi);
@@ -218,44 +336,57 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
   U.noise = noise;
   U.seed = seed;
   U.step = step;
-  const unsigned grid = (unsigned)tile_grid(a.ntiles);
+  const int c = f->cur;
+  PairBufs B{};
+  B.rstate = f->rstate[c];
+  B.rid = f->rid[c];
+  B.rnat = f->resident_nat ? f->rnat[c] : nullptr;
+  B.start = f->rstart[c];
+  B.snap = f->rsnap;
+  B.omega_f = f->romega;
+  B.compl_out = do_srk ? compl_out : nullptr;
+  B.tmp_state = f->tmp_state;
+  B.mv = f->mv;
+  B.farcnt = f->farcnt;
+  B.farslot = f->slot;
   if (f->timer) f->timer->begin(st);
-  if (P.s == 4 && srk)
-    fk_pairs<4, true, 4><<<grid, kFB, 0, st>>>(P, a, state_in, f->start, f->snap, f->omega_f, nat, nullptr,
-                                               compl_out, nullptr, nullptr, nullptr, U);
+  if (do_srk)
+    launch_pairs<4, true, 4>(f, st, P, B, U);
   else if (P.s == 4)
-    fk_pairs<4, false, 3><<<grid, kFB, 0, st>>>(P, a, state_in, f->start, f->snap, nullptr, nat, nullptr, nullptr,
-                                                nullptr, nullptr, nullptr, U);
+    launch_pairs<4, false, 3>(f, st, P, B, U);
   else
-    fk_pairs<3, false, 3><<<grid, kFB, 0, st>>>(P, a, state_in, f->start, f->snap, nullptr, nat, nullptr, nullptr,
-                                                nullptr, nullptr, nullptr, U);
+    launch_pairs<3, false, 3>(f, st, P, B, U);
   if (f->timer) f->timer->end(st);
+  if (P.s == 4)
+    resident_resort<4>(f, st, P, do_srk);
+  else
+    resident_resort<3>(f, st, P, false);
   FCHK(hipGetLastError());
   return SPSS_OK;
 }
 
 int fast_pairs_count(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, int *count) {
-  build(f, st, P, stage, false);
-  const FastArgs a = args_of(f);
+  build(f, st, P, stage, f->start, f->snap, nullptr);
+  PairBufs B = rebuilt_bufs(f, stage);
+  B.pcount = count;
   if (P.s == 4)
-    fk_pairs<4, false, 1><<<tile_grid(a.ntiles), kFB, 0, st>>>(P, a, stage, f->start, f->snap, nullptr, nullptr,
-                                                    nullptr, nullptr, count, nullptr, nullptr, StepArgs{});
+    launch_pairs<4, false, 1>(f, st, P, B, StepArgs{});
   else
-    fk_pairs<3, false, 1><<<tile_grid(a.ntiles), kFB, 0, st>>>(P, a, stage, f->start, f->snap, nullptr, nullptr,
-                                                    nullptr, nullptr, count, nullptr, nullptr, StepArgs{});
+    launch_pairs<3, false, 1>(f, st, P, B, StepArgs{});
   FCHK(hipGetLastError());
   return SPSS_OK;
 }
 
+// lists of the stage state of the preceding fast_pairs_count
 int fast_pairs_fill(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const int *pstart,
                     int *plist) {
-  const FastArgs a = args_of(f);
+  PairBufs B = rebuilt_bufs(f, stage);
+  B.pstart = pstart;
+  B.plist = plist;
   if (P.s == 4)
-    fk_pairs<4, false, 2><<<tile_grid(a.ntiles), kFB, 0, st>>>(P, a, stage, f->start, f->snap, nullptr, nullptr,
-                                                    nullptr, nullptr, nullptr, pstart, plist, StepArgs{});
+    launch_pairs<4, false, 2>(f, st, P, B, StepArgs{});
   else
-    fk_pairs<3, false, 2><<<tile_grid(a.ntiles), kFB, 0, st>>>(P, a, stage, f->start, f->snap, nullptr, nullptr,
-                                                    nullptr, nullptr, nullptr, pstart, plist, StepArgs{});
+    launch_pairs<3, false, 2>(f, st, P, B, StepArgs{});
   FCHK(hipGetLastError());
   return SPSS_OK;
 }

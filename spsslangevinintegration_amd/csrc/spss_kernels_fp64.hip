@@ -560,6 +560,13 @@ __global__ void k_fill_double(long m, double *__restrict__ p, double v) {
   if (k < m) p[k] = v;
 }
 
+// plain streaming copy (16-byte vectors): measures attainable HBM bandwidth and calibrates
+// the FETCH_SIZE / WRITE_SIZE counters against a kernel of known traffic
+__global__ __launch_bounds__(256) void k_copy16(long n16, const uint4 *__restrict__ src, uint4 *__restrict__ dst) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride) dst[k] = src[k];
+}
+
 __global__ void k_fill_int(long m, int *__restrict__ p, int v) {
   long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < m) p[k] = v;
@@ -679,6 +686,10 @@ void launch_accumulate(hipStream_t st, long m, const double *state, double *acc)
 
 void launch_fill_double(hipStream_t st, long m, double *p, double v) {
   if (m > 0) k_fill_double<<<dim3(nblk(m)), dim3(kBlock), 0, st>>>(m, p, v);
+}
+
+void launch_copy16(hipStream_t st, long n16, const void *src, void *dst) {
+  if (n16 > 0) k_copy16<<<dim3(256 * 16), dim3(256), 0, st>>>(n16, (const uint4 *)src, (uint4 *)dst);
 }
 
 void launch_fill_int(hipStream_t st, long m, int *p, int v) {

@@ -89,5 +89,6 @@ void launch_apply_pbc(hipStream_t st, int n, int s, double *state, double xs, do
 void launch_accumulate(hipStream_t st, long m, const double *state, double *acc);
 void launch_fill_double(hipStream_t st, long m, double *p, double v);
 void launch_fill_int(hipStream_t st, long m, int *p, int v);
+void launch_copy16(hipStream_t st, long n16, const void *src, void *dst);
 
 }  // namespace spss
