@@ -756,11 +756,10 @@ int spss_order_parameter(spss_system *h, int slot, double *re, double *im) {
 
 int spss_apply_periodic_boundary_conditions(spss_system *h) {
   if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
-  // elementwise, so it applies to the resident copy in cell order as well (cells are
-  // assigned from wrapped positions and do not change)
-  if (double *r = fast_resident_state(h->fast))
-    launch_apply_pbc(h->stream, h->n, h->s, r, h->P.xs, h->P.ys, h->P.xrs, h->P.yrs);
-  if (!h->master_stale) launch_apply_pbc(h->stream, h->n, h->s, h->state, h->P.xs, h->P.ys, h->P.xrs, h->P.yrs);
+  // a wrap can move a particle at x = -0 from the last cell to the first: the resident copy is
+  // rebuilt from the wrapped state at the next fused step
+  TRY(master_write(h, true));
+  launch_apply_pbc(h->stream, h->n, h->s, h->state, h->P.xs, h->P.ys, h->P.xrs, h->P.yrs);
   HIPCHECK(hipGetLastError());
   return SPSS_OK;
 }

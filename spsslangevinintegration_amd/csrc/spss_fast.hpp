@@ -26,8 +26,6 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
               double *compl_out, double dt, double D_phi, double friction, int noise_mode, const double *noise,
               uint64_t seed, uint64_t step);
 bool fast_is_resident(const FastPath *f);
-// FP64 state of the resident copy in cell order (null when not resident)
-double *fast_resident_state(FastPath *f);
 // writes the resident state back into `state` in original order; the copy stays resident
 int fast_resident_sync(FastPath *f, hipStream_t st, double *state);
 // forgets the resident copy (the next fused step re-enters from the caller's state)

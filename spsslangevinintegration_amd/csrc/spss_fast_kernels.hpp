@@ -5,11 +5,12 @@
 //  * rebuilt (MODE 0 RHS, MODE 1/2 neighbour export): every call sorts the particles
 //    of an FP64 stage state (original order) into cells -- fk_keys, a two-level scan,
 //    fk_scatter, fk_rank (stable: ascending particle index inside a cell);
-//  * resident (MODE 3 Euler, MODE 4 SRK): the FP64 state, the particle ids and the
+//  * resident (MODE 3, the stochastic steps): the FP64 state, the particle ids and the
 //    natural frequencies stay in cell order between steps.  The pair kernel writes the
-//    new state and each particle's move (one of the 3x3 neighbour cells) in place of
-//    the old order; fr_cell_counts, a two-level scan of the arrival counts and
-//    fr_scatter then move every particle to its new slot: inside a cell, particles that
+//    phasor sums of every particle in the old order; fr_move decides each particle's
+//    move (its new position depends on phase and omega only), fr_cell_counts and a
+//    two-level scan of the arrival counts give the new cell starts, and fr_scatter
+//    applies the step and writes every particle to its new slot: inside a cell, particles that
 //    stayed come first in their previous order, then arrivals grouped by the direction
 //    they came from, each group in its previous order, then the rare particles that jumped
 //    farther than the 3x3 block around their cell, by previous position (fr_far_bucket).
@@ -24,9 +25,10 @@ namespace spss {
 
 namespace {
 constexpr int kFB = 256;        // threads per pair workgroup
-constexpr int kStageMax = 1536; // staged particles per workgroup
+constexpr int kStageMax = 1024; // staged entries per workgroup (particles + row padding)
+constexpr int kPad = 4;         // padding entries after each staged row (>= the unroll - 1)
 constexpr int kMaxRows = 7;     // 2k+1 with k <= 3
-constexpr int kMaxCols = 64 + 6;
+constexpr int kMaxCols = 48;    // W + 2k
 constexpr int kScanItems = 8;   // counts per thread in the two-level scans
 }  // namespace
 
@@ -34,7 +36,6 @@ struct FastArgs {
   int n, ncf, k, W, chunks, ntiles;
   double H;
   float r2lo, r2hi;
-  float alpha_rev;
 };
 
 // buffers of one pair-kernel launch (unused ones are null)
@@ -45,7 +46,7 @@ struct PairBufs {
   const int *rid;        // resident: particle id of each sorted position
   const double *rnat;    // resident: natural frequency of each sorted position (or null)
   const int *start;      // [cells + 1] first sorted position of each cell
-  const float4 *snap;    // FP32 snapshot in cell order: offsets in the cell, phase in revolutions
+  const float4 *snap;    // FP32 snapshot in cell order: offsets in the cell, cos phi, sin phi
   const float *omega_f;  // SRK: omega of each sorted position
   const double *nat;     // rebuilt modes: natural frequency by particle id (or null)
   double *k_next;        // MODE 0, by particle id
@@ -53,13 +54,12 @@ struct PairBufs {
   int *pcount;           // MODE 1, by particle id
   const int *pstart;     // MODE 2, by particle id
   int *plist;            // MODE 2
-  double *tmp_state;     // MODE 3/4: new state of each (old) sorted position
-  int *mv;               // MODE 3/4: old cell * 16 + move of each (old) sorted position
-  int *farcnt;           // MODE 3/4: per cell, arrivals from farther than the 3x3 block (zeroed by the scan)
-  int *farslot;          // MODE 3/4: arrival slot of such a far move (unordered)
+  double2 *ssum;         // MODE 3: phasor sum S of each sorted position
+  double2 *tsum;         // MODE 3, SRK: omega-weighted phasor sum T
+  int *scnt;             // MODE 3: neighbour count
 };
 
-// the stochastic update of MODE 3 (Euler) and MODE 4 (SRK)
+// the stochastic update of fk_update (Euler or SRK)
 struct StepArgs {
   double dt, sqrt_dt, int_phi, c15, is3, sq2d, friction;
   int noise_mode;  // 0 none, 1 device array indexed by particle id, 3 Philox
@@ -105,11 +105,10 @@ __global__ void fk_keys(DevParams P, const double *__restrict__ stage, int ncf, 
   slot[i] = atomicAdd(count + c, 1);
 }
 
-// block-local exclusive scan of 2048 values per block; a value is count[c] plus, for the
-// resident re-sort, the near arrivals of cell c (either source may be null)
-__global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ count, const int *__restrict__ cnt9,
-                                                     int ncf, int m, int *__restrict__ start,
-                                                     int *__restrict__ blocksum) {
+// block-local exclusive scan of 2048 values per block; a value is count[c] + count2[c]
+// (count2 may be null)
+__global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ count, const int *__restrict__ count2,
+                                                     int m, int *__restrict__ start, int *__restrict__ blocksum) {
   __shared__ int sh[256];
   const int base = blockIdx.x * 256 * kScanItems + threadIdx.x * kScanItems;
   int v[kScanItems];
@@ -118,8 +117,8 @@ __global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ cou
   for (int q = 0; q < kScanItems; ++q) {
     v[q] = 0;
     if (base + q < m) {
-      if (count) v[q] += count[base + q];
-      if (cnt9) v[q] += new_count_of(cnt9, base + q, ncf);
+      v[q] = count[base + q];
+      if (count2) v[q] += count2[base + q];
     }
     s += v[q];
   }
@@ -140,9 +139,10 @@ __global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ cou
   if (threadIdx.x == 255) blocksum[blockIdx.x] = sh[255];
 }
 
-// adds the scanned block totals, writes start[m], zeroes count[] (when given) for the next build
-__global__ __launch_bounds__(256) void fk_scan_add(int *__restrict__ count, int m, int *__restrict__ start,
-                                                   const int *__restrict__ blocksum, int nblocks) {
+// adds the scanned block totals, writes start[m], zeroes count[] and count2[] for the next use
+__global__ __launch_bounds__(256) void fk_scan_add(int *__restrict__ count, int *__restrict__ count2, int m,
+                                                   int *__restrict__ start, const int *__restrict__ blocksum,
+                                                   int nblocks) {
   __shared__ int off;
   if (threadIdx.x == 0) {
     int s = 0;
@@ -156,7 +156,8 @@ __global__ __launch_bounds__(256) void fk_scan_add(int *__restrict__ count, int 
     const int c = base + q;
     if (c < m) {
       start[c] += off;
-      if (count) count[c] = 0;
+      count[c] = 0;
+      if (count2) count2[c] = 0;
     }
   }
 }
@@ -167,18 +168,15 @@ __global__ void fk_scatter(int n, const int *__restrict__ key, const int *__rest
   if (i < n) tmp[start[key[i]] + slot[i]] = i;
 }
 
-// snapshot entry of a particle: offsets inside cell (cx, cy) in FP64, rounded once;
-// phase in revolutions reduced to [0, 1) in FP64 before rounding
+// snapshot entry of a particle: offsets inside cell (cx, cy) and the unit phasor
+// (cos phi, sin phi), each computed in FP64 and rounded once to FP32
 template <int S>
-__device__ __forceinline__ float4 snap_entry(const DevParams &P, const double *st, int cx, int cy, double H,
-                                             int id) {
+__device__ __forceinline__ float4 snap_entry(const DevParams &P, const double *st, int cx, int cy, double H) {
   double xw, yw;
   apply_pbc(P, st[0], st[1], xw, yw);
-  const double offx = xw - (double)cx * H;
-  const double offy = yw - (double)cy * H;
-  const double rev = st[2] * (1.0 / (2.0 * M_PI));
-  const double red = rev - floor(rev);
-  return make_float4((float)offx, (float)offy, (float)red, __int_as_float(id));
+  double sn, cn;
+  sincos(st[2], &sn, &cn);
+  return make_float4((float)(xw - (double)cx * H), (float)(yw - (double)cy * H), (float)cn, (float)sn);
 }
 
 // stable rank + snapshot in sorted order
@@ -195,7 +193,7 @@ __global__ void fk_rank(DevParams P, const double *__restrict__ stage, int ncf, 
   const int p = b + r;
   sorted[p] = i;
   const double *st = stage + (long)i * S;
-  snap[p] = snap_entry<S>(P, st, c % ncf, c / ncf, H, i);
+  snap[p] = snap_entry<S>(P, st, c % ncf, c / ncf, H);
   if (omega_f) omega_f[p] = (S == 4) ? (float)st[3] : 0.f;
 }
 
@@ -208,12 +206,6 @@ __device__ __forceinline__ double philox_z(uint64_t seed, int i, uint32_t block,
   return z0;
 }
 
-__device__ __forceinline__ void two_sum_f(float a, float b, float &s, float &e) {
-  s = a + b;
-  float bb = s - a;
-  e = (a - (s - bb)) + (b - bb);
-}
-
 // FP64 re-test of a guard-band pair with the reference's formula
 // (x_i wrapped, x_j as stored, ClassCEffectiveParticleDistanceUnsigned, dr2 <= rho^2)
 __device__ __forceinline__ bool fp64_in_range(const DevParams &P, const double *pj, double x_i, double y_i) {
@@ -223,17 +215,51 @@ __device__ __forceinline__ bool fp64_in_range(const DevParams &P, const double *
   return dr2 <= P.rho2;
 }
 
-// MODE 0: RHS; 1: count neighbours; 2: write neighbour lists; 3: fused Euler; 4: fused SRK
+// inclusive prefix sum over the 64 lanes of a wave
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// MODE 0: RHS; 1: count neighbours; 2: write neighbour lists; 3: phasor sums of the resident
+// state for fk_update (the stochastic step)
+//
+// One workgroup per tile of W cells of one row.  The (2k+1) x (W+2k) cells around the tile
+// are staged into LDS (x re-based to the staged column 0 in FP64, rounded once; y as the
+// offset in its cell), each staged row followed by kPad entries far outside the box.  A
+// lane owns one particle of the tile and walks the (2k+1) contiguous LDS ranges of its
+// stencil four candidates at a time without per-candidate bounds: reads past the end of a
+// range land in farther cells of the same staged row (out of range, see below) or in the
+// padding, so the loop body has no branch on the common path.
+//
+// Range test: an FP32 distance below r2lo is in range, above r2hi out of range; the
+// guard band between them (rare) is decided by the reference's FP64 ClassC formula, and
+// only for candidates inside the lane's stencil.  A candidate beyond the stencil is at
+// least k cells = 1/int(1/rho) >= rho away in x, so its FP32 distance is never below r2lo.
+//
+// Coupling sums go through phasors:
+//   sum_j sin(phi_j - phi_i - alpha) = Im(e^{-i(phi_i + alpha)} S_i),  S_i = sum_j e^{i phi_j}
+//   sum_j (omega_j - omega_i) cos(phi_j - phi_i - alpha) = Re(e^{-i(phi_i + alpha)} (T_i - omega_i S_i)),
+//   T_i = sum_j omega_j e^{i phi_j},
+// with the phasor components rounded once from FP64 to FP32, accumulated in four FP32
+// partial sums per stencil row and combined in FP64.  The particle itself is among its
+// candidates exactly once (distinct stencil columns since int(1/rho) >= 3) and is removed
+// afterwards.
 template <int S, bool SRK, int MODE>
-__global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBufs B, StepArgs U) {
+__global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBufs B) {
   constexpr bool RES = MODE >= 3;
-  __shared__ float2 spos[kStageMax];               // x re-based to staged column 0, y offset in its cell
-  __shared__ float2 saux[kStageMax];               // phase in revolutions, global sorted position (bits)
+  constexpr bool SUMS = MODE == 0 || MODE >= 3;
+  constexpr bool LIST = MODE == 1 || MODE == 2;
+  __shared__ float4 sdat[kStageMax];             // staged x, y in its cell, cos phi, sin phi
   __shared__ float somega[SRK ? kStageMax : 1];
-  __shared__ uint8_t scol[kStageMax];              // staged column of each staged particle
-  __shared__ int cstart[kMaxRows][kMaxCols + 1];   // global sorted start of each staged cell
-  __shared__ int cpos[kMaxRows * kMaxCols + 1];    // LDS start of staged cell (r, q) at r*ncols+q
-  __shared__ int s_total;
+  __shared__ int sg[LIST ? kStageMax : 1];       // global sorted position (pair export)
+  __shared__ int cglob[kMaxRows * kMaxCols];     // global sorted start of staged cell (r, q) at r*ncols+q
+  __shared__ int ccnt[kMaxRows * kMaxCols];      // its particle count
+  __shared__ int cpos[kMaxRows * kMaxCols + 1];  // its LDS start; [ncells] = staged total with padding
 
   // XCD-aware tile order: consecutive tiles (neighbouring rows) share an XCD's L2
   const int per = gridDim.x / 8;  // the grid is a multiple of 8
@@ -249,55 +275,71 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
   const int ncf = A.ncf;
   const double H = A.H;
   const int *start = B.start;
-
-  // ---- staging plan, all threads: global extent of every staged cell
   const int ncells = nrows * ncols;
+
+  // ---- staging plan: global extent of every staged cell, then the LDS layout
   for (int e = threadIdx.x; e < ncells; e += kFB) {
     const int r = e / ncols, q = e % ncols;
     const int rrow = ((row + r - k) % ncf + ncf) % ncf;
     const int col = ((ca - k + q) % ncf + ncf) % ncf;
     const int c = rrow * ncf + col;
     const int g0 = start[c];
-    cstart[r][q] = g0;
-    cpos[e] = start[c + 1] - g0;  // count for now
+    cglob[e] = g0;
+    ccnt[e] = start[c + 1] - g0;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {  // exclusive prefix over at most 7 x 70 cells
-    int base = 0;
-    for (int e = 0; e < ncells; ++e) {
-      const int c = cpos[e];
-      cpos[e] = base;
-      base += c;
+  if (threadIdx.x < 64) {  // one wave: exclusive prefix of the LDS sizes (count + row padding)
+    const int lane = threadIdx.x;
+    constexpr int kPer = (kMaxRows * kMaxCols + 63) / 64;
+    int v[kPer];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = lane * kPer + q;
+      v[q] = (e < ncells) ? ccnt[e] + ((e % ncols == ncols - 1) ? kPad : 0) : 0;
+      s += v[q];
     }
-    cpos[ncells] = base;
-    s_total = base;
+    const int incl = wave_incl_scan(s, lane);
+    int run = incl - s;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = lane * kPer + q;
+      if (e < ncells) cpos[e] = run;
+      run += v[q];
+    }
+    if (lane == 63) cpos[ncells] = incl;
   }
   __syncthreads();
-  const int total = s_total;
+  const int total = cpos[ncells];
   const bool fits = total <= kStageMax;
 
   if (fits) {
-    // copy: staged entry e belongs to the last cell whose LDS start is <= e
+    // staged entry e belongs to the last cell whose LDS start is <= e
     for (int e = threadIdx.x; e < total; e += kFB) {
       int lo = 0, hi = ncells - 1;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (cpos[mid] <= e) lo = mid; else hi = mid - 1;
       }
-      const int r = lo / ncols, q = lo % ncols;
-      const int g = cstart[r][q] + (e - cpos[lo]);
-      const float4 v = B.snap[g];
-      const float xt = (float)((double)q * H + (double)v.x);
-      spos[e] = make_float2(xt, v.y);
-      saux[e] = make_float2(v.z, __int_as_float(g));
-      scol[e] = (uint8_t)q;
-      if (SRK) somega[e] = B.omega_f[g];
+      const int j = e - cpos[lo];
+      if (j < ccnt[lo]) {
+        const int q = lo % ncols;
+        const int g = cglob[lo] + j;
+        const float4 v = B.snap[g];
+        sdat[e] = make_float4((float)((double)q * H + (double)v.x), v.y, v.z, v.w);
+        if (SRK) somega[e] = B.omega_f[g];
+        if (LIST) sg[e] = g;
+      } else {  // row padding: never within range
+        sdat[e] = make_float4(1e30f, 0.f, 0.f, 0.f);
+        if (SRK) somega[e] = 0.f;
+        if (LIST) sg[e] = -1;
+      }
     }
   }
   __syncthreads();
 
-  // particles of the tile: staged row k (dr = 0), columns k .. k+W-1
-  const int gtb = cstart[k][k];
+  // particles of the tile: staged row k, columns k .. k+W-1 (contiguous in LDS and globally)
+  const int gtb = cglob[k * ncols + k];
   const int ntile = start[row * ncf + cb] - start[row * ncf + ca];
   const int ltb = fits ? cpos[k * ncols + k] : 0;
   const float r2lo = A.r2lo, r2hi = A.r2hi;
@@ -307,96 +349,117 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
     const int gp = gtb + u;  // global sorted position
     const int i = RES ? B.rid[gp] : B.sorted[gp];
     const double *pi = RES ? B.rstate + (long)gp * S : B.stage + (long)i * S;
-    double x_i64, y_i64;
-    apply_pbc(P, pi[0], pi[1], x_i64, y_i64);
-    float xi_t, yi, phi_i;
-    int qi;
-    if (fits) {
-      const float2 mp = spos[li], ma = saux[li];
-      xi_t = mp.x;
-      yi = mp.y;
-      phi_i = ma.x;
-      qi = scol[li];
-    } else {
-      const float4 mg = B.snap[gp];
-      xi_t = mg.x;  // offset in its own cell
-      yi = mg.y;
-      phi_i = mg.z;
-      qi = k;  // own cell column: last tile cell whose start <= gp
-      for (int q = k; q < k + W; ++q)
-        if (cstart[k][q] <= gp) qi = q;
+    // own staged column: last tile cell whose global start is <= gp
+    int qi = k;
+    {
+      int lo = k, hi = k + W - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cglob[k * ncols + mid] <= gp) lo = mid; else hi = mid - 1;
+      }
+      qi = lo;
     }
-    const float phia = phi_i + A.alpha_rev;
-    const float om_i = SRK ? (fits ? somega[li] : B.omega_f[gp]) : 0.f;
-    float acc = 0.f, accc = 0.f, cs = 0.f, csc = 0.f;
+    float4 me;
+    if (fits) {
+      me = sdat[li];
+    } else {
+      me = B.snap[gp];  // x as the offset in its own cell
+    }
+    const float xi_t = me.x, yi = me.y, ci = me.z, si = me.w;
+    double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
     int nnb = 0;
     int w = 0;
     const int wbase = (MODE == 2) ? B.pstart[i] : 0;
 
-    // a candidate within r2hi: guard-band re-test in FP64, then the coupling terms
-    auto use_pair = [&](float d2, int g, float phij, float omj) {
-      if (d2 >= r2lo) {
-        const double *pj = RES ? B.rstate + (long)g * S : B.stage + (long)B.sorted[g] * S;
-        if (!fp64_in_range(P, pj, x_i64, y_i64)) return;
-      }
-      ++nnb;
-      if (MODE == 0 || MODE >= 3) {
-        const float drev = phij - phia;
-        float s, e;
-        two_sum_f(acc, __builtin_amdgcn_sinf(drev), s, e);
-        acc = s;
-        accc += e;
-        if (SRK) {
-          two_sum_f(cs, (omj - om_i) * __builtin_amdgcn_cosf(drev), s, e);
-          cs = s;
-          csc += e;
-        }
-      } else if (MODE == 2) {
-        B.plist[wbase + w++] = B.sorted[g];
-      }
+    // FP64 decision of a guard-band candidate at global sorted position g
+    auto band_in = [&](int g) -> bool {
+      const double *pj = RES ? B.rstate + (long)g * S : B.stage + (long)B.sorted[g] * S;
+      double x_i64, y_i64;
+      apply_pbc(P, pi[0], pi[1], x_i64, y_i64);
+      return fp64_in_range(P, pj, x_i64, y_i64);
     };
 
     if (fits) {
-      if (MODE == 1 || MODE == 2) {
-        for (int r = 0; r < nrows; ++r) {
-          const float syi = (float)((double)yi - (double)(r - k) * H);
-          const int lo = cpos[r * ncols + qi - k];
-          const int hi = cpos[r * ncols + qi + k + 1];
-          for (int t = lo; t < hi; ++t) {
-            const float2 o = spos[t];
-            const float dx = o.x - xi_t;
-            const float dy = o.y - syi;
-            const float d2 = __builtin_fmaf(dy, dy, dx * dx);
-            if (d2 <= r2hi && t != li) {
-              const float2 a2 = saux[t];
-              use_pair(d2, __float_as_int(a2.y), a2.x, 0.f);
+      for (int r = 0; r < nrows; ++r) {
+        const float syi = (float)((double)yi - (double)(r - k) * H);
+        const int e0 = r * ncols + qi - k, e1 = r * ncols + qi + k;
+        const int lo = cpos[e0];
+        const int hi = cpos[e1] + ccnt[e1];  // end of the last stencil cell (before any padding)
+        float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an[4] = {0.f, 0.f, 0.f, 0.f};
+        float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int t = lo; t < hi; t += 4) {
+          float4 o[4];
+          float d2[4];
+          bool in[4], band[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            o[j] = sdat[t + j];
+            const float dx = o[j].x - xi_t;
+            const float dy = o[j].y - syi;
+            d2[j] = __builtin_fmaf(dy, dy, dx * dx);
+            in[j] = d2[j] <= r2hi;
+            band[j] = in[j] && d2[j] >= r2lo;
+          }
+          if (band[0] || band[1] || band[2] || band[3]) {  // rare: FP64 decision inside the stencil
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              if (band[j]) {
+                const int tj = t + j;
+                bool ok = tj < hi;
+                if (ok) {
+                  int lo2 = e0, hi2 = e1;  // stencil cell of tj
+                  while (lo2 < hi2) {
+                    const int mid = (lo2 + hi2 + 1) >> 1;
+                    if (cpos[mid] <= tj) lo2 = mid; else hi2 = mid - 1;
+                  }
+                  ok = band_in(cglob[lo2] + (tj - cpos[lo2]));
+                }
+                in[j] = ok;
+              }
+            }
+          }
+          if (SUMS) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const float m = in[j] ? 1.f : 0.f;
+              ax[j] = __builtin_fmaf(m, o[j].z, ax[j]);
+              ay[j] = __builtin_fmaf(m, o[j].w, ay[j]);
+              an[j] += m;
+              if (SRK) {
+                const float mo = m * somega[t + j];
+                bx[j] = __builtin_fmaf(mo, o[j].z, bx[j]);
+                by[j] = __builtin_fmaf(mo, o[j].w, by[j]);
+              }
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              if (in[j] && t + j < hi && t + j != li) {
+                ++nnb;
+                if (MODE == 2) B.plist[wbase + w++] = B.sorted[sg[t + j]];
+              }
             }
           }
         }
-      } else {
-        // the particle itself is among its candidates exactly once (distinct stencil
-        // columns since int(1/rho) >= 3): it is counted with d2 = 0 and removed below
-        for (int r = 0; r < nrows; ++r) {
-          const float syi = (float)((double)yi - (double)(r - k) * H);
-          const int lo = cpos[r * ncols + qi - k];
-          const int hi = cpos[r * ncols + qi + k + 1];
-#pragma unroll 2
-          for (int t = lo; t < hi; ++t) {
-            const float2 o = spos[t];
-            const float dx = o.x - xi_t;
-            const float dy = o.y - syi;
-            const float d2 = __builtin_fmaf(dy, dy, dx * dx);
-            if (d2 <= r2hi) {
-              const float2 a2 = saux[t];
-              use_pair(d2, __float_as_int(a2.y), a2.x, SRK ? somega[t] : 0.f);
-            }
+        if (SUMS) {
+          sx += ((double)ax[0] + (double)ax[1]) + ((double)ax[2] + (double)ax[3]);
+          sy += ((double)ay[0] + (double)ay[1]) + ((double)ay[2] + (double)ay[3]);
+          nnb += (int)(an[0] + an[1] + an[2] + an[3]);
+          if (SRK) {
+            tx += ((double)bx[0] + (double)bx[1]) + ((double)bx[2] + (double)bx[3]);
+            ty += ((double)by[0] + (double)by[1]) + ((double)by[2] + (double)by[3]);
           }
         }
+      }
+      if (SUMS) {  // remove the self term
         --nnb;
-        float s, e;
-        two_sum_f(acc, -__builtin_amdgcn_sinf(phi_i - phia), s, e);  // the self term added above
-        acc = s;
-        accc += e;
+        sx -= (double)ci;
+        sy -= (double)si;
+        if (SRK) {
+          const float omi = somega[li];
+          tx -= (double)(omi * ci);
+          ty -= (double)(omi * si);
+        }
       }
     } else {
       // global path (staging would overflow LDS): per stencil cell, own offsets shifted
@@ -405,30 +468,50 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
         const float syi = (float)((double)yi - (double)dr * H);
         for (int dc = -k; dc <= k; ++dc) {
           const float sxi = (float)((double)xi_t - (double)dc * H);
-          const int g0 = cstart[r][qi + dc];
-          const int rrow = ((row + dr) % ncf + ncf) % ncf;
-          const int col = ((ca - k + qi + dc) % ncf + ncf) % ncf;
-          const int g1 = start[rrow * ncf + col + 1];
+          const int e = r * ncols + qi + dc;
+          const int g0 = cglob[e], g1 = g0 + ccnt[e];
           for (int g = g0; g < g1; ++g) {
             if (g == gp) continue;
             const float4 o = B.snap[g];
             const float dx = o.x - sxi;
             const float dy = o.y - syi;
             const float d2 = __builtin_fmaf(dy, dy, dx * dx);
-            if (d2 <= r2hi) use_pair(d2, g, o.z, SRK ? B.omega_f[g] : 0.f);
+            if (d2 > r2hi) continue;
+            if (d2 >= r2lo && !band_in(g)) continue;
+            ++nnb;
+            if (SUMS) {
+              sx += (double)o.z;
+              sy += (double)o.w;
+              if (SRK) {
+                const float omj = B.omega_f[g];
+                tx += (double)(omj * o.z);
+                ty += (double)(omj * o.w);
+              }
+            } else if (MODE == 2) {
+              B.plist[wbase + w++] = B.sorted[g];
+            }
           }
         }
       }
     }
 
-    const double A64 = (double)acc + (double)accc;
     const double card = (double)nnb;
+    // e^{-i theta}, theta = phi_i + alpha, from the FP64 phase
+    double sn = 0.0, cn = 0.0, A64 = 0.0, C64 = 0.0;
+    if (MODE == 0) {
+      sincos(pi[2], &sn, &cn);
+      const double st = sn * P.cos_a + cn * P.sin_a;
+      const double ct = cn * P.cos_a - sn * P.sin_a;
+      A64 = ct * sy - st * sx;
+      if (SRK) {
+        const double wi = pi[3];
+        C64 = ct * (tx - wi * sx) + st * (ty - wi * sy);
+      }
+    }
     if (MODE == 1) {
       B.pcount[i] = nnb;
     } else if (MODE == 0) {
       const long ii = (long)i * S;
-      double sn, cn;
-      sincos(pi[2], &sn, &cn);
       const double nat_i = B.nat ? B.nat[i] : 0.0;
       B.k_next[ii] = P.v0 * cn;
       B.k_next[ii + 1] = P.v0 * sn;
@@ -441,92 +524,130 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
         } else if (nnb) {
           B.k_next[ii + 3] = -P.xi * omega_i + P.sigma * A64 / card;
         }
-        if (SRK && B.compl_out) {
-          double c = (double)cs + (double)csc;
-          if (nnb) c *= P.sigma / card;
-          B.compl_out[i] = c;
-        }
+        if (SRK && B.compl_out) B.compl_out[i] = nnb ? C64 * (P.sigma / card) : C64;
       } else {
         B.k_next[ii + 2] = nnb ? nat_i + P.sigma * A64 / card : nat_i;
       }
-    } else if (MODE >= 3) {
-      // fused stepper: the derivative of this particle as the zero-filled derivative vector
-      // of StochasticEulerStepperPtr.hpp:42-47 would hold it, then the update
-      double *out = B.tmp_state + (long)gp * S;
-      double sn, cn;
-      sincos(pi[2], &sn, &cn);
-      const double nat_i = B.rnat ? B.rnat[gp] : 0.0;
-      const double f0 = P.v0 * cn, f1 = P.v0 * sn;
-      if (S == 4) {
-        const double omega_i = pi[3];
-        const double f2 = omega_i;
-        double f3 = 0.0;
-        if (P.natfreq_local) {
-          f3 = nnb ? -P.xi * omega_i + nat_i + P.sigma * A64 / card : -P.xi * omega_i + nat_i;
-        } else if (nnb) {
-          f3 = -P.xi * omega_i + P.sigma * A64 / card;
-        }
-        if (MODE == 3) {  // StochasticEulerStepperPtr.hpp:50-67
-          const double z = (U.noise_mode == 1) ? U.noise[i] : (U.noise_mode == 3 ? philox_z(U.seed, i, 0, U.step) : 0.0);
-          const double dW = U.sqrt_dt * z;
-          out[0] = pi[0] + f0 * U.dt;
-          out[1] = pi[1] + f1 * U.dt;
-          out[2] = pi[2] + f2 * U.dt;
-          out[3] = pi[3] + (f3 * U.dt + U.int_phi * dW);
-        } else {  // StochasticRungeKuttaStepperPtr.hpp:50-71
-          double C = (double)cs + (double)csc;
-          if (nnb) C *= P.sigma / card;
-          if (B.compl_out) B.compl_out[i] = C;
-          double U1 = 0.0, U2 = 0.0;
-          if (U.noise_mode == 1) {
-            U1 = U.noise[2L * i];
-            U2 = U.noise[2L * i + 1];
-          } else if (U.noise_mode == 3) {
-            philox_normal2(U.seed, (uint32_t)i, 0, U.step, U1, U2);
-          }
-          const double dt = U.dt;
-          const double dW = U1 * U.sqrt_dt;
-          const double dZ = U.c15 * (U1 + U.is3 * U2);
-          out[0] = pi[0] + (f0 * dt - 0.5 * f2 * f1 * dt * dt);
-          out[1] = pi[1] + (f1 * dt + 0.5 * f2 * f0 * dt * dt);
-          out[2] = pi[2] + (f2 * dt + 0.5 * f3 * dt * dt + U.sq2d * dZ);
-          out[3] = pi[3] + (f3 * dt + 0.5 * (C - U.friction * f3) * dt * dt + U.sq2d * (dW - U.friction * dZ));
-        }
-      } else {  // model 3 Euler
-        const double f2 = nnb ? nat_i + P.sigma * A64 / card : nat_i;
-        const double z = (U.noise_mode == 1) ? U.noise[i] : (U.noise_mode == 3 ? philox_z(U.seed, i, 0, U.step) : 0.0);
-        out[0] = pi[0] + f0 * U.dt;
-        out[1] = pi[1] + f1 * U.dt;
-        out[2] = pi[2] + (f2 * U.dt + U.int_phi * (U.sqrt_dt * z));
-      }
-      // the particle's move from its cell (row, ca + qi - k)
-      double xw, yw;
-      apply_pbc(P, out[0], out[1], xw, yw);
-      const int nx = cell_coord(xw, ncf), ny = cell_coord(yw, ncf);
-      const int ocol = ca + qi - k;
-      int dxc = nx - ocol, dyc = ny - row;
-      if (dxc == ncf - 1) dxc = -1;
-      if (dxc == 1 - ncf) dxc = 1;
-      if (dyc == ncf - 1) dyc = -1;
-      if (dyc == 1 - ncf) dyc = 1;
-      int d;
-      if (dxc >= -1 && dxc <= 1 && dyc >= -1 && dyc <= 1) {
-        d = (dyc + 1) * 3 + (dxc + 1);
-      } else {
-        d = kFar;
-        B.farslot[gp] = atomicAdd(B.farcnt + ny * ncf + nx, 1);
-      }
-      B.mv[gp] = ((row * ncf + ocol) << 4) | d;
+    } else if (MODE == 3) {
+      B.ssum[gp] = make_double2(sx, sy);
+      if (SRK) B.tsum[gp] = make_double2(tx, ty);
+      B.scnt[gp] = nnb;
     }
   }
+}
+
+// The stochastic step of one particle from its phasor sums (StochasticEulerStepperPtr.hpp:50-67,
+// StochasticRungeKuttaStepperPtr.hpp:50-71): the derivative as the zero-filled derivative vector of
+// StochasticEulerStepperPtr.hpp:42-47 would hold it, then the update.  The new position depends on
+// the phase and omega only, so step_position gives it bit-identically without the sums.
+template <int S, bool SRK>
+__device__ __forceinline__ void step_position(const DevParams &P, const double *pi, const StepArgs &U, double &sn,
+                                              double &cn, double &x1, double &y1) {
+  sincos(pi[2], &sn, &cn);
+  const double f0 = P.v0 * cn, f1 = P.v0 * sn;
+  if (S == 4 && SRK) {
+    const double f2 = pi[3], dt = U.dt;
+    x1 = pi[0] + (f0 * dt - 0.5 * f2 * f1 * dt * dt);
+    y1 = pi[1] + (f1 * dt + 0.5 * f2 * f0 * dt * dt);
+  } else {
+    x1 = pi[0] + f0 * U.dt;
+    y1 = pi[1] + f1 * U.dt;
+  }
+}
+
+template <int S, bool SRK>
+__device__ __forceinline__ void step_particle(const DevParams &P, const double *pi, int i, double nat_i, double2 sm,
+                                              double2 tm, int nnb, const StepArgs &U, double *out, double &C) {
+  double sn, cn;
+  step_position<S, SRK>(P, pi, U, sn, cn, out[0], out[1]);
+  const double st = sn * P.cos_a + cn * P.sin_a;  // sin(phi_i + alpha)
+  const double ct = cn * P.cos_a - sn * P.sin_a;  // cos(phi_i + alpha)
+  const double A64 = ct * sm.y - st * sm.x;
+  const double card = (double)nnb;
+  C = 0.0;
+  if (S == 4) {
+    const double omega_i = pi[3];
+    const double f2 = omega_i;
+    double f3 = 0.0;
+    if (P.natfreq_local) {
+      f3 = nnb ? -P.xi * omega_i + nat_i + P.sigma * A64 / card : -P.xi * omega_i + nat_i;
+    } else if (nnb) {
+      f3 = -P.xi * omega_i + P.sigma * A64 / card;
+    }
+    if (!SRK) {
+      const double z = (U.noise_mode == 1) ? U.noise[i] : (U.noise_mode == 3 ? philox_z(U.seed, i, 0, U.step) : 0.0);
+      const double dW = U.sqrt_dt * z;
+      out[2] = pi[2] + f2 * U.dt;
+      out[3] = pi[3] + (f3 * U.dt + U.int_phi * dW);
+    } else {
+      const double C64 = ct * (tm.x - omega_i * sm.x) + st * (tm.y - omega_i * sm.y);
+      C = nnb ? C64 * (P.sigma / card) : C64;
+      double U1 = 0.0, U2 = 0.0;
+      if (U.noise_mode == 1) {
+        U1 = U.noise[2L * i];
+        U2 = U.noise[2L * i + 1];
+      } else if (U.noise_mode == 3) {
+        philox_normal2(U.seed, (uint32_t)i, 0, U.step, U1, U2);
+      }
+      const double dt = U.dt;
+      const double dW = U1 * U.sqrt_dt;
+      const double dZ = U.c15 * (U1 + U.is3 * U2);
+      out[2] = pi[2] + (f2 * dt + 0.5 * f3 * dt * dt + U.sq2d * dZ);
+      out[3] = pi[3] + (f3 * dt + 0.5 * (C - U.friction * f3) * dt * dt + U.sq2d * (dW - U.friction * dZ));
+    }
+  } else {  // model 3 Euler
+    const double f2 = nnb ? nat_i + P.sigma * A64 / card : nat_i;
+    const double z = (U.noise_mode == 1) ? U.noise[i] : (U.noise_mode == 3 ? philox_z(U.seed, i, 0, U.step) : 0.0);
+    out[2] = pi[2] + (f2 * U.dt + U.int_phi * (U.sqrt_dt * z));
+  }
+}
+
+template <int S>
+__device__ __forceinline__ int cell_of_position(const DevParams &P, double x, double y, int ncf) {
+  double xw, yw;
+  apply_pbc(P, x, y, xw, yw);
+  return cell_coord(yw, ncf) * ncf + cell_coord(xw, ncf);
 }
 
 // ---------------------------------------------------------------------------
 // resident re-sort
 
-// per cell: how many of its particles take each of the nine near moves
-__global__ void fr_cell_counts(int cells, const int *__restrict__ start, const int *__restrict__ mv,
-                               int *__restrict__ cnt9) {
+// the move of every resident particle this step: old cell * 16 + (dy+1)*3 + (dx+1), or kFar
+// with an unordered arrival slot in farcnt of the destination
+template <int S, bool SRK>
+__global__ __launch_bounds__(256) void fr_move(DevParams P, int n, int ncf, const double *__restrict__ rstate,
+                                               StepArgs U, int *__restrict__ mv, int *__restrict__ farcnt,
+                                               int *__restrict__ farslot) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double *pi = rstate + (long)p * S;
+  double sn, cn, x1, y1;
+  step_position<S, SRK>(P, pi, U, sn, cn, x1, y1);
+  double xw, yw;
+  apply_pbc(P, pi[0], pi[1], xw, yw);
+  const int ox = cell_coord(xw, ncf), oy = cell_coord(yw, ncf);
+  apply_pbc(P, x1, y1, xw, yw);
+  const int nx = cell_coord(xw, ncf), ny = cell_coord(yw, ncf);
+  int dxc = nx - ox, dyc = ny - oy;
+  if (dxc == ncf - 1) dxc = -1;
+  if (dxc == 1 - ncf) dxc = 1;
+  if (dyc == ncf - 1) dyc = -1;
+  if (dyc == 1 - ncf) dyc = 1;
+  int d;
+  if (dxc >= -1 && dxc <= 1 && dyc >= -1 && dyc <= 1) {
+    d = (dyc + 1) * 3 + (dxc + 1);
+  } else {
+    d = kFar;
+    farslot[p] = atomicAdd(farcnt + ny * ncf + nx, 1);
+  }
+  mv[p] = ((oy * ncf + ox) << 4) | d;
+}
+
+// per cell: how many of its particles take each of the nine near moves (cnt9), added into
+// the arrival count of every destination cell (newcnt, zeroed by the scan)
+__global__ __launch_bounds__(256) void fr_cell_counts(int cells, int ncf, const int *__restrict__ start,
+                                                      const int *__restrict__ mv, int *__restrict__ cnt9,
+                                                      int *__restrict__ newcnt) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cells) return;
   int v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -535,45 +656,55 @@ __global__ void fr_cell_counts(int cells, const int *__restrict__ start, const i
 #pragma unroll
     for (int e = 0; e < 9; ++e) v[e] += (d == e) ? 1 : 0;
   }
+  const int cx = c % ncf, cy = c / ncf;
 #pragma unroll
-  for (int d = 0; d < 9; ++d) cnt9[(long)c * 9 + d] = v[d];
-}
-
-template <int S>
-__device__ __forceinline__ int cell_of_state(const DevParams &P, const double *st, int ncf) {
-  double xw, yw;
-  apply_pbc(P, st[0], st[1], xw, yw);
-  return cell_coord(yw, ncf) * ncf + cell_coord(xw, ncf);
+  for (int d = 0; d < 9; ++d) {
+    cnt9[(long)c * 9 + d] = v[d];
+    if (v[d]) {
+      const int dx = (cx + d % 3 - 1 + ncf) % ncf, dy = (cy + d / 3 - 1 + ncf) % ncf;
+      atomicAdd(newcnt + dy * ncf + dx, v[d]);
+    }
+  }
 }
 
 // far arrivals: unordered bucket behind the near arrivals of their destination cell
-template <int S>
+template <int S, bool SRK>
 __global__ void fr_far_bucket(DevParams P, int n, int ncf, const int *__restrict__ mv,
-                              const double *__restrict__ tmp_state, const int *__restrict__ nstart,
+                              const double *__restrict__ rstate, StepArgs U, const int *__restrict__ nstart,
                               const int *__restrict__ cnt9, const int *__restrict__ farslot,
                               int *__restrict__ bucket) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n || (mv[p] & 15) != kFar) return;
-  const int cd = cell_of_state<S>(P, tmp_state + (long)p * S, ncf);
+  double sn, cn, x1, y1;
+  step_position<S, SRK>(P, rstate + (long)p * S, U, sn, cn, x1, y1);
+  const int cd = cell_of_position<S>(P, x1, y1, ncf);
   bucket[nstart[cd] + new_count_of(cnt9, cd, ncf) + farslot[p]] = p;
 }
 
-// move every particle to its new slot; write the new resident arrays and the snapshot
-template <int S>
-__global__ void fr_scatter(DevParams P, int n, int ncf, double H, const int *__restrict__ start,
-                           const int *__restrict__ nstart, const int *__restrict__ mv,
-                           const int *__restrict__ cnt9, const int *__restrict__ bucket,
-                           const double *__restrict__ tmp_state, const int *__restrict__ rid,
-                           const double *__restrict__ rnat, double *__restrict__ nstate, int *__restrict__ nrid,
-                           double *__restrict__ nnat, float4 *__restrict__ snap, float *__restrict__ omega_f) {
+// the step of every particle, written to its new slot with its id, natural frequency and
+// snapshot
+template <int S, bool SRK>
+__global__ __launch_bounds__(256) void fr_scatter(
+    DevParams P, int n, int ncf, double H, const int *__restrict__ start, const int *__restrict__ nstart,
+    const int *__restrict__ mv, const int *__restrict__ cnt9, const int *__restrict__ bucket,
+    const double *__restrict__ rstate, const int *__restrict__ rid, const double *__restrict__ rnat,
+    const double2 *__restrict__ ssum, const double2 *__restrict__ tsum, const int *__restrict__ scnt, StepArgs U,
+    double *__restrict__ compl_out, double *__restrict__ nstate, int *__restrict__ nrid, double *__restrict__ nnat,
+    float4 *__restrict__ snap, float *__restrict__ omega_f) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const int id = rid[p];
+  const double nat_i = rnat ? rnat[p] : 0.0;
+  double out[S];
+  double C;
+  step_particle<S, SRK>(P, rstate + (long)p * S, id, nat_i, ssum[p], SRK ? tsum[p] : make_double2(0.0, 0.0),
+                        scnt[p], U, out, C);
+  if (SRK && compl_out) compl_out[id] = C;
   const int m = mv[p];
   const int cs = m >> 4, d = m & 15;
-  const double *src = tmp_state + (long)p * S;
   int cd, pos;
   if (d == kFar) {
-    cd = cell_of_state<S>(P, src, ncf);
+    cd = cell_of_position<S>(P, out[0], out[1], ncf);
     const int b = nstart[cd] + new_count_of(cnt9, cd, ncf);
     const int e = nstart[cd + 1];
     int rank = 0;
@@ -592,12 +723,11 @@ __global__ void fr_scatter(DevParams P, int n, int ncf, double H, const int *__r
   }
   double *dst = nstate + (long)pos * S;
 #pragma unroll
-  for (int q = 0; q < S; ++q) dst[q] = src[q];
-  const int id = rid[p];
+  for (int q = 0; q < S; ++q) dst[q] = out[q];
   nrid[pos] = id;
-  if (rnat) nnat[pos] = rnat[p];
-  snap[pos] = snap_entry<S>(P, src, cd % ncf, cd / ncf, H, id);
-  if (omega_f) omega_f[pos] = (S == 4) ? (float)src[3] : 0.f;
+  if (rnat) nnat[pos] = nat_i;
+  snap[pos] = snap_entry<S>(P, out, cd % ncf, cd / ncf, H);
+  if (omega_f) omega_f[pos] = (S == 4) ? (float)out[3] : 0.f;
 }
 
 // entry into resident mode: gather the state (original order) into cell order

@@ -5,20 +5,22 @@
 //   scan         exclusive scan of the cell counts (two kernels)
 //   fk_scatter   unstable bucket fill
 //   fk_rank      stable position inside the cell (ascending particle index) -> sorted order,
-//                and the snapshot: FP32 offsets of the particle inside its cell, phase in
-//                revolutions reduced to [0,1); omega for the SRK force
+//                and the snapshot: FP32 offsets of the particle inside its cell and its
+//                unit phasor (cos phi, sin phi); omega for the SRK force
 //   fk_pairs     one workgroup per tile of W cells of one row: the (2k+1) rows x (W+2k)
 //                cells around it are staged into LDS with x re-based to the tile (FP64
 //                rounding, once per staged particle); each lane walks the (2k+1) contiguous
-//                row ranges of its particle in FP32 and accumulates the coupling terms with
-//                a two-sum (FP32 value + FP32 error).  Pairs whose FP32 distance lies in a
+//                row ranges of its particle in FP32 and accumulates the phasors of its
+//                neighbours in FP64 (coupling = Im(e^{-i(phi+alpha)} sum e^{i phi_j})).
+//                Pairs whose FP32 distance lies in a
 //                guard band around rho^2 are re-tested with the reference's FP64 ClassC
 //                formula, so the neighbour sets equal the FP64 ones exactly.
 //   self terms (v0 cos phi, v0 sin phi, omega, -xi omega + sigma A / n) in FP64.
 //
 // Fused stochastic steps keep the state resident in cell order (see spss_fast_kernels.hpp):
-//   fk_pairs (MODE 3/4) -> fr_cell_counts -> scan of the arrivals -> fr_far_bucket -> fr_scatter,
-// five launches per step and no host synchronisation.
+//   fk_pairs (MODE 3: phasor sums) -> fr_move -> fr_cell_counts -> scan of the arrivals
+//   -> fr_far_bucket -> fr_scatter (FP64 Euler/SRK update written to the new slots),
+// seven launches per step and no host synchronisation.
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -43,7 +45,6 @@ struct FastPath {
   double H = 1.0;
   int W = 8, chunks = 1, ntiles = 1;
   float r2lo = 0.f, r2hi = 0.f;
-  float alpha_rev = 0.f;
   // rebuilt cell list (RHS, pair export, entry into resident mode)
   int *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr, *sorted = nullptr;
   int *blocksum = nullptr;
@@ -60,8 +61,9 @@ struct FastPath {
   int *rstart[2] = {nullptr, nullptr};
   float4 *rsnap = nullptr;
   float *romega = nullptr;
-  double *tmp_state = nullptr;
-  int *mv = nullptr, *cnt9 = nullptr, *farcnt = nullptr;
+  double2 *ssum = nullptr, *tsum = nullptr;
+  int *scnt = nullptr;
+  int *mv = nullptr, *cnt9 = nullptr, *newcnt = nullptr, *farcnt = nullptr;
   hipStream_t st = nullptr;
   KernelTimer *timer = nullptr;
 };
@@ -117,10 +119,13 @@ int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, 
   f->H = 1.0 / (double)f->ncf;
   f->cells = f->ncf * f->ncf;
   f->nsb = (f->cells + 256 * kScanItems - 1) / (256 * kScanItems);
-  // tile width: ~200 particles per tile at the mean density, at most kMaxCols - 2k
+  // tile width: about one particle per thread at the mean density, with the staged
+  // (2k+1) x (W+2k) cells and their row padding within 80% of the LDS stage
   const double per_cell = (double)P.n / ((double)f->ncf * f->ncf);
-  int W = (int)std::lround(200.0 / std::max(per_cell, 1e-9));
-  const int wstage = (int)(0.75 * kStageMax / std::max(per_cell * (2 * f->k + 1), 1e-9)) - 2 * f->k;
+  const int nrows = 2 * f->k + 1;
+  int W = (int)std::lround((double)kFB / std::max(per_cell, 1e-9));
+  const int wstage =
+      (int)((0.8 * kStageMax - nrows * kPad) / std::max(per_cell * nrows, 1e-9)) - 2 * f->k;
   W = std::min(W, std::max(wstage, 1));
   W = std::max(1, std::min(W, std::min(kMaxCols - 2 * f->k, f->ncf)));
   f->W = W;
@@ -135,7 +140,6 @@ int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, 
   g = std::max(g, 64.0 * eps);
   f->r2lo = (float)(f->rho2 * (1.0 - g));
   f->r2hi = (float)(f->rho2 * (1.0 + g));
-  f->alpha_rev = (float)(P.alpha / (2.0 * M_PI));
   f->st = st;
   const int cells = f->cells;
   const size_t n = (size_t)P.n;
@@ -160,16 +164,20 @@ int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, 
   }
   A(falloc(&f->rsnap, n));
   A(falloc(&f->romega, n));
-  A(falloc(&f->tmp_state, n * P.s));
+  A(falloc(&f->ssum, n));
+  A(falloc(&f->tsum, n));
+  A(falloc(&f->scnt, n));
   A(falloc(&f->mv, n));
   A(falloc(&f->cnt9, (size_t)cells * 9));
   A(falloc(&f->farcnt, cells));
+  A(falloc(&f->newcnt, cells));
   if (e != hipSuccess) {
     fast_destroy(f);
     return ffail(std::string("hipMalloc: ") + hipGetErrorString(e));
   }
   launch_fill_int(st, cells, f->count, 0);
   launch_fill_int(st, cells, f->farcnt, 0);
+  launch_fill_int(st, cells, f->newcnt, 0);
   *out = f;
   return SPSS_OK;
 }
@@ -178,8 +186,8 @@ void fast_destroy(FastPath *f) {
   if (!f) return;
   void *b[] = {f->key,     f->slot,      f->count,     f->start,     f->tmp,       f->sorted,    f->blocksum,
                f->snap,    f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],    f->rid[1],    f->rnat[0],
-               f->rnat[1], f->rstart[0], f->rstart[1], f->rsnap,     f->romega,    f->tmp_state, f->mv,
-               f->cnt9,    f->farcnt};
+               f->rnat[1], f->rstart[0], f->rstart[1], f->rsnap,     f->romega,    f->newcnt,    f->mv,
+               f->cnt9,    f->farcnt,    f->ssum,      f->tsum,      f->scnt};
   for (void *p : b) hipFree(p);
   delete f;
 }
@@ -208,7 +216,6 @@ static FastArgs args_of(const FastPath *f) {
   a.H = f->H;
   a.r2lo = f->r2lo;
   a.r2hi = f->r2hi;
-  a.alpha_rev = f->alpha_rev;
   return a;
 }
 
@@ -221,8 +228,8 @@ static void build(FastPath *f, hipStream_t st, const DevParams &P, const double 
     fk_keys<4><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->key, f->slot, f->count);
   else
     fk_keys<3><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->key, f->slot, f->count);
-  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->ncf, f->cells, start, f->blocksum);
-  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->count, f->cells, start, f->blocksum, f->nsb);
+  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum);
+  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum, f->nsb);
   fk_scatter<<<nb, 256, 0, st>>>(n, f->key, f->slot, start, f->tmp);
   if (P.s == 4)
     fk_rank<4><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->H, f->key, start, f->tmp, f->sorted, snap, omega);
@@ -241,9 +248,8 @@ static PairBufs rebuilt_bufs(const FastPath *f, const double *stage) {
 }
 
 template <int S, bool SRK, int MODE>
-static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, const PairBufs &B,
-                         const StepArgs &U) {
-  fk_pairs<S, SRK, MODE><<<tile_grid(f->ntiles), kFB, 0, st>>>(P, args_of(f), B, U);
+static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, const PairBufs &B) {
+  fk_pairs<S, SRK, MODE><<<tile_grid(f->ntiles), kFB, 0, st>>>(P, args_of(f), B);
 }
 
 int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const double *nat,
@@ -256,19 +262,17 @@ int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stag
   B.compl_out = srk ? compl_out : nullptr;
   if (f->timer) f->timer->begin(st);
   if (P.s == 4 && srk)
-    launch_pairs<4, true, 0>(f, st, P, B, StepArgs{});
+    launch_pairs<4, true, 0>(f, st, P, B);
   else if (P.s == 4)
-    launch_pairs<4, false, 0>(f, st, P, B, StepArgs{});
+    launch_pairs<4, false, 0>(f, st, P, B);
   else
-    launch_pairs<3, false, 0>(f, st, P, B, StepArgs{});
+    launch_pairs<3, false, 0>(f, st, P, B);
   if (f->timer) f->timer->end(st);
   FCHK(hipGetLastError());
   return SPSS_OK;
 }
 
 bool fast_is_resident(const FastPath *f) { return f && f->resident; }
-
-double *fast_resident_state(FastPath *f) { return (f && f->resident) ? f->rstate[f->cur] : nullptr; }
 
 void fast_resident_drop(FastPath *f) {
   if (f) f->resident = false;
@@ -299,19 +303,24 @@ static void resident_enter(FastPath *f, hipStream_t st, const DevParams &P, cons
   f->resident_nat = nat != nullptr;
 }
 
-template <int S>
-static void resident_resort(FastPath *f, hipStream_t st, const DevParams &P, bool srk) {
+// one resident step after the phasor sums: moves, arrival counts, new cell starts, the step
+// applied and written to the new slots
+template <int S, bool SRK>
+static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const StepArgs &U, double *compl_out) {
   const int c = f->cur, o = 1 - c;
   const int n = f->n;
   const unsigned nb = nblocks(n);
-  fr_cell_counts<<<nblocks(f->cells), 256, 0, st>>>(f->cells, f->rstart[c], f->mv, f->cnt9);
-  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->farcnt, f->cnt9, f->ncf, f->cells, f->rstart[o], f->blocksum);
-  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->farcnt, f->cells, f->rstart[o], f->blocksum, f->nsb);
-  fr_far_bucket<S><<<nb, 256, 0, st>>>(P, n, f->ncf, f->mv, f->tmp_state, f->rstart[o], f->cnt9, f->slot, f->tmp);
-  fr_scatter<S><<<nb, 256, 0, st>>>(P, n, f->ncf, f->H, f->rstart[c], f->rstart[o], f->mv, f->cnt9, f->tmp,
-                                    f->tmp_state, f->rid[c], f->resident_nat ? f->rnat[c] : nullptr,
-                                    f->rstate[o], f->rid[o], f->rnat[o], f->rsnap,
-                                    (srk && S == 4) ? f->romega : nullptr);
+  const double *rnat = f->resident_nat ? f->rnat[c] : nullptr;
+  fr_move<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->rstate[c], U, f->mv, f->farcnt, f->slot);
+  fr_cell_counts<<<nblocks(f->cells), 256, 0, st>>>(f->cells, f->ncf, f->rstart[c], f->mv, f->cnt9, f->newcnt);
+  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum);
+  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum, f->nsb);
+  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->mv, f->rstate[c], U, f->rstart[o], f->cnt9, f->slot,
+                                           f->tmp);
+  fr_scatter<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->H, f->rstart[c], f->rstart[o], f->mv, f->cnt9, f->tmp,
+                                        f->rstate[c], f->rid[c], rnat, f->ssum, f->tsum, f->scnt, U,
+                                        SRK ? compl_out : nullptr, f->rstate[o], f->rid[o], f->rnat[o], f->rsnap,
+                                        SRK ? f->romega : nullptr);
   f->cur = o;
 }
 
@@ -340,27 +349,26 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
   PairBufs B{};
   B.rstate = f->rstate[c];
   B.rid = f->rid[c];
-  B.rnat = f->resident_nat ? f->rnat[c] : nullptr;
   B.start = f->rstart[c];
   B.snap = f->rsnap;
   B.omega_f = f->romega;
-  B.compl_out = do_srk ? compl_out : nullptr;
-  B.tmp_state = f->tmp_state;
-  B.mv = f->mv;
-  B.farcnt = f->farcnt;
-  B.farslot = f->slot;
+  B.ssum = f->ssum;
+  B.tsum = f->tsum;
+  B.scnt = f->scnt;
   if (f->timer) f->timer->begin(st);
   if (do_srk)
-    launch_pairs<4, true, 4>(f, st, P, B, U);
+    launch_pairs<4, true, 3>(f, st, P, B);
   else if (P.s == 4)
-    launch_pairs<4, false, 3>(f, st, P, B, U);
+    launch_pairs<4, false, 3>(f, st, P, B);
   else
-    launch_pairs<3, false, 3>(f, st, P, B, U);
+    launch_pairs<3, false, 3>(f, st, P, B);
   if (f->timer) f->timer->end(st);
-  if (P.s == 4)
-    resident_resort<4>(f, st, P, do_srk);
+  if (do_srk)
+    resident_step<4, true>(f, st, P, U, compl_out);
+  else if (P.s == 4)
+    resident_step<4, false>(f, st, P, U, nullptr);
   else
-    resident_resort<3>(f, st, P, false);
+    resident_step<3, false>(f, st, P, U, nullptr);
   FCHK(hipGetLastError());
   return SPSS_OK;
 }
@@ -370,9 +378,9 @@ int fast_pairs_count(FastPath *f, hipStream_t st, const DevParams &P, const doub
   PairBufs B = rebuilt_bufs(f, stage);
   B.pcount = count;
   if (P.s == 4)
-    launch_pairs<4, false, 1>(f, st, P, B, StepArgs{});
+    launch_pairs<4, false, 1>(f, st, P, B);
   else
-    launch_pairs<3, false, 1>(f, st, P, B, StepArgs{});
+    launch_pairs<3, false, 1>(f, st, P, B);
   FCHK(hipGetLastError());
   return SPSS_OK;
 }
@@ -384,9 +392,9 @@ int fast_pairs_fill(FastPath *f, hipStream_t st, const DevParams &P, const doubl
   B.pstart = pstart;
   B.plist = plist;
   if (P.s == 4)
-    launch_pairs<4, false, 2>(f, st, P, B, StepArgs{});
+    launch_pairs<4, false, 2>(f, st, P, B);
   else
-    launch_pairs<3, false, 2>(f, st, P, B, StepArgs{});
+    launch_pairs<3, false, 2>(f, st, P, B);
   FCHK(hipGetLastError());
   return SPSS_OK;
 }

@@ -5,7 +5,8 @@
   the box, clustered inputs (LDS staging overflow -> global path, list flushes),
   tiny grids;
 * one RHS: position slots within 1e-12 (FP64 self terms), coupling slot within
-  2e-5 absolute (FP32 pair terms, hardware sine);
+  1e-6 absolute (FP32 distance tests and phasor components summed in FP32 partials per
+  stencil row, combined in FP64; observed ~1e-7);
 * Euler trajectories fed identical noise: order parameter within 1e-4 of the FP64
   oracle after 300 steps at config-2 sizes;
 * determinism: identical inputs give bit-identical results.
@@ -16,7 +17,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 TOL_POS = 1e-12      # k slots 0..2: FP64 self terms
-TOL_COUPLING = 2e-5  # k slot 3: sigma * (FP32 pair sum) / n, absolute
+TOL_COUPLING = 1e-6  # k slot 3: sigma * (phasor pair sum) / n, absolute
 TOL_R = 1e-4         # order parameter vs FP64 oracle
 
 
@@ -101,7 +102,9 @@ def test_fp32c_rhs_vs_oracle(spss, model, n, rho, sub, alpha, cluster):
     for slot in range(model - 1):
         assert np.max(np.abs(kg[slot::model] - ko[slot::model])) <= TOL_POS, slot
     c = model - 1
-    assert np.max(np.abs(kg[c::model] - ko[c::model])) <= TOL_COUPLING * max(1.0, kw["sigma"])
+    err = np.max(np.abs(kg[c::model] - ko[c::model]))
+    print(f"  coupling slot max |gpu - oracle| = {err:.3e}", flush=True)
+    assert err <= TOL_COUPLING * max(1.0, kw["sigma"])
 
 
 def test_fp32c_srk_complementary_force(spss):
@@ -118,7 +121,8 @@ def test_fp32c_srk_complementary_force(spss):
     o.evaluate_rhs(3, st, ko.copy(), ko, 0.0, 0.01)
     g.evaluate_rhs(3, st, kg.copy(), kg, 0.0, 0.01)
     co, cg = o.complementary_force(), g.complementary_force()
-    assert np.max(np.abs(co - cg)) <= 2e-5 * max(1.0, np.max(np.abs(co)))
+    print(f"  complementary force max |gpu - oracle| = {np.max(np.abs(co - cg)):.3e}", flush=True)
+    assert np.max(np.abs(co - cg)) <= 1e-6 * max(1.0, np.max(np.abs(co)))
 
 
 def test_fp32c_euler_order_parameter_matches_fp64(spss):
