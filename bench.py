@@ -10,10 +10,11 @@ v0 = 0.005, dt = 0.002, cell list rebuilt every step, stochastic Euler with
 Philox noise (seed 0x5EED0003), FP32-compensated pair sums, FP64 state.
 
 Prints one JSON line.  `value` is whole-job particle-steps/s with the state
-resident in HBM.  `roofline` is the fused pair+update kernel timed with HIP
-events on its stream over the timed steps; `cpu_baseline` is the reference's
-own serial linked-list RHS (compiled from its sources into oracle/_ref) or,
-without that build, the oracle restatement, on a bounded sample.
+resident in HBM.  `roofline` is the pair kernel (fk_pairs, phasor sums of the
+resident state) timed with HIP events on its stream over the timed steps;
+`cpu_baseline` is the reference's own serial linked-list RHS + Euler update
+(compiled from its sources into oracle/_ref) or, without that build, the
+oracle restatement, on a bounded sample.
 """
 import argparse
 import json
@@ -30,9 +31,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s, N=1M cell list; batched vs plain at ≤1e-4 order-param err"
 HBM_PEAK = 8.0e12
-# algorithmic HBM bytes per particle of the fused pair+update kernel: FP64 state read and
-# write (2 x 32) + FP32 snapshot read (12) + particle index read (4)
-PAIR_KERNEL_BYTES = 80.0
+# algorithmic HBM bytes per particle of the pair kernel of one Euler step: FP32 snapshot
+# (x, y, cos phi, sin phi) read once (16), phasor sum (2 x FP64) and count written (20),
+# cell starts read once (4 B x 2 per cell, 62,500 cells)
+PAIR_KERNEL_BYTES = 16.0 + 20.0 + 8.0 * 62500.0 / 1048576.0
+# HBM bytes per launch of that kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled
+# by the copy-kernel calibration (profiles/fk_pairs_euler_N1M_hbm_traffic.txt)
+PAIR_KERNEL_TRAFFIC = 3.9748e7
+# VALU issue utilisation of that kernel (SQ_INSTS_VALU per CU-cycle, profiles/fk_pairs_euler_N1M_valu.txt)
+PAIR_KERNEL_VALU_UTIL = 0.99
 # algorithmic bytes per particle-step of the whole step (state 64, snapshot w+r 24,
 # key and sorted index w+r 16, cell table w+r 8 * 62500 / N)
 STEP_BYTES = 64.0 + 24.0 + 16.0 + 8.0 * 62500.0 / 1048576.0
@@ -154,10 +161,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": "N=1,048,576 uniform in the unit box, rho=0.004 cell list rebuilt every step, "
                                "stochastic Euler dt=0.002, m=0.5 sigma=1.5 D=0.02, Lorentzian(0,0.05) omega, "
-                               "FP32-compensated pair sums, FP64 state",
+                               "FP32 distance tests, phasor sums in FP32 partials combined in FP64, FP64 state",
                    "n": n, "rho": 0.004, "cell_subdivision": args.subdiv, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None},
+                     "frac": achieved / HBM_PEAK, "traffic": PAIR_KERNEL_TRAFFIC},
+        # the pair kernel is bound by VALU issue, not HBM (no MFMA work on this path)
+        "valu": {"kernel": "fk_pairs", "issue_util": PAIR_KERNEL_VALU_UTIL},
         "cpu_baseline": cpu,
         "pair_kernel_us": pair_ms * 1e3,
         "hbm_copy_gbps": copy_gbps,

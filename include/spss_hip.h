@@ -172,6 +172,17 @@ int spss_kernel_timer_stop(spss_system *sys, double *total_ms, long *launches);
  * (read + write bytes / time); no reference counterpart */
 int spss_device_copy_bandwidth(long bytes, int repeats, double *gbps);
 
+/* The counter-based noise of the kernels, on the host (no reference counterpart: the reference
+ * draws from std::mt19937, StochasticEulerStepperPtr.hpp:53-62).  One Philox4x32-10 block
+ * (Salmon et al., SC'11) of counter ctr[4] under key key[2]; and the two standard normals the
+ * kernels draw for particles [first, first + count) at `step` from counter block `block`
+ * (Euler: z0 of block 0; SRK: U1 = z0, U2 = z1 of block 0; model 6: z0, z1 of block 0 and
+ * z0 of block 1).  The uniforms are bit-identical to the device's; the Box-Muller transform
+ * uses the host libm and agrees with the device to a few ulp. */
+int spss_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+int spss_philox_normals(uint64_t seed, uint64_t step, uint32_t block, long first, long count, double *z0,
+                        double *z1);
+
 /* AveragingObserver (Observers/AveragingObserver.cpp:66-105): running FP64 sum of the state */
 int spss_averaging_reset(spss_system *sys);
 int spss_averaging_accumulate(spss_system *sys);

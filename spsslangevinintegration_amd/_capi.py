@@ -29,6 +29,7 @@ EXPORTS = [
     "spss_neighbour_pairs", "spss_order_parameter", "spss_apply_periodic_boundary_conditions",
     "spss_averaging_reset", "spss_averaging_accumulate", "spss_averaging_get",
     "spss_kernel_timer_start", "spss_kernel_timer_stop", "spss_device_copy_bandwidth",
+    "spss_philox4x32_10", "spss_philox_normals",
 ]
 
 
@@ -97,6 +98,8 @@ def load(path=None):
         "spss_kernel_timer_start": (i, [vp]),
         "spss_kernel_timer_stop": (i, [vp, C.POINTER(d), C.POINTER(l)]),
         "spss_device_copy_bandwidth": (i, [l, i, C.POINTER(d)]),
+        "spss_philox4x32_10": (i, [vp, vp, vp]),
+        "spss_philox_normals": (i, [C.c_uint64, C.c_uint64, C.c_uint32, l, l, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -125,3 +128,20 @@ def copy_bandwidth(nbytes=1 << 30, repeats=20):
     v = C.c_double()
     check(load().spss_device_copy_bandwidth(int(nbytes), int(repeats), C.byref(v)))
     return v.value
+
+
+def philox4x32_10(ctr, key):
+    """one Philox4x32-10 block (host), as the kernels compute it"""
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    check(load().spss_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data))
+    return out
+
+
+def philox_normals(seed, step, n, block=0, first=0):
+    """(z0, z1): the standard normals the kernels draw for particles first..first+n-1"""
+    z0 = np.zeros(n)
+    z1 = np.zeros(n)
+    check(load().spss_philox_normals(int(seed), int(step), int(block), int(first), int(n), ptr(z0), ptr(z1)))
+    return z0, z1

@@ -3,7 +3,8 @@
     python spsslangevinintegration_amd/build.py        # or __graft_entry__.build()
 
 Every .hip file under csrc/ is compiled to an object in _build/ (in parallel)
-and linked into spsslangevinintegration_amd/libspss_hip.so.  All device code is
+and linked into spsslangevinintegration_amd/libspss_hip.so; cpp/shim_check (the
+reference-named C++ classes driven like the reference's engine) links against it.  All device code is
 compiled with packed-FP32 instructions disabled and FP contraction off (FMAs
 are written explicitly where a kernel wants them).
 """
@@ -53,8 +54,16 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the C++ drop-in check program over the reference-named classes (host code, g++)
+    cpp = os.path.join(HERE, "cpp")
+    exe = os.path.join(cpp, "shim_check")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(os.path.dirname(HERE), "include"),
+           os.path.join(cpp, "shim_check.cpp"), "-L" + HERE, "-lspss_hip", "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on shim_check.cpp:\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print("built", LIB, file=sys.stderr)
+        print("built", LIB, "and", exe, file=sys.stderr)
     return LIB
 
 

@@ -1,0 +1,118 @@
+// shim_check.cpp -- drives the reference-named C++ classes of spss_reference_api.hpp the way
+// the reference's engine does, for the drop-in tests: reads an input state, runs one scenario,
+// writes the result as raw FP64.
+//
+//   shim_check rk4     n rho K dt in.bin out.bin               all-pairs RK4 (method AUTO: rho > 0.5)
+//   shim_check euler   n rho K dt in.bin normals.bin out.bin   linked list, injected normals
+//   shim_check srk     n rho K dt in.bin out.bin               linked list, Philox noise (seed 0x5EED0000)
+//   shim_check average n rho K dt in.bin out.bin               Euler + AveragingObserverPtr, time mean
+//   shim_check engine  n                 out.bin               SimulationEngineFirstOrderPtr, t_1 = 1
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "spss_reference_api.hpp"
+
+static std::vector<Real> read_file(const char *path, size_t count) {
+  std::vector<Real> v(count);
+  std::ifstream f(path, std::ios::binary);
+  f.read(reinterpret_cast<char *>(v.data()), (std::streamsize)(count * sizeof(Real)));
+  if (!f) throw std::runtime_error(std::string("cannot read ") + path);
+  return v;
+}
+
+static void write_file(const char *path, const std::vector<Real> &v) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char *>(v.data()), (std::streamsize)(v.size() * sizeof(Real)));
+}
+
+int main(int argc, char **argv) {
+  try {
+    if (argc < 3) throw std::runtime_error("usage: shim_check mode n ...");
+    const std::string mode = argv[1];
+    const int n = std::atoi(argv[2]), s = 4;
+    ThreadSharedMemory thread(argc, argv, n, s);
+    PeriodicBoundaryConditions pbc(&thread, n, s, 1.0, 1.0);
+    if (mode == "engine") {
+      SimulationEngineFirstOrderPtr engine(&thread, n, s);
+      SimulationEngineFirstOrderPtr::Run r;
+      r.t_1 = 1.0;
+      r.summary_every = 10;
+      engine.RunSimulation(r);
+      std::vector<Real> out;
+      for (const auto &e : engine.summary()) {
+        out.push_back(e.t);
+        out.push_back(e.order_parameter.real());
+        out.push_back(e.order_parameter.imag());
+      }
+      write_file(argv[3], engine.system_state());
+      write_file((std::string(argv[3]) + ".summary").c_str(), out);
+      return 0;
+    }
+    const double rho = std::atof(argv[3]);
+    const int K = std::atoi(argv[4]);
+    const double dt = std::atof(argv[5]);
+    std::vector<Real> state = read_file(argv[6], (size_t)n * s);
+    const Real v0 = 0.02, xi = 1.3, sigma = 2.0, alpha = 0.1, D_phi = 0.07;
+    if (mode == "rk4") {
+      ParticleSystemFirstOrderPtr system(&thread, v0, xi, sigma, rho, alpha, D_phi, pbc, n, s);
+      RungeKutta4StepperPtr stepper(&thread, n, s);
+      Real t = 0.0;
+      for (int k = 0; k < K; ++k) {
+        t += dt;
+        stepper.DoStep(system, state.data(), t, dt);
+      }
+      write_file(argv[7], state);
+    } else if (mode == "euler") {
+      spss_shim::default_options().method = SPSS_METHOD_LINKED_LIST;
+      std::vector<Real> z = read_file(argv[7], (size_t)n * K);
+      ParticleSystemFirstOrderPtr system(&thread, v0, xi, sigma, rho, alpha, D_phi, pbc, n, s);
+      StochasticEulerStepperPtr stepper(&thread, n, s);
+      Real t = 0.0;
+      for (int k = 0; k < K; ++k) {
+        t += dt;
+        std::vector<Real> zk(z.begin() + (size_t)k * n, z.begin() + (size_t)(k + 1) * n);
+        stepper.DoStep(system, state.data(), t, dt, zk);
+      }
+      write_file(argv[8], state);
+    } else if (mode == "srk") {
+      spss_shim::default_options().method = SPSS_METHOD_LINKED_LIST;
+      ParticleSystemFirstOrderPtr system(&thread, v0, xi, sigma, rho, alpha, D_phi, pbc, n, s);
+      StochasticRungeKuttaStepperPtr stepper(&thread, n, s);
+      Real t = 0.0;
+      for (int k = 0; k < K; ++k) {
+        t += dt;
+        stepper.DoStep(system, state.data(), t, dt);
+      }
+      std::vector<Real> out = state;
+      const std::vector<Real> &c = system.GetComplimentaryAlignmentForce();
+      out.insert(out.end(), c.begin(), c.end());
+      write_file(argv[7], out);
+    } else if (mode == "average") {
+      spss_shim::default_options().method = SPSS_METHOD_LINKED_LIST;
+      ParticleSystemFirstOrderPtr system(&thread, v0, xi, sigma, rho, alpha, D_phi, pbc, n, s);
+      system.SetHostSync(false);
+      StochasticEulerStepperPtr stepper(&thread, n, s);
+      AveragingObserverPtr observer(&thread, n, s);
+      observer.Bind(system);
+      Real t = 0.0;
+      for (int k = 0; k < K; ++k) {
+        t += dt;
+        stepper.DoStep(system, state.data(), t, dt);
+        observer(state.data(), (long)state.size(), t);
+      }
+      std::vector<Real> out = observer.AveragedSystemState();
+      system.DownloadState(state.data());
+      out.insert(out.end(), state.begin(), state.end());
+      write_file(argv[7], out);
+    } else {
+      throw std::runtime_error("unknown mode " + mode);
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "shim_check: %s\n", e.what());
+    return 1;
+  }
+}

@@ -1,0 +1,522 @@
+// spss_reference_api.hpp -- the reference's C++ class interface on top of the C ABI of
+// libspss_hip.so (include/spss_hip.h).  Header-only; link with -lspss_hip.
+//
+// A host program written against the reference classes
+//   Thread / ThreadSharedMemory            (Parallelization/Thread.hpp, ThreadSharedMemory.hpp)
+//   PeriodicBoundaryConditions             (BoundaryConfiguration/PeriodicBoundaryConditions.hpp:12-48)
+//   ParticleSystemFirstOrderPtr            (DynamicalSystems/ParticleSystemFirstOrderPtr.hpp:12-98)
+//   ParticleSystemSecondOrderPtr           (DynamicalSystems/ParticleSystemSecondOrderPtr.hpp:16-37)
+//   RungeKutta4StepperPtr                  (Steppers/RungeKutta4StepperPtr.hpp:15-35)
+//   StochasticEulerStepperPtr              (Steppers/StochasticEulerStepperPtr.hpp:22-73)
+//   StochasticRungeKuttaStepperPtr         (Steppers/StochasticRungeKuttaStepperPtr.hpp:22-76)
+//   AveragingObserverPtr                   (Observers/AveragingObserverPtr.hpp:13-40)
+//   SimulationEngineFirstOrderPtr          (ControlEngines/SimulationEngineFirstOrderPtr.hpp:15-40)
+// compiles against this header unchanged: same class names, constructor arguments and
+// method signatures.  The particle system owns one device handle; EvaluateRhs and the
+// steppers run on the GPU.
+//
+// Host memory semantics.  The reference keeps the state in host memory and a stepper updates
+// it in place.  Here every DoStep uploads the caller's state, steps on the device and writes
+// the result back (exact drop-in semantics, PCIe-bound).  A caller that does not read the
+// host state every step calls SetHostSync(false) on the system: DoStep then keeps the state
+// on the device and DownloadState() refreshes the host copy on demand (the engine below does
+// this, reading the order parameter on the device).
+//
+// Errors: the reference has none (void returns, asserts); here a failing call throws
+// std::runtime_error with spss_last_error().
+//
+// Noise: the stochastic steppers draw from the library's Philox4x32-10 stream (seed, step,
+// particle), not from the reference's std::mt19937; DoStep overloads taking the normals
+// reproduce the reference's draws when they are supplied.
+#pragma once
+
+#include <cmath>
+#include <complex>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "spss_hip.h"
+
+typedef double Real;
+typedef float RealOutput;
+
+namespace spss_shim {
+
+inline void check(int rc) {
+  if (rc != SPSS_OK) throw std::runtime_error(std::string("spss: ") + spss_last_error());
+}
+
+// options that the reference fixes in code (method dispatch, precision, seed)
+struct Options {
+  int method = SPSS_METHOD_AUTO;  // EvaluateRhs's dispatch on rho unless set
+  int precision = SPSS_PRECISION_FP64;
+  int verlet_mode = SPSS_VERLET_COMPAT;
+  double verlet_skin = 0.05;
+  int natfreq_local = 0;  // the reference applies natural frequencies in the global branch only
+  int cell_subdivision = 1;
+  uint64_t seed = 0x5EED0000ull;
+};
+
+inline Options &default_options() {
+  static Options o;
+  return o;
+}
+
+}  // namespace spss_shim
+
+// ---------------------------------------------------------------------------
+// one device: rank 0 of one, every particle owned (Thread.cpp:22-23)
+class Thread {
+ public:
+  Thread(int /*argc*/, char ** /*argv*/, int number_of_particles, int number_of_state_variables)
+      : n_(number_of_particles), s_(number_of_state_variables), loop_indices_(number_of_particles) {
+    for (int i = 0; i < n_; ++i) loop_indices_[i] = i;
+  }
+  virtual ~Thread() = default;
+  int GetRank() { return 0; }
+  int GetNumberOfMpichThreads() { return 1; }
+  int GetNumberOfParticlesPerMpichThread() { return n_; }
+  const std::vector<int> &GetLoopIndices() const { return loop_indices_; }
+  virtual bool IsRoot() const { return true; }
+  virtual void SynchronizeVector(std::vector<Real> &) {}
+  virtual void SynchronizeVectorThoughBuffer(std::vector<Real> &, std::vector<Real> &) {}
+  virtual void BroadcastVector(std::vector<Real> &) {}
+  virtual void SynchronizePrelinkedList(std::vector<int> &) const {}
+
+ protected:
+  int n_, s_;
+  std::vector<int> loop_indices_;
+};
+
+class ThreadSharedMemory : public Thread {
+ public:
+  ThreadSharedMemory(int argc, char **argv, int number_of_particles, int number_of_state_variables)
+      : Thread(argc, argv, number_of_particles, number_of_state_variables) {}
+  int GetRootRank() { return 0; }
+  int GetSharedRootRank() { return 0; }
+  bool IsRoot() const override { return true; }
+  bool IsSharedRoot() { return true; }
+  void SynchronizeVector(std::vector<Real> &) override {}
+  void SynchronizeVector(Real *const, long) {}
+  void BroadcastVector(std::vector<Real> &) override {}
+  void BroadcastVector(Real *const, long) {}
+};
+
+// ---------------------------------------------------------------------------
+// PeriodicBoundaryConditions.cpp:7-165: scalar helpers for host code (observers, output);
+// the kernels carry their own copies of these formulas
+class PeriodicBoundaryConditions {
+ public:
+  PeriodicBoundaryConditions(Thread *thread, int number_of_particles, int number_of_state_variables, Real x_size,
+                             Real y_size)
+      : thread_(thread), n_(number_of_particles), s_(number_of_state_variables), x_size_(x_size), y_size_(y_size),
+        x_rsize_(1.0 / x_size), y_rsize_(1.0 / y_size) {}
+
+  void ClassAEffectiveParticleDistance(Real x_i, Real y_i, Real x_j, Real y_j, Real &dx, Real &dy) const {
+    dx = x_j - x_i;
+    dx -= static_cast<int>(dx * 2.0 * x_rsize_) * x_size_;
+    dy = y_j - y_i;
+    dy -= static_cast<int>(dy * 2.0 * y_rsize_) * y_size_;
+  }
+  void ClassCEffectiveParticleDistanceUnsigned(Real x_i, Real y_i, Real x_j, Real y_j, Real &dx, Real &dy) const {
+    dx = std::fabs(x_j - x_i);
+    dx -= static_cast<int>(dx * x_rsize_ + 0.5) * x_size_;
+    dy = std::fabs(y_j - y_i);
+    dy -= static_cast<int>(dy * y_rsize_ + 0.5) * y_size_;
+  }
+  void ApplyPeriodicBoundaryConditions(Real x, Real y, Real &x_pbc, Real &y_pbc) const {
+    x_pbc = x - std::floor(x * x_rsize_) * x_size_;
+    y_pbc = y - std::floor(y * y_rsize_) * y_size_;
+  }
+  // wraps positions of a host state vector (output copies, PeriodicBoundaryConditions.cpp:127-155)
+  template <class T>
+  void ApplyPeriodicBoundaryConditions(std::vector<T> &system_state) {
+    for (int i = 0; i < n_; ++i) {
+      const long ii = (long)s_ * i;
+      system_state[ii] -= std::floor(system_state[ii] * x_rsize_) * x_size_;
+      system_state[ii + 1] -= std::floor(system_state[ii + 1] * y_rsize_) * y_size_;
+    }
+  }
+  Real GetXSize() const { return x_size_; }
+  Real GetYSize() const { return y_size_; }
+
+ private:
+  Thread *thread_;
+  int n_, s_;
+  Real x_size_, y_size_, x_rsize_, y_rsize_;
+};
+
+// ---------------------------------------------------------------------------
+namespace spss_shim {
+
+// one device handle and the host-side flags the reference exposes as public members
+class DeviceSystem {
+ public:
+  DeviceSystem(const spss_params &p) : p_(p) { check(spss_create(&p_, &h_)); }
+  ~DeviceSystem() { spss_destroy(h_); }
+  DeviceSystem(const DeviceSystem &) = delete;
+  DeviceSystem &operator=(const DeviceSystem &) = delete;
+
+  spss_system *handle() const { return h_; }
+  int n() const { return p_.n; }
+  int s() const { return p_.model; }
+  int method() const { return method_; }
+  void SetMethod(int m) { method_ = m; }
+  void SetHostSync(bool on) { host_sync_ = on; }
+  bool host_sync() const { return host_sync_; }
+
+  // k_next entries left untouched by the RHS (k[3] of isolated particles) keep their values
+  void Rhs(int method, const Real *state, const std::vector<Real> &k_prev, std::vector<Real> &k_next, Real k_coef,
+           Real dt, bool higher_order, bool &uses_verlet) {
+    check(spss_set_higher_order(h_, higher_order ? 1 : 0));
+    check(spss_evaluate_rhs(h_, method, state, k_prev.data(), k_next.data(), k_coef, dt));
+    int flag = 0;
+    check(spss_verlet_status(h_, nullptr, nullptr, &flag));
+    uses_verlet = flag != 0;
+  }
+  void Upload(const Real *state) {
+    if (host_sync_ || !resident_) check(spss_set_state(h_, state));
+    resident_ = true;
+  }
+  void Download(Real *state) {
+    if (host_sync_) check(spss_get_state(h_, state));
+  }
+  void DownloadState(Real *state) { check(spss_get_state(h_, state)); }
+
+ private:
+  spss_params p_;
+  spss_system *h_ = nullptr;
+  int method_ = SPSS_METHOD_AUTO;
+  bool host_sync_ = true;
+  bool resident_ = false;
+};
+
+inline spss_params make_params(int model, int n, Real v0, Real xi, Real sigma, Real rho, Real alpha, Real D_phi,
+                               Real xi_r, Real D_r, const PeriodicBoundaryConditions &pbc) {
+  spss_params p;
+  check(spss_params_default(&p));
+  const Options &o = default_options();
+  p.model = model;
+  p.n = n;
+  p.v0 = v0;
+  p.xi = xi;
+  p.sigma = sigma;
+  p.rho = rho;
+  p.alpha = alpha;
+  p.D_phi = D_phi;
+  p.xi_r = xi_r;
+  p.D_r = D_r;
+  p.x_size = pbc.GetXSize();
+  p.y_size = pbc.GetYSize();
+  p.verlet_skin = o.verlet_skin;
+  p.verlet_mode = o.verlet_mode;
+  p.natfreq_local = o.natfreq_local;
+  p.precision = o.precision;
+  p.cell_subdivision = o.cell_subdivision;
+  p.seed = o.seed;
+  return p;
+}
+
+}  // namespace spss_shim
+
+// ---------------------------------------------------------------------------
+// ParticleSystemFirstOrderPtr.hpp:12-98 (s = 4; s = 3 is the first-order phase model)
+class ParticleSystemFirstOrderPtr {
+ public:
+  ParticleSystemFirstOrderPtr(ThreadSharedMemory *thread, Real v_0, Real xi, Real sigma, Real rho, Real alpha,
+                              Real D_phi, PeriodicBoundaryConditions &pbc_config, int number_of_particles,
+                              int number_of_state_variables)
+      : uses_verlet_list_(false), is_higher_order_sde_integration_(false), thread_(thread),
+        pbc_config_(pbc_config), xi_(xi), rho_(rho), alpha_(alpha), D_phi_(D_phi),
+        dev_(spss_shim::make_params(number_of_state_variables, number_of_particles, v_0, xi, sigma, rho, alpha,
+                                    D_phi, 1.0, 0.0, pbc_config)) {
+    dev_.SetMethod(spss_shim::default_options().method);
+  }
+
+  void SetNaturalFrequencies(const std::vector<Real> &natural_frequencies) {
+    spss_shim::check(spss_set_natural_frequencies(dev_.handle(), natural_frequencies.data(),
+                                                  (int)natural_frequencies.size()));
+  }
+
+  void EvaluateRhs(const Real *const system_state, const std::vector<Real> &k_prev, std::vector<Real> &k_next,
+                   Real k_coef, Real dt) {
+    Eval(dev_.method(), system_state, k_prev, k_next, k_coef, dt);
+  }
+  void EvaluateInteractionsWithGlobalPolarOrderField(const Real *const s, const std::vector<Real> &kp,
+                                                     std::vector<Real> &kn, Real c, Real dt) {
+    Eval(SPSS_METHOD_GLOBAL_FIELD, s, kp, kn, c, dt);
+  }
+  void EvaluateInteractionsWithAllPairs(const Real *const s, const std::vector<Real> &kp, std::vector<Real> &kn,
+                                        Real c, Real dt) {
+    Eval(SPSS_METHOD_ALL_PAIRS, s, kp, kn, c, dt);
+  }
+  void EvaluateInteractionsWithLinkedList(const Real *const s, const std::vector<Real> &kp, std::vector<Real> &kn,
+                                          Real c, Real dt) {
+    Eval(SPSS_METHOD_LINKED_LIST, s, kp, kn, c, dt);
+  }
+  void EvaluateInteractionsWithVerletNeighborList(const Real *const s, const std::vector<Real> &kp,
+                                                  std::vector<Real> &kn, Real c, Real dt) {
+    Eval(SPSS_METHOD_VERLET, s, kp, kn, c, dt);
+  }
+  void EvaluateInteractionsWithLinkedListAndVerletNeighborList(const Real *const s, const std::vector<Real> &kp,
+                                                               std::vector<Real> &kn, Real c, Real dt) {
+    Eval(SPSS_METHOD_LINKED_LIST_VERLET, s, kp, kn, c, dt);
+  }
+
+  Real xi() const { return xi_; }
+  Real rho() const { return rho_; }
+  Real D_phi() const { return D_phi_; }
+  Real alpha() const { return alpha_; }
+
+  bool uses_verlet_list_;
+  void CalculateMaxDisplacement(const std::vector<Real> &total_displacement) {
+    spss_shim::check(spss_calculate_max_displacement(dev_.handle(), total_displacement.data()));
+  }
+  bool is_higher_order_sde_integration_;
+  const std::vector<Real> &GetComplimentaryAlignmentForce() const {
+    compl_.resize(dev_.n());
+    spss_shim::check(spss_get_complementary_force(dev_.handle(), compl_.data()));
+    return compl_;
+  }
+
+  // beyond the reference: device residency and method choice
+  spss_shim::DeviceSystem &device() { return dev_; }
+  void SetHostSync(bool on) { dev_.SetHostSync(on); }
+  void SetMethod(int method) { dev_.SetMethod(method); }
+  void DownloadState(Real *system_state) { dev_.DownloadState(system_state); }
+  std::complex<Real> OrderParameter() {
+    double re = 0.0, im = 0.0;
+    spss_shim::check(spss_order_parameter(dev_.handle(), dev_.s() == 6 ? 4 : 2, &re, &im));
+    return {re, im};
+  }
+
+ private:
+  void Eval(int method, const Real *s, const std::vector<Real> &kp, std::vector<Real> &kn, Real c, Real dt) {
+    dev_.Rhs(method, s, kp, kn, c, dt, is_higher_order_sde_integration_, uses_verlet_list_);
+  }
+
+  ThreadSharedMemory *thread_;
+  PeriodicBoundaryConditions &pbc_config_;
+  Real xi_, rho_, alpha_, D_phi_;
+  spss_shim::DeviceSystem dev_;
+  mutable std::vector<Real> compl_;
+};
+
+// ParticleSystemSecondOrderPtr.hpp:16-37 (s = 6)
+class ParticleSystemSecondOrderPtr {
+ public:
+  ParticleSystemSecondOrderPtr(ThreadSharedMemory *thread, Real v_0, Real xi_r, Real D_r, Real xi_phi, Real D_phi,
+                               Real sigma, Real rho, Real alpha, PeriodicBoundaryConditions &pbc_config,
+                               int number_of_particles, int number_of_state_variables)
+      : thread_(thread), pbc_config_(pbc_config), rho_(rho), alpha_(alpha), D_phi_(D_phi), D_r_(D_r),
+        dev_(spss_shim::make_params(number_of_state_variables, number_of_particles, v_0, xi_phi, sigma, rho, alpha,
+                                    D_phi, xi_r, D_r, pbc_config)) {
+    dev_.SetMethod(spss_shim::default_options().method);
+  }
+  void EvaluateRhs(const Real *const s, const std::vector<Real> &kp, std::vector<Real> &kn, Real c, Real dt) {
+    bool v = false;
+    dev_.Rhs(dev_.method(), s, kp, kn, c, dt, false, v);
+  }
+  void EvaluateInteractionsWithAllPairs(const Real *const s, const std::vector<Real> &kp, std::vector<Real> &kn,
+                                        Real c, Real dt) {
+    bool v = false;
+    dev_.Rhs(SPSS_METHOD_ALL_PAIRS, s, kp, kn, c, dt, false, v);
+  }
+  void EvaluateInteractionsWithLinkedList(const Real *const s, const std::vector<Real> &kp, std::vector<Real> &kn,
+                                          Real c, Real dt) {
+    bool v = false;
+    dev_.Rhs(SPSS_METHOD_LINKED_LIST, s, kp, kn, c, dt, false, v);
+  }
+  Real rho() const { return rho_; }
+  Real alpha() const { return alpha_; }
+  Real D_phi() const { return D_phi_; }
+  Real D_r() const { return D_r_; }
+
+  spss_shim::DeviceSystem &device() { return dev_; }
+  void SetHostSync(bool on) { dev_.SetHostSync(on); }
+  void SetMethod(int method) { dev_.SetMethod(method); }
+  void DownloadState(Real *system_state) { dev_.DownloadState(system_state); }
+
+ private:
+  ThreadSharedMemory *thread_;
+  PeriodicBoundaryConditions &pbc_config_;
+  Real rho_, alpha_, D_phi_, D_r_;
+  spss_shim::DeviceSystem dev_;
+};
+
+// ---------------------------------------------------------------------------
+// steppers: the step runs on the device (the k-buffers live in the handle)
+namespace spss_shim {
+template <class System>
+inline void stochastic_step(System &sys, Real *const state, Real dt, bool srk, const std::vector<Real> *normals) {
+  DeviceSystem &d = sys.device();
+  d.Upload(state);
+  const int mode = normals ? SPSS_NOISE_HOST : SPSS_NOISE_PHILOX;
+  const double *z = normals ? normals->data() : nullptr;
+  if (srk)
+    check(spss_srk_step(d.handle(), d.method(), dt, mode, z));
+  else
+    check(spss_euler_step(d.handle(), d.method(), dt, mode, z));
+  d.Download(state);
+}
+}  // namespace spss_shim
+
+// RungeKutta4StepperPtr.hpp:15-35
+class RungeKutta4StepperPtr {
+ public:
+  RungeKutta4StepperPtr(ThreadSharedMemory *thread, int number_of_particles, int number_of_state_variables)
+      : thread_(thread), n_(number_of_particles), s_(number_of_state_variables) {}
+  template <class System>
+  void DoStep(System &particle_system, Real *const system_state, Real /*t*/, Real dt) {
+    spss_shim::DeviceSystem &d = particle_system.device();
+    d.Upload(system_state);
+    spss_shim::check(spss_rk4_step(d.handle(), d.method(), dt));
+    d.Download(system_state);
+  }
+
+ private:
+  ThreadSharedMemory *thread_;
+  int n_, s_;
+};
+
+// StochasticEulerStepperPtr.hpp:22-73; normals: one per particle (s = 3, 4), three for s = 6
+class StochasticEulerStepperPtr {
+ public:
+  StochasticEulerStepperPtr(ThreadSharedMemory *thread, int number_of_particles, int number_of_state_variables)
+      : thread_(thread), n_(number_of_particles), s_(number_of_state_variables) {}
+  template <class System>
+  void DoStep(System &particle_system, Real *const system_state, Real /*t*/, Real dt) {
+    spss_shim::stochastic_step(particle_system, system_state, dt, false, nullptr);
+  }
+  template <class System>
+  void DoStep(System &particle_system, Real *const system_state, Real /*t*/, Real dt,
+              const std::vector<Real> &normals) {
+    spss_shim::stochastic_step(particle_system, system_state, dt, false, &normals);
+  }
+
+ private:
+  ThreadSharedMemory *thread_;
+  int n_, s_;
+};
+
+// StochasticRungeKuttaStepperPtr.hpp:22-76; normals: (U1, U2) per particle
+class StochasticRungeKuttaStepperPtr {
+ public:
+  StochasticRungeKuttaStepperPtr(ThreadSharedMemory *thread, int number_of_particles, int number_of_state_variables)
+      : thread_(thread), n_(number_of_particles), s_(number_of_state_variables) {}
+  void DoStep(ParticleSystemFirstOrderPtr &particle_system, Real *const system_state, Real /*t*/, Real dt) {
+    particle_system.is_higher_order_sde_integration_ = true;  // StochasticRungeKuttaStepperPtr.hpp:44
+    spss_shim::stochastic_step(particle_system, system_state, dt, true, nullptr);
+  }
+  void DoStep(ParticleSystemFirstOrderPtr &particle_system, Real *const system_state, Real /*t*/, Real dt,
+              const std::vector<Real> &normals) {
+    particle_system.is_higher_order_sde_integration_ = true;
+    spss_shim::stochastic_step(particle_system, system_state, dt, true, &normals);
+  }
+
+ private:
+  ThreadSharedMemory *thread_;
+  int n_, s_;
+};
+
+// ---------------------------------------------------------------------------
+// AveragingObserverPtr.hpp:13-40: running sum of the state on the device of the system it
+// observes (Bind), or of the host state it is given (uploaded through the bound system)
+class AveragingObserverPtr {
+ public:
+  AveragingObserverPtr(ThreadSharedMemory *thread, int number_of_particles, int number_of_state_variables)
+      : thread_(thread), n_(number_of_particles), s_(number_of_state_variables) {}
+  void Bind(ParticleSystemFirstOrderPtr &system) {
+    dev_ = &system.device();
+    spss_shim::check(spss_averaging_reset(dev_->handle()));
+  }
+  // with host sync the device already holds system_state; without it the device state is the
+  // current one by construction
+  void operator()(Real *const /*system_state*/, long /*size*/, Real /*t*/) {
+    if (!dev_) throw std::runtime_error("spss: AveragingObserverPtr needs Bind(particle_system)");
+    spss_shim::check(spss_averaging_accumulate(dev_->handle()));
+  }
+  // time-averaged state (sum / count)
+  std::vector<Real> AveragedSystemState() const {
+    std::vector<Real> sum((size_t)n_ * s_);
+    long count = 0;
+    spss_shim::check(spss_averaging_get(dev_->handle(), sum.data(), &count));
+    for (Real &v : sum) v /= (count > 0 ? (Real)count : 1.0);
+    return sum;
+  }
+
+ private:
+  ThreadSharedMemory *thread_;
+  int n_, s_;
+  spss_shim::DeviceSystem *dev_ = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// SimulationEngineFirstOrderPtr.cpp:96-165: random initial state (x, y ~ U[0,1), phi ~
+// U[0,2pi), omega ~ U[-sigma/xi, sigma/xi]), then `while (t <= t_1) { t += dt; DoStep; observe }`
+// with the stochastic Runge-Kutta stepper.  The run keeps the state on the device and reads
+// the order parameter there every `summary_every` steps (the binary observer's cadence).
+class SimulationEngineFirstOrderPtr {
+ public:
+  SimulationEngineFirstOrderPtr(ThreadSharedMemory *thread, int number_of_particles, int number_of_state_variables)
+      : thread_(thread), pbc_config_(thread, number_of_particles, number_of_state_variables, 1.0, 1.0),
+        n_(number_of_particles), s_(number_of_state_variables),
+        system_state_((size_t)number_of_particles * number_of_state_variables, 0.0) {}
+
+  struct Run {
+    Real v_0 = 1.0, xi = 0.1, sigma = 1.0, rho = 0.8, alpha = 0.3, D_phi = 0.01;  // :103-108
+    Real t_0 = 0.0, t_1 = 1000.0, dt = 0.01;                                        // :110-112
+    int summary_every = 100;
+    unsigned seed = 0;
+  };
+
+  void RunSimulation() { RunSimulation(Run{}); }
+  void RunSimulation(const Run &r) {
+    InitializeRandomSystemState(r.sigma, r.xi, r.seed);
+    StochasticRungeKuttaStepperPtr stepper(thread_, n_, s_);
+    ParticleSystemFirstOrderPtr particle_system(thread_, r.v_0, r.xi, r.sigma, r.rho, r.alpha, r.D_phi, pbc_config_,
+                                                n_, s_);
+    particle_system.SetHostSync(false);
+    summary_.clear();
+    Real t = r.t_0;
+    long k = 0;
+    while (t <= r.t_1) {
+      t += r.dt;
+      stepper.DoStep(particle_system, system_state_.data(), t, r.dt);
+      if (++k % r.summary_every == 0) summary_.push_back({t, particle_system.OrderParameter()});
+    }
+    particle_system.DownloadState(system_state_.data());
+  }
+  void RunContinuationMethod() {
+    throw std::runtime_error("spss: parameter continuation (dopri5) is not part of the device path");
+  }
+
+  const std::vector<Real> &system_state() const { return system_state_; }
+  struct Summary {
+    Real t;
+    std::complex<Real> order_parameter;
+  };
+  const std::vector<Summary> &summary() const { return summary_; }
+
+ private:
+  void InitializeRandomSystemState(Real sigma, Real xi, unsigned seed) {
+    std::mt19937 gen(seed);
+    std::uniform_real_distribution<Real> unif(0.0, 1.0);
+    std::uniform_real_distribution<Real> ang(0.0, 2.0 * M_PI);
+    std::uniform_real_distribution<Real> om(-sigma / xi, sigma / xi);
+    for (int i = 0; i < n_; ++i) {
+      system_state_[(size_t)s_ * i] = unif(gen);
+      system_state_[(size_t)s_ * i + 1] = unif(gen);
+      system_state_[(size_t)s_ * i + 2] = ang(gen);
+      if (s_ == 4) system_state_[(size_t)s_ * i + 3] = om(gen);
+    }
+  }
+
+  ThreadSharedMemory *thread_;
+  PeriodicBoundaryConditions pbc_config_;
+  int n_, s_;
+  std::vector<Real> system_state_;
+  std::vector<Summary> summary_;
+};

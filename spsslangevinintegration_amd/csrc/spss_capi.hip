@@ -792,6 +792,36 @@ int spss_device_copy_bandwidth(long bytes, int repeats, double *gbps) {
   return SPSS_OK;
 }
 
+int spss_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  if (!ctr || !key || !out) return fail(SPSS_ERR_INVALID, "NULL argument");
+  U4 r = philox4x32_10(U4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+  out[3] = r.w;
+  return SPSS_OK;
+}
+
+// the host twin of philox_normal2: identical counter, key and 53-bit uniforms
+int spss_philox_normals(uint64_t seed, uint64_t step, uint32_t block, long first, long count, double *z0,
+                        double *z1) {
+  if (count < 0 || first < 0 || (count > 0 && (!z0 || !z1))) return fail(SPSS_ERR_INVALID, "bad argument");
+  const double inv53 = 1.0 / 9007199254740992.0;
+  for (long k = 0; k < count; ++k) {
+    const uint32_t particle = (uint32_t)(first + k);
+    U4 r = philox4x32_10(U4{particle, block, (uint32_t)step, (uint32_t)(step >> 32)}, (uint32_t)seed,
+                         (uint32_t)(seed >> 32));
+    const uint64_t a = ((uint64_t)r.x << 21) ^ (uint64_t)(r.y >> 11);
+    const uint64_t b = ((uint64_t)r.z << 21) ^ (uint64_t)(r.w >> 11);
+    const double u1 = ((double)(a & ((1ull << 53) - 1)) + 1.0) * inv53;
+    const double u2 = (double)(b & ((1ull << 53) - 1)) * inv53;
+    const double rad = std::sqrt(-2.0 * std::log(u1));
+    z0[k] = rad * std::cos(2.0 * M_PI * u2);
+    z1[k] = rad * std::sin(2.0 * M_PI * u2);
+  }
+  return SPSS_OK;
+}
+
 int spss_kernel_timer_start(spss_system *h) {
   if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
   if (!h->timer) h->timer = new KernelTimer();

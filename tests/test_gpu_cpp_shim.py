@@ -1,0 +1,133 @@
+"""The reference-named C++ classes (spsslangevinintegration_amd/cpp/spss_reference_api.hpp)
+driven by a C++ host program the way the reference's engine drives its own classes
+(cpp/shim_check), against the oracle:
+
+* RungeKutta4StepperPtr::DoStep over ParticleSystemFirstOrderPtr, method AUTO (rho = 0.6:
+  the reference dispatches to all pairs), 10 steps: within 1e-12;
+* StochasticEulerStepperPtr::DoStep with injected normals, linked list: within 1e-12;
+* StochasticRungeKuttaStepperPtr::DoStep with the library's Philox draws (regenerated on the
+  host for the oracle) and GetComplimentaryAlignmentForce: within 1e-12;
+* AveragingObserverPtr bound to the system, host sync off: the time mean of the states;
+* SimulationEngineFirstOrderPtr::RunSimulation (the reference's parameters, t_1 = 1): the
+  time loop's step count and the order parameter it reads on the device, which for the last
+  summary is the one of its final state.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import pyoracle
+from spsslangevinintegration_amd import _capi
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "spsslangevinintegration_amd", "cpp", "shim_check")
+KW = dict(v0=0.02, xi=1.3, sigma=2.0, alpha=0.1, D_phi=0.07)  # shim_check.cpp's parameters
+
+
+def _state(n, seed):
+    rng = np.random.default_rng(seed)
+    st = np.zeros((n, 4))
+    st[:, 0], st[:, 1] = rng.random(n), rng.random(n)
+    st[:, 2] = rng.random(n) * 2 * np.pi
+    st[:, 3] = rng.normal(0, 0.3, n)
+    return st.reshape(-1)
+
+
+def _run(*args):
+    print("[gpu] shim_check", " ".join(str(a) for a in args), flush=True)
+    r = subprocess.run([EXE] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+
+def test_shim_rk4_auto_dispatch():
+    n, rho, K, dt = 1024, 0.6, 10, 0.01
+    st = _state(n, 1)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        st.tofile(fin)
+        _run("rk4", n, rho, K, dt, fin, fout)
+        got = np.fromfile(fout)
+    o = oracle.Oracle(model=4, n=n, rho=rho, **KW)
+    want = st.copy()
+    for _ in range(K):
+        o.rk4_step(0, want, dt)  # method 0: the reference's EvaluateRhs dispatch
+    assert np.max(np.abs(got - want)) <= 1e-12
+
+
+def test_shim_euler_injected_noise():
+    n, rho, K, dt = 4096, 0.03, 10, 0.005
+    st = _state(n, 2)
+    z = np.random.default_rng(3).normal(0, 1, (K, n))
+    with tempfile.TemporaryDirectory() as td:
+        fin, fz, fout = (os.path.join(td, f) for f in ("in.bin", "z.bin", "out.bin"))
+        st.tofile(fin)
+        z.tofile(fz)
+        _run("euler", n, rho, K, dt, fin, fz, fout)
+        got = np.fromfile(fout)
+    o = oracle.Oracle(model=4, n=n, rho=rho, **KW)
+    want = st.copy()
+    for k in range(K):
+        o.euler_step(3, want, dt, z[k])
+    assert np.max(np.abs(got - want)) <= 1e-12
+
+
+def test_shim_srk_philox_and_complementary_force():
+    n, rho, K, dt, seed = 4096, 0.03, 8, 0.005, 0x5EED0000
+    st = _state(n, 4)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        st.tofile(fin)
+        _run("srk", n, rho, K, dt, fin, fout)
+        raw = np.fromfile(fout)
+    got, compl = raw[:4 * n], raw[4 * n:]
+    o = oracle.Oracle(model=4, n=n, rho=rho, **KW)
+    want = st.copy()
+    for k in range(K):
+        z0, z1 = _capi.philox_normals(seed, k, n)
+        o.srk_step(3, want, dt, np.stack([z0, z1], axis=1).reshape(-1))
+    assert np.max(np.abs(got - want)) <= 1e-12
+    assert np.max(np.abs(compl - o.complementary_force())) <= 1e-12
+
+
+def test_shim_averaging_observer():
+    n, rho, K, dt, seed = 4096, 0.03, 6, 0.005, 0x5EED0000
+    st = _state(n, 5)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        st.tofile(fin)
+        _run("average", n, rho, K, dt, fin, fout)
+        raw = np.fromfile(fout)
+    mean, final = raw[:4 * n], raw[4 * n:]
+    o = oracle.Oracle(model=4, n=n, rho=rho, **KW)
+    cur = st.copy()
+    acc = np.zeros(4 * n)
+    for k in range(K):
+        z0, _ = _capi.philox_normals(seed, k, n)
+        o.euler_step(3, cur, dt, z0)
+        acc += cur
+    assert np.max(np.abs(final - cur)) <= 1e-12
+    assert np.max(np.abs(mean - acc / K)) <= 1e-12 * max(1.0, np.max(np.abs(acc)))
+
+
+def test_shim_engine_run_simulation():
+    n = 2048
+    with tempfile.TemporaryDirectory() as td:
+        fout = os.path.join(td, "out.bin")
+        _run("engine", n, fout)
+        final = np.fromfile(fout)
+        summ = np.fromfile(fout + ".summary").reshape(-1, 3)
+    assert np.all(np.isfinite(final))
+    # t_0 = 0, dt = 0.01, while (t <= 1): the FP64 loop runs 100 steps (t reaches
+    # 1.0000000000000007); summaries every 10, the last one of the final state
+    assert summ.shape[0] == 10
+    assert abs(summ[-1, 0] - 1.0) < 1e-12
+    r = np.hypot(summ[:, 1], summ[:, 2])
+    assert np.all((r >= 0) & (r <= 1))
+    re, im = pyoracle.order_parameter(final, n, 4, 2)
+    assert abs(re - summ[-1, 1]) <= 1e-12 and abs(im - summ[-1, 2]) <= 1e-12
