@@ -6,8 +6,8 @@ uniform in the unit box, rho = 0.004 (250 x 250 cells, ~52.7 neighbours), inerti
 phase model with m = 0.5, sigma = 1.5, D = 0.02 mapped onto the reference's
 variables (xi = 1/m = 2, sigma/m = 3, D/m^2 = 0.08, natural frequencies
 omega_i/m with omega_i ~ Lorentzian(0, 0.05) clipped at +-10), phi_dot(0) = 0,
-v0 = 0.005, dt = 0.002, cell list rebuilt every step, stochastic Euler with
-Philox noise (seed 0x5EED0003), FP32-compensated pair sums, FP64 state.
+v0 = 0.005, dt = 0.002, cell membership refreshed every step, stochastic Euler with
+Philox noise (seed 0x5EED0003), FP32 pair arithmetic (blocked sums: FP32 partials, FP64 totals), FP64 state.
 
 Prints one JSON line.  `value` is whole-job particle-steps/s with the state
 resident in HBM.  `roofline` is the pair kernel (fk_pairs, phasor sums of the
@@ -159,7 +159,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "N=1,048,576 uniform in the unit box, rho=0.004 cell list rebuilt every step, "
+        "config": {"workload": "N=1,048,576 uniform in the unit box, rho=0.004 cell list refreshed every step, "
                                "stochastic Euler dt=0.002, m=0.5 sigma=1.5 D=0.02, Lorentzian(0,0.05) omega, "
                                "FP32 distance tests, phasor sums in FP32 partials combined in FP64, FP64 state",
                    "n": n, "rho": 0.004, "cell_subdivision": args.subdiv, "parallelism": "single GPU"},

@@ -48,15 +48,18 @@ extern "C" {
 
 /* arithmetic of the pair sums */
 #define SPSS_PRECISION_FP64 0  /* FP64, the reference's operation order */
-#define SPSS_PRECISION_FP32C 1 /* FP32 pair math on cell-relative offsets, two-sum compensated
-                                  coupling sums, FP64 stage combination (cell-list methods) */
+#define SPSS_PRECISION_FP32C 1 /* linked-list method, models 3 and 4: FP32 distance tests on
+                                  cell-relative offsets (guard band decided in FP64, so the
+                                  neighbour sets equal the FP64 ones), coupling sums through the
+                                  neighbours' unit phasors in FP32 partial sums combined in FP64,
+                                  FP64 state and stage combination */
 
 /* Verlet semantics */
 #define SPSS_VERLET_COMPAT 0 /* reference: r_max = rho + 0.05, candidates from the 9 cells */
 #define SPSS_VERLET_EXACT 1  /* r_max = rho + verlet_skin, candidates from cells >= r_max */
 
 /* steppers */
-#define SPSS_STEPPER_RK4 0   /* RungeKutta4Stepper[Ptr]::DoStep, Steppers/RungeKutta4StepperPtr.cpp:68-115 */
+#define SPSS_STEPPER_RK4 0   /* RungeKutta4Stepper[Ptr]::DoStep, Steppers/RungeKutta4StepperPtr.cpp:31-82 */
 #define SPSS_STEPPER_EULER 1 /* StochasticEulerStepperPtr.hpp:40-73 (s=4), StochasticEulerStepper.hpp:33-51 (s=6) */
 #define SPSS_STEPPER_SRK 2   /* StochasticRungeKuttaStepperPtr.hpp:40-76 (s=4 only) */
 
@@ -107,12 +110,13 @@ int spss_abi_version(void);
 int spss_set_stream(spss_system *sys, void *hip_stream);
 int spss_synchronize(spss_system *sys);
 
-/* device-resident master state (SimulationEngineFirstOrderPtr.cpp:71 `new Real[s*N]`) */
+/* device-resident master state (SimulationEngineFirstOrderPtr.cpp:29 `new Real[system_state_size_]`) */
 int spss_set_state(spss_system *sys, const double *host_state);
 int spss_get_state(spss_system *sys, double *host_state);
 int spss_set_state_device(spss_system *sys, const double *device_state);
-/* current device state buffer; FP32C stochastic steps alternate between two buffers, so the
- * pointer is valid until the next step */
+/* the device buffer of the master state (s*n doubles, original particle order), for the caller
+ * to read or write between calls.  FP32C stochastic steps advance a copy kept in cell order;
+ * this call writes that copy back first and drops it (the next such step re-sorts). */
 double *spss_state_device(spss_system *sys);
 
 /* ParticleSystemFirstOrderPtr::SetNaturalFrequencies (ParticleSystemFirstOrderPtr.cpp:126-129) */
@@ -137,7 +141,7 @@ int spss_rk4_step(spss_system *sys, int method, double dt);
 int spss_euler_step(spss_system *sys, int method, double dt, int noise_mode, const double *noise);
 int spss_srk_step(spss_system *sys, int method, double dt, int noise_mode, const double *noise);
 /* `nsteps` steps of one stepper (the time loop of SimulationEngineFirstOrderPtr::RunSimulation,
- * SimulationEngineFirstOrderPtr.cpp:170-181, without observers).  Noise: NONE or PHILOX. */
+ * SimulationEngineFirstOrderPtr.cpp:128-137, without observers).  Noise: NONE or PHILOX. */
 int spss_run(spss_system *sys, int stepper, int method, long nsteps, double dt, int noise_mode);
 /* number of steps taken so far (the Philox step counter) */
 int spss_get_step(spss_system *sys, long *step);
@@ -157,14 +161,14 @@ int spss_get_verlet_lists(spss_system *sys, int *host_counts, int *host_flat, lo
 int spss_neighbour_pairs(spss_system *sys, int method, const double *host_stage, int32_t *host_pairs,
                          long cap, long *count);
 
-/* SaveSummaryStatistics' complex order parameter (BinaryObserverFirstOrderPtr.cpp:181-200),
+/* SaveSummaryStatistics' complex order parameter (BinaryObserverFirstOrderPtr.cpp:126-146),
  * accumulated in FP64 on the device: (1/N) sum exp(i * state[s*k + slot]) */
 int spss_order_parameter(spss_system *sys, int slot, double *re, double *im);
-/* ApplyPeriodicBoundaryConditions(Real*, long) on the device state (PeriodicBoundaryConditions.cpp:197-205) */
+/* ApplyPeriodicBoundaryConditions(Real*, long) on the device state (PeriodicBoundaryConditions.cpp:147-155) */
 int spss_apply_periodic_boundary_conditions(spss_system *sys);
 
 /* Instrumentation with no reference counterpart (the reference only prints wall-clock
- * seconds per output frame, BinaryObserverFirstOrderPtr.cpp:158-160): HIP events on the
+ * seconds per output frame, BinaryObserverFirstOrderPtr.cpp:103-105): HIP events on the
  * handle's stream around every launch of the RHS pair kernel between start and stop. */
 int spss_kernel_timer_start(spss_system *sys);
 int spss_kernel_timer_stop(spss_system *sys, double *total_ms, long *launches);
@@ -183,7 +187,8 @@ int spss_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 int spss_philox_normals(uint64_t seed, uint64_t step, uint32_t block, long first, long count, double *z0,
                         double *z1);
 
-/* AveragingObserver (Observers/AveragingObserver.cpp:66-105): running FP64 sum of the state */
+/* AveragingObserver::operator() and the sum behind SaveAveragedSystemState (Observers/AveragingObserver.cpp:29-37,
+ * 39-68; AveragingObserverPtr.cpp:46-79): running FP64 sum of the state and the step count */
 int spss_averaging_reset(spss_system *sys);
 int spss_averaging_accumulate(spss_system *sys);
 int spss_averaging_get(spss_system *sys, double *host_sum, long *count);

@@ -1,4 +1,4 @@
-// spss_fast.hpp -- the FP32-compensated cell-list path (SPSS_PRECISION_FP32C).
+// spss_fast.hpp -- the FP32 cell-list path (SPSS_PRECISION_FP32C): FP32 pair arithmetic, FP64 sums and state.
 #pragma once
 
 #include <hip/hip_runtime.h>

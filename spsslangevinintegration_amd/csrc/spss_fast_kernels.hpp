@@ -1,4 +1,4 @@
-// spss_fast_kernels.hpp -- device code of the FP32-compensated cell-list path
+// spss_fast_kernels.hpp -- device code of the FP32 cell-list path (SPSS_PRECISION_FP32C)
 // (included by spss_kernels_fast.hip; host logic lives there).
 //
 // Two ways to feed the pair kernel:

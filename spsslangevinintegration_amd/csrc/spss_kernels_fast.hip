@@ -1,4 +1,4 @@
-// spss_kernels_fast.hip -- FP32-compensated cell-list RHS (SPSS_PRECISION_FP32C), models 3 and 4.
+// spss_kernels_fast.hip -- FP32 cell-list path (SPSS_PRECISION_FP32C), models 3 and 4: host side.
 //
 // Data path of one RHS evaluation (stage state in FP64 AoS, original order):
 //   fk_keys      cell of every particle on a grid of side 1/(k*nc), histogram by atomics

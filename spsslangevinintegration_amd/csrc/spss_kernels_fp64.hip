@@ -540,7 +540,7 @@ __global__ void k_order_final(int nb, const double *__restrict__ partial, double
   out[q] = (s + c) / n;
 }
 
-// ApplyPeriodicBoundaryConditions(Real*, long), PeriodicBoundaryConditions.cpp:197-205
+// ApplyPeriodicBoundaryConditions(Real*, long), PeriodicBoundaryConditions.cpp:147-155
 __global__ void k_apply_pbc(int n, int s, double *__restrict__ state, double xs, double ys, double xrs, double yrs) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -549,7 +549,7 @@ __global__ void k_apply_pbc(int n, int s, double *__restrict__ state, double xs,
   state[ii + 1] -= floor(state[ii + 1] * yrs) * ys;
 }
 
-// AveragingObserver::operator(), AveragingObserver.cpp:66-74
+// AveragingObserver::operator(), AveragingObserver.cpp:29-37
 __global__ void k_accumulate(long m, const double *__restrict__ state, double *__restrict__ acc) {
   long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < m) acc[k] = state[k] + acc[k];

@@ -1,4 +1,4 @@
-"""FP32-compensated cell-list path (SPSS_PRECISION_FP32C) against the FP64 oracle.
+"""FP32 cell-list path (SPSS_PRECISION_FP32C: FP32 distance tests and phasor partial sums, FP64 totals and state) against the FP64 oracle.
 
 * neighbour sets: bit-exact against the brute-force FP64 set with the reference's
   ClassC formula (sorted (i, j) lists), for cell subdivisions 1-3, positions outside
