@@ -74,6 +74,101 @@ def time_steps(g, spss, stepper, steps, warmup, dt=0.002):
     return wall, kms / max(launches, 1)
 
 
+def ensemble_config(R=512, seed=0x5EED0004):
+    """The batched ensemble: R replicas with N_r = round(2048 * 32^u), u ~ U[0,1), sigma ~ U[0,4],
+    D ~ U[0,0.2], m in {0, 0.5, 1}; rho = 0.03, v0 = 0.01, Philox seed 0x5EED0004 + r; natural
+    frequencies omega ~ N(0, 0.2^2) (the inertial configuration's distribution).  m = 0 replicas
+    are the first-order model (s = 3); m > 0 map to model 4 with xi = 1/m, sigma/m, D/m^2, omega/m."""
+    rng = np.random.default_rng(seed)
+    ns = np.round(2048 * 32.0 ** rng.random(R)).astype(np.int64)
+    sigma = rng.uniform(0, 4, R)
+    D = rng.uniform(0, 0.2, R)
+    m = rng.choice([0.0, 0.5, 1.0], R)
+    seeds = seed + np.arange(R)
+    groups = {}
+    for model, sel in ((3, m == 0), (4, m > 0)):
+        idx = np.nonzero(sel)[0]
+        mm = np.where(m[idx] > 0, m[idx], 1.0)
+        xi = np.where(model == 4, 1.0 / mm, 0.0)
+        sig = sigma[idx] / (mm if model == 4 else 1.0)
+        Dp = D[idx] / (mm ** 2 if model == 4 else 1.0)
+        states, nats = [], []
+        for k, r in enumerate(idx):
+            st = np.zeros((ns[r], model))
+            st[:, 0], st[:, 1] = rng.random(ns[r]), rng.random(ns[r])
+            st[:, 2] = rng.random(ns[r]) * 2 * np.pi
+            states.append(st.reshape(-1))
+            w = rng.normal(0, 0.2, ns[r])
+            nats.append(w / mm[k] if model == 4 else w)
+        groups[model] = dict(idx=idx, ns=ns[idx], xi=xi, sigma=sig, D=Dp, seeds=seeds[idx], states=states, nats=nats)
+    return groups
+
+
+def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0.03, every=10):
+    """512-replica ensemble: batched run (autotuned tiles, graph replays) against the plain loop
+    (the same kernels through one single-system handle per replica, one after another)."""
+    groups = ensemble_config()
+    total_n = sum(int(g["ns"].sum()) for g in groups.values())
+    batches, tune = {}, {}
+    for model, g in groups.items():
+        b = spss.ReplicaBatch(model, g["ns"], g["xi"], g["sigma"], g["D"], g["seeds"], v0=v0, rho=rho)
+        b.set_natural_frequencies(np.concatenate(g["nats"]))
+        b.set_state(np.concatenate(g["states"]))
+        table, chosen = b.autotune(dt, probe_steps=10)
+        tune[model] = {"ms_per_step": [round(float(t), 4) for t in table], "chosen": int(chosen)}
+        batches[model] = b
+    # sample for the plain-loop comparison: plain_steps steps of every replica, both ways
+    t0 = time.perf_counter()
+    for b in batches.values():
+        b.run(plain_steps, dt, observe_every=every, observe_from=0, graph=True)
+    t_batch_sample = time.perf_counter() - t0
+    zb = {m: b.order_stats() for m, b in batches.items()}
+    t_plain, max_err = 0.0, 0.0
+    for model, g in groups.items():
+        for k in range(len(g["ns"])):
+            h = spss.ParticleSystem(model=model, n=int(g["ns"][k]), v0=v0, xi=g["xi"][k], sigma=g["sigma"][k],
+                                    rho=rho, D_phi=g["D"][k], natfreq_local=1, precision=spss.PRECISION_FP32C,
+                                    seed=int(g["seeds"][k]))
+            h.set_natural_frequencies(g["nats"][k])
+            h.set_state(g["states"][k])
+            h.synchronize()
+            rs = []
+            t0 = time.perf_counter()
+            for _ in range(plain_steps // every):
+                h.run(spss.STEPPER_EULER, spss.METHOD_LINKED_LIST, every, dt, spss.NOISE_PHILOX)
+                rs.append(np.hypot(*h.order_parameter(2)))
+            h.synchronize()
+            t_plain += time.perf_counter() - t0
+            max_err = max(max_err, abs(float(np.mean(rs)) - float(zb[model][0][k])))
+            h.close()
+    # the full batched run: time averages over the second half
+    for model, b in batches.items():
+        b.reset(0)
+        b.set_state(np.concatenate(groups[model]["states"]))
+    t0 = time.perf_counter()
+    for b in batches.values():
+        b.run(steps, dt, observe_every=every, observe_from=steps // 2, graph=True)
+    t_batch = time.perf_counter() - t0
+    R_mean = np.zeros(sum(len(g["ns"]) for g in groups.values()))
+    R_var = np.zeros_like(R_mean)
+    for model, b in batches.items():
+        mean, var, cnt, z, step = b.order_stats()
+        R_mean[groups[model]["idx"]] = mean
+        R_var[groups[model]["idx"]] = var
+        b.close()
+    return {
+        "replicas": int(R_mean.size), "particles": total_n, "steps": steps, "dt": dt,
+        "particle_steps_per_s": total_n * steps / t_batch,
+        "sample_steps": plain_steps,
+        "sample_batched_particle_steps_per_s": total_n * plain_steps / t_batch_sample,
+        "sample_plain_particle_steps_per_s": total_n * plain_steps / t_plain,
+        "speedup_vs_plain_loop": t_plain / t_batch_sample,
+        "order_param_max_abs_err_vs_plain": max_err,
+        "mean_R_over_replicas": float(R_mean.mean()),
+        "autotune": tune,
+    }
+
+
 def cpu_baseline(n=1 << 20, steps=3, warmup=1):
     """The reference's serial linked-list RHS + Euler update on the host (1 core)."""
     m, sigma, D = 0.5, 1.5, 0.02
@@ -116,6 +211,9 @@ def main():
     ap.add_argument("--subdiv", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--batch-steps", type=int, default=3000)
+    ap.add_argument("--plain-steps", type=int, default=100)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see the sharding design in "
@@ -142,6 +240,11 @@ def main():
     wall2, _ = time_steps(g2, spss, spss.STEPPER_SRK, srk_steps, 10)
     g2.close()
     print(f"[bench] {ms_step:.4f} ms/step, pair kernel {pair_ms * 1e3:.1f} us", file=sys.stderr, flush=True)
+    batched = None
+    if not args.no_batch:
+        print("[bench] batched ensemble", file=sys.stderr, flush=True)
+        batched = batched_ensemble(spss, steps=args.batch_steps, plain_steps=args.plain_steps)
+        print(f"[bench] batched {batched['speedup_vs_plain_loop']:.1f}x plain loop", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -173,6 +276,7 @@ def main():
         "step_roofline_frac": STEP_BYTES * value / HBM_PEAK,
         "pair_interactions_per_s": pairs / n * value if pairs else None,
         "srk_particle_steps_per_s": n * srk_steps / wall2,
+        "batched": batched,
     }
     g.close()
     print(json.dumps(line), flush=True)

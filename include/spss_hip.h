@@ -187,6 +187,53 @@ int spss_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 int spss_philox_normals(uint64_t seed, uint64_t step, uint32_t block, long first, long count, double *z0,
                         double *z1);
 
+/* ---- batches of independent replicas (no reference counterpart: the reference runs one
+ * system per job and sweeps parameters in a host loop, SimulationEngineFirstOrder.cpp:105-113).
+ * The replicas of a batch share the model (3 or 4), v0, rho, alpha and the cell grid; each has
+ * its own particle count, phase friction, coupling, noise intensity and Philox seed.  FP32 cell-
+ * list path (linked-list neighbours, stochastic Euler, Philox noise).  The state and the natural
+ * frequencies are the replicas' arrays one after another (AoS FP64, reference layout).  Replica
+ * r of a batch steps exactly like a single system (spss_create with precision FP32C, the same
+ * parameters and seed) run through spss_run with SPSS_NOISE_PHILOX. */
+typedef struct spss_batch spss_batch;
+typedef struct spss_replica {
+  int n;          /* particles */
+  double xi;      /* phase friction (model 4) */
+  double sigma;   /* coupling */
+  double D_phi;   /* noise intensity */
+  uint64_t seed;  /* Philox key */
+} spss_replica;
+typedef struct spss_batch_params {
+  int model;            /* 3 or 4 */
+  double v0;
+  double rho;
+  double alpha;
+  int cell_subdivision; /* cells of side 1/(k*int(1/rho)); 0 -> 1 */
+  int natfreq_local;    /* 1: natural frequencies in the local branches (model 4) */
+} spss_batch_params;
+int spss_batch_create(const spss_batch_params *bp, int nrep, const spss_replica *reps, spss_batch **out);
+int spss_batch_destroy(spss_batch *b);
+int spss_batch_set_state(spss_batch *b, const double *host_state);
+int spss_batch_get_state(spss_batch *b, double *host_state);
+int spss_batch_set_natural_frequencies(spss_batch *b, const double *host_w);
+/* clear the time averages and set the step counter (the Philox step of the next step) */
+int spss_batch_reset(spss_batch *b, long step);
+/* `nsteps` stochastic Euler steps of every replica.  Every `observe_every` steps (0: never)
+ * each replica's order parameter Z = <e^{i phi}> is read on the device; readings taken once the
+ * step counter has reached `observe_from` enter the time averages of |Z| and |Z|^2.  use_graph:
+ * the step sequence is captured in a HIP graph once and replayed. */
+int spss_batch_run(spss_batch *b, long nsteps, double dt, long observe_every, long observe_from, int use_graph);
+/* per replica: time-mean |Z|, its variance, the number of readings, the last Z; the step counter */
+int spss_batch_order_stats(spss_batch *b, double *mean_R, double *var_R, long *count, double *last_re,
+                           double *last_im, long *step);
+/* tile width (cells per pair workgroup) of every replica */
+int spss_batch_get_tile_widths(spss_batch *b, int *W);
+int spss_batch_set_tile_widths(spss_batch *b, const int *W);
+/* autotuner: times `probe_steps` steps for 6 tile-width scales (0.5x .. 2x the default
+ * particles per workgroup), keeps the fastest; ms per step into table_ms[<= table_cap], the
+ * chosen index into *chosen.  State and step counter are restored, the averages cleared. */
+int spss_batch_autotune(spss_batch *b, double dt, int probe_steps, double *table_ms, int table_cap, int *chosen);
+
 /* AveragingObserver::operator() and the sum behind SaveAveragedSystemState (Observers/AveragingObserver.cpp:29-37,
  * 39-68; AveragingObserverPtr.cpp:46-79): running FP64 sum of the state and the step count */
 int spss_averaging_reset(spss_system *sys);

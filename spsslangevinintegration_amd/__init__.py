@@ -14,3 +14,4 @@ from ._capi import (METHOD_ALL_PAIRS, METHOD_AUTO, METHOD_GLOBAL_FIELD, METHOD_L
 _capi.load()
 
 from .system import ParticleSystem  # noqa: E402,F401
+from .batch import ReplicaBatch  # noqa: E402,F401

@@ -30,6 +30,9 @@ EXPORTS = [
     "spss_averaging_reset", "spss_averaging_accumulate", "spss_averaging_get",
     "spss_kernel_timer_start", "spss_kernel_timer_stop", "spss_device_copy_bandwidth",
     "spss_philox4x32_10", "spss_philox_normals",
+    "spss_batch_create", "spss_batch_destroy", "spss_batch_set_state", "spss_batch_get_state",
+    "spss_batch_set_natural_frequencies", "spss_batch_reset", "spss_batch_run", "spss_batch_order_stats",
+    "spss_batch_get_tile_widths", "spss_batch_set_tile_widths", "spss_batch_autotune",
 ]
 
 
@@ -42,6 +45,16 @@ class SpssParams(C.Structure):
         ("verlet_mode", C.c_int), ("natfreq_local", C.c_int), ("precision", C.c_int),
         ("cell_subdivision", C.c_int), ("seed", C.c_uint64),
     ]
+
+
+class SpssReplica(C.Structure):
+    _fields_ = [("n", C.c_int), ("xi", C.c_double), ("sigma", C.c_double), ("D_phi", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class SpssBatchParams(C.Structure):
+    _fields_ = [("model", C.c_int), ("v0", C.c_double), ("rho", C.c_double), ("alpha", C.c_double),
+                ("cell_subdivision", C.c_int), ("natfreq_local", C.c_int)]
 
 
 class SpssError(RuntimeError):
@@ -100,6 +113,17 @@ def load(path=None):
         "spss_device_copy_bandwidth": (i, [l, i, C.POINTER(d)]),
         "spss_philox4x32_10": (i, [vp, vp, vp]),
         "spss_philox_normals": (i, [C.c_uint64, C.c_uint64, C.c_uint32, l, l, vp, vp]),
+        "spss_batch_create": (i, [C.POINTER(SpssBatchParams), i, vp, C.POINTER(vp)]),
+        "spss_batch_destroy": (i, [vp]),
+        "spss_batch_set_state": (i, [vp, vp]),
+        "spss_batch_get_state": (i, [vp, vp]),
+        "spss_batch_set_natural_frequencies": (i, [vp, vp]),
+        "spss_batch_reset": (i, [vp, l]),
+        "spss_batch_run": (i, [vp, l, d, l, l, i]),
+        "spss_batch_order_stats": (i, [vp, vp, vp, vp, vp, vp, C.POINTER(l)]),
+        "spss_batch_get_tile_widths": (i, [vp, vp]),
+        "spss_batch_set_tile_widths": (i, [vp, vp]),
+        "spss_batch_autotune": (i, [vp, d, i, vp, i, C.POINTER(i)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -28,6 +28,11 @@ int fail(int code, const std::string &msg) {
 }
 }  // namespace
 
+namespace spss {
+// the message spss_last_error() returns, for the other translation units of the library
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace spss
+
 #define HIPCHECK(call)                                                                           \
   do {                                                                                           \
     hipError_t e_ = (call);                                                                      \

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "spss_device.hpp"
 #include "spss_launch.hpp"
 
@@ -11,6 +13,33 @@ namespace spss {
 struct FastPath;
 
 int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv);
+
+// per-replica coefficients of the stochastic step of a batch (a single system takes them from
+// its params): phase friction xi, coupling sigma, sqrt(2 D) for Euler and SRK, Philox key
+struct Coef {
+  double xi, sigma, int_phi, sq2d;
+  uint64_t seed;
+};
+// a batch of R independent replicas on one cell grid, particles stored replica after replica
+// (rep_n[r] each, summing to P.n); coef (R entries) makes it a batch stepped by
+// fast_batch_step with per-replica xi, sigma, noise and seed
+int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv, int R,
+                         const int *rep_n, const Coef *coef);
+// tile width (cells per pair workgroup) of every replica: read and set (the autotuner's knob)
+int fast_tile_widths(FastPath *f, std::vector<int> *W);
+int fast_set_tile_widths(FastPath *f, const std::vector<int> &W);
+int fast_replica_count(const FastPath *f);
+// one stochastic Euler step of every replica with Philox noise (per-replica seed, particle index
+// inside the replica, step from the device counter, which the call advances)
+int fast_batch_step(FastPath *f, hipStream_t st, const DevParams &P, const double *state, const double *nat,
+                    double dt);
+// per-replica order parameter of the current state; |Z|, |Z|^2 and a count are added to the
+// running sums once the step counter has reached `from`
+int fast_batch_observe(FastPath *f, hipStream_t st, uint64_t from);
+// clear the running sums and set the step counter
+int fast_batch_reset(FastPath *f, hipStream_t st, uint64_t step);
+// running sums (3 per replica: sum |Z|, sum |Z|^2, count), last Z (2 per replica), step counter
+int fast_batch_stats(FastPath *f, hipStream_t st, double *racc, double *zlast, uint64_t *step);
 void fast_destroy(FastPath *f);
 void fast_set_stream(FastPath *f, hipStream_t st);
 void fast_set_timer(FastPath *f, KernelTimer *t);

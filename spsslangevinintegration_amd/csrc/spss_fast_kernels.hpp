@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 #include "spss_device.hpp"
+#include "spss_fast.hpp"
 
 namespace spss {
 
@@ -33,10 +34,29 @@ constexpr int kScanItems = 8;   // counts per thread in the two-level scans
 }  // namespace
 
 struct FastArgs {
-  int n, ncf, k, W, chunks, ntiles;
+  int n, ncf, k, ntiles, cpr;  // cpr: cells per replica (ncf * ncf)
+  const int4 *tiles;           // per tile: first cell of its replica, row, first column, end column
   double H;
   float r2lo, r2hi;
 };
+
+// Replicas: independent systems of the same cell grid stored one after another (particles in
+// replica order, cells [r * cpr, (r + 1) * cpr) for replica r).  A single system is one replica.
+constexpr int kChunkShift = 11;  // rep_chunk[p >> kChunkShift]: replica holding position p rounded down
+struct Layout {
+  const int *rep_first;  // [R + 1] first particle of each replica
+  const int *rep_chunk;
+  int cpr;
+};
+
+__device__ __forceinline__ int replica_of(const Layout &L, int p) {
+  int r = L.rep_chunk[p >> kChunkShift];
+  while (L.rep_first[r + 1] <= p) ++r;
+  return r;
+}
+
+__device__ __forceinline__ int cell_base(int c, int cpr) { return (c / cpr) * cpr; }
+
 
 // buffers of one pair-kernel launch (unused ones are null)
 struct PairBufs {
@@ -65,6 +85,8 @@ struct StepArgs {
   int noise_mode;  // 0 none, 1 device array indexed by particle id, 3 Philox
   const double *noise;
   uint64_t seed, step;
+  const unsigned long long *step_dev;  // when set, the Philox step is read here (graph replays)
+  const Coef *coef;                    // when set, per-replica xi, sigma, noise intensities, seed
 };
 
 // ---------------------------------------------------------------------------
@@ -75,18 +97,19 @@ constexpr int kFar = 9;
 
 __device__ __forceinline__ int move_order(int d) { return d == 4 ? 0 : (d < 4 ? d + 1 : d); }
 
-// source cell of an arrival into cell c by move d
-__device__ __forceinline__ int move_source(int c, int d, int ncf) {
-  const int cx = c % ncf, cy = c / ncf;
+// source cell of an arrival into cell c by move d (within c's replica)
+__device__ __forceinline__ int move_source(int c, int d, int ncf, int cpr) {
+  const int base = cell_base(c, cpr), l = c - base;
+  const int cx = l % ncf, cy = l / ncf;
   const int sx = (cx - (d % 3 - 1) + ncf) % ncf;
   const int sy = (cy - (d / 3 - 1) + ncf) % ncf;
-  return sy * ncf + sx;
+  return base + sy * ncf + sx;
 }
 
-__device__ __forceinline__ int new_count_of(const int *cnt9, int c, int ncf) {
+__device__ __forceinline__ int new_count_of(const int *cnt9, int c, int ncf, int cpr) {
   int s = 0;
 #pragma unroll
-  for (int d = 0; d < 9; ++d) s += cnt9[(long)move_source(c, d, ncf) * 9 + d];
+  for (int d = 0; d < 9; ++d) s += cnt9[(long)move_source(c, d, ncf, cpr) * 9 + d];
   return s;
 }
 
@@ -94,13 +117,13 @@ __device__ __forceinline__ int new_count_of(const int *cnt9, int c, int ncf) {
 // rebuilt cell list
 
 template <int S>
-__global__ void fk_keys(DevParams P, const double *__restrict__ stage, int ncf, int *__restrict__ key,
+__global__ void fk_keys(DevParams P, Layout L, const double *__restrict__ stage, int ncf, int *__restrict__ key,
                         int *__restrict__ slot, int *__restrict__ count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   double xw, yw;
   apply_pbc(P, stage[(long)i * S], stage[(long)i * S + 1], xw, yw);
-  const int c = cell_coord(yw, ncf) * ncf + cell_coord(xw, ncf);
+  const int c = replica_of(L, i) * L.cpr + cell_coord(yw, ncf) * ncf + cell_coord(xw, ncf);
   key[i] = c;
   slot[i] = atomicAdd(count + c, 1);
 }
@@ -181,7 +204,7 @@ __device__ __forceinline__ float4 snap_entry(const DevParams &P, const double *s
 
 // stable rank + snapshot in sorted order
 template <int S>
-__global__ void fk_rank(DevParams P, const double *__restrict__ stage, int ncf, double H,
+__global__ void fk_rank(DevParams P, const double *__restrict__ stage, int ncf, int cpr, double H,
                         const int *__restrict__ key, const int *__restrict__ start, const int *__restrict__ tmp,
                         int *__restrict__ sorted, float4 *__restrict__ snap, float *__restrict__ omega_f) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -193,7 +216,8 @@ __global__ void fk_rank(DevParams P, const double *__restrict__ stage, int ncf, 
   const int p = b + r;
   sorted[p] = i;
   const double *st = stage + (long)i * S;
-  snap[p] = snap_entry<S>(P, st, c % ncf, c / ncf, H);
+  const int l = c - cell_base(c, cpr);
+  snap[p] = snap_entry<S>(P, st, l % ncf, l / ncf, H);
   if (omega_f) omega_f[p] = (S == 4) ? (float)st[3] : 0.f;
 }
 
@@ -265,9 +289,8 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
   const int per = gridDim.x / 8;  // the grid is a multiple of 8
   const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (tile >= A.ntiles) return;
-  const int row = tile / A.chunks;
-  const int ca = (tile % A.chunks) * A.W;
-  const int cb = min(ca + A.W, A.ncf);
+  const int4 td = A.tiles[tile];
+  const int cbase = td.x, row = td.y, ca = td.z, cb = td.w;
   const int W = cb - ca;
   const int k = A.k;
   const int nrows = 2 * k + 1;
@@ -282,7 +305,7 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
     const int r = e / ncols, q = e % ncols;
     const int rrow = ((row + r - k) % ncf + ncf) % ncf;
     const int col = ((ca - k + q) % ncf + ncf) % ncf;
-    const int c = rrow * ncf + col;
+    const int c = cbase + rrow * ncf + col;
     const int g0 = start[c];
     cglob[e] = g0;
     ccnt[e] = start[c + 1] - g0;
@@ -340,7 +363,7 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
 
   // particles of the tile: staged row k, columns k .. k+W-1 (contiguous in LDS and globally)
   const int gtb = cglob[k * ncols + k];
-  const int ntile = start[row * ncf + cb] - start[row * ncf + ca];
+  const int ntile = start[cbase + row * ncf + cb] - start[cbase + row * ncf + ca];
   const int ltb = fits ? cpos[k * ncols + k] : 0;
   const float r2lo = A.r2lo, r2hi = A.r2hi;
 
@@ -556,8 +579,9 @@ __device__ __forceinline__ void step_position(const DevParams &P, const double *
 }
 
 template <int S, bool SRK>
-__device__ __forceinline__ void step_particle(const DevParams &P, const double *pi, int i, double nat_i, double2 sm,
-                                              double2 tm, int nnb, const StepArgs &U, double *out, double &C) {
+__device__ __forceinline__ void step_particle(const DevParams &P, const Coef &K, const double *pi, int i,
+                                              double nat_i, double2 sm, double2 tm, int nnb, const StepArgs &U,
+                                              uint64_t step, double *out, double &C) {
   double sn, cn;
   step_position<S, SRK>(P, pi, U, sn, cn, out[0], out[1]);
   const double st = sn * P.cos_a + cn * P.sin_a;  // sin(phi_i + alpha)
@@ -570,35 +594,35 @@ __device__ __forceinline__ void step_particle(const DevParams &P, const double *
     const double f2 = omega_i;
     double f3 = 0.0;
     if (P.natfreq_local) {
-      f3 = nnb ? -P.xi * omega_i + nat_i + P.sigma * A64 / card : -P.xi * omega_i + nat_i;
+      f3 = nnb ? -K.xi * omega_i + nat_i + K.sigma * A64 / card : -K.xi * omega_i + nat_i;
     } else if (nnb) {
-      f3 = -P.xi * omega_i + P.sigma * A64 / card;
+      f3 = -K.xi * omega_i + K.sigma * A64 / card;
     }
     if (!SRK) {
-      const double z = (U.noise_mode == 1) ? U.noise[i] : (U.noise_mode == 3 ? philox_z(U.seed, i, 0, U.step) : 0.0);
+      const double z = (U.noise_mode == 1) ? U.noise[i] : (U.noise_mode == 3 ? philox_z(K.seed, i, 0, step) : 0.0);
       const double dW = U.sqrt_dt * z;
       out[2] = pi[2] + f2 * U.dt;
-      out[3] = pi[3] + (f3 * U.dt + U.int_phi * dW);
+      out[3] = pi[3] + (f3 * U.dt + K.int_phi * dW);
     } else {
       const double C64 = ct * (tm.x - omega_i * sm.x) + st * (tm.y - omega_i * sm.y);
-      C = nnb ? C64 * (P.sigma / card) : C64;
+      C = nnb ? C64 * (K.sigma / card) : C64;
       double U1 = 0.0, U2 = 0.0;
       if (U.noise_mode == 1) {
         U1 = U.noise[2L * i];
         U2 = U.noise[2L * i + 1];
       } else if (U.noise_mode == 3) {
-        philox_normal2(U.seed, (uint32_t)i, 0, U.step, U1, U2);
+        philox_normal2(K.seed, (uint32_t)i, 0, step, U1, U2);
       }
       const double dt = U.dt;
       const double dW = U1 * U.sqrt_dt;
       const double dZ = U.c15 * (U1 + U.is3 * U2);
-      out[2] = pi[2] + (f2 * dt + 0.5 * f3 * dt * dt + U.sq2d * dZ);
-      out[3] = pi[3] + (f3 * dt + 0.5 * (C - U.friction * f3) * dt * dt + U.sq2d * (dW - U.friction * dZ));
+      out[2] = pi[2] + (f2 * dt + 0.5 * f3 * dt * dt + K.sq2d * dZ);
+      out[3] = pi[3] + (f3 * dt + 0.5 * (C - K.xi * f3) * dt * dt + K.sq2d * (dW - K.xi * dZ));
     }
   } else {  // model 3 Euler
-    const double f2 = nnb ? nat_i + P.sigma * A64 / card : nat_i;
-    const double z = (U.noise_mode == 1) ? U.noise[i] : (U.noise_mode == 3 ? philox_z(U.seed, i, 0, U.step) : 0.0);
-    out[2] = pi[2] + (f2 * U.dt + U.int_phi * (U.sqrt_dt * z));
+    const double f2 = nnb ? nat_i + K.sigma * A64 / card : nat_i;
+    const double z = (U.noise_mode == 1) ? U.noise[i] : (U.noise_mode == 3 ? philox_z(K.seed, i, 0, step) : 0.0);
+    out[2] = pi[2] + (f2 * U.dt + K.int_phi * (U.sqrt_dt * z));
   }
 }
 
@@ -615,9 +639,9 @@ __device__ __forceinline__ int cell_of_position(const DevParams &P, double x, do
 // the move of every resident particle this step: old cell * 16 + (dy+1)*3 + (dx+1), or kFar
 // with an unordered arrival slot in farcnt of the destination
 template <int S, bool SRK>
-__global__ __launch_bounds__(256) void fr_move(DevParams P, int n, int ncf, const double *__restrict__ rstate,
-                                               StepArgs U, int *__restrict__ mv, int *__restrict__ farcnt,
-                                               int *__restrict__ farslot) {
+__global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int ncf,
+                                               const double *__restrict__ rstate, StepArgs U, int *__restrict__ mv,
+                                               int *__restrict__ farcnt, int *__restrict__ farslot) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const double *pi = rstate + (long)p * S;
@@ -628,6 +652,7 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, int n, int ncf, cons
   const int ox = cell_coord(xw, ncf), oy = cell_coord(yw, ncf);
   apply_pbc(P, x1, y1, xw, yw);
   const int nx = cell_coord(xw, ncf), ny = cell_coord(yw, ncf);
+  const int base = replica_of(L, p) * L.cpr;
   int dxc = nx - ox, dyc = ny - oy;
   if (dxc == ncf - 1) dxc = -1;
   if (dxc == 1 - ncf) dxc = 1;
@@ -638,14 +663,14 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, int n, int ncf, cons
     d = (dyc + 1) * 3 + (dxc + 1);
   } else {
     d = kFar;
-    farslot[p] = atomicAdd(farcnt + ny * ncf + nx, 1);
+    farslot[p] = atomicAdd(farcnt + base + ny * ncf + nx, 1);
   }
-  mv[p] = ((oy * ncf + ox) << 4) | d;
+  mv[p] = ((base + oy * ncf + ox) << 4) | d;
 }
 
 // per cell: how many of its particles take each of the nine near moves (cnt9), added into
 // the arrival count of every destination cell (newcnt, zeroed by the scan)
-__global__ __launch_bounds__(256) void fr_cell_counts(int cells, int ncf, const int *__restrict__ start,
+__global__ __launch_bounds__(256) void fr_cell_counts(int cells, int ncf, int cpr, const int *__restrict__ start,
                                                       const int *__restrict__ mv, int *__restrict__ cnt9,
                                                       int *__restrict__ newcnt) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -656,36 +681,39 @@ __global__ __launch_bounds__(256) void fr_cell_counts(int cells, int ncf, const 
 #pragma unroll
     for (int e = 0; e < 9; ++e) v[e] += (d == e) ? 1 : 0;
   }
-  const int cx = c % ncf, cy = c / ncf;
+  const int base = cell_base(c, cpr), l = c - base;
+  const int cx = l % ncf, cy = l / ncf;
 #pragma unroll
   for (int d = 0; d < 9; ++d) {
     cnt9[(long)c * 9 + d] = v[d];
     if (v[d]) {
       const int dx = (cx + d % 3 - 1 + ncf) % ncf, dy = (cy + d / 3 - 1 + ncf) % ncf;
-      atomicAdd(newcnt + dy * ncf + dx, v[d]);
+      atomicAdd(newcnt + base + dy * ncf + dx, v[d]);
     }
   }
 }
 
 // far arrivals: unordered bucket behind the near arrivals of their destination cell
 template <int S, bool SRK>
-__global__ void fr_far_bucket(DevParams P, int n, int ncf, const int *__restrict__ mv,
+__global__ void fr_far_bucket(DevParams P, int n, int ncf, int cpr, const int *__restrict__ mv,
                               const double *__restrict__ rstate, StepArgs U, const int *__restrict__ nstart,
                               const int *__restrict__ cnt9, const int *__restrict__ farslot,
                               int *__restrict__ bucket) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n || (mv[p] & 15) != kFar) return;
+  if (p >= n) return;
+  const int m = mv[p];
+  if ((m & 15) != kFar) return;
   double sn, cn, x1, y1;
   step_position<S, SRK>(P, rstate + (long)p * S, U, sn, cn, x1, y1);
-  const int cd = cell_of_position<S>(P, x1, y1, ncf);
-  bucket[nstart[cd] + new_count_of(cnt9, cd, ncf) + farslot[p]] = p;
+  const int cd = cell_base(m >> 4, cpr) + cell_of_position<S>(P, x1, y1, ncf);
+  bucket[nstart[cd] + new_count_of(cnt9, cd, ncf, cpr) + farslot[p]] = p;
 }
 
 // the step of every particle, written to its new slot with its id, natural frequency and
 // snapshot
 template <int S, bool SRK>
 __global__ __launch_bounds__(256) void fr_scatter(
-    DevParams P, int n, int ncf, double H, const int *__restrict__ start, const int *__restrict__ nstart,
+    DevParams P, Layout L, int n, int ncf, double H, const int *__restrict__ start, const int *__restrict__ nstart,
     const int *__restrict__ mv, const int *__restrict__ cnt9, const int *__restrict__ bucket,
     const double *__restrict__ rstate, const int *__restrict__ rid, const double *__restrict__ rnat,
     const double2 *__restrict__ ssum, const double2 *__restrict__ tsum, const int *__restrict__ scnt, StepArgs U,
@@ -693,19 +721,33 @@ __global__ __launch_bounds__(256) void fr_scatter(
     float4 *__restrict__ snap, float *__restrict__ omega_f) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const int cpr = L.cpr;
+  const int m = mv[p];
+  const int cs = m >> 4, d = m & 15;
+  const int r = cs / cpr, base = r * cpr;
+  Coef K;
+  if (U.coef) {
+    K = U.coef[r];
+  } else {
+    K.xi = P.xi;
+    K.sigma = P.sigma;
+    K.int_phi = U.int_phi;
+    K.sq2d = U.sq2d;
+    K.seed = U.seed;
+  }
+  const uint64_t step = U.step_dev ? (uint64_t)*U.step_dev : U.step;
   const int id = rid[p];
   const double nat_i = rnat ? rnat[p] : 0.0;
   double out[S];
   double C;
-  step_particle<S, SRK>(P, rstate + (long)p * S, id, nat_i, ssum[p], SRK ? tsum[p] : make_double2(0.0, 0.0),
-                        scnt[p], U, out, C);
+  // noise and injected draws are indexed by the particle's index inside its replica
+  step_particle<S, SRK>(P, K, rstate + (long)p * S, id - L.rep_first[r], nat_i, ssum[p],
+                        SRK ? tsum[p] : make_double2(0.0, 0.0), scnt[p], U, step, out, C);
   if (SRK && compl_out) compl_out[id] = C;
-  const int m = mv[p];
-  const int cs = m >> 4, d = m & 15;
   int cd, pos;
   if (d == kFar) {
-    cd = cell_of_position<S>(P, out[0], out[1], ncf);
-    const int b = nstart[cd] + new_count_of(cnt9, cd, ncf);
+    cd = base + cell_of_position<S>(P, out[0], out[1], ncf);
+    const int b = nstart[cd] + new_count_of(cnt9, cd, ncf, cpr);
     const int e = nstart[cd + 1];
     int rank = 0;
     for (int q = b; q < e; ++q) rank += (bucket[q] < p) ? 1 : 0;
@@ -713,21 +755,68 @@ __global__ __launch_bounds__(256) void fr_scatter(
   } else {
     int rank = 0;
     for (int q = start[cs]; q < p; ++q) rank += ((mv[q] & 15) == d) ? 1 : 0;
+    const int ls = cs - base;
     const int dcx = d % 3 - 1, dcy = d / 3 - 1;
-    cd = ((cs / ncf + dcy + ncf) % ncf) * ncf + (cs % ncf + dcx + ncf) % ncf;
-    int base = 0;
+    cd = base + ((ls / ncf + dcy + ncf) % ncf) * ncf + (ls % ncf + dcx + ncf) % ncf;
+    int b0 = 0;
     const int od = move_order(d);
     for (int e = 0; e < 9; ++e)
-      if (move_order(e) < od) base += cnt9[(long)move_source(cd, e, ncf) * 9 + e];
-    pos = nstart[cd] + base + rank;
+      if (move_order(e) < od) b0 += cnt9[(long)move_source(cd, e, ncf, cpr) * 9 + e];
+    pos = nstart[cd] + b0 + rank;
   }
   double *dst = nstate + (long)pos * S;
 #pragma unroll
   for (int q = 0; q < S; ++q) dst[q] = out[q];
   nrid[pos] = id;
   if (rnat) nnat[pos] = nat_i;
-  snap[pos] = snap_entry<S>(P, out, cd % ncf, cd / ncf, H);
+  const int ld = cd - base;
+  snap[pos] = snap_entry<S>(P, out, ld % ncf, ld / ncf, H);
   if (omega_f) omega_f[pos] = (S == 4) ? (float)out[3] : 0.f;
+}
+
+// advance the device step counter of graph-replayed runs
+__global__ void fb_step_inc(unsigned long long *step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1ull;
+}
+
+// per replica: the phase order parameter Z = (1/N) sum e^{i phi} of the resident snapshot,
+// summed in FP64 in a fixed tree; when the step counter has reached `from`, |Z| and |Z|^2 are
+// added to the replica's running sums (one block per replica)
+__global__ __launch_bounds__(256) void fb_order_stats(Layout L, const float4 *__restrict__ snap,
+                                                      const unsigned long long *__restrict__ step,
+                                                      unsigned long long from, double *__restrict__ zlast,
+                                                      double *__restrict__ racc) {
+  __shared__ double sc[256], ss[256];
+  const int r = blockIdx.x;
+  const int b = L.rep_first[r], e = L.rep_first[r + 1];
+  double c = 0.0, s = 0.0;
+  for (int p = b + threadIdx.x; p < e; p += 256) {
+    const float4 v = snap[p];
+    c += (double)v.z;
+    s += (double)v.w;
+  }
+  sc[threadIdx.x] = c;
+  ss[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      sc[threadIdx.x] += sc[threadIdx.x + off];
+      ss[threadIdx.x] += ss[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double inv = 1.0 / (double)(e - b);
+    const double zr = sc[0] * inv, zi = ss[0] * inv;
+    zlast[2 * r] = zr;
+    zlast[2 * r + 1] = zi;
+    if (*step >= from) {
+      const double R2 = zr * zr + zi * zi;
+      racc[3 * r] += sqrt(R2);
+      racc[3 * r + 1] += R2;
+      racc[3 * r + 2] += 1.0;
+    }
+  }
 }
 
 // entry into resident mode: gather the state (original order) into cell order

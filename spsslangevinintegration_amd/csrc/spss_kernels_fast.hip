@@ -41,10 +41,18 @@ thread_local std::string g_fast_err;
 struct FastPath {
   int n = 0, s = 4;
   double rho = 0.0, rho2 = 0.0;
-  int nc = 1, k = 1, ncf = 1, cells = 1, nsb = 1;
+  int nc = 1, k = 1, ncf = 1, cpr = 1, cells = 1, nsb = 1;
   double H = 1.0;
-  int W = 8, chunks = 1, ntiles = 1;
   float r2lo = 0.f, r2hi = 0.f;
+  // replicas (one for a single system) and the tiles of the pair kernel
+  int R = 1;
+  std::vector<int> rep_first_h, W_h;
+  int *rep_first = nullptr, *rep_chunk = nullptr;
+  int4 *tiles = nullptr;
+  int ntiles = 0, tiles_cap = 0;
+  Coef *coef = nullptr;                    // per-replica coefficients (batches)
+  unsigned long long *step_dev = nullptr;  // device step counter (batches)
+  double *zlast = nullptr, *racc = nullptr;
   // rebuilt cell list (RHS, pair export, entry into resident mode)
   int *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr, *sorted = nullptr;
   int *blocksum = nullptr;
@@ -89,7 +97,49 @@ static hipError_t falloc(T **p, size_t count) {
   return hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
 }
 
-int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv) {
+// default tile width of a replica: about one particle per thread at its mean density, with
+// the staged (2k+1) x (W+2k) cells and their row padding within 85% of the LDS stage
+static int default_tile_width(double per_cell, int k, int ncf) {
+  const int nrows = 2 * k + 1;
+  int W = (int)std::lround((double)kFB / std::max(per_cell, 1e-9));
+  const int wstage = (int)((0.85 * kStageMax - nrows * kPad) / std::max(per_cell * nrows, 1e-9)) - 2 * k;
+  W = std::min(W, std::max(wstage, 1));
+  return std::max(1, std::min(W, std::min(kMaxCols - 2 * k, ncf)));
+}
+
+// tile table for per-replica tile widths, and the guard band of the widest tile: the FP32
+// distance error bound (coordinates <= (W+2k+1) cells from the staged origin, rounded once;
+// offsets < one cell) times a safety factor of 8
+static int set_tiles(FastPath *f, const std::vector<int> &W) {
+  std::vector<int4> t;
+  int wmax = 1;
+  for (int r = 0; r < f->R; ++r) {
+    const int w = std::max(1, std::min(W[r], std::min(kMaxCols - 2 * f->k, f->ncf)));
+    wmax = std::max(wmax, w);
+    for (int row = 0; row < f->ncf; ++row)
+      for (int ca = 0; ca < f->ncf; ca += w) t.push_back(make_int4(r * f->cpr, row, ca, std::min(ca + w, f->ncf)));
+  }
+  if ((int)t.size() > f->tiles_cap) {
+    hipFree(f->tiles);
+    f->tiles = nullptr;
+    FCHK(hipMalloc(&f->tiles, sizeof(int4) * t.size()));
+    f->tiles_cap = (int)t.size();
+  }
+  FCHK(hipMemcpy(f->tiles, t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice));
+  f->ntiles = (int)t.size();
+  f->W_h = W;
+  const double span = (double)(wmax + 2 * f->k + 1) * f->H;
+  const double eps = std::ldexp(1.0, -24);
+  const double dxerr = 2.0 * span * eps + 2.0 * f->H * eps;
+  double g = 8.0 * (2.0 * dxerr / f->rho + 4.0 * eps);
+  g = std::max(g, 64.0 * eps);
+  f->r2lo = (float)(f->rho2 * (1.0 - g));
+  f->r2hi = (float)(f->rho2 * (1.0 + g));
+  return SPSS_OK;
+}
+
+int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv, int R,
+                         const int *rep_n, const Coef *coef) {
   *out = nullptr;
   if (subdiv < 1 || subdiv > 3) {
     g_fast_err = "cell_subdivision must be 1, 2 or 3";
@@ -104,8 +154,9 @@ int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, 
                  "more than once (use the FP64 precision there)";
     return SPSS_ERR_UNSUPPORTED;
   }
-  if ((long)nc * subdiv * nc * subdiv >= (1L << 27)) {
-    g_fast_err = "the FP32C cell list supports at most 2^27 cells (int(1/rho) * cell_subdivision < 11585)";
+  const long cpr = (long)nc * subdiv * nc * subdiv;
+  if (R < 1 || cpr * R >= (1L << 27)) {
+    g_fast_err = "the FP32C cell list supports at most 2^27 cells over all replicas";
     return SPSS_ERR_UNSUPPORTED;
   }
   FastPath *f = new FastPath();
@@ -117,30 +168,26 @@ int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, 
   f->k = subdiv;
   f->ncf = nc * subdiv;
   f->H = 1.0 / (double)f->ncf;
-  f->cells = f->ncf * f->ncf;
+  f->cpr = (int)cpr;
+  f->R = R;
+  f->cells = (int)(cpr * R);
   f->nsb = (f->cells + 256 * kScanItems - 1) / (256 * kScanItems);
-  // tile width: about one particle per thread at the mean density, with the staged
-  // (2k+1) x (W+2k) cells and their row padding within 80% of the LDS stage
-  const double per_cell = (double)P.n / ((double)f->ncf * f->ncf);
-  const int nrows = 2 * f->k + 1;
-  int W = (int)std::lround((double)kFB / std::max(per_cell, 1e-9));
-  const int wstage =
-      (int)((0.8 * kStageMax - nrows * kPad) / std::max(per_cell * nrows, 1e-9)) - 2 * f->k;
-  W = std::min(W, std::max(wstage, 1));
-  W = std::max(1, std::min(W, std::min(kMaxCols - 2 * f->k, f->ncf)));
-  f->W = W;
-  f->chunks = (f->ncf + W - 1) / W;
-  f->ntiles = f->chunks * f->ncf;
-  // guard band: FP32 distance error bound (coordinates <= (W+2k) cells from the staged
-  // origin, rounded once; offsets < one cell) times a safety factor of 8
-  const double span = (double)(W + 2 * f->k + 1) * f->H;
-  const double eps = std::ldexp(1.0, -24);
-  const double dxerr = 2.0 * span * eps + 2.0 * f->H * eps;
-  double g = 8.0 * (2.0 * dxerr / rho + 4.0 * eps);
-  g = std::max(g, 64.0 * eps);
-  f->r2lo = (float)(f->rho2 * (1.0 - g));
-  f->r2hi = (float)(f->rho2 * (1.0 + g));
   f->st = st;
+  // replica offsets and the coarse position -> replica table
+  f->rep_first_h.assign(R + 1, 0);
+  for (int r = 0; r < R; ++r) f->rep_first_h[r + 1] = f->rep_first_h[r] + rep_n[r];
+  if (f->rep_first_h[R] != P.n) {
+    delete f;
+    g_fast_err = "replica sizes do not add up to n";
+    return SPSS_ERR_INVALID;
+  }
+  const int nchunk = (P.n >> kChunkShift) + 1;
+  std::vector<int> chunk(nchunk, 0);
+  for (int q = 0, r = 0; q < nchunk; ++q) {
+    const int p = q << kChunkShift;
+    while (r + 1 < R && f->rep_first_h[r + 1] <= p) ++r;
+    chunk[q] = r;
+  }
   const int cells = f->cells;
   const size_t n = (size_t)P.n;
   hipError_t e = hipSuccess;
@@ -171,25 +218,66 @@ int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, 
   A(falloc(&f->cnt9, (size_t)cells * 9));
   A(falloc(&f->farcnt, cells));
   A(falloc(&f->newcnt, cells));
+  A(falloc(&f->rep_first, R + 1));
+  A(falloc(&f->rep_chunk, nchunk));
+  if (coef) {
+    A(falloc(&f->coef, R));
+    A(falloc(&f->step_dev, 1));
+    A(falloc(&f->zlast, 2 * R));
+    A(falloc(&f->racc, 3 * R));
+  }
+  if (e == hipSuccess) e = hipMemcpy(f->rep_first, f->rep_first_h.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(f->rep_chunk, chunk.data(), sizeof(int) * nchunk, hipMemcpyHostToDevice);
+  if (e == hipSuccess && coef) e = hipMemcpy(f->coef, coef, sizeof(Coef) * R, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     fast_destroy(f);
-    return ffail(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return ffail(std::string("hipMalloc/hipMemcpy: ") + hipGetErrorString(e));
+  }
+  std::vector<int> W(R);
+  for (int r = 0; r < R; ++r) W[r] = default_tile_width((double)rep_n[r] / (double)cpr, f->k, f->ncf);
+  int rc = set_tiles(f, W);
+  if (rc != SPSS_OK) {
+    fast_destroy(f);
+    return rc;
   }
   launch_fill_int(st, cells, f->count, 0);
   launch_fill_int(st, cells, f->farcnt, 0);
   launch_fill_int(st, cells, f->newcnt, 0);
+  if (coef) {
+    launch_fill_double(st, 3L * R, f->racc, 0.0);
+    launch_fill_double(st, 2L * R, f->zlast, 0.0);
+  }
   *out = f;
   return SPSS_OK;
 }
 
+int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv) {
+  const int n = P.n;
+  return fast_create_replicas(out, st, P, rho, nc, subdiv, 1, &n, nullptr);
+}
+
 void fast_destroy(FastPath *f) {
   if (!f) return;
-  void *b[] = {f->key,     f->slot,      f->count,     f->start,     f->tmp,       f->sorted,    f->blocksum,
-               f->snap,    f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],    f->rid[1],    f->rnat[0],
-               f->rnat[1], f->rstart[0], f->rstart[1], f->rsnap,     f->romega,    f->newcnt,    f->mv,
-               f->cnt9,    f->farcnt,    f->ssum,      f->tsum,      f->scnt};
+  void *b[] = {f->key,       f->slot,      f->count,     f->start,   f->tmp,       f->sorted,    f->blocksum,
+               f->snap,      f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],  f->rid[1],    f->rnat[0],
+               f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mv,
+               f->cnt9,      f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
+               f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc};
   for (void *p : b) hipFree(p);
   delete f;
+}
+
+int fast_tile_widths(FastPath *f, std::vector<int> *W) {
+  *W = f->W_h;
+  return SPSS_OK;
+}
+
+int fast_set_tile_widths(FastPath *f, const std::vector<int> &W) {
+  if ((int)W.size() != f->R) {
+    g_fast_err = "one tile width per replica";
+    return SPSS_ERR_INVALID;
+  }
+  return set_tiles(f, W);
 }
 
 void fast_set_stream(FastPath *f, hipStream_t st) {
@@ -210,13 +298,21 @@ static FastArgs args_of(const FastPath *f) {
   a.n = f->n;
   a.ncf = f->ncf;
   a.k = f->k;
-  a.W = f->W;
-  a.chunks = f->chunks;
   a.ntiles = f->ntiles;
+  a.cpr = f->cpr;
+  a.tiles = f->tiles;
   a.H = f->H;
   a.r2lo = f->r2lo;
   a.r2hi = f->r2hi;
   return a;
+}
+
+static Layout layout_of(const FastPath *f) {
+  Layout L;
+  L.rep_first = f->rep_first;
+  L.rep_chunk = f->rep_chunk;
+  L.cpr = f->cpr;
+  return L;
 }
 
 // sorted order, cell starts and snapshot of a stage state (original order)
@@ -224,17 +320,18 @@ static void build(FastPath *f, hipStream_t st, const DevParams &P, const double 
                   float *omega) {
   const int n = f->n;
   const unsigned nb = nblocks(n);
+  const Layout L = layout_of(f);
   if (P.s == 4)
-    fk_keys<4><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->key, f->slot, f->count);
+    fk_keys<4><<<nb, 256, 0, st>>>(P, L, stage, f->ncf, f->key, f->slot, f->count);
   else
-    fk_keys<3><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->key, f->slot, f->count);
+    fk_keys<3><<<nb, 256, 0, st>>>(P, L, stage, f->ncf, f->key, f->slot, f->count);
   fk_scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum);
   fk_scan_add<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum, f->nsb);
   fk_scatter<<<nb, 256, 0, st>>>(n, f->key, f->slot, start, f->tmp);
   if (P.s == 4)
-    fk_rank<4><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->H, f->key, start, f->tmp, f->sorted, snap, omega);
+    fk_rank<4><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->cpr, f->H, f->key, start, f->tmp, f->sorted, snap, omega);
   else
-    fk_rank<3><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->H, f->key, start, f->tmp, f->sorted, snap, nullptr);
+    fk_rank<3><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->cpr, f->H, f->key, start, f->tmp, f->sorted, snap, nullptr);
 }
 
 static PairBufs rebuilt_bufs(const FastPath *f, const double *stage) {
@@ -311,40 +408,28 @@ static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const
   const int n = f->n;
   const unsigned nb = nblocks(n);
   const double *rnat = f->resident_nat ? f->rnat[c] : nullptr;
-  fr_move<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->rstate[c], U, f->mv, f->farcnt, f->slot);
-  fr_cell_counts<<<nblocks(f->cells), 256, 0, st>>>(f->cells, f->ncf, f->rstart[c], f->mv, f->cnt9, f->newcnt);
+  const Layout L = layout_of(f);
+  fr_move<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->rstate[c], U, f->mv, f->farcnt, f->slot);
+  fr_cell_counts<<<nblocks(f->cells), 256, 0, st>>>(f->cells, f->ncf, f->cpr, f->rstart[c], f->mv, f->cnt9,
+                                                    f->newcnt);
   fk_scan_local<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum);
   fk_scan_add<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum, f->nsb);
-  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->mv, f->rstate[c], U, f->rstart[o], f->cnt9, f->slot,
-                                           f->tmp);
-  fr_scatter<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->H, f->rstart[c], f->rstart[o], f->mv, f->cnt9, f->tmp,
+  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, f->mv, f->rstate[c], U, f->rstart[o], f->cnt9,
+                                           f->slot, f->tmp);
+  fr_scatter<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], f->mv, f->cnt9, f->tmp,
                                         f->rstate[c], f->rid[c], rnat, f->ssum, f->tsum, f->scnt, U,
                                         SRK ? compl_out : nullptr, f->rstate[o], f->rid[o], f->rnat[o], f->rsnap,
                                         SRK ? f->romega : nullptr);
   f->cur = o;
 }
 
-int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat,
-              double *compl_out, double dt, double D_phi, double friction, int noise_mode, const double *noise,
-              uint64_t seed, uint64_t step) {
-  const bool do_srk = srk && P.s == 4;
+static int step_impl(FastPath *f, hipStream_t st, const DevParams &P, bool do_srk, const double *state,
+                     const double *nat, double *compl_out, const StepArgs &U) {
   // the SRK step needs the FP32 omega of every particle, which an Euler-entered copy lacks
   if (f->resident && (nat != nullptr) != f->resident_nat) f->resident = false;
   if (f->resident && do_srk && !f->romega_valid) f->resident = false;
   if (!f->resident) resident_enter(f, st, P, state, nat, do_srk);
   f->romega_valid = do_srk;
-  StepArgs U{};
-  U.dt = dt;
-  U.sqrt_dt = std::sqrt(dt);
-  U.int_phi = std::sqrt(2.0 * D_phi);
-  U.c15 = 0.5 * std::pow(dt, 1.5);
-  U.is3 = 1.0 / std::sqrt(3.0);
-  U.sq2d = std::sqrt(2.0 * D_phi);
-  U.friction = friction;
-  U.noise_mode = noise_mode;
-  U.noise = noise;
-  U.seed = seed;
-  U.step = step;
   const int c = f->cur;
   PairBufs B{};
   B.rstate = f->rstate[c];
@@ -372,6 +457,76 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
   FCHK(hipGetLastError());
   return SPSS_OK;
 }
+
+static StepArgs step_args(double dt, double D_phi, double friction, int noise_mode, const double *noise,
+                          uint64_t seed, uint64_t step) {
+  StepArgs U{};
+  U.dt = dt;
+  U.sqrt_dt = std::sqrt(dt);
+  U.int_phi = std::sqrt(2.0 * D_phi);
+  U.c15 = 0.5 * std::pow(dt, 1.5);
+  U.is3 = 1.0 / std::sqrt(3.0);
+  U.sq2d = std::sqrt(2.0 * D_phi);
+  U.friction = friction;
+  U.noise_mode = noise_mode;
+  U.noise = noise;
+  U.seed = seed;
+  U.step = step;
+  return U;
+}
+
+int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat,
+              double *compl_out, double dt, double D_phi, double friction, int noise_mode, const double *noise,
+              uint64_t seed, uint64_t step) {
+  const StepArgs U = step_args(dt, D_phi, friction, noise_mode, noise, seed, step);
+  return step_impl(f, st, P, srk && P.s == 4, state, nat, compl_out, U);
+}
+
+// ---------------------------------------------------------------------------
+// replica batches: per-replica coefficients, Philox step from the device counter
+
+int fast_batch_step(FastPath *f, hipStream_t st, const DevParams &P, const double *state, const double *nat,
+                    double dt) {
+  if (!f->coef) {
+    g_fast_err = "not a replica batch";
+    return SPSS_ERR_INVALID;
+  }
+  StepArgs U = step_args(dt, 0.0, 0.0, 3, nullptr, 0, 0);
+  U.coef = f->coef;
+  U.step_dev = f->step_dev;
+  int rc = step_impl(f, st, P, false, state, nat, nullptr, U);
+  if (rc != SPSS_OK) return rc;
+  fb_step_inc<<<1, 64, 0, st>>>(f->step_dev);
+  FCHK(hipGetLastError());
+  return SPSS_OK;
+}
+
+int fast_batch_observe(FastPath *f, hipStream_t st, uint64_t from) {
+  fb_order_stats<<<f->R, 256, 0, st>>>(layout_of(f), f->rsnap, f->step_dev, (unsigned long long)from, f->zlast,
+                                       f->racc);
+  FCHK(hipGetLastError());
+  return SPSS_OK;
+}
+
+int fast_batch_reset(FastPath *f, hipStream_t st, uint64_t step) {
+  launch_fill_double(st, 3L * f->R, f->racc, 0.0);
+  const unsigned long long v = step;
+  FCHK(hipMemcpyAsync(f->step_dev, &v, sizeof(v), hipMemcpyHostToDevice, st));
+  FCHK(hipStreamSynchronize(st));
+  return SPSS_OK;
+}
+
+int fast_batch_stats(FastPath *f, hipStream_t st, double *racc, double *zlast, uint64_t *step) {
+  unsigned long long v = 0;
+  if (racc) FCHK(hipMemcpyAsync(racc, f->racc, sizeof(double) * 3 * f->R, hipMemcpyDeviceToHost, st));
+  if (zlast) FCHK(hipMemcpyAsync(zlast, f->zlast, sizeof(double) * 2 * f->R, hipMemcpyDeviceToHost, st));
+  FCHK(hipMemcpyAsync(&v, f->step_dev, sizeof(v), hipMemcpyDeviceToHost, st));
+  FCHK(hipStreamSynchronize(st));
+  if (step) *step = v;
+  return SPSS_OK;
+}
+
+int fast_replica_count(const FastPath *f) { return f->R; }
 
 int fast_pairs_count(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, int *count) {
   build(f, st, P, stage, f->start, f->snap, nullptr);

@@ -4,9 +4,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "spss_device.hpp"
 
 namespace spss {
+
+// records the message spss_last_error() returns and passes the code through
+int set_error(int code, const std::string &msg);
 
 // HIP-event timer around one kernel family, enabled on request (bench instrumentation):
 // records an event pair per launch on the launching stream, sums the durations on read.

@@ -1,0 +1,124 @@
+"""Batches of independent replicas (spss_batch_*, ReplicaBatch) against single systems.
+
+Replica r of a batch must step exactly like a single FP32C system with the same parameters,
+natural frequencies and Philox seed run through spss_run: bit-identical states, whatever the
+tile widths (default, set by hand, or picked by the autotuner) and whether the steps run
+eagerly or as graph replays.  The per-replica order-parameter readings equal the order
+parameter of the replica's state (FP32 phasors summed in FP64: within 1e-6) and their time
+averages follow the readings of a single system.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def spss():
+    import spsslangevinintegration_amd as m
+    return m
+
+
+def _replicas(model, R, seed):
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(2048, 9000, R)
+    m = rng.choice([0.5, 1.0], R)
+    sigma = rng.uniform(0, 4, R)
+    D = rng.uniform(0, 0.2, R)
+    if model == 4:
+        xi, sig, Dp = 1.0 / m, sigma / m, D / m ** 2
+    else:
+        xi, sig, Dp = np.zeros(R), sigma, D
+    seeds = 0x5EED0004 + np.arange(R)
+    states, nats = [], []
+    for r in range(R):
+        st = np.zeros((ns[r], model))
+        st[:, 0], st[:, 1] = rng.random(ns[r]), rng.random(ns[r])
+        st[:, 2] = rng.random(ns[r]) * 2 * np.pi
+        states.append(st.reshape(-1))
+        w = rng.normal(0, 0.2, ns[r])
+        nats.append(w / m[r] if model == 4 else w)
+    return ns, xi, sig, Dp, seeds, states, nats
+
+
+def _singles(spss, model, ns, xi, sig, Dp, seeds, states, nats, K, dt, v0, rho):
+    out = []
+    for r in range(len(ns)):
+        g = spss.ParticleSystem(model=model, n=int(ns[r]), v0=v0, xi=xi[r], sigma=sig[r], rho=rho, D_phi=Dp[r],
+                                natfreq_local=1, precision=spss.PRECISION_FP32C, seed=int(seeds[r]))
+        g.set_natural_frequencies(nats[r])
+        g.set_state(states[r])
+        g.run(spss.STEPPER_EULER, spss.METHOD_LINKED_LIST, K, dt, spss.NOISE_PHILOX)
+        out.append(g.get_state())
+        g.close()
+    return out
+
+
+@pytest.mark.parametrize("model,graph", [(4, True), (4, False), (3, True)])
+def test_batch_equals_single_systems(spss, model, graph):
+    R, K, dt, v0, rho = 6, 24, 0.005, 0.01, 0.03
+    print(f"[gpu] batch model={model} graph={graph}", flush=True)
+    ns, xi, sig, Dp, seeds, states, nats = _replicas(model, R, 10 + model)
+    b = spss.ReplicaBatch(model, ns, xi, sig, Dp, seeds, v0=v0, rho=rho)
+    b.set_natural_frequencies(np.concatenate(nats))
+    b.set_state(np.concatenate(states))
+    b.run(K, dt, observe_every=4, observe_from=8, graph=graph)
+    got = b.get_state()
+    want = _singles(spss, model, ns, xi, sig, Dp, seeds, states, nats, K, dt, v0, rho)
+    for r in range(R):
+        assert np.array_equal(b.replica_state(got, r), want[r]), r
+    mean, var, cnt, z, step = b.order_stats()
+    assert step == K
+    # readings after steps 4, 8, ..., 24; averaged from step 8 on (5 readings)
+    assert np.all(cnt == 5)
+    for r in range(R):
+        re, im = pyoracle.order_parameter(want[r], int(ns[r]), model, 2)
+        assert abs(z[r] - complex(re, im)) <= 1e-6
+    assert np.all(mean >= 0) and np.all(mean <= 1) and np.all(var >= 0)
+
+
+def test_batch_time_averages_follow_readings(spss):
+    model, R, dt, v0, rho = 4, 3, 0.005, 0.01, 0.03
+    ns, xi, sig, Dp, seeds, states, nats = _replicas(model, R, 21)
+    b = spss.ReplicaBatch(model, ns, xi, sig, Dp, seeds, v0=v0, rho=rho)
+    b.set_natural_frequencies(np.concatenate(nats))
+    b.set_state(np.concatenate(states))
+    b.run(30, dt, observe_every=3, observe_from=9, graph=True)
+    mean, var, cnt, z, step = b.order_stats()
+    for r in range(R):
+        g = spss.ParticleSystem(model=model, n=int(ns[r]), v0=v0, xi=xi[r], sigma=sig[r], rho=rho, D_phi=Dp[r],
+                                natfreq_local=1, precision=spss.PRECISION_FP32C, seed=int(seeds[r]))
+        g.set_natural_frequencies(nats[r])
+        g.set_state(states[r])
+        rs = []
+        for k in range(10):
+            g.run(spss.STEPPER_EULER, spss.METHOD_LINKED_LIST, 3, dt, spss.NOISE_PHILOX)
+            if 3 * (k + 1) >= 9:
+                rs.append(np.hypot(*g.order_parameter(2)))
+        rs = np.array(rs)
+        assert cnt[r] == len(rs)
+        assert abs(mean[r] - rs.mean()) <= 1e-6
+        assert abs(var[r] - rs.var()) <= 1e-6
+
+
+def test_batch_tile_widths_and_autotune_do_not_change_results(spss):
+    model, R, K, dt, v0, rho = 4, 5, 12, 0.005, 0.01, 0.03
+    ns, xi, sig, Dp, seeds, states, nats = _replicas(model, R, 31)
+    outs = []
+    for mode in ("default", "manual", "tuned"):
+        b = spss.ReplicaBatch(model, ns, xi, sig, Dp, seeds, v0=v0, rho=rho)
+        b.set_natural_frequencies(np.concatenate(nats))
+        b.set_state(np.concatenate(states))
+        if mode == "manual":
+            b.set_tile_widths(np.array([1, 3, 7, 11, 33]))
+        elif mode == "tuned":
+            table, chosen = b.autotune(dt, probe_steps=6)
+            print(f"  autotune ms/step {np.round(table, 4)} chosen {chosen}", flush=True)
+            assert np.all(table > 0) and 0 <= chosen < 6
+            assert b.order_stats()[4] == 0  # step counter restored
+        b.run(K, dt, observe_every=0, graph=True)
+        outs.append(b.get_state())
+        b.close()
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
