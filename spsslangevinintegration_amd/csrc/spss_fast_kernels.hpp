@@ -36,6 +36,8 @@ constexpr int kScanItems = 8;   // counts per thread in the two-level scans
 struct FastArgs {
   int n, ncf, k, ntiles, cpr;  // cpr: cells per replica (ncf * ncf)
   const int4 *tiles;           // per tile: first cell of its replica, row, first column, end column
+  int *ovf_list;               // tiles whose staged cells overflow the LDS stage (fk_pairs_ovf runs them)
+  int *ovf_count;              // their number, zeroed by fk_scan_add
   double H;
   float r2lo, r2hi;
 };
@@ -163,10 +165,12 @@ __global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ cou
 }
 
 // adds the scanned block totals, writes start[m], zeroes count[] and count2[] for the next use
+// (and the overflow-tile counter of the pair kernel)
 __global__ __launch_bounds__(256) void fk_scan_add(int *__restrict__ count, int *__restrict__ count2, int m,
                                                    int *__restrict__ start, const int *__restrict__ blocksum,
-                                                   int nblocks) {
+                                                   int nblocks, int *__restrict__ ovf_count) {
   __shared__ int off;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ovf_count) *ovf_count = 0;
   if (threadIdx.x == 0) {
     int s = 0;
     for (int b = 0; b < (int)blockIdx.x; ++b) s += blocksum[b];
@@ -249,8 +253,139 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
   return v;
 }
 
+// per particle after its pair sums: the RHS (MODE 0), the neighbour count (MODE 1) or the
+// phasor sums of a stochastic step (MODE 3)
+template <int S, bool SRK, int MODE>
+__device__ __forceinline__ void finish_particle(const DevParams &P, const PairBufs &B, int i, int gp,
+                                                const double *pi, double sx, double sy, double tx, double ty,
+                                                int nnb) {
+  const double card = (double)nnb;
+  // e^{-i theta}, theta = phi_i + alpha, from the FP64 phase
+  double sn = 0.0, cn = 0.0, A64 = 0.0, C64 = 0.0;
+  if (MODE == 0) {
+    sincos(pi[2], &sn, &cn);
+    const double st = sn * P.cos_a + cn * P.sin_a;
+    const double ct = cn * P.cos_a - sn * P.sin_a;
+    A64 = ct * sy - st * sx;
+    if (SRK) {
+      const double wi = pi[3];
+      C64 = ct * (tx - wi * sx) + st * (ty - wi * sy);
+    }
+  }
+  if (MODE == 1) {
+    B.pcount[i] = nnb;
+  } else if (MODE == 0) {
+    const long ii = (long)i * S;
+    const double nat_i = B.nat ? B.nat[i] : 0.0;
+    B.k_next[ii] = P.v0 * cn;
+    B.k_next[ii + 1] = P.v0 * sn;
+    if (S == 4) {
+      const double omega_i = pi[3];
+      B.k_next[ii + 2] = omega_i;
+      if (P.natfreq_local) {
+        if (nnb) B.k_next[ii + 3] = -P.xi * omega_i + nat_i + P.sigma * A64 / card;
+        else B.k_next[ii + 3] = -P.xi * omega_i + nat_i;
+      } else if (nnb) {
+        B.k_next[ii + 3] = -P.xi * omega_i + P.sigma * A64 / card;
+      }
+      if (SRK && B.compl_out) B.compl_out[i] = nnb ? C64 * (P.sigma / card) : C64;
+    } else {
+      B.k_next[ii + 2] = nnb ? nat_i + P.sigma * A64 / card : nat_i;
+    }
+  } else if (MODE == 3) {
+    B.ssum[gp] = make_double2(sx, sy);
+    if (SRK) B.tsum[gp] = make_double2(tx, ty);
+    B.scnt[gp] = nnb;
+  }
+}
+
+template <int S, bool SRK, int MODE>
+struct PairLds {
+  float4 sdat[kStageMax];                          // staged x, y in its cell, cos phi, sin phi
+  float somega[SRK ? kStageMax : 1];
+  int sg[(MODE == 1 || MODE == 2) ? kStageMax : 1];  // global sorted position (pair export)
+  int cglob[kMaxRows * kMaxCols];                  // global sorted start of staged cell (r, q) at r*ncols+q
+  int ccnt[kMaxRows * kMaxCols];                   // its particle count
+  int cpos[kMaxRows * kMaxCols + 1];               // its LDS start; [cells] = staged total with padding
+};
+
+// FP64 decision of a guard-band candidate (global sorted position g) for particle pi
+template <int S, int MODE>
+__device__ __forceinline__ bool band_in(const DevParams &P, const PairBufs &B, const double *pi, int g) {
+  const double *pj = (MODE >= 3) ? B.rstate + (long)g * S : B.stage + (long)B.sorted[g] * S;
+  double x_i64, y_i64;
+  apply_pbc(P, pi[0], pi[1], x_i64, y_i64);
+  return fp64_in_range(P, pj, x_i64, y_i64);
+}
+
+// The candidates staged at LDS positions [lo, hi) (plus far-away padding up to hi + 3) against
+// one particle, four at a time: FP32 distance tests with the FP64 decision in the guard band,
+// phasor sums in 4 FP32 partial slots (slot = (t - lo) mod 4), counts and list entries.
+// gof(t) gives the global sorted position of LDS entry t for band decisions.
+template <int S, bool SRK, int MODE, class GOF>
+__device__ __forceinline__ void scan_range(const DevParams &P, const PairBufs &B, const float4 *sdat,
+                                           const float *somega, int lo, int hi, float xi_t, float syi,
+                                           float r2lo, float r2hi, const double *pi, int self_t, GOF gof,
+                                           float *ax, float *ay, float *an, float *bx, float *by, int &nnb,
+                                           int &w, int wbase) {
+  constexpr bool SUMS = MODE == 0 || MODE >= 3;
+  for (int t = lo; t < hi; t += 4) {
+    float4 o[4];
+    bool in[4], band[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      o[j] = sdat[t + j];
+      const float dx = o[j].x - xi_t;
+      const float dy = o[j].y - syi;
+      const float d2 = __builtin_fmaf(dy, dy, dx * dx);
+      in[j] = d2 <= r2hi;
+      band[j] = in[j] && d2 >= r2lo;
+    }
+    if (band[0] || band[1] || band[2] || band[3]) {  // rare: FP64 decision inside the stencil
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (band[j]) in[j] = (t + j < hi) && band_in<S, MODE>(P, B, pi, gof(t + j));
+    }
+    if (SUMS) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float m = in[j] ? 1.f : 0.f;
+        ax[j] = __builtin_fmaf(m, o[j].z, ax[j]);
+        ay[j] = __builtin_fmaf(m, o[j].w, ay[j]);
+        an[j] += m;
+        if (SRK) {
+          const float mo = m * somega[t + j];
+          bx[j] = __builtin_fmaf(mo, o[j].z, bx[j]);
+          by[j] = __builtin_fmaf(mo, o[j].w, by[j]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (in[j] && t + j < hi && t + j != self_t) {
+          ++nnb;
+          if (MODE == 2) B.plist[wbase + w++] = B.sorted[gof(t + j)];
+        }
+      }
+    }
+  }
+}
+
+template <bool SRK>
+__device__ __forceinline__ void flush_row(const float *ax, const float *ay, const float *an, const float *bx,
+                                          const float *by, double &sx, double &sy, double &tx, double &ty,
+                                          int &nnb) {
+  sx += ((double)ax[0] + (double)ax[1]) + ((double)ax[2] + (double)ax[3]);
+  sy += ((double)ay[0] + (double)ay[1]) + ((double)ay[2] + (double)ay[3]);
+  nnb += (int)(an[0] + an[1] + an[2] + an[3]);
+  if (SRK) {
+    tx += ((double)bx[0] + (double)bx[1]) + ((double)bx[2] + (double)bx[3]);
+    ty += ((double)by[0] + (double)by[1]) + ((double)by[2] + (double)by[3]);
+  }
+}
+
 // MODE 0: RHS; 1: count neighbours; 2: write neighbour lists; 3: phasor sums of the resident
-// state for fk_update (the stochastic step)
+// state for the stochastic step
 //
 // One workgroup per tile of W cells of one row.  The (2k+1) x (W+2k) cells around the tile
 // are staged into LDS (x re-based to the staged column 0 in FP64, rounded once; y as the
@@ -258,7 +393,8 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 // lane owns one particle of the tile and walks the (2k+1) contiguous LDS ranges of its
 // stencil four candidates at a time without per-candidate bounds: reads past the end of a
 // range land in farther cells of the same staged row (out of range, see below) or in the
-// padding, so the loop body has no branch on the common path.
+// padding, so the loop body has no branch on the common path.  A tile whose cells do not fit
+// the stage is appended to an overflow list for fk_pairs_ovf.
 //
 // Range test: an FP32 distance below r2lo is in range, above r2hi out of range; the
 // guard band between them (rare) is decided by the reference's FP64 ClassC formula, and
@@ -272,23 +408,16 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 // with the phasor components rounded once from FP64 to FP32, accumulated in four FP32
 // partial sums per stencil row and combined in FP64.  The particle itself is among its
 // candidates exactly once (distinct stencil columns since int(1/rho) >= 3) and is removed
-// afterwards.
+// afterwards.  A particle's candidates, their order and grouping depend on its own stencil
+// cells only (not on the tile), so every tiling and the overflow pass give identical sums.
 template <int S, bool SRK, int MODE>
-__global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBufs B) {
+__device__ __forceinline__ void pairs_tile(const DevParams &P, const FastArgs &A, const PairBufs &B, int tile,
+                                           PairLds<S, SRK, MODE> &L) {
   constexpr bool RES = MODE >= 3;
   constexpr bool SUMS = MODE == 0 || MODE >= 3;
   constexpr bool LIST = MODE == 1 || MODE == 2;
-  __shared__ float4 sdat[kStageMax];             // staged x, y in its cell, cos phi, sin phi
-  __shared__ float somega[SRK ? kStageMax : 1];
-  __shared__ int sg[LIST ? kStageMax : 1];       // global sorted position (pair export)
-  __shared__ int cglob[kMaxRows * kMaxCols];     // global sorted start of staged cell (r, q) at r*ncols+q
-  __shared__ int ccnt[kMaxRows * kMaxCols];      // its particle count
-  __shared__ int cpos[kMaxRows * kMaxCols + 1];  // its LDS start; [ncells] = staged total with padding
-
-  // XCD-aware tile order: consecutive tiles (neighbouring rows) share an XCD's L2
-  const int per = gridDim.x / 8;  // the grid is a multiple of 8
-  const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (tile >= A.ntiles) return;
+  float4 *sdat = L.sdat;
+  int *cglob = L.cglob, *ccnt = L.ccnt, *cpos = L.cpos;
   const int4 td = A.tiles[tile];
   const int cbase = td.x, row = td.y, ca = td.z, cb = td.w;
   const int W = cb - ca;
@@ -315,15 +444,15 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
     const int lane = threadIdx.x;
     constexpr int kPer = (kMaxRows * kMaxCols + 63) / 64;
     int v[kPer];
-    int s = 0;
+    int sum = 0;
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       const int e = lane * kPer + q;
       v[q] = (e < ncells) ? ccnt[e] + ((e % ncols == ncols - 1) ? kPad : 0) : 0;
-      s += v[q];
+      sum += v[q];
     }
-    const int incl = wave_incl_scan(s, lane);
-    int run = incl - s;
+    const int incl = wave_incl_scan(sum, lane);
+    int run = incl - sum;
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       const int e = lane * kPer + q;
@@ -334,29 +463,30 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
   }
   __syncthreads();
   const int total = cpos[ncells];
-  const bool fits = total <= kStageMax;
+  if (total > kStageMax) {  // uniform: every thread reads the same total
+    if (threadIdx.x == 0) A.ovf_list[atomicAdd(A.ovf_count, 1)] = tile;
+    return;
+  }
 
-  if (fits) {
-    // staged entry e belongs to the last cell whose LDS start is <= e
-    for (int e = threadIdx.x; e < total; e += kFB) {
-      int lo = 0, hi = ncells - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (cpos[mid] <= e) lo = mid; else hi = mid - 1;
-      }
-      const int j = e - cpos[lo];
-      if (j < ccnt[lo]) {
-        const int q = lo % ncols;
-        const int g = cglob[lo] + j;
-        const float4 v = B.snap[g];
-        sdat[e] = make_float4((float)((double)q * H + (double)v.x), v.y, v.z, v.w);
-        if (SRK) somega[e] = B.omega_f[g];
-        if (LIST) sg[e] = g;
-      } else {  // row padding: never within range
-        sdat[e] = make_float4(1e30f, 0.f, 0.f, 0.f);
-        if (SRK) somega[e] = 0.f;
-        if (LIST) sg[e] = -1;
-      }
+  // staged entry e belongs to the last cell whose LDS start is <= e
+  for (int e = threadIdx.x; e < total; e += kFB) {
+    int lo = 0, hi = ncells - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (cpos[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    const int j = e - cpos[lo];
+    if (j < ccnt[lo]) {
+      const int q = lo % ncols;
+      const int g = cglob[lo] + j;
+      const float4 v = B.snap[g];
+      sdat[e] = make_float4((float)((double)q * H + (double)v.x), v.y, v.z, v.w);
+      if (SRK) L.somega[e] = B.omega_f[g];
+      if (LIST) L.sg[e] = g;
+    } else {  // row padding: never within range
+      sdat[e] = make_float4(1e30f, 0.f, 0.f, 0.f);
+      if (SRK) L.somega[e] = 0.f;
+      if (LIST) L.sg[e] = -1;
     }
   }
   __syncthreads();
@@ -364,11 +494,11 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
   // particles of the tile: staged row k, columns k .. k+W-1 (contiguous in LDS and globally)
   const int gtb = cglob[k * ncols + k];
   const int ntile = start[cbase + row * ncf + cb] - start[cbase + row * ncf + ca];
-  const int ltb = fits ? cpos[k * ncols + k] : 0;
+  const int ltb = cpos[k * ncols + k];
   const float r2lo = A.r2lo, r2hi = A.r2hi;
 
   for (int u = threadIdx.x; u < ntile; u += kFB) {
-    const int li = ltb + u;  // LDS index (fits)
+    const int li = ltb + u;  // LDS index
     const int gp = gtb + u;  // global sorted position
     const int i = RES ? B.rid[gp] : B.sorted[gp];
     const double *pi = RES ? B.rstate + (long)gp * S : B.stage + (long)i * S;
@@ -382,179 +512,168 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
       }
       qi = lo;
     }
-    float4 me;
-    if (fits) {
-      me = sdat[li];
-    } else {
-      me = B.snap[gp];  // x as the offset in its own cell
-    }
+    const float4 me = sdat[li];
     const float xi_t = me.x, yi = me.y, ci = me.z, si = me.w;
     double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
     int nnb = 0;
     int w = 0;
     const int wbase = (MODE == 2) ? B.pstart[i] : 0;
 
-    // FP64 decision of a guard-band candidate at global sorted position g
-    auto band_in = [&](int g) -> bool {
-      const double *pj = RES ? B.rstate + (long)g * S : B.stage + (long)B.sorted[g] * S;
-      double x_i64, y_i64;
-      apply_pbc(P, pi[0], pi[1], x_i64, y_i64);
-      return fp64_in_range(P, pj, x_i64, y_i64);
-    };
-
-    if (fits) {
-      for (int r = 0; r < nrows; ++r) {
-        const float syi = (float)((double)yi - (double)(r - k) * H);
-        const int e0 = r * ncols + qi - k, e1 = r * ncols + qi + k;
-        const int lo = cpos[e0];
-        const int hi = cpos[e1] + ccnt[e1];  // end of the last stencil cell (before any padding)
-        float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an[4] = {0.f, 0.f, 0.f, 0.f};
-        float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int t = lo; t < hi; t += 4) {
-          float4 o[4];
-          float d2[4];
-          bool in[4], band[4];
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            o[j] = sdat[t + j];
-            const float dx = o[j].x - xi_t;
-            const float dy = o[j].y - syi;
-            d2[j] = __builtin_fmaf(dy, dy, dx * dx);
-            in[j] = d2[j] <= r2hi;
-            band[j] = in[j] && d2[j] >= r2lo;
-          }
-          if (band[0] || band[1] || band[2] || band[3]) {  // rare: FP64 decision inside the stencil
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              if (band[j]) {
-                const int tj = t + j;
-                bool ok = tj < hi;
-                if (ok) {
-                  int lo2 = e0, hi2 = e1;  // stencil cell of tj
-                  while (lo2 < hi2) {
-                    const int mid = (lo2 + hi2 + 1) >> 1;
-                    if (cpos[mid] <= tj) lo2 = mid; else hi2 = mid - 1;
-                  }
-                  ok = band_in(cglob[lo2] + (tj - cpos[lo2]));
-                }
-                in[j] = ok;
-              }
-            }
-          }
-          if (SUMS) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const float m = in[j] ? 1.f : 0.f;
-              ax[j] = __builtin_fmaf(m, o[j].z, ax[j]);
-              ay[j] = __builtin_fmaf(m, o[j].w, ay[j]);
-              an[j] += m;
-              if (SRK) {
-                const float mo = m * somega[t + j];
-                bx[j] = __builtin_fmaf(mo, o[j].z, bx[j]);
-                by[j] = __builtin_fmaf(mo, o[j].w, by[j]);
-              }
-            }
-          } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              if (in[j] && t + j < hi && t + j != li) {
-                ++nnb;
-                if (MODE == 2) B.plist[wbase + w++] = B.sorted[sg[t + j]];
-              }
-            }
-          }
+    for (int r = 0; r < nrows; ++r) {
+      const float syi = (float)((double)yi - (double)(r - k) * H);
+      const int e0 = r * ncols + qi - k, e1 = r * ncols + qi + k;
+      const int lo = cpos[e0];
+      const int hi = cpos[e1] + ccnt[e1];  // end of the last stencil cell (before any padding)
+      // global position of staged entry t inside this row's stencil (band decisions, lists)
+      auto gof = [&](int t) -> int {
+        int lo2 = e0, hi2 = e1;
+        while (lo2 < hi2) {
+          const int mid = (lo2 + hi2 + 1) >> 1;
+          if (cpos[mid] <= t) lo2 = mid; else hi2 = mid - 1;
         }
-        if (SUMS) {
-          sx += ((double)ax[0] + (double)ax[1]) + ((double)ax[2] + (double)ax[3]);
-          sy += ((double)ay[0] + (double)ay[1]) + ((double)ay[2] + (double)ay[3]);
-          nnb += (int)(an[0] + an[1] + an[2] + an[3]);
-          if (SRK) {
-            tx += ((double)bx[0] + (double)bx[1]) + ((double)bx[2] + (double)bx[3]);
-            ty += ((double)by[0] + (double)by[1]) + ((double)by[2] + (double)by[3]);
-          }
-        }
-      }
-      if (SUMS) {  // remove the self term
-        --nnb;
-        sx -= (double)ci;
-        sy -= (double)si;
-        if (SRK) {
-          const float omi = somega[li];
-          tx -= (double)(omi * ci);
-          ty -= (double)(omi * si);
-        }
-      }
-    } else {
-      // global path (staging would overflow LDS): per stencil cell, own offsets shifted
-      for (int r = 0; r < nrows; ++r) {
-        const int dr = r - k;
-        const float syi = (float)((double)yi - (double)dr * H);
-        for (int dc = -k; dc <= k; ++dc) {
-          const float sxi = (float)((double)xi_t - (double)dc * H);
-          const int e = r * ncols + qi + dc;
-          const int g0 = cglob[e], g1 = g0 + ccnt[e];
-          for (int g = g0; g < g1; ++g) {
-            if (g == gp) continue;
-            const float4 o = B.snap[g];
-            const float dx = o.x - sxi;
-            const float dy = o.y - syi;
-            const float d2 = __builtin_fmaf(dy, dy, dx * dx);
-            if (d2 > r2hi) continue;
-            if (d2 >= r2lo && !band_in(g)) continue;
-            ++nnb;
-            if (SUMS) {
-              sx += (double)o.z;
-              sy += (double)o.w;
-              if (SRK) {
-                const float omj = B.omega_f[g];
-                tx += (double)(omj * o.z);
-                ty += (double)(omj * o.w);
-              }
-            } else if (MODE == 2) {
-              B.plist[wbase + w++] = B.sorted[g];
-            }
-          }
-        }
-      }
+        return LIST ? L.sg[t] : cglob[lo2] + (t - cpos[lo2]);
+      };
+      float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an[4] = {0.f, 0.f, 0.f, 0.f};
+      float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
+      scan_range<S, SRK, MODE>(P, B, sdat, L.somega, lo, hi, xi_t, syi, r2lo, r2hi, pi, li, gof, ax, ay, an, bx, by,
+                               nnb, w, wbase);
+      if (SUMS) flush_row<SRK>(ax, ay, an, bx, by, sx, sy, tx, ty, nnb);
     }
-
-    const double card = (double)nnb;
-    // e^{-i theta}, theta = phi_i + alpha, from the FP64 phase
-    double sn = 0.0, cn = 0.0, A64 = 0.0, C64 = 0.0;
-    if (MODE == 0) {
-      sincos(pi[2], &sn, &cn);
-      const double st = sn * P.cos_a + cn * P.sin_a;
-      const double ct = cn * P.cos_a - sn * P.sin_a;
-      A64 = ct * sy - st * sx;
+    if (SUMS) {  // remove the self term
+      --nnb;
+      sx -= (double)ci;
+      sy -= (double)si;
       if (SRK) {
-        const double wi = pi[3];
-        C64 = ct * (tx - wi * sx) + st * (ty - wi * sy);
+        const float omi = L.somega[li];
+        tx -= (double)(omi * ci);
+        ty -= (double)(omi * si);
       }
     }
-    if (MODE == 1) {
-      B.pcount[i] = nnb;
-    } else if (MODE == 0) {
-      const long ii = (long)i * S;
-      const double nat_i = B.nat ? B.nat[i] : 0.0;
-      B.k_next[ii] = P.v0 * cn;
-      B.k_next[ii + 1] = P.v0 * sn;
-      if (S == 4) {
-        const double omega_i = pi[3];
-        B.k_next[ii + 2] = omega_i;
-        if (P.natfreq_local) {
-          if (nnb) B.k_next[ii + 3] = -P.xi * omega_i + nat_i + P.sigma * A64 / card;
-          else B.k_next[ii + 3] = -P.xi * omega_i + nat_i;
-        } else if (nnb) {
-          B.k_next[ii + 3] = -P.xi * omega_i + P.sigma * A64 / card;
+    finish_particle<S, SRK, MODE>(P, B, i, gp, pi, sx, sy, tx, ty, nnb);
+  }
+}
+
+template <int S, bool SRK, int MODE>
+__global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBufs B) {
+  __shared__ PairLds<S, SRK, MODE> L;
+  // XCD-aware tile order: consecutive tiles (neighbouring rows) share an XCD's L2
+  const int per = gridDim.x / 8;  // the grid is a multiple of 8
+  const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (tile >= A.ntiles) return;
+  pairs_tile<S, SRK, MODE>(P, A, B, tile, L);
+}
+
+template <bool SRK>
+struct OvfLds {
+  float4 sdat[kStageMax];
+  float somega[SRK ? kStageMax : 1];
+  int sg[kStageMax];
+  int cglob[kMaxRows], ccnt[kMaxRows];
+};
+
+// The tiles fk_pairs left in the overflow list (dense clusters), a grid-stride loop over them.
+// A tile is handled one cell at a time: the particles of the cell (up to kFB per pass) share one
+// stencil, so each stencil row is staged by itself in chunks of kStageMax - kPad entries that
+// start at multiples of 4 of the row range, which keeps every lane's candidate order and 4-slot
+// grouping identical to fk_pairs.
+template <int S, bool SRK, int MODE>
+__global__ __launch_bounds__(kFB) void fk_pairs_ovf(DevParams P, FastArgs A, PairBufs B) {
+  constexpr bool RES = MODE >= 3;
+  constexpr bool SUMS = MODE == 0 || MODE >= 3;
+  constexpr int kChunk = kStageMax - kPad;
+  __shared__ OvfLds<SRK> L;
+  const int cnt = *A.ovf_count;
+  const int k = A.k, nrows = 2 * k + 1, ncf = A.ncf;
+  const double H = A.H;
+  const float r2lo = A.r2lo, r2hi = A.r2hi;
+  const int *start = B.start;
+  for (int it = blockIdx.x; it < cnt; it += gridDim.x) {
+    const int4 td = A.tiles[A.ovf_list[it]];
+    const int cbase = td.x, row = td.y;
+    for (int q = td.z; q < td.w; ++q) {
+      const int cown = cbase + row * ncf + q;
+      const int gp0 = start[cown], gp1 = start[cown + 1];
+      for (int pc = gp0; pc < gp1; pc += kFB) {
+        const int gp = pc + threadIdx.x;
+        const bool act = gp < gp1;
+        const int i = act ? (RES ? B.rid[gp] : B.sorted[gp]) : 0;
+        const double *pi = act ? (RES ? B.rstate + (long)gp * S : B.stage + (long)i * S) : nullptr;
+        const float4 me = act ? B.snap[gp] : make_float4(0.f, 0.f, 0.f, 0.f);
+        // x re-based to the window's column 0 (the particle's own column is k)
+        const float xi_t = (float)((double)k * H + (double)me.x);
+        const float yi = me.y, ci = me.z, si = me.w;
+        double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
+        int nnb = 0;
+        int w = 0;
+        const int wbase = (MODE == 2 && act) ? B.pstart[i] : 0;
+        int self_t = -1;
+        for (int r = 0; r < nrows; ++r) {
+          __syncthreads();  // the previous row's stage is consumed
+          if (threadIdx.x < nrows) {
+            const int dc = threadIdx.x;
+            const int rrow = ((row + r - k) % ncf + ncf) % ncf;
+            const int col = ((q - k + dc) % ncf + ncf) % ncf;
+            const int c = cbase + rrow * ncf + col;
+            L.cglob[dc] = start[c];
+            L.ccnt[dc] = start[c + 1] - start[c];
+          }
+          __syncthreads();
+          int len = 0;
+          for (int dc = 0; dc < nrows; ++dc) len += L.ccnt[dc];
+          const float syi = (float)((double)yi - (double)(r - k) * H);
+          float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an[4] = {0.f, 0.f, 0.f, 0.f};
+          float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
+          for (int c0 = 0; c0 < len; c0 += kChunk) {
+            const int clen = min(kChunk, len - c0);
+            if (c0 > 0) __syncthreads();
+            for (int e = threadIdx.x; e < clen + kPad; e += kFB) {
+              if (e < clen) {
+                const int t = c0 + e;
+                int dc = 0, base = 0;
+                while (t >= base + L.ccnt[dc]) base += L.ccnt[dc++];
+                const int g = L.cglob[dc] + (t - base);
+                const float4 v = B.snap[g];
+                L.sdat[e] = make_float4((float)((double)dc * H + (double)v.x), v.y, v.z, v.w);
+                if (SRK) L.somega[e] = B.omega_f[g];
+                L.sg[e] = g;
+              } else {
+                L.sdat[e] = make_float4(1e30f, 0.f, 0.f, 0.f);
+                if (SRK) L.somega[e] = 0.f;
+                L.sg[e] = -1;
+              }
+            }
+            __syncthreads();
+            if (act) {
+              int st = -1;  // the particle itself, in its own row
+              if (r == k) {
+                const int s0 = L.cglob[k] <= gp && gp < L.cglob[k] + L.ccnt[k] ? gp - L.cglob[k] : -1;
+                int base = 0;
+                for (int dc = 0; dc < k; ++dc) base += L.ccnt[dc];
+                if (s0 >= 0 && base + s0 >= c0 && base + s0 < c0 + clen) st = base + s0 - c0;
+              }
+              if (st >= 0) self_t = st;
+              auto gof = [&](int t) -> int { return L.sg[t]; };
+              scan_range<S, SRK, MODE>(P, B, L.sdat, L.somega, 0, clen, xi_t, syi, r2lo, r2hi, pi, st, gof, ax,
+                                       ay, an, bx, by, nnb, w, wbase);
+            }
+          }
+          if (act && SUMS) flush_row<SRK>(ax, ay, an, bx, by, sx, sy, tx, ty, nnb);
         }
-        if (SRK && B.compl_out) B.compl_out[i] = nnb ? C64 * (P.sigma / card) : C64;
-      } else {
-        B.k_next[ii + 2] = nnb ? nat_i + P.sigma * A64 / card : nat_i;
+        if (act) {
+          if (SUMS) {  // remove the self term
+            --nnb;
+            sx -= (double)ci;
+            sy -= (double)si;
+            if (SRK) {
+              const float omi = B.omega_f[gp];
+              tx -= (double)(omi * ci);
+              ty -= (double)(omi * si);
+            }
+          }
+          finish_particle<S, SRK, MODE>(P, B, i, gp, pi, sx, sy, tx, ty, nnb);
+        }
+        (void)self_t;
       }
-    } else if (MODE == 3) {
-      B.ssum[gp] = make_double2(sx, sy);
-      if (SRK) B.tsum[gp] = make_double2(tx, ty);
-      B.scnt[gp] = nnb;
     }
   }
 }

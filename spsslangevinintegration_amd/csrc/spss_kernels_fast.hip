@@ -49,6 +49,7 @@ struct FastPath {
   std::vector<int> rep_first_h, W_h;
   int *rep_first = nullptr, *rep_chunk = nullptr;
   int4 *tiles = nullptr;
+  int *ovf_list = nullptr, *ovf_count = nullptr;  // tiles left to the overflow pass
   int ntiles = 0, tiles_cap = 0;
   Coef *coef = nullptr;                    // per-replica coefficients (batches)
   unsigned long long *step_dev = nullptr;  // device step counter (batches)
@@ -121,8 +122,11 @@ static int set_tiles(FastPath *f, const std::vector<int> &W) {
   }
   if ((int)t.size() > f->tiles_cap) {
     hipFree(f->tiles);
+    hipFree(f->ovf_list);
     f->tiles = nullptr;
+    f->ovf_list = nullptr;
     FCHK(hipMalloc(&f->tiles, sizeof(int4) * t.size()));
+    FCHK(hipMalloc(&f->ovf_list, sizeof(int) * t.size()));
     f->tiles_cap = (int)t.size();
   }
   FCHK(hipMemcpy(f->tiles, t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice));
@@ -219,6 +223,7 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->farcnt, cells));
   A(falloc(&f->newcnt, cells));
   A(falloc(&f->rep_first, R + 1));
+  A(falloc(&f->ovf_count, 1));
   A(falloc(&f->rep_chunk, nchunk));
   if (coef) {
     A(falloc(&f->coef, R));
@@ -243,6 +248,7 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   launch_fill_int(st, cells, f->count, 0);
   launch_fill_int(st, cells, f->farcnt, 0);
   launch_fill_int(st, cells, f->newcnt, 0);
+  launch_fill_int(st, 1, f->ovf_count, 0);
   if (coef) {
     launch_fill_double(st, 3L * R, f->racc, 0.0);
     launch_fill_double(st, 2L * R, f->zlast, 0.0);
@@ -262,7 +268,7 @@ void fast_destroy(FastPath *f) {
                f->snap,      f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],  f->rid[1],    f->rnat[0],
                f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mv,
                f->cnt9,      f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
-               f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc};
+               f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc,      f->ovf_list,  f->ovf_count};
   for (void *p : b) hipFree(p);
   delete f;
 }
@@ -301,6 +307,8 @@ static FastArgs args_of(const FastPath *f) {
   a.ntiles = f->ntiles;
   a.cpr = f->cpr;
   a.tiles = f->tiles;
+  a.ovf_list = f->ovf_list;
+  a.ovf_count = f->ovf_count;
   a.H = f->H;
   a.r2lo = f->r2lo;
   a.r2hi = f->r2hi;
@@ -326,7 +334,7 @@ static void build(FastPath *f, hipStream_t st, const DevParams &P, const double 
   else
     fk_keys<3><<<nb, 256, 0, st>>>(P, L, stage, f->ncf, f->key, f->slot, f->count);
   fk_scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum);
-  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum, f->nsb);
+  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum, f->nsb, f->ovf_count);
   fk_scatter<<<nb, 256, 0, st>>>(n, f->key, f->slot, start, f->tmp);
   if (P.s == 4)
     fk_rank<4><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->cpr, f->H, f->key, start, f->tmp, f->sorted, snap, omega);
@@ -344,9 +352,12 @@ static PairBufs rebuilt_bufs(const FastPath *f, const double *stage) {
   return B;
 }
 
+// the pair kernel, then the pass over the tiles whose cells overflowed its LDS stage (usually none)
 template <int S, bool SRK, int MODE>
 static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, const PairBufs &B) {
-  fk_pairs<S, SRK, MODE><<<tile_grid(f->ntiles), kFB, 0, st>>>(P, args_of(f), B);
+  const FastArgs a = args_of(f);
+  fk_pairs<S, SRK, MODE><<<tile_grid(f->ntiles), kFB, 0, st>>>(P, a, B);
+  fk_pairs_ovf<S, SRK, MODE><<<std::min(f->ntiles, 1024), kFB, 0, st>>>(P, a, B);
 }
 
 int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const double *nat,
@@ -413,7 +424,8 @@ static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const
   fr_cell_counts<<<nblocks(f->cells), 256, 0, st>>>(f->cells, f->ncf, f->cpr, f->rstart[c], f->mv, f->cnt9,
                                                     f->newcnt);
   fk_scan_local<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum);
-  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum, f->nsb);
+  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum, f->nsb,
+                                      f->ovf_count);
   fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, f->mv, f->rstate[c], U, f->rstart[o], f->cnt9,
                                            f->slot, f->tmp);
   fr_scatter<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], f->mv, f->cnt9, f->tmp,
@@ -543,6 +555,7 @@ int fast_pairs_count(FastPath *f, hipStream_t st, const DevParams &P, const doub
 // lists of the stage state of the preceding fast_pairs_count
 int fast_pairs_fill(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const int *pstart,
                     int *plist) {
+  launch_fill_int(st, 1, f->ovf_count, 0);  // the count pass left its overflow list behind
   PairBufs B = rebuilt_bufs(f, stage);
   B.pstart = pstart;
   B.plist = plist;

@@ -104,8 +104,16 @@ def test_batch_time_averages_follow_readings(spss):
 
 
 def test_batch_tile_widths_and_autotune_do_not_change_results(spss):
+    """Also with a dense spot (600 particles inside one cell of replica 1): tiles wide enough to
+    overflow the LDS stage there take the global-memory walk, narrow ones stay staged; both
+    visit the same candidates in the same order."""
     model, R, K, dt, v0, rho = 4, 5, 12, 0.005, 0.01, 0.03
     ns, xi, sig, Dp, seeds, states, nats = _replicas(model, R, 31)
+    rng = np.random.default_rng(5)
+    s1 = states[1].reshape(-1, model)
+    s1[:600, 0] = 0.5 + 0.02 * rng.random(600)
+    s1[:600, 1] = 0.5 + 0.02 * rng.random(600)
+    states[1] = s1.reshape(-1)
     outs = []
     for mode in ("default", "manual", "tuned"):
         b = spss.ReplicaBatch(model, ns, xi, sig, Dp, seeds, v0=v0, rho=rho)
