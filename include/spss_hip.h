@@ -234,6 +234,35 @@ int spss_batch_set_tile_widths(spss_batch *b, const int *W);
  * chosen index into *chosen.  State and step counter are restored, the averages cleared. */
 int spss_batch_autotune(spss_batch *b, double dt, int probe_steps, double *table_ms, int table_cap, int *chosen);
 
+/* ---- the network of the learned sweep screen (no reference counterpart).  A fully connected
+ * FP32 network of widths dims[0] -> ... -> dims[ndims-1] (at most 8 layers, widths <= 4096):
+ * GELU (erf form) after every hidden layer, linear output, mean-squared-error loss, Adam.
+ * Parameters are one flat float array, layer after layer: W_l [dims[l+1] x dims[l]] row-major,
+ * then b_l.  Rows of inputs and targets are row-major float arrays.  All kernels sum in a fixed
+ * order without atomics: results are reproducible bit for bit. */
+typedef struct spss_mlp spss_mlp;
+/* W and b ~ U(-1/sqrt(in), 1/sqrt(in)) from Philox4x32-10 keyed by `seed`; max_batch bounds the
+ * rows of one forward/backward pass */
+int spss_mlp_create(spss_mlp **out, const int *dims, int ndims, int max_batch, uint64_t seed);
+int spss_mlp_destroy(spss_mlp *m);
+int spss_mlp_param_count(const spss_mlp *m, long *count);
+int spss_mlp_set_params(spss_mlp *m, const float *params, long count);
+int spss_mlp_get_params(spss_mlp *m, float *params, long count);
+/* outputs of n rows (any n; processed max_batch rows at a time) */
+int spss_mlp_predict(spss_mlp *m, const float *x, long n, float *y);
+/* gradient of the loss mean((y - t)^2) over batch x dims[last] outputs w.r.t. every parameter
+ * (flat layout), and the loss */
+int spss_mlp_gradients(spss_mlp *m, const float *x, const float *t, int batch, float *grad, float *loss);
+/* training rows kept on the device; the trainer walks them in order, `batch` at a time, wrapping */
+int spss_mlp_set_data(spss_mlp *m, const float *x, const float *t, long n);
+/* nsteps Adam steps (torch.optim.Adam's update, no weight decay); the loss of each step's batch
+ * (before its update) into losses[nsteps] (may be NULL); nsteps <= 65536 */
+int spss_mlp_train(spss_mlp *m, int nsteps, int batch, float lr, float beta1, float beta2, float eps,
+                   float *losses);
+/* zero the Adam moments and step count */
+int spss_mlp_reset_optimizer(spss_mlp *m);
+int spss_mlp_adam_steps(const spss_mlp *m, long *steps);
+
 /* AveragingObserver::operator() and the sum behind SaveAveragedSystemState (Observers/AveragingObserver.cpp:29-37,
  * 39-68; AveragingObserverPtr.cpp:46-79): running FP64 sum of the state and the step count */
 int spss_averaging_reset(spss_system *sys);

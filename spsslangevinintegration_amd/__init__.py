@@ -15,3 +15,4 @@ _capi.load()
 
 from .system import ParticleSystem  # noqa: E402,F401
 from .batch import ReplicaBatch  # noqa: E402,F401
+from .screen import MLP, SweepScreen  # noqa: E402,F401

@@ -1,0 +1,68 @@
+"""What the learned sweep screen gains over the plain sweep.
+
+The network (trained on the GPU by the library's own kernels on batched runs of random
+parameter points) predicts the steady-state mean |Z| along sigma sweeps; the screen simulates
+only a window around the predicted crossing of mean |Z| = 0.5 and widens it whenever the
+simulated points disagree with the prediction or do not bracket the crossing.  Against the
+plain sweep (every point simulated) it must find the same crossing (within one grid step) with
+at most 40% of the simulations.  Every point is simulated with its own seed, so a point the
+screen simulates has exactly the value the plain sweep gives it.  The network must also beat
+the constant predictor on held-out points.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIM = dict(steps=2000, dt=0.005, observe_every=10, observe_from=1000)
+
+
+@pytest.fixture(scope="module")
+def trained():
+    import spsslangevinintegration_amd as spss
+    from spsslangevinintegration_amd import screen as S
+    rng = np.random.default_rng(0x5C4EE)
+    sc = spss.SweepScreen(seed=0x5EED0005, sim_kwargs=SIM)
+    train = S.random_points(384, rng, seed0=0x5EED2000)
+    y = sc.simulate(train)
+    losses = sc.fit(train, y, steps=4000, batch=128, lr=2e-3)
+    print(f"training: {train.size} points, loss {losses[:20].mean():.4f} -> {losses[-20:].mean():.5f}")
+    return sc, S, y
+
+
+def test_network_beats_the_constant_predictor(trained):
+    sc, S, y_train = trained
+    held = S.random_points(128, np.random.default_rng(77), seed0=0x5EED3000)
+    y = sc.simulate(held)
+    pred = sc.predict(held)
+    rmse = np.sqrt(np.mean((pred[:, 0] - y[:, 0]) ** 2))
+    rmse_const = np.sqrt(np.mean((y_train[:, 0].mean() - y[:, 0]) ** 2))
+    print(f"held-out mean |Z|: network RMSE {rmse:.4f}, constant predictor {rmse_const:.4f}")
+    assert rmse < 0.5 * rmse_const
+
+
+def test_screened_sweep_finds_the_plain_crossing_with_fewer_simulations(trained):
+    sc, S, _ = trained
+    rng = np.random.default_rng(99)
+    G = 40
+    sig = np.linspace(0.0, 3.0, G)
+    plain_sims = screened_sims = 0
+    for k in range(4):
+        D, m, s = rng.uniform(0.0, 0.15), rng.uniform(0.3, 1.0), rng.uniform(0.05, 0.25)
+        N = int(2 ** rng.uniform(11.2, 12.8))
+        line = S.points(sig, D, m, s, N, 0.0, 0x5EED4000 + 100 * k + np.arange(G, dtype=np.uint64))
+        before = sc.simulations
+        x_plain, R = sc.plain_sweep(line)
+        plain_sims += sc.simulations - before
+        before = sc.simulations
+        x_scr, idx, R_scr, err = sc.screened_sweep(line, window=6)
+        screened_sims += sc.simulations - before
+        print(f"line {k}: D={D:.3f} m={m:.2f} s={s:.3f} N={N}: plain crossing {x_plain}, screened {x_scr} "
+              f"from {idx.size} points (largest prediction error {err:.3f})")
+        np.testing.assert_array_equal(R_scr, R[idx])  # same seeds, same simulations
+        if x_plain is None:
+            assert x_scr is None
+        else:
+            assert x_scr is not None and abs(x_scr - x_plain) <= sig[1] - sig[0] + 1e-12
+    print(f"simulations: plain {plain_sims}, screened {screened_sims} ({screened_sims / plain_sims:.2f})")
+    assert screened_sims <= 0.4 * plain_sims
