@@ -147,6 +147,16 @@ int spss_run(spss_system *sys, int stepper, int method, long nsteps, double dt, 
 int spss_get_step(spss_system *sys, long *step);
 int spss_set_step(spss_system *sys, long step);
 
+/* cell-size autotuner of the FP32C linked-list path (no reference counterpart): times
+ * `probe_steps` stochastic Euler (stepper = SPSS_STEPPER_EULER) or Runge-Kutta steps with cells
+ * of side 1/(k*int(1/rho)) for k = 1, 2, 3 and keeps the fastest; ms per step into
+ * table_ms[k-1] (-1 for a grid that is too fine; at most table_cap entries), the chosen k into
+ * *chosen.  State, step counter and complementary force are restored.  Neighbour sets do not
+ * depend on k; the coupling sums change within the FP32C tolerance (candidate order). */
+int spss_autotune_cells(spss_system *sys, int stepper, double dt, int probe_steps, double *table_ms, int table_cap,
+                        int *chosen);
+int spss_get_cell_subdivision(spss_system *sys, int *k);
+
 /* CalculateMaxDisplacement (ParticleSystemFirstOrderPtr.cpp:836-861) on a host total-displacement
  * vector, and the Verlet bookkeeping it updates */
 int spss_calculate_max_displacement(spss_system *sys, const double *host_total_displacement);

@@ -29,7 +29,7 @@ EXPORTS = [
     "spss_neighbour_pairs", "spss_order_parameter", "spss_apply_periodic_boundary_conditions",
     "spss_averaging_reset", "spss_averaging_accumulate", "spss_averaging_get",
     "spss_kernel_timer_start", "spss_kernel_timer_stop", "spss_device_copy_bandwidth",
-    "spss_philox4x32_10", "spss_philox_normals",
+    "spss_philox4x32_10", "spss_philox_normals", "spss_autotune_cells", "spss_get_cell_subdivision",
     "spss_batch_create", "spss_batch_destroy", "spss_batch_set_state", "spss_batch_get_state",
     "spss_batch_set_natural_frequencies", "spss_batch_reset", "spss_batch_run", "spss_batch_order_stats",
     "spss_batch_get_tile_widths", "spss_batch_set_tile_widths", "spss_batch_autotune",
@@ -116,6 +116,8 @@ def load(path=None):
         "spss_device_copy_bandwidth": (i, [l, i, C.POINTER(d)]),
         "spss_philox4x32_10": (i, [vp, vp, vp]),
         "spss_philox_normals": (i, [C.c_uint64, C.c_uint64, C.c_uint32, l, l, vp, vp]),
+        "spss_autotune_cells": (i, [vp, i, d, i, vp, i, C.POINTER(i)]),
+        "spss_get_cell_subdivision": (i, [vp, C.POINTER(i)]),
         "spss_batch_create": (i, [C.POINTER(SpssBatchParams), i, vp, C.POINTER(vp)]),
         "spss_batch_destroy": (i, [vp]),
         "spss_batch_set_state": (i, [vp, vp]),

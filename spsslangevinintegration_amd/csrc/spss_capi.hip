@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -638,6 +639,100 @@ int spss_get_step(spss_system *h, long *step) {
 int spss_set_step(spss_system *h, long step) {
   if (!h || step < 0) return fail(SPSS_ERR_INVALID, "bad argument");
   h->step = step;
+  return SPSS_OK;
+}
+
+int spss_autotune_cells(spss_system *h, int stepper, double dt, int probe_steps, double *table_ms, int table_cap,
+                        int *chosen) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (h->p.precision != SPSS_PRECISION_FP32C || !h->fast)
+    return fail(SPSS_ERR_UNSUPPORTED, "the cell autotuner needs the FP32C linked-list path");
+  if (stepper != SPSS_STEPPER_EULER && stepper != SPSS_STEPPER_SRK)
+    return fail(SPSS_ERR_INVALID, "the cell autotuner times the stochastic Euler or Runge-Kutta step");
+  if (probe_steps < 1 || !(dt > 0.0)) return fail(SPSS_ERR_INVALID, "probe_steps >= 1 and dt > 0 required");
+  // what the probes change and must be put back: the state, the complementary force, the
+  // step counter and the higher-order flags
+  TRY(sync_master(h));
+  const size_t m = (size_t)h->s * h->n;
+  double *keep = nullptr, *keep_c = nullptr;
+  TRY(dalloc_n(&keep, m));
+  if (dalloc_n(&keep_c, h->n) != SPSS_OK) {
+    hipFree(keep);
+    return SPSS_ERR_DEVICE;
+  }
+  HIPCHECK(hipMemcpyAsync(keep, h->state, m * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  HIPCHECK(hipMemcpyAsync(keep_c, h->compl_f, h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  const long step0 = h->step;
+  const bool ho0 = h->higher_order;
+  const int pho0 = h->P.higher_order;
+  auto restore = [&]() -> int {
+    TRY(master_write(h, false));
+    HIPCHECK(hipMemcpyAsync(h->state, keep, m * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHECK(hipMemcpyAsync(h->compl_f, keep_c, h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    h->step = step0;
+    h->higher_order = ho0;
+    h->P.higher_order = pho0;
+    return SPSS_OK;
+  };
+  int best = h->p.cell_subdivision;
+  double best_ms = 1e300;
+  int rc = SPSS_OK;
+  for (int k = 1; k <= 3 && rc == SPSS_OK; ++k) {
+    FastPath *f = nullptr;
+    if (fast_create(&f, h->stream, h->P, h->p.rho, h->ncx, k) != SPSS_OK) {  // grid too fine: not a candidate
+      if (k - 1 < table_cap && table_ms) table_ms[k - 1] = -1.0;
+      continue;
+    }
+    if ((rc = restore()) != SPSS_OK) {
+      fast_destroy(f);
+      break;
+    }
+    fast_destroy(h->fast);
+    h->fast = f;
+    h->p.cell_subdivision = k;
+    const int warm = 2;
+    double ms = 0.0;
+    for (int s = 0; s < warm + probe_steps && rc == SPSS_OK; ++s) {
+      if (s == warm) {
+        if (hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(SPSS_ERR_DEVICE, "probe step failed");
+        ms = -std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+      }
+      if (rc == SPSS_OK)
+        rc = stepper == SPSS_STEPPER_EULER ? euler_step(h, SPSS_METHOD_LINKED_LIST, dt, SPSS_NOISE_PHILOX, nullptr)
+                                           : srk_step(h, SPSS_METHOD_LINKED_LIST, dt, SPSS_NOISE_PHILOX, nullptr);
+    }
+    if (rc == SPSS_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(SPSS_ERR_DEVICE, "probe failed");
+    if (rc != SPSS_OK) break;
+    ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    ms /= probe_steps;
+    if (k - 1 < table_cap && table_ms) table_ms[k - 1] = ms;
+    if (ms < best_ms) best_ms = ms, best = k;
+  }
+  // keep the fastest grid (or the original one after a failure) with the original state
+  if (rc == SPSS_OK && best != h->p.cell_subdivision) {
+    FastPath *f = nullptr;
+    rc = fast_create(&f, h->stream, h->P, h->p.rho, h->ncx, best);
+    if (rc == SPSS_OK) {
+      fast_destroy(h->fast);
+      h->fast = f;
+      h->p.cell_subdivision = best;
+    } else {
+      rc = fail(rc, fast_last_error());
+    }
+  }
+  const int rc2 = restore();
+  hipFree(keep);
+  hipFree(keep_c);
+  if (rc != SPSS_OK) return rc;
+  if (rc2 != SPSS_OK) return rc2;
+  if (chosen) *chosen = h->p.cell_subdivision;
+  return SPSS_OK;
+}
+
+int spss_get_cell_subdivision(spss_system *h, int *k) {
+  if (!h || !k) return fail(SPSS_ERR_INVALID, "NULL argument");
+  *k = h->p.cell_subdivision;
   return SPSS_OK;
 }
 

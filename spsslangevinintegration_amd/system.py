@@ -134,6 +134,22 @@ class ParticleSystem:
         A.check(self.lib.spss_neighbour_pairs(self.h, method, A.ptr(st), A.ptr(out), cnt.value, C.byref(cnt)))
         return out[: 2 * cnt.value].reshape(-1, 2)
 
+    # ---- cell-size autotuner (FP32C)
+    def autotune_cells(self, stepper, dt, probe_steps=10):
+        """times the FP32C step at cell subdivisions 1, 2, 3 and keeps the fastest; returns
+        (ms per step for each, -1 where the grid is too fine; the chosen subdivision)"""
+        table = np.zeros(3)
+        k = C.c_int()
+        A.check(self.lib.spss_autotune_cells(self.h, int(stepper), float(dt), int(probe_steps), A.ptr(table), 3,
+                                             C.byref(k)))
+        return table, k.value
+
+    @property
+    def cell_subdivision(self):
+        k = C.c_int()
+        A.check(self.lib.spss_get_cell_subdivision(self.h, C.byref(k)))
+        return k.value
+
     # ---- instrumentation
     def kernel_timer_start(self):
         A.check(self.lib.spss_kernel_timer_start(self.h))
