@@ -31,13 +31,18 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s, N=1M cell list; batched vs plain at ≤1e-4 order-param err"
 HBM_PEAK = 8.0e12
-# algorithmic HBM bytes per particle of the pair kernel of one Euler step: FP32 snapshot
-# (x, y, cos phi, sin phi) read once (16), phasor sum (2 x FP64) and count written (20),
-# cell starts read once (4 B x 2 per cell, 62,500 cells)
-PAIR_KERNEL_BYTES = 16.0 + 20.0 + 8.0 * 62500.0 / 1048576.0
+
+
+def pair_kernel_bytes(n, cells):
+    """algorithmic HBM bytes per particle of the pair kernel of one Euler step: FP32 snapshot
+    (x, y, cos phi, sin phi) read once (16), phasor sum (2 x FP64) and count written (20); per
+    cell its start read (4 B x 2) and its nine move counters cleared (36 B)"""
+    return 16.0 + 20.0 + (8.0 + 36.0) * cells / n
+
+
 # HBM bytes per launch of that kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled
-# by the copy-kernel calibration (profiles/fk_pairs_euler_N1M_hbm_traffic.txt)
-PAIR_KERNEL_TRAFFIC = 3.9748e7
+# by the copy-kernel calibration, by cell subdivision (profiles/fk_pairs_euler_N1M_hbm_traffic.txt)
+PAIR_KERNEL_TRAFFIC = {1: 3.9748e7}
 # VALU issue utilisation of that kernel (SQ_INSTS_VALU per CU-cycle, profiles/fk_pairs_euler_N1M_valu.txt)
 PAIR_KERNEL_VALU_UTIL = 0.99
 # algorithmic bytes per particle-step of the whole step (state 64, snapshot w+r 24,
@@ -208,7 +213,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--subdiv", type=int, default=1)
+    ap.add_argument("--subdiv", type=int, default=0, help="cell subdivision 1-3; 0: the autotuner picks it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--batch-steps", type=int, default=3000)
@@ -225,17 +230,24 @@ def main():
 
     # attainable HBM bandwidth of the build's own streaming copy kernel (1 GiB, read + write)
     copy_gbps = _capi.copy_bandwidth(1 << 30, 10)
-    g, kw, st0, natf = headline_system(spss, subdiv=args.subdiv)
+    g, kw, st0, natf = headline_system(spss, subdiv=max(args.subdiv, 1))
     n = kw["n"]
+    tune = None
+    if args.subdiv == 0:  # cell size picked by the autotuner (outside the timed region)
+        table, k = g.autotune_cells(spss.STEPPER_EULER, 0.002, probe_steps=20)
+        tune = {"ms_per_step_by_subdivision": [round(float(t), 4) for t in table], "chosen": int(k)}
+        print(f"[bench] cell autotuner: {tune}", file=sys.stderr, flush=True)
+    subdiv = g.cell_subdivision
+    cells = (250 * subdiv) ** 2
     print(f"[bench] headline n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
     wall, pair_ms = time_steps(g, spss, spss.STEPPER_EULER, args.steps, args.warmup)
     value = n * args.steps / wall
     ms_step = wall / args.steps * 1e3
     # neighbour count of the final state (pair-interactions/s = sum_i n_i * steps / s)
     pairs = C_pairs(g, spss)
-    achieved = PAIR_KERNEL_BYTES * n / (pair_ms * 1e-3)
+    achieved = pair_kernel_bytes(n, cells) * n / (pair_ms * 1e-3)
     # secondary: the SRK stepper (complementary force, 2 draws per particle)
-    g2, _, _, _ = headline_system(spss, subdiv=args.subdiv)
+    g2, _, _, _ = headline_system(spss, subdiv=subdiv)
     srk_steps = max(args.steps // 5, 10)
     wall2, _ = time_steps(g2, spss, spss.STEPPER_SRK, srk_steps, 10)
     g2.close()
@@ -265,9 +277,10 @@ def main():
         "config": {"workload": "N=1,048,576 uniform in the unit box, rho=0.004 cell list refreshed every step, "
                                "stochastic Euler dt=0.002, m=0.5 sigma=1.5 D=0.02, Lorentzian(0,0.05) omega, "
                                "FP32 distance tests, phasor sums in FP32 partials combined in FP64, FP64 state",
-                   "n": n, "rho": 0.004, "cell_subdivision": args.subdiv, "parallelism": "single GPU"},
+                   "n": n, "rho": 0.004, "cell_subdivision": subdiv, "cell_autotune": tune,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": PAIR_KERNEL_TRAFFIC},
+                     "frac": achieved / HBM_PEAK, "traffic": PAIR_KERNEL_TRAFFIC.get(subdiv)},
         # the pair kernel is bound by VALU issue, not HBM (no MFMA work on this path)
         "valu": {"kernel": "fk_pairs", "issue_util": PAIR_KERNEL_VALU_UTIL},
         "cpu_baseline": cpu,

@@ -8,8 +8,9 @@
 //  * resident (MODE 3, the stochastic steps): the FP64 state, the particle ids and the
 //    natural frequencies stay in cell order between steps.  The pair kernel writes the
 //    phasor sums of every particle in the old order; fr_move decides each particle's
-//    move (its new position depends on phase and omega only), fr_cell_counts and a
-//    two-level scan of the arrival counts give the new cell starts, and fr_scatter
+//    move (its new position depends on phase and omega only) and counts the moves per
+//    (cell, direction), a two-level scan of the arrival counts (gathered from those counts)
+//    gives the new cell starts, and fr_scatter
 //    applies the step and writes every particle to its new slot: inside a cell, particles that
 //    stayed come first in their previous order, then arrivals grouped by the direction
 //    they came from, each group in its previous order, then the rare particles that jumped
@@ -79,6 +80,7 @@ struct PairBufs {
   double2 *ssum;         // MODE 3: phasor sum S of each sorted position
   double2 *tsum;         // MODE 3, SRK: omega-weighted phasor sum T
   int *scnt;             // MODE 3: neighbour count
+  int *cnt9;             // MODE 3: move counts per (cell, direction), zeroed here for fr_move
 };
 
 // the stochastic update of fk_update (Euler or SRK)
@@ -130,10 +132,12 @@ __global__ void fk_keys(DevParams P, Layout L, const double *__restrict__ stage,
   slot[i] = atomicAdd(count + c, 1);
 }
 
-// block-local exclusive scan of 2048 values per block; a value is count[c] + count2[c]
-// (count2 may be null)
+// block-local exclusive scan of 2048 values per block; a value is count[c] + count2[c] (count2
+// may be null), plus, when cnt9 is given, the near arrivals of cell c gathered from the move
+// counts of its nine source cells
 __global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ count, const int *__restrict__ count2,
-                                                     int m, int *__restrict__ start, int *__restrict__ blocksum) {
+                                                     const int *__restrict__ cnt9, int ncf, int cpr, int m,
+                                                     int *__restrict__ start, int *__restrict__ blocksum) {
   __shared__ int sh[256];
   const int base = blockIdx.x * 256 * kScanItems + threadIdx.x * kScanItems;
   int v[kScanItems];
@@ -144,6 +148,7 @@ __global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ cou
     if (base + q < m) {
       v[q] = count[base + q];
       if (count2) v[q] += count2[base + q];
+      if (cnt9) v[q] += new_count_of(cnt9, base + q, ncf, cpr);
     }
     s += v[q];
   }
@@ -428,6 +433,11 @@ __device__ __forceinline__ void pairs_tile(const DevParams &P, const FastArgs &A
   const double H = A.H;
   const int *start = B.start;
   const int ncells = nrows * ncols;
+
+  if (MODE == 3) {  // this step's move counts of the tile's cells start from zero
+    int *z = B.cnt9 + (long)(cbase + row * ncf + ca) * 9;
+    for (int e = threadIdx.x; e < 9 * W; e += kFB) z[e] = 0;
+  }
 
   // ---- staging plan: global extent of every staged cell, then the LDS layout
   for (int e = threadIdx.x; e < ncells; e += kFB) {
@@ -756,59 +766,51 @@ __device__ __forceinline__ int cell_of_position(const DevParams &P, double x, do
 // resident re-sort
 
 // the move of every resident particle this step: old cell * 16 + (dy+1)*3 + (dx+1), or kFar
-// with an unordered arrival slot in farcnt of the destination
+// with an unordered arrival slot in farcnt of the destination.  The near moves are counted per
+// (old cell, direction) into cnt9 (zeroed by the pair kernel of this step): particles lie in
+// cell order, so a wave's equal (cell, direction) keys form runs and one lane per run adds the
+// run's length.
 template <int S, bool SRK>
 __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int ncf,
                                                const double *__restrict__ rstate, StepArgs U, int *__restrict__ mv,
-                                               int *__restrict__ farcnt, int *__restrict__ farslot) {
+                                               int *__restrict__ farcnt, int *__restrict__ farslot,
+                                               int *__restrict__ cnt9) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const double *pi = rstate + (long)p * S;
-  double sn, cn, x1, y1;
-  step_position<S, SRK>(P, pi, U, sn, cn, x1, y1);
-  double xw, yw;
-  apply_pbc(P, pi[0], pi[1], xw, yw);
-  const int ox = cell_coord(xw, ncf), oy = cell_coord(yw, ncf);
-  apply_pbc(P, x1, y1, xw, yw);
-  const int nx = cell_coord(xw, ncf), ny = cell_coord(yw, ncf);
-  const int base = replica_of(L, p) * L.cpr;
-  int dxc = nx - ox, dyc = ny - oy;
-  if (dxc == ncf - 1) dxc = -1;
-  if (dxc == 1 - ncf) dxc = 1;
-  if (dyc == ncf - 1) dyc = -1;
-  if (dyc == 1 - ncf) dyc = 1;
-  int d;
-  if (dxc >= -1 && dxc <= 1 && dyc >= -1 && dyc <= 1) {
-    d = (dyc + 1) * 3 + (dxc + 1);
-  } else {
-    d = kFar;
-    farslot[p] = atomicAdd(farcnt + base + ny * ncf + nx, 1);
-  }
-  mv[p] = ((base + oy * ncf + ox) << 4) | d;
-}
-
-// per cell: how many of its particles take each of the nine near moves (cnt9), added into
-// the arrival count of every destination cell (newcnt, zeroed by the scan)
-__global__ __launch_bounds__(256) void fr_cell_counts(int cells, int ncf, int cpr, const int *__restrict__ start,
-                                                      const int *__restrict__ mv, int *__restrict__ cnt9,
-                                                      int *__restrict__ newcnt) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cells) return;
-  int v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int p = start[c]; p < start[c + 1]; ++p) {
-    const int d = mv[p] & 15;
-#pragma unroll
-    for (int e = 0; e < 9; ++e) v[e] += (d == e) ? 1 : 0;
-  }
-  const int base = cell_base(c, cpr), l = c - base;
-  const int cx = l % ncf, cy = l / ncf;
-#pragma unroll
-  for (int d = 0; d < 9; ++d) {
-    cnt9[(long)c * 9 + d] = v[d];
-    if (v[d]) {
-      const int dx = (cx + d % 3 - 1 + ncf) % ncf, dy = (cy + d / 3 - 1 + ncf) % ncf;
-      atomicAdd(newcnt + base + dy * ncf + dx, v[d]);
+  const bool valid = p < n;
+  int cell = -1, d = kFar;  // invalid lanes: a key of their own that is never counted
+  if (valid) {
+    const double *pi = rstate + (long)p * S;
+    double sn, cn, x1, y1;
+    step_position<S, SRK>(P, pi, U, sn, cn, x1, y1);
+    double xw, yw;
+    apply_pbc(P, pi[0], pi[1], xw, yw);
+    const int ox = cell_coord(xw, ncf), oy = cell_coord(yw, ncf);
+    apply_pbc(P, x1, y1, xw, yw);
+    const int nx = cell_coord(xw, ncf), ny = cell_coord(yw, ncf);
+    const int base = replica_of(L, p) * L.cpr;
+    int dxc = nx - ox, dyc = ny - oy;
+    if (dxc == ncf - 1) dxc = -1;
+    if (dxc == 1 - ncf) dxc = 1;
+    if (dyc == ncf - 1) dyc = -1;
+    if (dyc == 1 - ncf) dyc = 1;
+    if (dxc >= -1 && dxc <= 1 && dyc >= -1 && dyc <= 1) {
+      d = (dyc + 1) * 3 + (dxc + 1);
+    } else {
+      d = kFar;
+      farslot[p] = atomicAdd(farcnt + base + ny * ncf + nx, 1);
     }
+    cell = base + oy * ncf + ox;
+    mv[p] = (cell << 4) | d;
+  }
+  // runs of equal keys inside the wave (every lane takes part: no early return above)
+  const int lane = threadIdx.x & 63;
+  const int pc = __shfl_up(cell, 1, 64), pd = __shfl_up(d, 1, 64);
+  const bool head = lane == 0 || pc != cell || pd != d;
+  const unsigned long long heads = __ballot(head);
+  if (head && valid && d != kFar) {
+    const unsigned long long later = lane == 63 ? 0ull : heads >> (lane + 1);
+    const int len = later ? __ffsll((long long)later) : 64 - lane;
+    atomicAdd(cnt9 + (long)cell * 9 + d, len);
   }
 }
 

@@ -18,7 +18,7 @@
 //   self terms (v0 cos phi, v0 sin phi, omega, -xi omega + sigma A / n) in FP64.
 //
 // Fused stochastic steps keep the state resident in cell order (see spss_fast_kernels.hpp):
-//   fk_pairs (MODE 3: phasor sums) -> fr_move -> fr_cell_counts -> scan of the arrivals
+//   fk_pairs (MODE 3: phasor sums) -> fr_move (moves and their counts) -> scan of the arrivals
 //   -> fr_far_bucket -> fr_scatter (FP64 Euler/SRK update written to the new slots),
 // seven launches per step and no host synchronisation.
 #include <algorithm>
@@ -72,7 +72,7 @@ struct FastPath {
   float *romega = nullptr;
   double2 *ssum = nullptr, *tsum = nullptr;
   int *scnt = nullptr;
-  int *mv = nullptr, *cnt9 = nullptr, *newcnt = nullptr, *farcnt = nullptr;
+  int *mv = nullptr, *cnt9 = nullptr, *farcnt = nullptr;
   hipStream_t st = nullptr;
   KernelTimer *timer = nullptr;
 };
@@ -221,7 +221,6 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->mv, n));
   A(falloc(&f->cnt9, (size_t)cells * 9));
   A(falloc(&f->farcnt, cells));
-  A(falloc(&f->newcnt, cells));
   A(falloc(&f->rep_first, R + 1));
   A(falloc(&f->ovf_count, 1));
   A(falloc(&f->rep_chunk, nchunk));
@@ -247,7 +246,7 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   }
   launch_fill_int(st, cells, f->count, 0);
   launch_fill_int(st, cells, f->farcnt, 0);
-  launch_fill_int(st, cells, f->newcnt, 0);
+  launch_fill_int(st, (long)cells * 9, f->cnt9, 0);
   launch_fill_int(st, 1, f->ovf_count, 0);
   if (coef) {
     launch_fill_double(st, 3L * R, f->racc, 0.0);
@@ -266,7 +265,7 @@ void fast_destroy(FastPath *f) {
   if (!f) return;
   void *b[] = {f->key,       f->slot,      f->count,     f->start,   f->tmp,       f->sorted,    f->blocksum,
                f->snap,      f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],  f->rid[1],    f->rnat[0],
-               f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mv,
+               f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->mv,
                f->cnt9,      f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
                f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc,      f->ovf_list,  f->ovf_count};
   for (void *p : b) hipFree(p);
@@ -333,7 +332,7 @@ static void build(FastPath *f, hipStream_t st, const DevParams &P, const double 
     fk_keys<4><<<nb, 256, 0, st>>>(P, L, stage, f->ncf, f->key, f->slot, f->count);
   else
     fk_keys<3><<<nb, 256, 0, st>>>(P, L, stage, f->ncf, f->key, f->slot, f->count);
-  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum);
+  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, nullptr, 0, 1, f->cells, start, f->blocksum);
   fk_scan_add<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum, f->nsb, f->ovf_count);
   fk_scatter<<<nb, 256, 0, st>>>(n, f->key, f->slot, start, f->tmp);
   if (P.s == 4)
@@ -420,11 +419,10 @@ static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const
   const unsigned nb = nblocks(n);
   const double *rnat = f->resident_nat ? f->rnat[c] : nullptr;
   const Layout L = layout_of(f);
-  fr_move<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->rstate[c], U, f->mv, f->farcnt, f->slot);
-  fr_cell_counts<<<nblocks(f->cells), 256, 0, st>>>(f->cells, f->ncf, f->cpr, f->rstart[c], f->mv, f->cnt9,
-                                                    f->newcnt);
-  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum);
-  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum, f->nsb,
+  fr_move<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->rstate[c], U, f->mv, f->farcnt, f->slot, f->cnt9);
+  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->farcnt, nullptr, f->cnt9, f->ncf, f->cpr, f->cells, f->rstart[o],
+                                        f->blocksum);
+  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->farcnt, nullptr, f->cells, f->rstart[o], f->blocksum, f->nsb,
                                       f->ovf_count);
   fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, f->mv, f->rstate[c], U, f->rstart[o], f->cnt9,
                                            f->slot, f->tmp);
@@ -452,6 +450,7 @@ static int step_impl(FastPath *f, hipStream_t st, const DevParams &P, bool do_sr
   B.ssum = f->ssum;
   B.tsum = f->tsum;
   B.scnt = f->scnt;
+  B.cnt9 = f->cnt9;
   if (f->timer) f->timer->begin(st);
   if (do_srk)
     launch_pairs<4, true, 3>(f, st, P, B);

@@ -4,12 +4,15 @@ reference, which sweeps parameters in a host loop, SimulationEngineFirstOrder.cp
 
 * `MLP` wraps the spss_mlp_* entry points: forward, backward and Adam run as the library's own
   HIP kernels on the device.
-* `SweepScreen` maps a replica's parameters (sigma, D, m, omega spread, N, initial coherence)
-  to (time-mean |Z|, standard deviation of |Z|), simulates points with `ReplicaBatch`, trains
-  the network on them and locates the crossing of a level of mean |Z| along a sigma sweep with
-  few simulations.  The simulation is both the fallback and the judge: every point the screen
-  reports on is simulated, and a line whose simulated points disagree with the prediction by
-  more than `max_error` or do not bracket the crossing is widened until they do, up to the
+* `SweepScreen` maps a replica's parameters (sigma, D, m, v0, rho and the density N) to the
+  time-mean and the standard deviation of |Z| in the steady state, simulates points with
+  `ReplicaBatch`, and trains the network on them.  Two uses: `screen` simulates, out of many
+  points, only those where the predicted |d mean|Z| / d sigma| is large or the predicted
+  spread of |Z| exceeds a bound, and scores the network against those simulations;
+  `screened_sweep` locates the crossing of a level of mean |Z| along a sigma sweep with few
+  simulations.  The simulation is both the fallback and the judge: every point the screen
+  reports on is simulated, and a sweep whose simulated points do not bracket the crossing (or
+  disagree with the prediction by more than `max_error`) is widened until they do, up to the
   plain sweep.
 """
 import ctypes as C
@@ -19,7 +22,7 @@ import numpy as np
 from . import _capi as A
 from .batch import ReplicaBatch
 
-FEATURES = ("sigma", "D", "m", "omega_std", "log2_N", "coherence0")
+FEATURES = ("sigma", "D", "m", "v0", "rho", "log2_N")
 OUTPUTS = ("mean_R", "std_R")
 DIMS = (6, 128, 128, 128, 2)
 
@@ -98,24 +101,33 @@ class MLP:
         return n.value
 
 
-def points(sigma, D, m, omega_std, N, coherence0, seed):
-    """structured array of sweep points (broadcast over the arguments)"""
-    cols = np.broadcast_arrays(*(np.asarray(v) for v in (sigma, D, m, omega_std, N, coherence0, seed)))
-    p = np.zeros(cols[0].size, dtype=[("sigma", "f8"), ("D", "f8"), ("m", "f8"), ("omega_std", "f8"),
-                                      ("N", "i8"), ("coherence0", "f8"), ("seed", "u8")])
+def points(sigma, D, m, v0, rho, N, seed, omega_std=0.2, coherence0=1.0):
+    """structured array of sweep points (broadcast over the arguments).  By default runs start
+    from identical phases (coherence0 = 1): the steady state is then that of the synchronised
+    branch, which survives where coupling beats noise and frequency spread and decays
+    elsewhere -- a transition that shows within a few thousand steps even with local coupling,
+    where synchrony grown from random phases would need far longer runs."""
+    vals = (sigma, D, m, v0, rho, N, seed, omega_std, coherence0)
+    cols = np.broadcast_arrays(*(np.asarray(v) for v in vals))
+    p = np.zeros(cols[0].size, dtype=[("sigma", "f8"), ("D", "f8"), ("m", "f8"), ("v0", "f8"), ("rho", "f8"),
+                                      ("N", "i8"), ("seed", "u8"), ("omega_std", "f8"), ("coherence0", "f8")])
     for name, c in zip(p.dtype.names, cols):
         p[name] = c.reshape(-1)
     return p
 
 
-def random_points(n, rng, sigma=(0.0, 3.0), D=(0.0, 0.2), m=(0.2, 1.0), omega_std=(0.05, 0.3), N=(2048, 8192),
-                  coherence0=(0.0, 1.0), seed0=0x5EED1000):
-    """n points drawn uniformly from the ranges (N log-uniform)"""
-    u = rng.random((n, 6))
-    lo_hi = np.array([sigma, D, m, omega_std, (np.log2(N[0]), np.log2(N[1])), coherence0], dtype=np.float64)
-    v = lo_hi[:, 0] + u * (lo_hi[:, 1] - lo_hi[:, 0])
-    return points(v[:, 0], v[:, 1], v[:, 2], v[:, 3], np.round(2.0 ** v[:, 4]).astype(np.int64), v[:, 5],
-                  seed0 + np.arange(n, dtype=np.uint64))
+def random_points(n, rng, sigma=(0.0, 3.0), D=(0.0, 0.2), m=(0.2, 1.0), v0=(0.005, 0.01, 0.02),
+                  rho=(0.02, 0.03, 0.04), N=(2048, 8192), seed0=0x5EED1000, **fixed):
+    """n points: sigma, D, m uniform in their ranges, N log-uniform, v0 and rho drawn from the
+    given values (points of one (v0, rho) share a batch)"""
+    u = rng.random((n, 4))
+    s = sigma[0] + u[:, 0] * (sigma[1] - sigma[0])
+    d = D[0] + u[:, 1] * (D[1] - D[0])
+    mm = m[0] + u[:, 2] * (m[1] - m[0])
+    nn = np.round(2.0 ** (np.log2(N[0]) + u[:, 3] * (np.log2(N[1]) - np.log2(N[0])))).astype(np.int64)
+    vv = rng.choice(np.asarray(v0, dtype=np.float64), n)
+    rr = rng.choice(np.asarray(rho, dtype=np.float64), n)
+    return points(s, d, mm, vv, rr, nn, seed0 + np.arange(n, dtype=np.uint64), **fixed)
 
 
 def initial_state(p, model):
@@ -131,14 +143,14 @@ def initial_state(p, model):
     return st.reshape(-1), (w / p["m"] if model == 4 else w)
 
 
-def simulate(pts, steps=2000, dt=0.005, observe_every=10, observe_from=1000, v0=0.01, rho=0.03):
-    """(mean |Z|, std |Z|) per point from full simulations, all points of a model in one batch.
+def simulate(pts, steps=3000, dt=0.005, observe_every=10, observe_from=1500):
+    """(mean |Z|, std |Z|) per point from full simulations: one batch per (model, v0, rho).
     m > 0 runs model 4 with xi = 1/m, sigma/m, D/m^2, omega/m; m = 0 the first-order model 3."""
     out = np.zeros((pts.size, 2))
-    for model, sel in ((3, pts["m"] == 0), (4, pts["m"] > 0)):
-        idx = np.nonzero(sel)[0]
-        if idx.size == 0:
-            continue
+    model_of = np.where(pts["m"] > 0, 4, 3)
+    keys = sorted(set(zip(model_of.tolist(), pts["v0"].tolist(), pts["rho"].tolist())))
+    for model, v0, rho in keys:
+        idx = np.nonzero((model_of == model) & (pts["v0"] == v0) & (pts["rho"] == rho))[0]
         q = pts[idx]
         mm = np.where(q["m"] > 0, q["m"], 1.0)
         xi = 1.0 / mm if model == 4 else np.zeros(idx.size)
@@ -160,7 +172,8 @@ def simulate(pts, steps=2000, dt=0.005, observe_every=10, observe_from=1000, v0=
 class SweepScreen:
     """network screen over sweep points; see the module docstring"""
 
-    RANGES = np.array([(0.0, 3.0), (0.0, 0.2), (0.0, 1.0), (0.0, 0.3), (11.0, 13.0), (0.0, 1.0)])
+    # feature ranges mapped onto [-1, 1]: sigma, D, m, v0, rho, log2 N
+    RANGES = np.array([(0.0, 4.0), (0.0, 0.2), (0.0, 1.0), (0.0, 0.02), (0.0, 0.05), (11.0, 16.0)])
 
     def __init__(self, seed=0x5EED0005, sim_kwargs=None, net=None):
         self.net = net or MLP(DIMS, max_batch=1024, seed=seed)
@@ -168,8 +181,8 @@ class SweepScreen:
         self.simulations = 0
 
     def features(self, pts):
-        raw = np.stack([pts["sigma"], pts["D"], pts["m"], pts["omega_std"], np.log2(pts["N"].astype(np.float64)),
-                        pts["coherence0"]], axis=1)
+        raw = np.stack([pts["sigma"], pts["D"], pts["m"], pts["v0"], pts["rho"],
+                        np.log2(pts["N"].astype(np.float64))], axis=1)
         lo, hi = self.RANGES[:, 0], self.RANGES[:, 1]
         return (2.0 * (raw - lo) / (hi - lo) - 1.0).astype(np.float32)
 
@@ -188,6 +201,26 @@ class SweepScreen:
 
     def predict(self, pts):
         return self.net.predict(self.features(pts)).astype(np.float64)
+
+    def slope(self, pts, h=0.02):
+        """d mean|Z| / d sigma of the network (central difference)"""
+        up, dn = pts.copy(), pts.copy()
+        up["sigma"] += h
+        dn["sigma"] -= h
+        return (self.predict(up)[:, 0] - self.predict(dn)[:, 0]) / (2.0 * h)
+
+    def screen(self, pts, slope_min=0.5, spread_max=0.05, simulate=True):
+        """predict every point; simulate only those with |d mean|Z| / d sigma| > slope_min or a
+        predicted spread of |Z| > spread_max.  Returns a dict: the prediction for all points, the
+        indices selected, their simulated values and the network's error on them."""
+        pred = self.predict(pts)
+        sel = np.nonzero((np.abs(self.slope(pts)) > slope_min) | (pred[:, 1] > spread_max))[0]
+        out = {"predicted": pred, "selected": sel}
+        if simulate and sel.size:
+            sim = self.simulate(pts[sel])
+            out["simulated"] = sim
+            out["error"] = np.abs(pred[sel] - sim)
+        return out
 
     @staticmethod
     def crossing(sigma, R, level):

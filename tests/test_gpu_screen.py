@@ -14,7 +14,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SIM = dict(steps=2000, dt=0.005, observe_every=10, observe_from=1000)
+SIM = dict(steps=3000, dt=0.005, observe_every=10, observe_from=1500)
 
 
 @pytest.fixture(scope="module")
@@ -48,16 +48,17 @@ def test_screened_sweep_finds_the_plain_crossing_with_fewer_simulations(trained)
     sig = np.linspace(0.0, 3.0, G)
     plain_sims = screened_sims = 0
     for k in range(4):
-        D, m, s = rng.uniform(0.0, 0.15), rng.uniform(0.3, 1.0), rng.uniform(0.05, 0.25)
+        D, m = rng.uniform(0.0, 0.15), rng.uniform(0.3, 1.0)
+        v0, rho = rng.choice([0.005, 0.01, 0.02]), rng.choice([0.02, 0.03, 0.04])
         N = int(2 ** rng.uniform(11.2, 12.8))
-        line = S.points(sig, D, m, s, N, 0.0, 0x5EED4000 + 100 * k + np.arange(G, dtype=np.uint64))
+        line = S.points(sig, D, m, v0, rho, N, 0x5EED4000 + 100 * k + np.arange(G, dtype=np.uint64))
         before = sc.simulations
         x_plain, R = sc.plain_sweep(line)
         plain_sims += sc.simulations - before
         before = sc.simulations
         x_scr, idx, R_scr, err = sc.screened_sweep(line, window=6)
         screened_sims += sc.simulations - before
-        print(f"line {k}: D={D:.3f} m={m:.2f} s={s:.3f} N={N}: plain crossing {x_plain}, screened {x_scr} "
+        print(f"line {k}: D={D:.3f} m={m:.2f} v0={v0} rho={rho} N={N}: plain crossing {x_plain}, screened {x_scr} "
               f"from {idx.size} points (largest prediction error {err:.3f})")
         np.testing.assert_array_equal(R_scr, R[idx])  # same seeds, same simulations
         if x_plain is None:
@@ -66,3 +67,20 @@ def test_screened_sweep_finds_the_plain_crossing_with_fewer_simulations(trained)
             assert x_scr is not None and abs(x_scr - x_plain) <= sig[1] - sig[0] + 1e-12
     print(f"simulations: plain {plain_sims}, screened {screened_sims} ({screened_sims / plain_sims:.2f})")
     assert screened_sims <= 0.4 * plain_sims
+
+
+def test_screen_selects_the_transition_points(trained):
+    """out of random points, the screen simulates a minority and still covers most of the points
+    the plain method (simulate everything) finds in the transition region"""
+    sc, S, _ = trained
+    pts = S.random_points(256, np.random.default_rng(123), seed0=0x5EED5000)
+    plain = sc.simulate(pts)
+    out = sc.screen(pts, slope_min=0.3, spread_max=0.05, simulate=False)
+    sel = out["selected"]
+    near = np.nonzero(np.abs(plain[:, 0] - 0.5) < 0.2)[0]
+    recall = np.isin(near, sel).mean() if near.size else 1.0
+    err = np.abs(out["predicted"][:, 0] - plain[:, 0])
+    print(f"screen: {sel.size} of {pts.size} simulated; {near.size} transition points, recall {recall:.2f}; "
+          f"network error median {np.median(err):.3f}, on selected {np.median(err[sel]) if sel.size else 0:.3f}")
+    assert sel.size <= 0.6 * pts.size
+    assert recall >= 0.7

@@ -10,9 +10,9 @@ class _Net:
     def __init__(self, shift):
         self.shift = shift
 
-    def predict(self, x):  # x[:, 0] is sigma mapped to [-1, 1] over [0, 3]
-        sig = (x[:, 0].astype(np.float64) + 1.0) * 1.5
-        return np.stack([_curve(sig - self.shift), np.zeros_like(sig)], axis=1).astype(np.float32)
+    def predict(self, x):  # x[:, 0] is sigma mapped to [-1, 1] over [0, 4]
+        sig = (x[:, 0].astype(np.float64) + 1.0) * 2.0
+        return np.stack([_curve(sig - self.shift), np.full_like(sig, 0.01)], axis=1).astype(np.float32)
 
 
 def _curve(sig, xc=1.3):
@@ -30,7 +30,7 @@ class _Screen(S.SweepScreen):
 
 
 def _line(G=40):
-    return S.points(np.linspace(0, 3, G), 0.05, 0.5, 0.1, 4096, 0.0, np.arange(G, dtype=np.uint64))
+    return S.points(np.linspace(0, 3, G), 0.05, 0.5, 0.01, 0.03, 4096, np.arange(G, dtype=np.uint64))
 
 
 @pytest.mark.parametrize("shift", [0.0, 0.3, -0.5, 1.2])
@@ -78,14 +78,14 @@ def test_crossing_interpolates_and_handles_the_edges():
 
 def test_features_map_the_ranges_to_unit_interval():
     sc = _Screen(0.0, _curve)
-    p = S.points([0.0, 3.0], [0.0, 0.2], [0.0, 1.0], [0.0, 0.3], [2048, 8192], [0.0, 1.0], [1, 2])
+    p = S.points([0.0, 4.0], [0.0, 0.2], [0.0, 1.0], [0.0, 0.02], [0.0, 0.05], [2048, 65536], [1, 2])
     f = sc.features(p)
     np.testing.assert_allclose(f[0], -1.0)
     np.testing.assert_allclose(f[1], 1.0)
 
 
 def test_initial_state_depends_only_on_the_point():
-    p = S.points(1.0, 0.1, 0.5, 0.2, 2048, 0.5, 123)[0]
+    p = S.points(1.0, 0.1, 0.5, 0.01, 0.03, 2048, 123, coherence0=0.5)[0]
     a, wa = S.initial_state(p, 4)
     b, wb = S.initial_state(p, 4)
     np.testing.assert_array_equal(a, b)
@@ -94,3 +94,24 @@ def test_initial_state_depends_only_on_the_point():
     assert np.all(st[:, 2] < np.pi + 1e-12) and np.all(st[:, 3] == 0.0)
     R = abs(np.mean(np.exp(1j * st[:, 2])))
     assert abs(R - 2 / np.pi) < 0.05  # phases U[0, pi): |<e^{i phi}>| = 2/pi
+
+
+def test_screen_simulates_where_the_slope_is_steep():
+    line = _line(80)
+    sc = _Screen(0.0, _curve)
+    out = sc.screen(line, slope_min=0.5, spread_max=0.05)
+    sig = line["sigma"][out["selected"]]
+    slope = _curve(line["sigma"]) * (1 - _curve(line["sigma"])) / 0.1  # exact derivative of the curve
+    want = np.nonzero(slope > 0.5)[0]
+    assert abs(out["selected"].size - want.size) <= 2 and np.all(np.abs(sig - 1.3) < 0.6)
+    assert sc.simulations == out["selected"].size
+    np.testing.assert_allclose(out["error"][:, 0], 0.0, atol=1e-6)
+    sc2 = _Screen(0.0, _curve)
+    out2 = sc2.screen(line, slope_min=0.5, spread_max=0.005)  # every predicted spread 0.01 > 0.005
+    assert out2["selected"].size == line.size
+
+
+def test_random_points_cover_the_ranges():
+    p = S.random_points(2000, np.random.default_rng(1))
+    assert p["sigma"].min() >= 0 and p["sigma"].max() <= 3 and set(np.unique(p["rho"])) == {0.02, 0.03, 0.04}
+    assert p["N"].min() >= 2048 and p["N"].max() <= 8192 and len(np.unique(p["seed"])) == 2000
