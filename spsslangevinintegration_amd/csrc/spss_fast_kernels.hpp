@@ -9,7 +9,7 @@
 //    natural frequencies stay in cell order between steps.  The pair kernel writes the
 //    phasor sums of every particle in the old order; fr_move decides each particle's
 //    move (its new position depends on phase and omega only) and counts the moves per
-//    (cell, direction), a two-level scan of the arrival counts (gathered from those counts)
+//    (cell, direction) and the arrivals per cell, a two-level scan of the arrival counts
 //    gives the new cell starts, and fr_scatter
 //    applies the step and writes every particle to its new slot: inside a cell, particles that
 //    stayed come first in their previous order, then arrivals grouped by the direction
@@ -37,7 +37,9 @@ constexpr int kScanItems = 8;   // counts per thread in the two-level scans
 struct FastArgs {
   int n, ncf, k, ntiles, cpr;  // cpr: cells per replica (ncf * ncf)
   const int4 *tiles;           // per tile: first cell of its replica, row, first column, end column
-  int *ovf_list;               // tiles whose staged cells overflow the LDS stage (fk_pairs_ovf runs them)
+  int4 *ovf_list;              // work of tiles whose staged cells overflow the LDS stage, for fk_pairs_ovf:
+                               // {first cell of the replica, row, ca, cb} a column span that fits, or
+                               // {.., .., column, -1 - j} particles j*kFB.. of a cell that does not
   int *ovf_count;              // their number, zeroed by fk_scan_add
   double H;
   float r2lo, r2hi;
@@ -132,12 +134,10 @@ __global__ void fk_keys(DevParams P, Layout L, const double *__restrict__ stage,
   slot[i] = atomicAdd(count + c, 1);
 }
 
-// block-local exclusive scan of 2048 values per block; a value is count[c] + count2[c] (count2
-// may be null), plus, when cnt9 is given, the near arrivals of cell c gathered from the move
-// counts of its nine source cells
+// block-local exclusive scan of 2048 values per block; a value is count[c] + count2[c]
+// (count2 may be null)
 __global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ count, const int *__restrict__ count2,
-                                                     const int *__restrict__ cnt9, int ncf, int cpr, int m,
-                                                     int *__restrict__ start, int *__restrict__ blocksum) {
+                                                     int m, int *__restrict__ start, int *__restrict__ blocksum) {
   __shared__ int sh[256];
   const int base = blockIdx.x * 256 * kScanItems + threadIdx.x * kScanItems;
   int v[kScanItems];
@@ -148,7 +148,6 @@ __global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ cou
     if (base + q < m) {
       v[q] = count[base + q];
       if (count2) v[q] += count2[base + q];
-      if (cnt9) v[q] += new_count_of(cnt9, base + q, ncf, cpr);
     }
     s += v[q];
   }
@@ -389,6 +388,37 @@ __device__ __forceinline__ void flush_row(const float *ax, const float *ay, cons
   }
 }
 
+// The work of a span of cells whose staged cells overflow the LDS stage, appended to the overflow
+// list: the widest column spans that fit, and for a column that does not fit by itself its
+// particles in chunks of kFB.  ccnt holds the particles of every staged cell (row-major over the
+// 2k+1 rows and the W+2k staged columns); colsum is LDS scratch for W+2k column sums.
+__device__ __forceinline__ void split_span(const FastArgs &A, int cbase, int row, int ca, int cb, const int *ccnt,
+                                           int *colsum) {
+  const int k = A.k, nrows = 2 * k + 1, ncols = cb - ca + 2 * k;
+  __syncthreads();  // every thread has read the staged total that colsum overwrites
+  for (int q = threadIdx.x; q < ncols; q += kFB) {
+    int sum = 0;
+    for (int r = 0; r < nrows; ++r) sum += ccnt[r * ncols + q];
+    colsum[q] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int a = ca; a < cb;) {
+    int tot = nrows * kPad;
+    for (int q = a - ca; q <= a - ca + 2 * k; ++q) tot += colsum[q];
+    int b = a + 1;
+    if (tot <= kStageMax) {
+      while (b < cb && tot + colsum[b - ca + 2 * k] <= kStageMax) tot += colsum[b++ - ca + 2 * k];
+      A.ovf_list[atomicAdd(A.ovf_count, 1)] = make_int4(cbase, row, a, b);
+    } else {
+      const int nch = (ccnt[k * ncols + (a - ca) + k] + kFB - 1) / kFB;
+      const int at = atomicAdd(A.ovf_count, nch);
+      for (int j = 0; j < nch; ++j) A.ovf_list[at + j] = make_int4(cbase, row, a, -1 - j);
+    }
+    a = b;
+  }
+}
+
 // MODE 0: RHS; 1: count neighbours; 2: write neighbour lists; 3: phasor sums of the resident
 // state for the stochastic step
 //
@@ -399,7 +429,8 @@ __device__ __forceinline__ void flush_row(const float *ax, const float *ay, cons
 // stencil four candidates at a time without per-candidate bounds: reads past the end of a
 // range land in farther cells of the same staged row (out of range, see below) or in the
 // padding, so the loop body has no branch on the common path.  A tile whose cells do not fit
-// the stage is appended to an overflow list for fk_pairs_ovf.
+// the stage is split into smaller work for fk_pairs_ovf (split_span).  pairs_span runs the cells
+// [ca, cb) of one row; it returns false when their staged cells do not fit.
 //
 // Range test: an FP32 distance below r2lo is in range, above r2hi out of range; the
 // guard band between them (rare) is decided by the reference's FP64 ClassC formula, and
@@ -415,16 +446,14 @@ __device__ __forceinline__ void flush_row(const float *ax, const float *ay, cons
 // candidates exactly once (distinct stencil columns since int(1/rho) >= 3) and is removed
 // afterwards.  A particle's candidates, their order and grouping depend on its own stencil
 // cells only (not on the tile), so every tiling and the overflow pass give identical sums.
-template <int S, bool SRK, int MODE>
-__device__ __forceinline__ void pairs_tile(const DevParams &P, const FastArgs &A, const PairBufs &B, int tile,
-                                           PairLds<S, SRK, MODE> &L) {
+template <int S, bool SRK, int MODE, bool SPLIT>
+__device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A, const PairBufs &B, int cbase,
+                                           int row, int ca, int cb, PairLds<S, SRK, MODE> &L) {
   constexpr bool RES = MODE >= 3;
   constexpr bool SUMS = MODE == 0 || MODE >= 3;
   constexpr bool LIST = MODE == 1 || MODE == 2;
   float4 *sdat = L.sdat;
   int *cglob = L.cglob, *ccnt = L.ccnt, *cpos = L.cpos;
-  const int4 td = A.tiles[tile];
-  const int cbase = td.x, row = td.y, ca = td.z, cb = td.w;
   const int W = cb - ca;
   const int k = A.k;
   const int nrows = 2 * k + 1;
@@ -433,11 +462,6 @@ __device__ __forceinline__ void pairs_tile(const DevParams &P, const FastArgs &A
   const double H = A.H;
   const int *start = B.start;
   const int ncells = nrows * ncols;
-
-  if (MODE == 3) {  // this step's move counts of the tile's cells start from zero
-    int *z = B.cnt9 + (long)(cbase + row * ncf + ca) * 9;
-    for (int e = threadIdx.x; e < 9 * W; e += kFB) z[e] = 0;
-  }
 
   // ---- staging plan: global extent of every staged cell, then the LDS layout
   for (int e = threadIdx.x; e < ncells; e += kFB) {
@@ -474,8 +498,8 @@ __device__ __forceinline__ void pairs_tile(const DevParams &P, const FastArgs &A
   __syncthreads();
   const int total = cpos[ncells];
   if (total > kStageMax) {  // uniform: every thread reads the same total
-    if (threadIdx.x == 0) A.ovf_list[atomicAdd(A.ovf_count, 1)] = tile;
-    return;
+    if (SPLIT) split_span(A, cbase, row, ca, cb, ccnt, cpos);
+    return false;
   }
 
   // staged entry e belongs to the last cell whose LDS start is <= e
@@ -561,6 +585,7 @@ __device__ __forceinline__ void pairs_tile(const DevParams &P, const FastArgs &A
     }
     finish_particle<S, SRK, MODE>(P, B, i, gp, pi, sx, sy, tx, ty, nnb);
   }
+  return true;
 }
 
 template <int S, bool SRK, int MODE>
@@ -570,7 +595,13 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
   const int per = gridDim.x / 8;  // the grid is a multiple of 8
   const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (tile >= A.ntiles) return;
-  pairs_tile<S, SRK, MODE>(P, A, B, tile, L);
+  const int4 td = A.tiles[tile];
+  pairs_span<S, SRK, MODE, true>(P, A, B, td.x, td.y, td.z, td.w, L);  // overflow: split for fk_pairs_ovf
+  if (MODE == 3) {  // this step's move counts of the tile's cells start from zero (stored last: loads
+                    // and stores share one wait counter, so stores up front would delay the staging)
+    int *z = B.cnt9 + (long)(td.x + td.y * A.ncf + td.z) * 9;
+    for (int e = threadIdx.x; e < 9 * (td.w - td.z); e += kFB) z[e] = 0;
+  }
 }
 
 template <bool SRK>
@@ -581,110 +612,122 @@ struct OvfLds {
   int cglob[kMaxRows], ccnt[kMaxRows];
 };
 
-// The tiles fk_pairs left in the overflow list (dense clusters), a grid-stride loop over them.
-// A tile is handled one cell at a time: the particles of the cell (up to kFB per pass) share one
-// stencil, so each stencil row is staged by itself in chunks of kStageMax - kPad entries that
-// start at multiples of 4 of the row range, which keeps every lane's candidate order and 4-slot
-// grouping identical to fk_pairs.
+// A cell whose own stencil does not fit the LDS stage (a dense cluster): its particles
+// chunk*kFB .. (chunk+1)*kFB - 1 share one stencil; each stencil row is staged by itself in chunks of kStageMax - kPad
+// entries that start at multiples of 4 of the row range, which keeps every lane's candidate
+// order and 4-slot grouping identical to fk_pairs.
 template <int S, bool SRK, int MODE>
-__global__ __launch_bounds__(kFB) void fk_pairs_ovf(DevParams P, FastArgs A, PairBufs B) {
+__device__ __forceinline__ void pairs_dense_cell(const DevParams &P, const FastArgs &A, const PairBufs &B,
+                                                 int cbase, int row, int q, int chunk, OvfLds<SRK> &L) {
   constexpr bool RES = MODE >= 3;
   constexpr bool SUMS = MODE == 0 || MODE >= 3;
   constexpr int kChunk = kStageMax - kPad;
-  __shared__ OvfLds<SRK> L;
-  const int cnt = *A.ovf_count;
   const int k = A.k, nrows = 2 * k + 1, ncf = A.ncf;
   const double H = A.H;
   const float r2lo = A.r2lo, r2hi = A.r2hi;
   const int *start = B.start;
-  for (int it = blockIdx.x; it < cnt; it += gridDim.x) {
-    const int4 td = A.tiles[A.ovf_list[it]];
-    const int cbase = td.x, row = td.y;
-    for (int q = td.z; q < td.w; ++q) {
-      const int cown = cbase + row * ncf + q;
-      const int gp0 = start[cown], gp1 = start[cown + 1];
-      for (int pc = gp0; pc < gp1; pc += kFB) {
-        const int gp = pc + threadIdx.x;
-        const bool act = gp < gp1;
-        const int i = act ? (RES ? B.rid[gp] : B.sorted[gp]) : 0;
-        const double *pi = act ? (RES ? B.rstate + (long)gp * S : B.stage + (long)i * S) : nullptr;
-        const float4 me = act ? B.snap[gp] : make_float4(0.f, 0.f, 0.f, 0.f);
-        // x re-based to the window's column 0 (the particle's own column is k)
-        const float xi_t = (float)((double)k * H + (double)me.x);
-        const float yi = me.y, ci = me.z, si = me.w;
-        double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
-        int nnb = 0;
-        int w = 0;
-        const int wbase = (MODE == 2 && act) ? B.pstart[i] : 0;
-        int self_t = -1;
-        for (int r = 0; r < nrows; ++r) {
-          __syncthreads();  // the previous row's stage is consumed
-          if (threadIdx.x < nrows) {
-            const int dc = threadIdx.x;
-            const int rrow = ((row + r - k) % ncf + ncf) % ncf;
-            const int col = ((q - k + dc) % ncf + ncf) % ncf;
-            const int c = cbase + rrow * ncf + col;
-            L.cglob[dc] = start[c];
-            L.ccnt[dc] = start[c + 1] - start[c];
-          }
-          __syncthreads();
-          int len = 0;
-          for (int dc = 0; dc < nrows; ++dc) len += L.ccnt[dc];
-          const float syi = (float)((double)yi - (double)(r - k) * H);
-          float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an[4] = {0.f, 0.f, 0.f, 0.f};
-          float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
-          for (int c0 = 0; c0 < len; c0 += kChunk) {
-            const int clen = min(kChunk, len - c0);
-            if (c0 > 0) __syncthreads();
-            for (int e = threadIdx.x; e < clen + kPad; e += kFB) {
-              if (e < clen) {
-                const int t = c0 + e;
-                int dc = 0, base = 0;
-                while (t >= base + L.ccnt[dc]) base += L.ccnt[dc++];
-                const int g = L.cglob[dc] + (t - base);
-                const float4 v = B.snap[g];
-                L.sdat[e] = make_float4((float)((double)dc * H + (double)v.x), v.y, v.z, v.w);
-                if (SRK) L.somega[e] = B.omega_f[g];
-                L.sg[e] = g;
-              } else {
-                L.sdat[e] = make_float4(1e30f, 0.f, 0.f, 0.f);
-                if (SRK) L.somega[e] = 0.f;
-                L.sg[e] = -1;
-              }
-            }
-            __syncthreads();
-            if (act) {
-              int st = -1;  // the particle itself, in its own row
-              if (r == k) {
-                const int s0 = L.cglob[k] <= gp && gp < L.cglob[k] + L.ccnt[k] ? gp - L.cglob[k] : -1;
-                int base = 0;
-                for (int dc = 0; dc < k; ++dc) base += L.ccnt[dc];
-                if (s0 >= 0 && base + s0 >= c0 && base + s0 < c0 + clen) st = base + s0 - c0;
-              }
-              if (st >= 0) self_t = st;
-              auto gof = [&](int t) -> int { return L.sg[t]; };
-              scan_range<S, SRK, MODE>(P, B, L.sdat, L.somega, 0, clen, xi_t, syi, r2lo, r2hi, pi, st, gof, ax,
-                                       ay, an, bx, by, nnb, w, wbase);
-            }
-          }
-          if (act && SUMS) flush_row<SRK>(ax, ay, an, bx, by, sx, sy, tx, ty, nnb);
-        }
-        if (act) {
-          if (SUMS) {  // remove the self term
-            --nnb;
-            sx -= (double)ci;
-            sy -= (double)si;
-            if (SRK) {
-              const float omi = B.omega_f[gp];
-              tx -= (double)(omi * ci);
-              ty -= (double)(omi * si);
-            }
-          }
-          finish_particle<S, SRK, MODE>(P, B, i, gp, pi, sx, sy, tx, ty, nnb);
-        }
-        (void)self_t;
+  const int cown = cbase + row * ncf + q;
+  const int gp1 = start[cown + 1];
+  {
+    const int pc = start[cown] + chunk * kFB;
+    const int gp = pc + threadIdx.x;
+    const bool act = gp < gp1;
+    const int i = act ? (RES ? B.rid[gp] : B.sorted[gp]) : 0;
+    const double *pi = act ? (RES ? B.rstate + (long)gp * S : B.stage + (long)i * S) : nullptr;
+    const float4 me = act ? B.snap[gp] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // x re-based to the window's column 0 (the particle's own column is k)
+    const float xi_t = (float)((double)k * H + (double)me.x);
+    const float yi = me.y, ci = me.z, si = me.w;
+    double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
+    int nnb = 0;
+    int w = 0;
+    const int wbase = (MODE == 2 && act) ? B.pstart[i] : 0;
+    for (int r = 0; r < nrows; ++r) {
+      __syncthreads();  // the previous row's stage is consumed
+      if (threadIdx.x < nrows) {
+        const int dc = threadIdx.x;
+        const int rrow = ((row + r - k) % ncf + ncf) % ncf;
+        const int col = ((q - k + dc) % ncf + ncf) % ncf;
+        const int c = cbase + rrow * ncf + col;
+        L.cglob[dc] = start[c];
+        L.ccnt[dc] = start[c + 1] - start[c];
       }
+      __syncthreads();
+      int len = 0;
+      for (int dc = 0; dc < nrows; ++dc) len += L.ccnt[dc];
+      const float syi = (float)((double)yi - (double)(r - k) * H);
+      float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an[4] = {0.f, 0.f, 0.f, 0.f};
+      float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
+      for (int c0 = 0; c0 < len; c0 += kChunk) {
+        const int clen = min(kChunk, len - c0);
+        if (c0 > 0) __syncthreads();
+        for (int e = threadIdx.x; e < clen + kPad; e += kFB) {
+          if (e < clen) {
+            const int t = c0 + e;
+            int dc = 0, base = 0;
+            while (t >= base + L.ccnt[dc]) base += L.ccnt[dc++];
+            const int g = L.cglob[dc] + (t - base);
+            const float4 v = B.snap[g];
+            L.sdat[e] = make_float4((float)((double)dc * H + (double)v.x), v.y, v.z, v.w);
+            if (SRK) L.somega[e] = B.omega_f[g];
+            L.sg[e] = g;
+          } else {
+            L.sdat[e] = make_float4(1e30f, 0.f, 0.f, 0.f);
+            if (SRK) L.somega[e] = 0.f;
+            L.sg[e] = -1;
+          }
+        }
+        __syncthreads();
+        if (act) {
+          int st = -1;  // the particle itself, in its own row
+          if (r == k) {
+            const int s0 = L.cglob[k] <= gp && gp < L.cglob[k] + L.ccnt[k] ? gp - L.cglob[k] : -1;
+            int base = 0;
+            for (int dc = 0; dc < k; ++dc) base += L.ccnt[dc];
+            if (s0 >= 0 && base + s0 >= c0 && base + s0 < c0 + clen) st = base + s0 - c0;
+          }
+          auto gof = [&](int t) -> int { return L.sg[t]; };
+          scan_range<S, SRK, MODE>(P, B, L.sdat, L.somega, 0, clen, xi_t, syi, r2lo, r2hi, pi, st, gof, ax,
+                                   ay, an, bx, by, nnb, w, wbase);
+        }
+      }
+      if (act && SUMS) flush_row<SRK>(ax, ay, an, bx, by, sx, sy, tx, ty, nnb);
     }
+    if (act) {
+      if (SUMS) {  // remove the self term
+        --nnb;
+        sx -= (double)ci;
+        sy -= (double)si;
+        if (SRK) {
+          const float omi = B.omega_f[gp];
+          tx -= (double)(omi * ci);
+          ty -= (double)(omi * si);
+        }
+      }
+      finish_particle<S, SRK, MODE>(P, B, i, gp, pi, sx, sy, tx, ty, nnb);
+    }
+  }
+}
+
+// The work fk_pairs split off overflowing tiles (dense clusters), a grid-stride loop over the
+// items: column spans that fit the stage run like fk_pairs, chunks of dense cells through
+// pairs_dense_cell.
+template <int S, bool SRK, int MODE>
+__global__ __launch_bounds__(kFB) void fk_pairs_ovf(DevParams P, FastArgs A, PairBufs B) {
+  // the span stage and the dense-cell stage are never live together: one LDS buffer holds either
+  constexpr size_t kLds = sizeof(PairLds<S, SRK, MODE>) > sizeof(OvfLds<SRK>) ? sizeof(PairLds<S, SRK, MODE>)
+                                                                               : sizeof(OvfLds<SRK>);
+  __shared__ __attribute__((aligned(16))) unsigned char lds[kLds];
+  PairLds<S, SRK, MODE> &Ls = *reinterpret_cast<PairLds<S, SRK, MODE> *>(lds);
+  OvfLds<SRK> &Lc = *reinterpret_cast<OvfLds<SRK> *>(lds);
+  const int cnt = *A.ovf_count;
+  for (int it = blockIdx.x; it < cnt; it += gridDim.x) {
+    const int4 w = A.ovf_list[it];
+    __syncthreads();  // the previous item's LDS is consumed
+    if (w.w >= 0)
+      pairs_span<S, SRK, MODE, false>(P, A, B, w.x, w.y, w.z, w.w, Ls);
+    else
+      pairs_dense_cell<S, SRK, MODE>(P, A, B, w.x, w.y, w.z, -1 - w.w, Lc);
   }
 }
 
@@ -767,17 +810,17 @@ __device__ __forceinline__ int cell_of_position(const DevParams &P, double x, do
 
 // the move of every resident particle this step: old cell * 16 + (dy+1)*3 + (dx+1), or kFar
 // with an unordered arrival slot in farcnt of the destination.  The near moves are counted per
-// (old cell, direction) into cnt9 (zeroed by the pair kernel of this step): particles lie in
-// cell order, so a wave's equal (cell, direction) keys form runs and one lane per run adds the
-// run's length.
+// (old cell, direction) into cnt9 (zeroed by the pair kernel of this step) and per destination
+// into newcnt (zeroed by the scan): particles lie in cell order, so a wave's equal (cell,
+// direction) keys form runs and one lane per run adds the run's length.
 template <int S, bool SRK>
 __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int ncf,
                                                const double *__restrict__ rstate, StepArgs U, int *__restrict__ mv,
                                                int *__restrict__ farcnt, int *__restrict__ farslot,
-                                               int *__restrict__ cnt9) {
+                                               int *__restrict__ cnt9, int *__restrict__ newcnt) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = p < n;
-  int cell = -1, d = kFar;  // invalid lanes: a key of their own that is never counted
+  int cell = -1, d = kFar, dest = 0;  // invalid lanes: a key of their own that is never counted
   if (valid) {
     const double *pi = rstate + (long)p * S;
     double sn, cn, x1, y1;
@@ -800,6 +843,7 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int
       farslot[p] = atomicAdd(farcnt + base + ny * ncf + nx, 1);
     }
     cell = base + oy * ncf + ox;
+    dest = base + ny * ncf + nx;
     mv[p] = (cell << 4) | d;
   }
   // runs of equal keys inside the wave (every lane takes part: no early return above)
@@ -811,6 +855,7 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int
     const unsigned long long later = lane == 63 ? 0ull : heads >> (lane + 1);
     const int len = later ? __ffsll((long long)later) : 64 - lane;
     atomicAdd(cnt9 + (long)cell * 9 + d, len);
+    atomicAdd(newcnt + dest, len);
   }
 }
 

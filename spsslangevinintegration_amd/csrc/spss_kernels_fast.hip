@@ -49,7 +49,8 @@ struct FastPath {
   std::vector<int> rep_first_h, W_h;
   int *rep_first = nullptr, *rep_chunk = nullptr;
   int4 *tiles = nullptr;
-  int *ovf_list = nullptr, *ovf_count = nullptr;  // tiles left to the overflow pass
+  int4 *ovf_list = nullptr;  // work split off overflowing tiles
+  int *ovf_count = nullptr;
   int ntiles = 0, tiles_cap = 0;
   Coef *coef = nullptr;                    // per-replica coefficients (batches)
   unsigned long long *step_dev = nullptr;  // device step counter (batches)
@@ -72,7 +73,7 @@ struct FastPath {
   float *romega = nullptr;
   double2 *ssum = nullptr, *tsum = nullptr;
   int *scnt = nullptr;
-  int *mv = nullptr, *cnt9 = nullptr, *farcnt = nullptr;
+  int *mv = nullptr, *cnt9 = nullptr, *newcnt = nullptr, *farcnt = nullptr;
   hipStream_t st = nullptr;
   KernelTimer *timer = nullptr;
 };
@@ -122,11 +123,8 @@ static int set_tiles(FastPath *f, const std::vector<int> &W) {
   }
   if ((int)t.size() > f->tiles_cap) {
     hipFree(f->tiles);
-    hipFree(f->ovf_list);
     f->tiles = nullptr;
-    f->ovf_list = nullptr;
     FCHK(hipMalloc(&f->tiles, sizeof(int4) * t.size()));
-    FCHK(hipMalloc(&f->ovf_list, sizeof(int) * t.size()));
     f->tiles_cap = (int)t.size();
   }
   FCHK(hipMemcpy(f->tiles, t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice));
@@ -221,8 +219,11 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->mv, n));
   A(falloc(&f->cnt9, (size_t)cells * 9));
   A(falloc(&f->farcnt, cells));
+  A(falloc(&f->newcnt, cells));
   A(falloc(&f->rep_first, R + 1));
   A(falloc(&f->ovf_count, 1));
+  // overflow work: at most one span per cell, or one item per cell and kFB particles
+  A(falloc(&f->ovf_list, (size_t)cells + n / kFB + 1));
   A(falloc(&f->rep_chunk, nchunk));
   if (coef) {
     A(falloc(&f->coef, R));
@@ -247,6 +248,7 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   launch_fill_int(st, cells, f->count, 0);
   launch_fill_int(st, cells, f->farcnt, 0);
   launch_fill_int(st, (long)cells * 9, f->cnt9, 0);
+  launch_fill_int(st, cells, f->newcnt, 0);
   launch_fill_int(st, 1, f->ovf_count, 0);
   if (coef) {
     launch_fill_double(st, 3L * R, f->racc, 0.0);
@@ -265,7 +267,7 @@ void fast_destroy(FastPath *f) {
   if (!f) return;
   void *b[] = {f->key,       f->slot,      f->count,     f->start,   f->tmp,       f->sorted,    f->blocksum,
                f->snap,      f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],  f->rid[1],    f->rnat[0],
-               f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->mv,
+               f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mv,
                f->cnt9,      f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
                f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc,      f->ovf_list,  f->ovf_count};
   for (void *p : b) hipFree(p);
@@ -332,7 +334,7 @@ static void build(FastPath *f, hipStream_t st, const DevParams &P, const double 
     fk_keys<4><<<nb, 256, 0, st>>>(P, L, stage, f->ncf, f->key, f->slot, f->count);
   else
     fk_keys<3><<<nb, 256, 0, st>>>(P, L, stage, f->ncf, f->key, f->slot, f->count);
-  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, nullptr, 0, 1, f->cells, start, f->blocksum);
+  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum);
   fk_scan_add<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum, f->nsb, f->ovf_count);
   fk_scatter<<<nb, 256, 0, st>>>(n, f->key, f->slot, start, f->tmp);
   if (P.s == 4)
@@ -356,7 +358,7 @@ template <int S, bool SRK, int MODE>
 static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, const PairBufs &B) {
   const FastArgs a = args_of(f);
   fk_pairs<S, SRK, MODE><<<tile_grid(f->ntiles), kFB, 0, st>>>(P, a, B);
-  fk_pairs_ovf<S, SRK, MODE><<<std::min(f->ntiles, 1024), kFB, 0, st>>>(P, a, B);
+  fk_pairs_ovf<S, SRK, MODE><<<1024, kFB, 0, st>>>(P, a, B);
 }
 
 int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const double *nat,
@@ -419,10 +421,10 @@ static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const
   const unsigned nb = nblocks(n);
   const double *rnat = f->resident_nat ? f->rnat[c] : nullptr;
   const Layout L = layout_of(f);
-  fr_move<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->rstate[c], U, f->mv, f->farcnt, f->slot, f->cnt9);
-  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->farcnt, nullptr, f->cnt9, f->ncf, f->cpr, f->cells, f->rstart[o],
-                                        f->blocksum);
-  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->farcnt, nullptr, f->cells, f->rstart[o], f->blocksum, f->nsb,
+  fr_move<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->rstate[c], U, f->mv, f->farcnt, f->slot, f->cnt9,
+                                      f->newcnt);
+  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum);
+  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum, f->nsb,
                                       f->ovf_count);
   fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, f->mv, f->rstate[c], U, f->rstart[o], f->cnt9,
                                            f->slot, f->tmp);
