@@ -82,7 +82,6 @@ struct PairBufs {
   double2 *ssum;         // MODE 3: phasor sum S of each sorted position
   double2 *tsum;         // MODE 3, SRK: omega-weighted phasor sum T
   int *scnt;             // MODE 3: neighbour count
-  int *cnt9;             // MODE 3: move counts per (cell, direction), zeroed here for fr_move
 };
 
 // the stochastic update of fk_update (Euler or SRK)
@@ -136,10 +135,10 @@ __global__ void fk_keys(DevParams P, Layout L, const double *__restrict__ stage,
 
 // block-local exclusive scan of 2048 values per block; a value is count[c] + count2[c]
 // (count2 may be null)
-__global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ count, const int *__restrict__ count2,
-                                                     int m, int *__restrict__ start, int *__restrict__ blocksum) {
+__device__ __forceinline__ void scan_local_block(const int *__restrict__ count, const int *__restrict__ count2, int m,
+                                                 int *__restrict__ start, int *__restrict__ blocksum, int blk) {
   __shared__ int sh[256];
-  const int base = blockIdx.x * 256 * kScanItems + threadIdx.x * kScanItems;
+  const int base = blk * 256 * kScanItems + threadIdx.x * kScanItems;
   int v[kScanItems];
   int s = 0;
 #pragma unroll
@@ -165,7 +164,12 @@ __global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ cou
     if (base + q < m) start[base + q] = run;
     run += v[q];
   }
-  if (threadIdx.x == 255) blocksum[blockIdx.x] = sh[255];
+  if (threadIdx.x == 255) blocksum[blk] = sh[255];
+}
+
+__global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ count, const int *__restrict__ count2,
+                                                     int m, int *__restrict__ start, int *__restrict__ blocksum) {
+  scan_local_block(count, count2, m, start, blocksum, blockIdx.x);
 }
 
 // adds the scanned block totals, writes start[m], zeroes count[] and count2[] for the next use
@@ -597,11 +601,6 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
   if (tile >= A.ntiles) return;
   const int4 td = A.tiles[tile];
   pairs_span<S, SRK, MODE, true>(P, A, B, td.x, td.y, td.z, td.w, L);  // overflow: split for fk_pairs_ovf
-  if (MODE == 3) {  // this step's move counts of the tile's cells start from zero (stored last: loads
-                    // and stores share one wait counter, so stores up front would delay the staging)
-    int *z = B.cnt9 + (long)(td.x + td.y * A.ncf + td.z) * 9;
-    for (int e = threadIdx.x; e < 9 * (td.w - td.z); e += kFB) z[e] = 0;
-  }
 }
 
 template <bool SRK>
@@ -713,7 +712,8 @@ __device__ __forceinline__ void pairs_dense_cell(const DevParams &P, const FastA
 // items: column spans that fit the stage run like fk_pairs, chunks of dense cells through
 // pairs_dense_cell.
 template <int S, bool SRK, int MODE>
-__global__ __launch_bounds__(kFB) void fk_pairs_ovf(DevParams P, FastArgs A, PairBufs B) {
+__device__ __forceinline__ void overflow_items(const DevParams &P, const FastArgs &A, const PairBufs &B, int first,
+                                               int stride) {
   // the span stage and the dense-cell stage are never live together: one LDS buffer holds either
   constexpr size_t kLds = sizeof(PairLds<S, SRK, MODE>) > sizeof(OvfLds<SRK>) ? sizeof(PairLds<S, SRK, MODE>)
                                                                                : sizeof(OvfLds<SRK>);
@@ -721,7 +721,7 @@ __global__ __launch_bounds__(kFB) void fk_pairs_ovf(DevParams P, FastArgs A, Pai
   PairLds<S, SRK, MODE> &Ls = *reinterpret_cast<PairLds<S, SRK, MODE> *>(lds);
   OvfLds<SRK> &Lc = *reinterpret_cast<OvfLds<SRK> *>(lds);
   const int cnt = *A.ovf_count;
-  for (int it = blockIdx.x; it < cnt; it += gridDim.x) {
+  for (int it = first; it < cnt; it += stride) {
     const int4 w = A.ovf_list[it];
     __syncthreads();  // the previous item's LDS is consumed
     if (w.w >= 0)
@@ -729,6 +729,23 @@ __global__ __launch_bounds__(kFB) void fk_pairs_ovf(DevParams P, FastArgs A, Pai
     else
       pairs_dense_cell<S, SRK, MODE>(P, A, B, w.x, w.y, w.z, -1 - w.w, Lc);
   }
+}
+
+template <int S, bool SRK, int MODE>
+__global__ __launch_bounds__(kFB) void fk_pairs_ovf(DevParams P, FastArgs A, PairBufs B) {
+  overflow_items<S, SRK, MODE>(P, A, B, blockIdx.x, gridDim.x);
+}
+
+// the resident step's first scan pass and, in the blocks after the first nsb, the overflow work of
+// the pair kernel (its sums are needed only by fr_scatter), saving a dependent launch per step
+template <int S, bool SRK>
+__global__ __launch_bounds__(kFB) void fr_scan_overflow(DevParams P, FastArgs A, PairBufs B, const int *count,
+                                                        const int *count2, int m, int *start, int *blocksum,
+                                                        int nsb) {
+  if ((int)blockIdx.x < nsb)
+    scan_local_block(count, count2, m, start, blocksum, blockIdx.x);
+  else
+    overflow_items<S, SRK, 3>(P, A, B, blockIdx.x - nsb, gridDim.x - nsb);
 }
 
 // The stochastic step of one particle from its phasor sums (StochasticEulerStepperPtr.hpp:50-67,
@@ -810,7 +827,7 @@ __device__ __forceinline__ int cell_of_position(const DevParams &P, double x, do
 
 // the move of every resident particle this step: old cell * 16 + (dy+1)*3 + (dx+1), or kFar
 // with an unordered arrival slot in farcnt of the destination.  The near moves are counted per
-// (old cell, direction) into cnt9 (zeroed by the pair kernel of this step) and per destination
+// (old cell, direction) into cnt9 (cleared by the previous step's fr_far_bucket) and per destination
 // into newcnt (zeroed by the scan): particles lie in cell order, so a wave's equal (cell,
 // direction) keys form runs and one lane per run adds the run's length.
 template <int S, bool SRK>
@@ -859,13 +876,16 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int
   }
 }
 
-// far arrivals: unordered bucket behind the near arrivals of their destination cell
+// far arrivals: unordered bucket behind the near arrivals of their destination cell; clears the
+// next step's move counters
 template <int S, bool SRK>
 __global__ void fr_far_bucket(DevParams P, int n, int ncf, int cpr, const int *__restrict__ mv,
                               const double *__restrict__ rstate, StepArgs U, const int *__restrict__ nstart,
                               const int *__restrict__ cnt9, const int *__restrict__ farslot,
-                              int *__restrict__ bucket) {
+                              int *__restrict__ bucket, int *__restrict__ cnt9_next, long nclear) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
+  // the other move-count buffer (last read by the previous step) starts the next step at zero
+  for (long e = p; e < nclear; e += (long)gridDim.x * blockDim.x) cnt9_next[e] = 0;
   if (p >= n) return;
   const int m = mv[p];
   if ((m & 15) != kFar) return;

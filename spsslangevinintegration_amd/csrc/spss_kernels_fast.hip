@@ -73,7 +73,8 @@ struct FastPath {
   float *romega = nullptr;
   double2 *ssum = nullptr, *tsum = nullptr;
   int *scnt = nullptr;
-  int *mv = nullptr, *cnt9 = nullptr, *newcnt = nullptr, *farcnt = nullptr;
+  int *cnt9[2] = {nullptr, nullptr};  // move counts per (cell, direction), by step parity (cur)
+  int *mv = nullptr, *newcnt = nullptr, *farcnt = nullptr;
   hipStream_t st = nullptr;
   KernelTimer *timer = nullptr;
 };
@@ -217,7 +218,8 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->tsum, n));
   A(falloc(&f->scnt, n));
   A(falloc(&f->mv, n));
-  A(falloc(&f->cnt9, (size_t)cells * 9));
+  A(falloc(&f->cnt9[0], (size_t)cells * 9));
+  A(falloc(&f->cnt9[1], (size_t)cells * 9));
   A(falloc(&f->farcnt, cells));
   A(falloc(&f->newcnt, cells));
   A(falloc(&f->rep_first, R + 1));
@@ -247,7 +249,8 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   }
   launch_fill_int(st, cells, f->count, 0);
   launch_fill_int(st, cells, f->farcnt, 0);
-  launch_fill_int(st, (long)cells * 9, f->cnt9, 0);
+  launch_fill_int(st, (long)cells * 9, f->cnt9[0], 0);
+  launch_fill_int(st, (long)cells * 9, f->cnt9[1], 0);
   launch_fill_int(st, cells, f->newcnt, 0);
   launch_fill_int(st, 1, f->ovf_count, 0);
   if (coef) {
@@ -268,7 +271,7 @@ void fast_destroy(FastPath *f) {
   void *b[] = {f->key,       f->slot,      f->count,     f->start,   f->tmp,       f->sorted,    f->blocksum,
                f->snap,      f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],  f->rid[1],    f->rnat[0],
                f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mv,
-               f->cnt9,      f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
+               f->cnt9[0],   f->cnt9[1],   f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
                f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc,      f->ovf_list,  f->ovf_count};
   for (void *p : b) hipFree(p);
   delete f;
@@ -299,6 +302,9 @@ void fast_set_timer(FastPath *f, KernelTimer *t) {
 static int tile_grid(int ntiles) { return ((ntiles + 7) / 8) * 8; }
 
 static unsigned nblocks(long n) { return (unsigned)((n + 255) / 256); }
+
+// workgroups of the overflow pass (grid-stride over its items; 4 per CU)
+constexpr int kOverflowBlocks = 1024;
 
 static FastArgs args_of(const FastPath *f) {
   FastArgs a;
@@ -358,7 +364,8 @@ template <int S, bool SRK, int MODE>
 static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, const PairBufs &B) {
   const FastArgs a = args_of(f);
   fk_pairs<S, SRK, MODE><<<tile_grid(f->ntiles), kFB, 0, st>>>(P, a, B);
-  fk_pairs_ovf<S, SRK, MODE><<<1024, kFB, 0, st>>>(P, a, B);
+  // the resident step (MODE 3) runs the overflow work inside its first scan launch (fr_scan_overflow)
+  if (MODE != 3) fk_pairs_ovf<S, SRK, MODE><<<kOverflowBlocks, kFB, 0, st>>>(P, a, B);
 }
 
 int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const double *nat,
@@ -415,20 +422,22 @@ static void resident_enter(FastPath *f, hipStream_t st, const DevParams &P, cons
 // one resident step after the phasor sums: moves, arrival counts, new cell starts, the step
 // applied and written to the new slots
 template <int S, bool SRK>
-static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const StepArgs &U, double *compl_out) {
+static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const StepArgs &U, double *compl_out,
+                          const PairBufs &B) {
   const int c = f->cur, o = 1 - c;
   const int n = f->n;
   const unsigned nb = nblocks(n);
   const double *rnat = f->resident_nat ? f->rnat[c] : nullptr;
   const Layout L = layout_of(f);
-  fr_move<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->rstate[c], U, f->mv, f->farcnt, f->slot, f->cnt9,
+  fr_move<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->rstate[c], U, f->mv, f->farcnt, f->slot, f->cnt9[c],
                                       f->newcnt);
-  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum);
+  fr_scan_overflow<S, SRK><<<f->nsb + kOverflowBlocks, kFB, 0, st>>>(P, args_of(f), B, f->newcnt, f->farcnt,
+                                                                     f->cells, f->rstart[o], f->blocksum, f->nsb);
   fk_scan_add<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum, f->nsb,
                                       f->ovf_count);
-  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, f->mv, f->rstate[c], U, f->rstart[o], f->cnt9,
-                                           f->slot, f->tmp);
-  fr_scatter<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], f->mv, f->cnt9, f->tmp,
+  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, f->mv, f->rstate[c], U, f->rstart[o],
+                                           f->cnt9[c], f->slot, f->tmp, f->cnt9[o], (long)f->cells * 9);
+  fr_scatter<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], f->mv, f->cnt9[c], f->tmp,
                                         f->rstate[c], f->rid[c], rnat, f->ssum, f->tsum, f->scnt, U,
                                         SRK ? compl_out : nullptr, f->rstate[o], f->rid[o], f->rnat[o], f->rsnap,
                                         SRK ? f->romega : nullptr);
@@ -452,7 +461,6 @@ static int step_impl(FastPath *f, hipStream_t st, const DevParams &P, bool do_sr
   B.ssum = f->ssum;
   B.tsum = f->tsum;
   B.scnt = f->scnt;
-  B.cnt9 = f->cnt9;
   if (f->timer) f->timer->begin(st);
   if (do_srk)
     launch_pairs<4, true, 3>(f, st, P, B);
@@ -462,11 +470,11 @@ static int step_impl(FastPath *f, hipStream_t st, const DevParams &P, bool do_sr
     launch_pairs<3, false, 3>(f, st, P, B);
   if (f->timer) f->timer->end(st);
   if (do_srk)
-    resident_step<4, true>(f, st, P, U, compl_out);
+    resident_step<4, true>(f, st, P, U, compl_out, B);
   else if (P.s == 4)
-    resident_step<4, false>(f, st, P, U, nullptr);
+    resident_step<4, false>(f, st, P, U, nullptr, B);
   else
-    resident_step<3, false>(f, st, P, U, nullptr);
+    resident_step<3, false>(f, st, P, U, nullptr, B);
   FCHK(hipGetLastError());
   return SPSS_OK;
 }
