@@ -36,15 +36,15 @@ HBM_PEAK = 8.0e12
 def pair_kernel_bytes(n, cells):
     """algorithmic HBM bytes per particle of the pair kernel of one Euler step: FP32 snapshot
     (x, y, cos phi, sin phi) read once (16), phasor sum (2 x FP64) and count written (20); per
-    cell its start read (4 B x 2) and its nine move counters cleared (36 B)"""
-    return 16.0 + 20.0 + (8.0 + 36.0) * cells / n
+    cell its start read (4 B x 2)"""
+    return 16.0 + 20.0 + 8.0 * cells / n
 
 
 # HBM bytes per launch of that kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled
 # by the copy-kernel calibration, by cell subdivision (profiles/fk_pairs_euler_N1M_hbm_traffic.txt)
-PAIR_KERNEL_TRAFFIC = {1: 3.9748e7}
+PAIR_KERNEL_TRAFFIC = {1: 3.9923e7}
 # VALU issue utilisation of that kernel (SQ_INSTS_VALU per CU-cycle, profiles/fk_pairs_euler_N1M_valu.txt)
-PAIR_KERNEL_VALU_UTIL = 0.99
+PAIR_KERNEL_VALU_UTIL = 0.96
 # algorithmic bytes per particle-step of the whole step (state 64, snapshot w+r 24,
 # key and sorted index w+r 16, cell table w+r 8 * 62500 / N)
 STEP_BYTES = 64.0 + 24.0 + 16.0 + 8.0 * 62500.0 / 1048576.0
