@@ -273,6 +273,29 @@ int spss_mlp_train(spss_mlp *m, int nsteps, int batch, float lr, float beta1, fl
 int spss_mlp_reset_optimizer(spss_mlp *m);
 int spss_mlp_adam_steps(const spss_mlp *m, long *steps);
 
+/* ---- the reference's output files and restart.
+ * A frame is a float time and the s*n state values as floats, as written by
+ * BinaryObserverFirstOrderPtr::SaveSystemState (BinaryObserverFirstOrderPtr.cpp:108-121): for
+ * models 3 and 4 x and y wrapped into the box by the float overload of
+ * ApplyPeriodicBoundaryConditions (PeriodicBoundaryConditions.cpp:137-145) and phi wrapped by
+ * phi -= floor(phi / 2 pi) * 2 pi, both in double on the float value; model 6 the raw float copy
+ * (BinaryObserverSecondOrderPtr.cpp:108-113).  spss_frame fills host_frame[0] with t and
+ * host_frame[1 .. s*n] with the state; spss_frame_append appends the frame to a binary file. */
+int spss_frame(spss_system *sys, double t, float *host_frame);
+int spss_frame_append(spss_system *sys, const char *path, double t);
+/* SaveSummaryStatistics (BinaryObserverFirstOrderPtr.cpp:126-146; model 6 phi in slot 4,
+ * BinaryObserverSecondOrderPtr.cpp:175-193): Z = sum_i ((float) cos phi_i, (float) sin phi_i)
+ * accumulated in complex<float> in particle order, divided by n; |Z| and arg Z as floats.
+ * spss_summary_append appends the line "t \t |Z| \t arg Z" (default stream formatting, %g). */
+int spss_summary(spss_system *sys, float *abs_z, float *arg_z);
+int spss_summary_append(spss_system *sys, const char *path, double t);
+/* SimulationEngineFirstOrderPtr::InitializeSystemStateFromPreviousSolution
+ * (SimulationEngineFirstOrderPtr.cpp:70-94): the last complete frame of a file of frames of
+ * s*n values; its floats widened to double.  spss_load_last_frame sets the system's state
+ * from it and returns its time. */
+int spss_read_last_frame(const char *path, int s, int n, float *t, double *state);
+int spss_load_last_frame(spss_system *sys, const char *path, double *t);
+
 /* AveragingObserver::operator() and the sum behind SaveAveragedSystemState (Observers/AveragingObserver.cpp:29-37,
  * 39-68; AveragingObserverPtr.cpp:46-79): running FP64 sum of the state and the step count */
 int spss_averaging_reset(spss_system *sys);

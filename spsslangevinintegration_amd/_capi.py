@@ -36,6 +36,8 @@ EXPORTS = [
     "spss_mlp_create", "spss_mlp_destroy", "spss_mlp_param_count", "spss_mlp_set_params", "spss_mlp_get_params",
     "spss_mlp_predict", "spss_mlp_gradients", "spss_mlp_set_data", "spss_mlp_train", "spss_mlp_reset_optimizer",
     "spss_mlp_adam_steps",
+    "spss_frame", "spss_frame_append", "spss_summary", "spss_summary_append", "spss_read_last_frame",
+    "spss_load_last_frame",
 ]
 
 
@@ -140,6 +142,12 @@ def load(path=None):
         "spss_mlp_train": (i, [vp, i, i, C.c_float, C.c_float, C.c_float, C.c_float, vp]),
         "spss_mlp_reset_optimizer": (i, [vp]),
         "spss_mlp_adam_steps": (i, [vp, C.POINTER(l)]),
+        "spss_frame": (i, [vp, d, vp]),
+        "spss_frame_append": (i, [vp, C.c_char_p, d]),
+        "spss_summary": (i, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "spss_summary_append": (i, [vp, C.c_char_p, d]),
+        "spss_read_last_frame": (i, [C.c_char_p, i, i, C.POINTER(C.c_float), vp]),
+        "spss_load_last_frame": (i, [vp, C.c_char_p, C.POINTER(d)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -177,6 +185,14 @@ def philox4x32_10(ctr, key):
     out = np.zeros(4, dtype=np.uint32)
     check(load().spss_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data))
     return out
+
+
+def read_last_frame(path, s, n):
+    """(t, state) of the last complete frame of a trajectory file (float time, s*n floats)"""
+    t = C.c_float()
+    st = np.zeros(s * n)
+    check(load().spss_read_last_frame(os.fsencode(path), int(s), int(n), C.byref(t), ptr(st)))
+    return t.value, st
 
 
 def philox_normals(seed, step, n, block=0, first=0):

@@ -7,6 +7,9 @@
 //   shim_check srk     n rho K dt in.bin out.bin               linked list, Philox noise (seed 0x5EED0000)
 //   shim_check average n rho K dt in.bin out.bin               Euler + AveragingObserverPtr, time mean
 //   shim_check engine  n                 out.bin               SimulationEngineFirstOrderPtr, t_1 = 1
+//   shim_check files   n folder          out.bin               the same run writing the reference's trajectory and
+//                                                              summary files into folder (BinaryObserverFirstOrderPtr),
+//                                                              then InitializeSystemStateFromPreviousSolution on them
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -49,6 +52,21 @@ int main(int argc, char **argv) {
       }
       write_file(argv[3], engine.system_state());
       write_file((std::string(argv[3]) + ".summary").c_str(), out);
+      return 0;
+    }
+    if (mode == "files") {
+      spss_shim::default_options().output_folder = std::string(argv[3]) + "/";
+      SimulationEngineFirstOrderPtr engine(&thread, n, s);
+      SimulationEngineFirstOrderPtr::Run r;
+      r.t_1 = 1.0;
+      r.write_files = true;
+      engine.RunSimulation(r);
+      write_file(argv[4], engine.system_state());
+      SimulationEngineFirstOrderPtr restarted(&thread, n, s);
+      restarted.InitializeSystemStateFromPreviousSolution(engine.files()[0]);
+      write_file((std::string(argv[4]) + ".restart").c_str(), restarted.system_state());
+      std::printf("%s\n%s\n%.9g\n", engine.files()[0].c_str(), engine.files()[1].c_str(),
+                  (double)restarted.started_from());
       return 0;
     }
     const double rho = std::atof(argv[3]);

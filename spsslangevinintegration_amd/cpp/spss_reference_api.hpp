@@ -32,7 +32,10 @@
 
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <memory>
 #include <random>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -57,6 +60,9 @@ struct Options {
   int natfreq_local = 0;  // the reference applies natural frequencies in the global branch only
   int cell_subdivision = 1;
   uint64_t seed = 0x5EED0000ull;
+  // where the binary observers write; the reference's folder is fixed per machine
+  // (BinaryObserverFirstOrderPtr.cpp:42-52)
+  std::string output_folder = "./";
 };
 
 inline Options &default_options() {
@@ -454,6 +460,103 @@ class AveragingObserverPtr {
 };
 
 // ---------------------------------------------------------------------------
+// BinaryObserverFirstOrderPtr.cpp:19-146 and BinaryObserverSecondOrderPtr.cpp:16-193: the
+// trajectory file (every `threshold`-th SaveSystemState call: a float time and the float state,
+// models 3/4 wrapped into the box and phi into [0, 2 pi)) and the summary file (every
+// threshold-th SaveSummaryStatistics call: "t \t |Z| \t arg Z"), named from the parameters as
+// the reference names them, in default_options().output_folder.  The frames and sums come from
+// the device state of the bound particle system (Bind); the host pointer arguments are kept for
+// the reference's signatures.
+namespace spss_shim {
+class BinaryFiles {
+ public:
+  BinaryFiles(const std::string &sim, const std::string &summary, int threshold)
+      : sim_(sim), summary_(summary), threshold_(threshold) {
+    std::remove(sim_.c_str());
+    std::remove(summary_.c_str());
+  }
+  void Bind(DeviceSystem &dev) { dev_ = &dev; }
+  void SaveSystemState(Real t) {
+    if (!(counter_[0]++ % threshold_)) check(spss_frame_append(device().handle(), sim_.c_str(), t));
+  }
+  void SaveSummaryStatistics(Real t) {
+    if (!(counter_[1]++ % threshold_)) check(spss_summary_append(device().handle(), summary_.c_str(), t));
+  }
+  const std::string &simulation_file_name() const { return sim_; }
+  const std::string &summary_statistics_file_name() const { return summary_; }
+
+ private:
+  DeviceSystem &device() const {
+    if (!dev_) throw std::runtime_error("spss: the binary observer needs Bind(particle_system)");
+    return *dev_;
+  }
+  std::string sim_, summary_;
+  int threshold_;
+  long counter_[2] = {0, 0};
+  DeviceSystem *dev_ = nullptr;
+};
+}  // namespace spss_shim
+
+class BinaryObserverFirstOrderPtr {
+ public:
+  BinaryObserverFirstOrderPtr(ThreadSharedMemory *thread, Real v_0, Real xi, Real sigma, Real rho, Real alpha,
+                              Real D_phi, PeriodicBoundaryConditions &pbc_config, int number_of_particles,
+                              int number_of_state_variables, Real dt, int trial)
+      : files_(name(v_0, xi, sigma, rho, alpha, D_phi, number_of_particles, trial, "", ".bin"),
+               name(v_0, xi, sigma, rho, alpha, D_phi, number_of_particles, trial, "", ".txt"), 100) {
+    (void)thread, (void)pbc_config, (void)number_of_state_variables, (void)dt;
+  }
+  void Bind(ParticleSystemFirstOrderPtr &system) { files_.Bind(system.device()); }
+  void SaveSystemState(Real *const /*system_state*/, long /*size*/, Real t) { files_.SaveSystemState(t); }
+  void SaveSummaryStatistics(const Real *const /*system_state*/, long /*size*/, Real t) {
+    files_.SaveSummaryStatistics(t);
+  }
+  const std::string &simulation_file_name() const { return files_.simulation_file_name(); }
+  const std::string &summary_statistics_file_name() const { return files_.summary_statistics_file_name(); }
+
+ private:
+  static std::string name(Real v_0, Real xi, Real sigma, Real rho, Real alpha, Real D_phi, int n, int trial,
+                          const char *prefix, const char *ext) {
+    std::ostringstream b;  // BinaryObserverFirstOrderPtr.cpp:54-60, 70-76
+    b << spss_shim::default_options().output_folder << prefix << "v0_" << v_0 << "_xi_" << xi << "_sigma_" << sigma
+      << "_rho_" << rho << "_alpha_" << alpha << "_Dphi_" << D_phi << "_N_" << n << "_" << 0 << "_" << trial << ext;
+    return b.str();
+  }
+  spss_shim::BinaryFiles files_;
+};
+
+class BinaryObserverSecondOrderPtr {
+ public:
+  BinaryObserverSecondOrderPtr(ThreadSharedMemory *thread, Real v_0, Real xi_r, Real D_r, Real xi_phi, Real D_phi,
+                               Real sigma, Real rho, Real alpha, PeriodicBoundaryConditions &pbc_config,
+                               int number_of_particles, int number_of_state_variables, Real dt, int trial)
+      : files_(name(v_0, xi_r, D_r, xi_phi, D_phi, sigma, rho, alpha, number_of_particles, trial, "", ".bin"),
+               name(v_0, xi_r, D_r, xi_phi, D_phi, sigma, rho, alpha, number_of_particles, trial,
+                    "summary_statistics_", ".txt"),
+               10) {
+    (void)thread, (void)pbc_config, (void)number_of_state_variables, (void)dt;
+  }
+  void Bind(ParticleSystemSecondOrderPtr &system) { files_.Bind(system.device()); }
+  void SaveSystemState(Real *const /*system_state*/, long /*size*/, Real t) { files_.SaveSystemState(t); }
+  void SaveSummaryStatistics(const Real *const /*system_state*/, long /*size*/, Real t) {
+    files_.SaveSummaryStatistics(t);
+  }
+  const std::string &simulation_file_name() const { return files_.simulation_file_name(); }
+  const std::string &summary_statistics_file_name() const { return files_.summary_statistics_file_name(); }
+
+ private:
+  static std::string name(Real v_0, Real xi_r, Real D_r, Real xi_phi, Real D_phi, Real sigma, Real rho, Real alpha,
+                          int n, int trial, const char *prefix, const char *ext) {
+    std::ostringstream b;  // BinaryObserverSecondOrderPtr.cpp:51-57, 66-72
+    b << spss_shim::default_options().output_folder << prefix << "v0_" << v_0 << "_xir_" << xi_r << "_Dr_" << D_r
+      << "_xiphi_" << xi_phi << "_Dphi_" << D_phi << "_sigma_" << sigma << "_rho_" << rho << "_alpha_" << alpha
+      << "_N_" << n << "_" << 0 << "_" << trial << ext;
+    return b.str();
+  }
+  spss_shim::BinaryFiles files_;
+};
+
+// ---------------------------------------------------------------------------
 // SimulationEngineFirstOrderPtr.cpp:96-165: random initial state (x, y ~ U[0,1), phi ~
 // U[0,2pi), omega ~ U[-sigma/xi, sigma/xi]), then `while (t <= t_1) { t += dt; DoStep; observe }`
 // with the stochastic Runge-Kutta stepper.  The run keeps the state on the device and reads
@@ -470,6 +573,9 @@ class SimulationEngineFirstOrderPtr {
     Real t_0 = 0.0, t_1 = 1000.0, dt = 0.01;                                        // :110-112
     int summary_every = 100;
     unsigned seed = 0;
+    // write the reference's trajectory and summary files (BinaryObserverFirstOrderPtr, :125-138)
+    bool write_files = false;
+    int trial = 0;
   };
 
   void RunSimulation() { RunSimulation(Run{}); }
@@ -480,18 +586,42 @@ class SimulationEngineFirstOrderPtr {
                                                 n_, s_);
     particle_system.SetHostSync(false);
     summary_.clear();
+    std::unique_ptr<BinaryObserverFirstOrderPtr> observer;
+    if (r.write_files) {
+      observer.reset(new BinaryObserverFirstOrderPtr(thread_, r.v_0, r.xi, r.sigma, r.rho, r.alpha, r.D_phi,
+                                                     pbc_config_, n_, s_, r.dt, r.trial));
+      observer->Bind(particle_system);
+      files_ = {observer->simulation_file_name(), observer->summary_statistics_file_name()};
+    }
     Real t = r.t_0;
     long k = 0;
+    if (observer) {
+      particle_system.device().Upload(system_state_.data());  // the device holds the initial state
+      observer->SaveSystemState(system_state_.data(), (long)system_state_.size(), t);
+    }
     while (t <= r.t_1) {
       t += r.dt;
       stepper.DoStep(particle_system, system_state_.data(), t, r.dt);
       if (++k % r.summary_every == 0) summary_.push_back({t, particle_system.OrderParameter()});
+      if (observer) {
+        observer->SaveSystemState(system_state_.data(), (long)system_state_.size(), t);
+        observer->SaveSummaryStatistics(system_state_.data(), (long)system_state_.size(), t);
+      }
     }
     particle_system.DownloadState(system_state_.data());
   }
+  // the trajectory and summary files of the last run with write_files
+  const std::vector<std::string> &files() const { return files_; }
   void RunContinuationMethod() {
     throw std::runtime_error("spss: parameter continuation (dopri5) is not part of the device path");
   }
+  // SimulationEngineFirstOrderPtr.cpp:70-94: the state from the last frame of a trajectory file
+  void InitializeSystemStateFromPreviousSolution(const std::string &file_name) {
+    float t = 0.f;
+    spss_shim::check(spss_read_last_frame(file_name.c_str(), s_, n_, &t, system_state_.data()));
+    started_from_ = t;
+  }
+  Real started_from() const { return started_from_; }
 
   const std::vector<Real> &system_state() const { return system_state_; }
   struct Summary {
@@ -519,4 +649,6 @@ class SimulationEngineFirstOrderPtr {
   int n_, s_;
   std::vector<Real> system_state_;
   std::vector<Summary> summary_;
+  std::vector<std::string> files_;
+  Real started_from_ = 0.0;
 };

@@ -854,6 +854,28 @@ int spss_order_parameter(spss_system *h, int slot, double *re, double *im) {
   return SPSS_OK;
 }
 
+}  // extern "C"
+
+namespace spss {
+// the master state brought up to date (the resident copy of the fused steps is kept), for the
+// other translation units of the library (the observers)
+int system_view(spss_system *h, SystemView *v) {
+  if (!h || !v) return fail(SPSS_ERR_INVALID, "NULL argument");
+  TRY(sync_master(h));
+  v->stream = h->stream;
+  v->state = h->state;
+  v->n = h->n;
+  v->s = h->s;
+  v->x_size = h->P.xs;
+  v->y_size = h->P.ys;
+  v->x_rsize = h->P.xrs;
+  v->y_rsize = h->P.yrs;
+  return SPSS_OK;
+}
+}  // namespace spss
+
+extern "C" {
+
 int spss_apply_periodic_boundary_conditions(spss_system *h) {
   if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
   // a wrap can move a particle at x = -0 from the last cell to the first: the resident copy is

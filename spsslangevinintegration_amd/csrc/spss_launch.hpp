@@ -13,6 +13,18 @@ namespace spss {
 // records the message spss_last_error() returns and passes the code through
 int set_error(int code, const std::string &msg);
 
+// what the observers need of a handle: its stream and its master state, up to date
+struct SystemView {
+  hipStream_t stream;
+  const double *state;  // device, AoS FP64, original particle order
+  int n, s;
+  double x_size, y_size, x_rsize, y_rsize;
+};
+}  // namespace spss
+struct spss_system;
+namespace spss {
+int system_view(spss_system *h, SystemView *v);
+
 // HIP-event timer around one kernel family, enabled on request (bench instrumentation):
 // records an event pair per launch on the launching stream, sums the durations on read.
 struct KernelTimer {

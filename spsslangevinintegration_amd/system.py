@@ -4,6 +4,7 @@ Method names follow the reference classes: EvaluateRhs, the three steppers'
 DoStep, SetNaturalFrequencies, CalculateMaxDisplacement, ... in snake case.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -133,6 +134,32 @@ class ParticleSystem:
         out = np.zeros(2 * max(cnt.value, 1), dtype=np.int32)
         A.check(self.lib.spss_neighbour_pairs(self.h, method, A.ptr(st), A.ptr(out), cnt.value, C.byref(cnt)))
         return out[: 2 * cnt.value].reshape(-1, 2)
+
+    # ---- output files of the reference's observers, restart
+    def frame(self, t):
+        """the frame the reference writes: float t, then the s*n state values as floats
+        (models 3/4 wrapped into the box and phi into [0, 2 pi))"""
+        out = np.empty(1 + self.s * self.n, dtype=np.float32)
+        A.check(self.lib.spss_frame(self.h, float(t), A.ptr(out)))
+        return out
+
+    def append_frame(self, path, t):
+        A.check(self.lib.spss_frame_append(self.h, os.fsencode(path), float(t)))
+
+    def summary(self):
+        """(|Z|, arg Z) as floats, Z accumulated in complex<float> in particle order"""
+        a, g = C.c_float(), C.c_float()
+        A.check(self.lib.spss_summary(self.h, C.byref(a), C.byref(g)))
+        return a.value, g.value
+
+    def append_summary(self, path, t):
+        A.check(self.lib.spss_summary_append(self.h, os.fsencode(path), float(t)))
+
+    def load_last_frame(self, path):
+        """restart from the last frame of a trajectory file; returns its time"""
+        t = C.c_double()
+        A.check(self.lib.spss_load_last_frame(self.h, os.fsencode(path), C.byref(t)))
+        return t.value
 
     # ---- cell-size autotuner (FP32C)
     def autotune_cells(self, stepper, dt, probe_steps=10):

@@ -131,3 +131,33 @@ def test_shim_engine_run_simulation():
     assert np.all((r >= 0) & (r <= 1))
     re, im = pyoracle.order_parameter(final, n, 4, 2)
     assert abs(re - summ[-1, 1]) <= 1e-12 and abs(im - summ[-1, 2]) <= 1e-12
+
+
+def test_shim_engine_writes_the_reference_files_and_restarts_from_them():
+    """BinaryObserverFirstOrderPtr through the engine's run (write_files): the trajectory holds
+    the initial frame and every 100th step's frame (the run has 100 steps), each the reference's
+    float wrap of the state; the summary file one line per 100th call; the restart reads the
+    last frame back (InitializeSystemStateFromPreviousSolution)."""
+    from oracle import observer_oracle as OO
+    n = 2048
+    with tempfile.TemporaryDirectory() as td:
+        fout = os.path.join(td, "out.bin")
+        res = subprocess.run([EXE, "files", str(n), td, fout], capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stderr
+        traj, summ, t_restart = res.stdout.split("\n")[:3]
+        assert os.path.basename(traj) == "v0_1_xi_0.1_sigma_1_rho_0.8_alpha_0.3_Dphi_0.01_N_2048_0_0.bin"
+        assert os.path.basename(summ) == "v0_1_xi_0.1_sigma_1_rho_0.8_alpha_0.3_Dphi_0.01_N_2048_0_0.txt"
+        final = np.fromfile(fout)
+        restart = np.fromfile(fout + ".restart")
+        assert os.path.getsize(traj) == 2 * (1 + 4 * n) * 4
+        frames = np.fromfile(traj, dtype=np.float32).reshape(2, 1 + 4 * n)
+        lines = open(summ).read().splitlines()
+    # the last step's time 1.0000000000000007 as a float
+    want = OO.frame(final, 4, 1.0000000000000007)
+    np.testing.assert_array_equal(frames[1], want)
+    assert frames[0][0] == 0.0 and np.all(frames[0][1::4] >= 0) and np.all(frames[0][1::4] <= 1)
+    np.testing.assert_array_equal(restart, want[1:].astype(np.float64))
+    assert float(t_restart) == float(np.float32(1.0000000000000007))
+    assert len(lines) == 1  # SaveSummaryStatistics is called after each step: calls 0 and 100 -> 0 only
+    t_s, a_s, g_s = lines[0].split("\t")
+    assert float(t_s) == float("%g" % np.float32(0.01))
