@@ -619,7 +619,22 @@ int spss_run(spss_system *h, int stepper, int method, long nsteps, double dt, in
   if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
   if (noise_mode != SPSS_NOISE_NONE && noise_mode != SPSS_NOISE_PHILOX)
     return fail(SPSS_ERR_INVALID, "spss_run draws noise from NONE or PHILOX only");
-  for (long k = 0; k < nsteps; ++k) {
+  long k = 0;
+  // fused FP32C stochastic steps with Philox noise: the first step (which may enter the resident
+  // order) runs eagerly, the bulk is replayed from a HIP graph, the remainder runs eagerly
+  const bool srk = stepper == SPSS_STEPPER_SRK && h->s == 4;
+  if ((stepper == SPSS_STEPPER_EULER || srk) && noise_mode == SPSS_NOISE_PHILOX && use_fused(h, method) &&
+      nsteps > 1) {
+    TRY(srk ? srk_step(h, method, dt, noise_mode, nullptr) : euler_step(h, method, dt, noise_mode, nullptr));
+    k = 1;
+    const long g = fast_run_graph(h->fast, h->stream, h->P, srk ? 1 : 0, h->state, h->have_nat ? h->nat : nullptr,
+                                  srk ? h->compl_f : nullptr, dt, h->p.D_phi, h->p.xi, h->p.seed,
+                                  (uint64_t)h->step, nsteps - k);
+    if (g < 0) return fail(SPSS_ERR_DEVICE, fast_last_error());
+    h->step += g;
+    k += g;
+  }
+  for (; k < nsteps; ++k) {
     switch (stepper) {
       case SPSS_STEPPER_RK4: TRY(rk4_step(h, method, dt)); break;
       case SPSS_STEPPER_EULER: TRY(euler_step(h, method, dt, noise_mode, nullptr)); break;

@@ -56,6 +56,13 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
               uint64_t seed, uint64_t step);
 bool fast_is_resident(const FastPath *f);
 // writes the resident state back into `state` in original order; the copy stays resident
+// Fused steps replayed from a HIP graph (Philox noise from the device step counter): runs the
+// largest multiple of the graph's step count not above nsteps, starting at Philox step `step`,
+// and returns how many steps it ran (0: not applicable now, e.g. not resident yet or the kernel
+// timer is on; -1: a HIP error, see fast_last_error).  The caller runs the rest with fast_step.
+long fast_run_graph(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat,
+                    double *compl_out, double dt, double D_phi, double friction, uint64_t seed, uint64_t step,
+                    long nsteps);
 int fast_resident_sync(FastPath *f, hipStream_t st, double *state);
 // forgets the resident copy (the next fused step re-enters from the caller's state)
 void fast_resident_drop(FastPath *f);

@@ -53,7 +53,16 @@ struct FastPath {
   int *ovf_count = nullptr;
   int ntiles = 0, tiles_cap = 0;
   Coef *coef = nullptr;                    // per-replica coefficients (batches)
-  unsigned long long *step_dev = nullptr;  // device step counter (batches)
+  unsigned long long *step_dev = nullptr;  // device step counter (batches, graph-replayed steps)
+  int gen = 0;                             // bumped whenever launch shapes change (tile table)
+  // graph of kGraphSteps fused steps of a single system and what it was captured with
+  hipGraph_t sg_graph = nullptr;
+  hipGraphExec_t sg_exec = nullptr;
+  int sg_srk = -1, sg_cur = -1, sg_gen = -1, sg_higher = -1;
+  double sg_dt = 0.0, sg_dphi = 0.0, sg_friction = 0.0;
+  uint64_t sg_seed = 0;
+  const double *sg_state = nullptr, *sg_nat = nullptr;
+  double *sg_compl = nullptr;
   double *zlast = nullptr, *racc = nullptr;
   // rebuilt cell list (RHS, pair export, entry into resident mode)
   int *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr, *sorted = nullptr;
@@ -100,11 +109,13 @@ static hipError_t falloc(T **p, size_t count) {
   return hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
 }
 
-// default tile width of a replica: about one particle per thread at its mean density, with
+// default tile width of a replica: about 0.8 particles per thread at its mean density, with
 // the staged (2k+1) x (W+2k) cells and their row padding within 85% of the LDS stage
 static int default_tile_width(double per_cell, int k, int ncf) {
   const int nrows = 2 * k + 1;
-  int W = (int)std::lround((double)kFB / std::max(per_cell, 1e-9));
+  // about 80% of the workgroup's lanes: a tile just above kFB particles would need a second pass
+  // in which one wave walks the whole stencil for a handful of particles
+  int W = (int)(0.8 * kFB / std::max(per_cell, 1e-9));
   const int wstage = (int)((0.85 * kStageMax - nrows * kPad) / std::max(per_cell * nrows, 1e-9)) - 2 * k;
   W = std::min(W, std::max(wstage, 1));
   return std::max(1, std::min(W, std::min(kMaxCols - 2 * k, ncf)));
@@ -131,6 +142,7 @@ static int set_tiles(FastPath *f, const std::vector<int> &W) {
   FCHK(hipMemcpy(f->tiles, t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice));
   f->ntiles = (int)t.size();
   f->W_h = W;
+  ++f->gen;
   const double span = (double)(wmax + 2 * f->k + 1) * f->H;
   const double eps = std::ldexp(1.0, -24);
   const double dxerr = 2.0 * span * eps + 2.0 * f->H * eps;
@@ -227,9 +239,9 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   // overflow work: at most one span per cell, or one item per cell and kFB particles
   A(falloc(&f->ovf_list, (size_t)cells + n / kFB + 1));
   A(falloc(&f->rep_chunk, nchunk));
+  A(falloc(&f->step_dev, 1));
   if (coef) {
     A(falloc(&f->coef, R));
-    A(falloc(&f->step_dev, 1));
     A(falloc(&f->zlast, 2 * R));
     A(falloc(&f->racc, 3 * R));
   }
@@ -268,6 +280,8 @@ int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, 
 
 void fast_destroy(FastPath *f) {
   if (!f) return;
+  if (f->sg_exec) hipGraphExecDestroy(f->sg_exec);
+  if (f->sg_graph) hipGraphDestroy(f->sg_graph);
   void *b[] = {f->key,       f->slot,      f->count,     f->start,   f->tmp,       f->sorted,    f->blocksum,
                f->snap,      f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],  f->rid[1],    f->rnat[0],
                f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mv,
@@ -501,6 +515,86 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
               uint64_t seed, uint64_t step) {
   const StepArgs U = step_args(dt, D_phi, friction, noise_mode, noise, seed, step);
   return step_impl(f, st, P, srk && P.s == 4, state, nat, compl_out, U);
+}
+
+// ---------------------------------------------------------------------------
+// graph-replayed fused steps of a single system: kGraphSteps steps (even, so that the double
+// buffers end where they started) captured once, each reading its Philox step from the device
+// counter and advancing it, then replayed; launch gaps between the six kernels of a step vanish
+static constexpr int kGraphSteps = 10;
+
+__global__ void fs_set_counter(unsigned long long *c, unsigned long long v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = v;
+}
+
+static void drop_step_graph(FastPath *f) {
+  if (f->sg_exec) hipGraphExecDestroy(f->sg_exec);
+  if (f->sg_graph) hipGraphDestroy(f->sg_graph);
+  f->sg_exec = nullptr;
+  f->sg_graph = nullptr;
+}
+
+long fast_run_graph(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat,
+                    double *compl_out, double dt, double D_phi, double friction, uint64_t seed, uint64_t step,
+                    long nsteps) {
+  const bool do_srk = srk && P.s == 4;
+  const long reps = nsteps / kGraphSteps;
+  // only from the resident order in the form this stepper needs, and not while the kernel timer
+  // records events around the pair kernel
+  if (reps < 1 || f->timer || !f->resident || (nat != nullptr) != f->resident_nat || (do_srk && !f->romega_valid))
+    return 0;
+  const bool same = f->sg_exec && f->sg_srk == (int)do_srk && f->sg_cur == f->cur && f->sg_gen == f->gen &&
+                    f->sg_higher == P.higher_order && f->sg_dt == dt && f->sg_dphi == D_phi &&
+                    f->sg_friction == friction && f->sg_seed == seed && f->sg_state == state && f->sg_nat == nat &&
+                    f->sg_compl == compl_out;
+  if (!same) {
+    drop_step_graph(f);
+    StepArgs U = step_args(dt, D_phi, friction, 3, nullptr, seed, 0);
+    U.step_dev = f->step_dev;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return 0;
+    const int cur0 = f->cur;
+    int rc = SPSS_OK;
+    for (int k = 0; k < kGraphSteps && rc == SPSS_OK; ++k) {
+      rc = step_impl(f, st, P, do_srk, state, nat, compl_out, U);
+      fb_step_inc<<<1, 64, 0, st>>>(f->step_dev);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc != SPSS_OK || e != hipSuccess || f->cur != cur0) {
+      if (g) hipGraphDestroy(g);
+      (void)hipGetLastError();
+      if (rc == SPSS_OK) g_fast_err = std::string("capture of the step graph failed: ") + hipGetErrorString(e);
+      return -1;
+    }
+    f->sg_graph = g;
+    const hipError_t ei = hipGraphInstantiate(&f->sg_exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      drop_step_graph(f);
+      g_fast_err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
+      return -1;
+    }
+    f->sg_srk = do_srk;
+    f->sg_cur = f->cur;
+    f->sg_gen = f->gen;
+    f->sg_higher = P.higher_order;
+    f->sg_dt = dt;
+    f->sg_dphi = D_phi;
+    f->sg_friction = friction;
+    f->sg_seed = seed;
+    f->sg_state = state;
+    f->sg_nat = nat;
+    f->sg_compl = compl_out;
+  }
+  fs_set_counter<<<1, 64, 0, st>>>(f->step_dev, (unsigned long long)step);
+  for (long r = 0; r < reps; ++r) {
+    const hipError_t el = hipGraphLaunch(f->sg_exec, st);
+    if (el != hipSuccess) {
+      g_fast_err = std::string("hipGraphLaunch: ") + hipGetErrorString(el);
+      return -1;
+    }
+  }
+  f->romega_valid = do_srk;
+  return reps * kGraphSteps;
 }
 
 // ---------------------------------------------------------------------------
