@@ -10,6 +10,7 @@
 //   shim_check files   n folder          out.bin               the same run writing the reference's trajectory and
 //                                                              summary files into folder (BinaryObserverFirstOrderPtr),
 //                                                              then InitializeSystemStateFromPreviousSolution on them
+//   shim_check continue n folder in.traj out.bin               RunContinuationMethod from in.traj: 2 alphas, t_1 = 0.05
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -52,6 +53,18 @@ int main(int argc, char **argv) {
       }
       write_file(argv[3], engine.system_state());
       write_file((std::string(argv[3]) + ".summary").c_str(), out);
+      return 0;
+    }
+    if (mode == "continue") {
+      spss_shim::default_options().output_folder = std::string(argv[3]) + "/";
+      SimulationEngineFirstOrderPtr engine(&thread, n, s);
+      SimulationEngineFirstOrderPtr::Continuation c;
+      c.t_1 = 0.05;
+      c.n_alpha = 2;
+      c.initial_file = argv[4];
+      engine.RunContinuationMethod(c);
+      write_file(argv[5], engine.system_state());
+      for (const std::string &f : engine.continuation_files()) std::printf("%s\n", f.c_str());
       return 0;
     }
     if (mode == "files") {

@@ -612,9 +612,40 @@ class SimulationEngineFirstOrderPtr {
   }
   // the trajectory and summary files of the last run with write_files
   const std::vector<std::string> &files() const { return files_; }
-  void RunContinuationMethod() {
-    throw std::runtime_error("spss: parameter continuation (dopri5) is not part of the device path");
+  // SimulationEngineFirstOrderPtr.cpp:167-217: from the last frame of a trajectory file, RK4 runs
+  // for alpha = alpha_0 - alpha_step * ia, ia = 1 .. n_alpha, each continuing from the previous
+  // one's state, with a summary line per step and the final frame in its own observer files
+  struct Continuation {
+    Real v_0 = 1.0, xi = 0.1, sigma = 1.0, rho = 0.8, D_phi = 0.0;  // :174-179
+    Real t_0 = 0.0, t_1 = 200.0, dt = 0.01;                         // :181-183
+    Real alpha_0 = 0.6, alpha_step = 0.05;                          // :191
+    int n_alpha = 5, trial = 0;
+    std::string initial_file;
+  };
+  void RunContinuationMethod(const Continuation &c) {
+    InitializeSystemStateFromPreviousSolution(c.initial_file);
+    for (int ia = 1; ia <= c.n_alpha; ++ia) {
+      const Real alpha = c.alpha_0 - c.alpha_step * ia;
+      RungeKutta4StepperPtr stepper(thread_, n_, s_);
+      ParticleSystemFirstOrderPtr particle_system(thread_, c.v_0, c.xi, c.sigma, c.rho, alpha, c.D_phi, pbc_config_,
+                                                  n_, s_);
+      particle_system.SetHostSync(false);
+      BinaryObserverFirstOrderPtr observer(thread_, c.v_0, c.xi, c.sigma, c.rho, alpha, c.D_phi, pbc_config_, n_, s_,
+                                           c.dt, c.trial);
+      observer.Bind(particle_system);
+      particle_system.device().Upload(system_state_.data());
+      Real t = c.t_0;
+      while (t <= c.t_1) {
+        t += c.dt;
+        stepper.DoStep(particle_system, system_state_.data(), t, c.dt);
+        observer.SaveSummaryStatistics(system_state_.data(), (long)system_state_.size(), t);
+      }
+      observer.SaveSystemState(system_state_.data(), (long)system_state_.size(), t);
+      particle_system.DownloadState(system_state_.data());
+      continuation_files_.push_back(observer.simulation_file_name());
+    }
   }
+  const std::vector<std::string> &continuation_files() const { return continuation_files_; }
   // SimulationEngineFirstOrderPtr.cpp:70-94: the state from the last frame of a trajectory file
   void InitializeSystemStateFromPreviousSolution(const std::string &file_name) {
     float t = 0.f;
@@ -649,6 +680,6 @@ class SimulationEngineFirstOrderPtr {
   int n_, s_;
   std::vector<Real> system_state_;
   std::vector<Summary> summary_;
-  std::vector<std::string> files_;
+  std::vector<std::string> files_, continuation_files_;
   Real started_from_ = 0.0;
 };

@@ -161,3 +161,36 @@ def test_shim_engine_writes_the_reference_files_and_restarts_from_them():
     assert len(lines) == 1  # SaveSummaryStatistics is called after each step: calls 0 and 100 -> 0 only
     t_s, a_s, g_s = lines[0].split("\t")
     assert float(t_s) == float("%g" % np.float32(0.01))
+
+
+def test_shim_parameter_continuation_from_a_trajectory_file():
+    """SimulationEngineFirstOrderPtr::RunContinuationMethod (SimulationEngineFirstOrderPtr.cpp:
+    167-217) from the last frame of a trajectory file: RK4 for alpha = 0.55 then 0.5, each from
+    the previous state with fresh stepper buffers, against the oracle fed the same widened floats"""
+    from oracle import observer_oracle as OO
+    n = 1000
+    rng = np.random.default_rng(44)
+    with tempfile.TemporaryDirectory() as td:
+        traj = os.path.join(td, "start.bin")
+        OO.write_frames(traj, [OO.frame(_state(n, 40), 4, 1.0), OO.frame(_state(n, 41) + rng.normal(0, 1e-3, 4 * n),
+                                                                          4, 3.0)])
+        fout = os.path.join(td, "out.bin")
+        res = subprocess.run([EXE, "continue", str(n), td, traj, fout], capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stderr
+        files = res.stdout.split()
+        got = np.fromfile(fout)
+        frames = [np.fromfile(f, dtype=np.float32) for f in files]
+        _, st = OO.read_last_frame(traj, 4, n)
+    assert [os.path.basename(f) for f in files] == [
+        "v0_1_xi_0.1_sigma_1_rho_0.8_alpha_0.55_Dphi_0_N_1000_0_0.bin",
+        "v0_1_xi_0.1_sigma_1_rho_0.8_alpha_0.5_Dphi_0_N_1000_0_0.bin"]
+    for ia, fr in zip((1, 2), frames):
+        o = oracle.Oracle(model=4, n=n, v0=1.0, xi=0.1, sigma=1.0, rho=0.8, alpha=0.6 - 0.05 * ia, D_phi=0.0)
+        t = 0.0
+        while t <= 0.05:
+            t += 0.01
+            o.rk4_step(0, st, 0.01)
+        want = OO.frame(st, 4, t)
+        assert fr.size == want.size and fr[0] == want[0]
+        np.testing.assert_allclose(fr[1:], want[1:], rtol=3e-7, atol=3e-7)
+    assert np.max(np.abs(got - st)) <= 1e-12
