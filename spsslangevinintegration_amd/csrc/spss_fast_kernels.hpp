@@ -42,7 +42,9 @@ struct FastArgs {
                                // {.., .., column, -1 - j} particles j*kFB.. of a cell that does not
   int *ovf_count;              // their number, zeroed by fk_scan_add
   double H;
-  float r2lo, r2hi;
+  // guard band of the FP32 range test: |d^2 - r2c| <= r2h is decided in FP64; r2k is a power of
+  // two with r2k * r2h > 4, so clamp(-r2k * (d^2 - r2c), 0, 1) is exactly 1 below the band, 0 above
+  float r2c, r2h, r2k;
 };
 
 // Replicas: independent systems of the same cell grid stored one after another (particles in
@@ -329,48 +331,56 @@ __device__ __forceinline__ bool band_in(const DevParams &P, const PairBufs &B, c
 // The candidates staged at LDS positions [lo, hi) (plus far-away padding up to hi + 3) against
 // one particle, four at a time: FP32 distance tests with the FP64 decision in the guard band,
 // phasor sums in 4 FP32 partial slots (slot = (t - lo) mod 4), counts and list entries.
+// dd = d^2 - r2c is formed as fma(dy, dy, fma(dx, dx, -r2c)); its rounding error near the band is
+// at most that of fma(dy, dy, dx * dx) (a few ulps of r2c), which the band width covers.  The
+// in-range weight m is clamp(-r2k * dd, 0, 1): exactly 1 or 0 outside the band (one VALU op with
+// the output clamp), replaced by the FP64 decision inside it.
 // gof(t) gives the global sorted position of LDS entry t for band decisions.
 template <int S, bool SRK, int MODE, class GOF>
 __device__ __forceinline__ void scan_range(const DevParams &P, const PairBufs &B, const float4 *sdat,
                                            const float *somega, int lo, int hi, float xi_t, float syi,
-                                           float r2lo, float r2hi, const double *pi, int self_t, GOF gof,
+                                           float r2c, float r2h, float r2k, const double *pi, int self_t, GOF gof,
                                            float *ax, float *ay, float *an, float *bx, float *by, int &nnb,
                                            int &w, int wbase) {
   constexpr bool SUMS = MODE == 0 || MODE >= 3;
-  for (int t = lo; t < hi; t += 4) {
+  const float4 *p = sdat + lo;
+  const float4 *const pe = sdat + hi;
+  for (; p < pe; p += 4) {
     float4 o[4];
-    bool in[4], band[4];
+    float m[4];
+    bool band[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      o[j] = sdat[t + j];
+      o[j] = p[j];
       const float dx = o[j].x - xi_t;
       const float dy = o[j].y - syi;
-      const float d2 = __builtin_fmaf(dy, dy, dx * dx);
-      in[j] = d2 <= r2hi;
-      band[j] = in[j] && d2 >= r2lo;
+      const float dd = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, -r2c));
+      band[j] = __builtin_fabsf(dd) <= r2h;
+      m[j] = __builtin_amdgcn_fmed3f(-r2k * dd, 0.f, 1.f);
     }
     if (band[0] || band[1] || band[2] || band[3]) {  // rare: FP64 decision inside the stencil
+      const int t = (int)(p - sdat);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (band[j]) in[j] = (t + j < hi) && band_in<S, MODE>(P, B, pi, gof(t + j));
+        if (band[j]) m[j] = ((t + j < hi) && band_in<S, MODE>(P, B, pi, gof(t + j))) ? 1.f : 0.f;
     }
     if (SUMS) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const float m = in[j] ? 1.f : 0.f;
-        ax[j] = __builtin_fmaf(m, o[j].z, ax[j]);
-        ay[j] = __builtin_fmaf(m, o[j].w, ay[j]);
-        an[j] += m;
+        ax[j] = __builtin_fmaf(m[j], o[j].z, ax[j]);
+        ay[j] = __builtin_fmaf(m[j], o[j].w, ay[j]);
+        an[j] += m[j];
         if (SRK) {
-          const float mo = m * somega[t + j];
+          const float mo = m[j] * somega[(p - sdat) + j];
           bx[j] = __builtin_fmaf(mo, o[j].z, bx[j]);
           by[j] = __builtin_fmaf(mo, o[j].w, by[j]);
         }
       }
     } else {
+      const int t = (int)(p - sdat);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        if (in[j] && t + j < hi && t + j != self_t) {
+        if (m[j] != 0.f && t + j < hi && t + j != self_t) {
           ++nnb;
           if (MODE == 2) B.plist[wbase + w++] = B.sorted[gof(t + j)];
         }
@@ -436,10 +446,10 @@ __device__ __forceinline__ void split_span(const FastArgs &A, int cbase, int row
 // the stage is split into smaller work for fk_pairs_ovf (split_span).  pairs_span runs the cells
 // [ca, cb) of one row; it returns false when their staged cells do not fit.
 //
-// Range test: an FP32 distance below r2lo is in range, above r2hi out of range; the
-// guard band between them (rare) is decided by the reference's FP64 ClassC formula, and
-// only for candidates inside the lane's stencil.  A candidate beyond the stencil is at
-// least k cells = 1/int(1/rho) >= rho away in x, so its FP32 distance is never below r2lo.
+// Range test: an FP32 squared distance below the guard band r2c +- r2h is in range, above it
+// out of range; the band (rare) is decided by the reference's FP64 ClassC formula, and only
+// for candidates inside the lane's stencil.  A candidate beyond the stencil is at least
+// k cells = 1/int(1/rho) >= rho away in x, so its FP32 distance is never below the band.
 //
 // Coupling sums go through phasors:
 //   sum_j sin(phi_j - phi_i - alpha) = Im(e^{-i(phi_i + alpha)} S_i),  S_i = sum_j e^{i phi_j}
@@ -533,7 +543,7 @@ __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A
   const int gtb = cglob[k * ncols + k];
   const int ntile = start[cbase + row * ncf + cb] - start[cbase + row * ncf + ca];
   const int ltb = cpos[k * ncols + k];
-  const float r2lo = A.r2lo, r2hi = A.r2hi;
+  const float r2c = A.r2c, r2h = A.r2h, r2k = A.r2k;
 
   for (int u = threadIdx.x; u < ntile; u += kFB) {
     const int li = ltb + u;  // LDS index
@@ -573,7 +583,7 @@ __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A
       };
       float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an[4] = {0.f, 0.f, 0.f, 0.f};
       float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
-      scan_range<S, SRK, MODE>(P, B, sdat, L.somega, lo, hi, xi_t, syi, r2lo, r2hi, pi, li, gof, ax, ay, an, bx, by,
+      scan_range<S, SRK, MODE>(P, B, sdat, L.somega, lo, hi, xi_t, syi, r2c, r2h, r2k, pi, li, gof, ax, ay, an, bx, by,
                                nnb, w, wbase);
       if (SUMS) flush_row<SRK>(ax, ay, an, bx, by, sx, sy, tx, ty, nnb);
     }
@@ -623,7 +633,7 @@ __device__ __forceinline__ void pairs_dense_cell(const DevParams &P, const FastA
   constexpr int kChunk = kStageMax - kPad;
   const int k = A.k, nrows = 2 * k + 1, ncf = A.ncf;
   const double H = A.H;
-  const float r2lo = A.r2lo, r2hi = A.r2hi;
+  const float r2c = A.r2c, r2h = A.r2h, r2k = A.r2k;
   const int *start = B.start;
   const int cown = cbase + row * ncf + q;
   const int gp1 = start[cown + 1];
@@ -686,7 +696,7 @@ __device__ __forceinline__ void pairs_dense_cell(const DevParams &P, const FastA
             if (s0 >= 0 && base + s0 >= c0 && base + s0 < c0 + clen) st = base + s0 - c0;
           }
           auto gof = [&](int t) -> int { return L.sg[t]; };
-          scan_range<S, SRK, MODE>(P, B, L.sdat, L.somega, 0, clen, xi_t, syi, r2lo, r2hi, pi, st, gof, ax,
+          scan_range<S, SRK, MODE>(P, B, L.sdat, L.somega, 0, clen, xi_t, syi, r2c, r2h, r2k, pi, st, gof, ax,
                                    ay, an, bx, by, nnb, w, wbase);
         }
       }

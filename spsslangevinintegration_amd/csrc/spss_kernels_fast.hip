@@ -43,7 +43,7 @@ struct FastPath {
   double rho = 0.0, rho2 = 0.0;
   int nc = 1, k = 1, ncf = 1, cpr = 1, cells = 1, nsb = 1;
   double H = 1.0;
-  float r2lo = 0.f, r2hi = 0.f;
+  float r2c = 0.f, r2h = 0.f, r2k = 0.f;
   // replicas (one for a single system) and the tiles of the pair kernel
   int R = 1;
   std::vector<int> rep_first_h, W_h;
@@ -148,8 +148,9 @@ static int set_tiles(FastPath *f, const std::vector<int> &W) {
   const double dxerr = 2.0 * span * eps + 2.0 * f->H * eps;
   double g = 8.0 * (2.0 * dxerr / f->rho + 4.0 * eps);
   g = std::max(g, 64.0 * eps);
-  f->r2lo = (float)(f->rho2 * (1.0 - g));
-  f->r2hi = (float)(f->rho2 * (1.0 + g));
+  f->r2c = (float)f->rho2;
+  f->r2h = (float)(f->rho2 * g);
+  f->r2k = (float)std::ldexp(1.0, std::ilogb(4.0 / (double)f->r2h) + 1);  // power of two, r2k * r2h > 4
   return SPSS_OK;
 }
 
@@ -331,8 +332,9 @@ static FastArgs args_of(const FastPath *f) {
   a.ovf_list = f->ovf_list;
   a.ovf_count = f->ovf_count;
   a.H = f->H;
-  a.r2lo = f->r2lo;
-  a.r2hi = f->r2hi;
+  a.r2c = f->r2c;
+  a.r2h = f->r2h;
+  a.r2k = f->r2k;
   return a;
 }
 
