@@ -10,7 +10,8 @@
 //    phasor sums of every particle in the old order; fr_move decides each particle's
 //    move (its new position depends on phase and omega only) and counts the moves per
 //    (cell, direction) and the arrivals per cell, a two-level scan of the arrival counts
-//    gives the new cell starts, and fr_scatter
+//    gives the new cell starts (block-local in fr_scan_overflow, whose last scan block also
+//    scans the block totals; the totals are added in fr_far_bucket), and fr_scatter
 //    applies the step and writes every particle to its new slot: inside a cell, particles that
 //    stayed come first in their previous order, then arrivals grouped by the direction
 //    they came from, each group in its previous order, then the rare particles that jumped
@@ -32,6 +33,7 @@ constexpr int kPad = 4;         // padding entries after each staged row (>= the
 constexpr int kMaxRows = 7;     // 2k+1 with k <= 3
 constexpr int kMaxCols = 48;    // W + 2k
 constexpr int kScanItems = 8;   // counts per thread in the two-level scans
+constexpr int kRowTileCols = 1024;  // rows of at most this many cells get per-step tiles
 }  // namespace
 
 struct FastArgs {
@@ -40,7 +42,7 @@ struct FastArgs {
   int4 *ovf_list;              // work of tiles whose staged cells overflow the LDS stage, for fk_pairs_ovf:
                                // {first cell of the replica, row, ca, cb} a column span that fits, or
                                // {.., .., column, -1 - j} particles j*kFB.. of a cell that does not
-  int *ovf_count;              // their number, zeroed by fk_scan_add
+  int *ovf_count;              // their number, zeroed by fk_scan_add or fr_far_bucket
   double H;
   // guard band of the FP32 range test: |d^2 - r2c| <= r2h is decided in FP64; r2k is a power of
   // two with r2k * r2h > 4, so clamp(-r2k * (d^2 - r2c), 0, 1) is exactly 1 below the band, 0 above
@@ -93,6 +95,7 @@ struct StepArgs {
   const double *noise;
   uint64_t seed, step;
   const unsigned long long *step_dev;  // when set, the Philox step is read here (graph replays)
+  unsigned long long *step_inc;        // when set, fr_move advances this counter by one first
   const Coef *coef;                    // when set, per-replica xi, sigma, noise intensities, seed
 };
 
@@ -196,6 +199,59 @@ __global__ __launch_bounds__(256) void fk_scan_add(int *__restrict__ count, int 
       count[c] = 0;
       if (count2) count2[c] = 0;
     }
+  }
+}
+
+// One scan block of the resident step: block-local starts into lstart and the block total into
+// blocksum; the last block to finish (counted in *done, which it resets) writes the exclusive
+// prefix of the nsb block totals to boff[0..nsb].  The block totals are published by a
+// device-scope fence before the count and read after one, so no block waits for another.
+__device__ __forceinline__ void scan_local_last(const int *__restrict__ count, const int *__restrict__ count2, int m,
+                                                int *__restrict__ lstart, int *blocksum, int blk, int nsb, int *done,
+                                                int *__restrict__ boff) {
+  scan_local_block(count, count2, m, lstart, blocksum, blk);
+  __shared__ int last;
+  __shared__ int carry;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    last = atomicAdd(done, 1) == nsb - 1;
+    carry = 0;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  __shared__ int sh[256];
+  for (int b0 = 0; b0 < nsb; b0 += 256 * kScanItems) {
+    const int base = b0 + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+      v[q] = base + q < nsb ? blocksum[base + q] : 0;
+      sum += v[q];
+    }
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    int run = carry + sh[threadIdx.x] - sum;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+      if (base + q < nsb) boff[base + q] = run;
+      run += v[q];
+    }
+    __syncthreads();  // every thread has read carry and sh
+    if (threadIdx.x == 255) carry += sh[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    boff[nsb] = carry;
+    *done = 0;
   }
 }
 
@@ -610,6 +666,7 @@ __global__ __launch_bounds__(kFB) void fk_pairs(DevParams P, FastArgs A, PairBuf
   const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (tile >= A.ntiles) return;
   const int4 td = A.tiles[tile];
+  if (td.z >= td.w) return;  // an unused row-tile slot
   pairs_span<S, SRK, MODE, true>(P, A, B, td.x, td.y, td.z, td.w, L);  // overflow: split for fk_pairs_ovf
 }
 
@@ -750,10 +807,10 @@ __global__ __launch_bounds__(kFB) void fk_pairs_ovf(DevParams P, FastArgs A, Pai
 // the pair kernel (its sums are needed only by fr_scatter), saving a dependent launch per step
 template <int S, bool SRK>
 __global__ __launch_bounds__(kFB) void fr_scan_overflow(DevParams P, FastArgs A, PairBufs B, const int *count,
-                                                        const int *count2, int m, int *start, int *blocksum,
-                                                        int nsb) {
+                                                        const int *count2, int m, int *lstart, int *blocksum,
+                                                        int nsb, int *done, int *boff) {
   if ((int)blockIdx.x < nsb)
-    scan_local_block(count, count2, m, start, blocksum, blockIdx.x);
+    scan_local_last(count, count2, m, lstart, blocksum, blockIdx.x, nsb, done, boff);
   else
     overflow_items<S, SRK, 3>(P, A, B, blockIdx.x - nsb, gridDim.x - nsb);
 }
@@ -846,6 +903,9 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int
                                                int *__restrict__ farcnt, int *__restrict__ farslot,
                                                int *__restrict__ cnt9, int *__restrict__ newcnt) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  // graph-replayed steps: the counter read by this step's fr_scatter moves on here, one kernel
+  // after the previous step's fr_scatter read it
+  if (U.step_inc && p == 0) *U.step_inc += 1ull;
   const bool valid = p < n;
   int cell = -1, d = kFar, dest = 0;  // invalid lanes: a key of their own that is never counted
   if (valid) {
@@ -886,23 +946,82 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int
   }
 }
 
-// far arrivals: unordered bucket behind the near arrivals of their destination cell; clears the
-// next step's move counters
+// The tiles of one row of cells for the next pair kernel, built by one wave from the row's cell
+// starts P[0..ncf] (LDS): greedy spans of whole cells, each as wide as possible with at most kFB
+// particles and at most wcap (<= 64) columns, span i in slot i of the row's cmax slots, empty
+// spans in the unused slots.  Full spans keep the workgroup's waves full where fixed-width
+// tiles leave the last wave of most tiles part-empty.  With cmax * wcap >= ncf every span stays
+// within wcap columns: a span is widened past the particle bound (a dense row) only as far as
+// the remaining slots need to cover the rest of the row.
+__device__ __forceinline__ void row_tiles(const int *P, int cbase, int row, int ncf, int wcap, int cmax,
+                                          int4 *__restrict__ slots) {
+  const int lane = threadIdx.x & 63;
+  int a = 0, i = 0;
+  while (a < ncf) {  // invariant: ncf - a <= (cmax - i) * wcap
+    const int e = a + 1 + lane;
+    // true for a leading run of lanes: P is non-decreasing
+    const bool fits = lane < wcap && e <= ncf && P[e] - P[a] <= kFB;
+    const int b = max(a + max(__popcll(__ballot(fits)), 1), ncf - (cmax - i - 1) * wcap);
+    if (lane == 0) slots[i] = make_int4(cbase, row, a, b);
+    a = b;
+    ++i;
+  }
+  for (int j = i + lane; j < cmax; j += 64) slots[j] = make_int4(cbase, row, 0, 0);
+}
+
+// rows [4 * blockIdx.x, 4 * blockIdx.x + 4) of a single system, one wave each; start(c) gives
+// the starts of the cell order the next pair kernel reads (c = 0 .. ncf * ncf).  Uniform per
+// workgroup, so the barrier is reached by all or none of its threads.
+template <class START>
+__device__ __forceinline__ void rows_to_tiles(START start, int ncf, int wcap, int cmax, int4 *__restrict__ tiles) {
+  __shared__ int P[4][kRowTileCols + 1];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + w;
+  if (row < ncf)
+    for (int c = lane; c <= ncf; c += 64) P[w][c] = start(row * ncf + c);
+  __syncthreads();
+  if (row < ncf) row_tiles(P[w], 0, row, ncf, wcap, cmax, tiles + (long)row * cmax);
+}
+
+// the row tiles of a cell order with final starts (entry into the resident order)
+__global__ __launch_bounds__(256) void fk_row_tiles(const int *__restrict__ start, int ncf, int wcap, int cmax,
+                                                    int4 *__restrict__ tiles) {
+  rows_to_tiles([&](int c) { return start[c]; }, ncf, wcap, cmax, tiles);
+}
+
+// far arrivals: unordered bucket behind the near arrivals of their destination cell.  Also the
+// second scan pass of the new cell starts (nstart[c] = lstart[c] + boff[block of c], from the
+// block-local starts and the prefix of the block totals left by fr_scan_overflow), the clearing
+// of the arrival counts it summed, of the next step's move counters and of the overflow count.
 template <int S, bool SRK>
-__global__ void fr_far_bucket(DevParams P, int n, int ncf, int cpr, const int *__restrict__ mv,
-                              const double *__restrict__ rstate, StepArgs U, const int *__restrict__ nstart,
-                              const int *__restrict__ cnt9, const int *__restrict__ farslot,
-                              int *__restrict__ bucket, int *__restrict__ cnt9_next, long nclear) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void fr_far_bucket(DevParams P, int n, int ncf, int cpr, const int *__restrict__ mv,
+                              const double *__restrict__ rstate, StepArgs U, const int *__restrict__ lstart,
+                              const int *__restrict__ boff, int m, int *__restrict__ nstart, int *__restrict__ count,
+                              int *__restrict__ count2, int *__restrict__ ovf_count, const int *__restrict__ cnt9,
+                              const int *__restrict__ farslot, int *__restrict__ bucket, int *__restrict__ cnt9_next,
+                              long nclear) {
+  constexpr int kBlock = 256 * kScanItems;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
   // the other move-count buffer (last read by the previous step) starts the next step at zero
-  for (long e = p; e < nclear; e += (long)gridDim.x * blockDim.x) cnt9_next[e] = 0;
+  for (long e = p; e < nclear; e += stride) cnt9_next[e] = 0;
+  for (long c = p; c <= m; c += stride) {
+    if (c < m) {
+      nstart[c] = lstart[c] + boff[c / kBlock];
+      count[c] = 0;
+      count2[c] = 0;
+    } else {
+      nstart[m] = boff[(m + kBlock - 1) / kBlock];
+    }
+  }
+  if (p == 0) *ovf_count = 0;
   if (p >= n) return;
-  const int m = mv[p];
-  if ((m & 15) != kFar) return;
+  const int mk = mv[p];
+  if ((mk & 15) != kFar) return;
   double sn, cn, x1, y1;
   step_position<S, SRK>(P, rstate + (long)p * S, U, sn, cn, x1, y1);
-  const int cd = cell_base(m >> 4, cpr) + cell_of_position<S>(P, x1, y1, ncf);
-  bucket[nstart[cd] + new_count_of(cnt9, cd, ncf, cpr) + farslot[p]] = p;
+  const int cd = cell_base(mk >> 4, cpr) + cell_of_position<S>(P, x1, y1, ncf);
+  bucket[lstart[cd] + boff[cd / kBlock] + new_count_of(cnt9, cd, ncf, cpr) + farslot[p]] = p;
 }
 
 // the step of every particle, written to its new slot with its id, natural frequency and
@@ -914,7 +1033,10 @@ __global__ __launch_bounds__(256) void fr_scatter(
     const double *__restrict__ rstate, const int *__restrict__ rid, const double *__restrict__ rnat,
     const double2 *__restrict__ ssum, const double2 *__restrict__ tsum, const int *__restrict__ scnt, StepArgs U,
     double *__restrict__ compl_out, double *__restrict__ nstate, int *__restrict__ nrid, double *__restrict__ nnat,
-    float4 *__restrict__ snap, float *__restrict__ omega_f) {
+    float4 *__restrict__ snap, float *__restrict__ omega_f, int4 *__restrict__ rtiles, int wcap, int cmax) {
+  // the next pair kernel's row tiles (single systems) from the new starts, in the first
+  // workgroups: their serial span search hides behind the other workgroups' memory traffic
+  if (rtiles && (int)blockIdx.x * 4 < ncf) rows_to_tiles([&](int c) { return nstart[c]; }, ncf, wcap, cmax, rtiles);
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int cpr = L.cpr;

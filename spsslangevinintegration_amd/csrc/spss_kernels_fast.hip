@@ -52,6 +52,9 @@ struct FastPath {
   int4 *ovf_list = nullptr;  // work split off overflowing tiles
   int *ovf_count = nullptr;
   int ntiles = 0, tiles_cap = 0;
+  // single systems: per-step row tiles of the resident order (cmax slots per row of cells)
+  int4 *rtiles = nullptr;
+  int rt_cmax = 0, rt_wcap = 0;
   Coef *coef = nullptr;                    // per-replica coefficients (batches)
   unsigned long long *step_dev = nullptr;  // device step counter (batches, graph-replayed steps)
   int gen = 0;                             // bumped whenever launch shapes change (tile table)
@@ -67,6 +70,9 @@ struct FastPath {
   // rebuilt cell list (RHS, pair export, entry into resident mode)
   int *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr, *sorted = nullptr;
   int *blocksum = nullptr;
+  // the resident step's scan: block-local starts, the prefix of the block totals, the count of
+  // finished scan blocks (zero between steps)
+  int *lstart = nullptr, *boff = nullptr, *scan_done = nullptr;
   float4 *snap = nullptr;
   float *omega_f = nullptr;
   // resident copy of the fused steps (double-buffered where a step permutes)
@@ -217,6 +223,18 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->tmp, n));
   A(falloc(&f->sorted, n));
   A(falloc(&f->blocksum, f->nsb + 1));
+  A(falloc(&f->lstart, cells));
+  A(falloc(&f->boff, f->nsb + 1));
+  A(falloc(&f->scan_done, 1));
+  if (R == 1 && f->ncf <= kRowTileCols) {
+    // slots per row: the greedy spans of a row 1.5x the mean, at about 94% of kFB each, plus 2,
+    // and at least enough spans of rt_wcap columns to cover a row
+    const double row_mean = (double)P.n / (double)f->ncf;
+    f->rt_wcap = std::min(64, kMaxCols - 2 * f->k);
+    f->rt_cmax = std::max((f->ncf + f->rt_wcap - 1) / f->rt_wcap,
+                          std::min(f->ncf, (int)std::ceil(1.5 * row_mean / (0.94 * kFB)) + 2));
+    A(falloc(&f->rtiles, (size_t)f->ncf * f->rt_cmax));
+  }
   A(falloc(&f->snap, n));
   A(falloc(&f->omega_f, n));
   for (int b = 0; b < 2; ++b) {
@@ -266,6 +284,7 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   launch_fill_int(st, (long)cells * 9, f->cnt9[1], 0);
   launch_fill_int(st, cells, f->newcnt, 0);
   launch_fill_int(st, 1, f->ovf_count, 0);
+  launch_fill_int(st, 1, f->scan_done, 0);
   if (coef) {
     launch_fill_double(st, 3L * R, f->racc, 0.0);
     launch_fill_double(st, 2L * R, f->zlast, 0.0);
@@ -287,7 +306,8 @@ void fast_destroy(FastPath *f) {
                f->snap,      f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],  f->rid[1],    f->rnat[0],
                f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mv,
                f->cnt9[0],   f->cnt9[1],   f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
-               f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc,      f->ovf_list,  f->ovf_count};
+               f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc,      f->ovf_list,  f->ovf_count,
+               f->lstart,    f->boff,      f->scan_done, f->rtiles};
   for (void *p : b) hipFree(p);
   delete f;
 }
@@ -378,8 +398,12 @@ static PairBufs rebuilt_bufs(const FastPath *f, const double *stage) {
 // the pair kernel, then the pass over the tiles whose cells overflowed its LDS stage (usually none)
 template <int S, bool SRK, int MODE>
 static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, const PairBufs &B) {
-  const FastArgs a = args_of(f);
-  fk_pairs<S, SRK, MODE><<<tile_grid(f->ntiles), kFB, 0, st>>>(P, a, B);
+  FastArgs a = args_of(f);
+  if (MODE == 3 && f->rtiles) {  // the resident order's row tiles, rebuilt every step
+    a.tiles = f->rtiles;
+    a.ntiles = f->ncf * f->rt_cmax;
+  }
+  fk_pairs<S, SRK, MODE><<<tile_grid(a.ntiles), kFB, 0, st>>>(P, a, B);
   // the resident step (MODE 3) runs the overflow work inside its first scan launch (fr_scan_overflow)
   if (MODE != 3) fk_pairs_ovf<S, SRK, MODE><<<kOverflowBlocks, kFB, 0, st>>>(P, a, B);
 }
@@ -431,6 +455,8 @@ static void resident_enter(FastPath *f, hipStream_t st, const DevParams &P, cons
     fr_gather<4><<<nb, 256, 0, st>>>(f->n, state, f->sorted, nat, f->rstate[c], f->rid[c], f->rnat[c]);
   else
     fr_gather<3><<<nb, 256, 0, st>>>(f->n, state, f->sorted, nat, f->rstate[c], f->rid[c], f->rnat[c]);
+  if (f->rtiles)
+    fk_row_tiles<<<(f->ncf + 3) / 4, 256, 0, st>>>(f->rstart[c], f->ncf, f->rt_wcap, f->rt_cmax, f->rtiles);
   f->resident = true;
   f->resident_nat = nat != nullptr;
 }
@@ -448,15 +474,17 @@ static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const
   fr_move<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->rstate[c], U, f->mv, f->farcnt, f->slot, f->cnt9[c],
                                       f->newcnt);
   fr_scan_overflow<S, SRK><<<f->nsb + kOverflowBlocks, kFB, 0, st>>>(P, args_of(f), B, f->newcnt, f->farcnt,
-                                                                     f->cells, f->rstart[o], f->blocksum, f->nsb);
-  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->newcnt, f->farcnt, f->cells, f->rstart[o], f->blocksum, f->nsb,
-                                      f->ovf_count);
-  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, f->mv, f->rstate[c], U, f->rstart[o],
-                                           f->cnt9[c], f->slot, f->tmp, f->cnt9[o], (long)f->cells * 9);
-  fr_scatter<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], f->mv, f->cnt9[c], f->tmp,
+                                                                     f->cells, f->lstart, f->blocksum, f->nsb,
+                                                                     f->scan_done, f->boff);
+  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, f->mv, f->rstate[c], U, f->lstart, f->boff,
+                                           f->cells, f->rstart[o], f->newcnt, f->farcnt, f->ovf_count, f->cnt9[c],
+                                           f->slot, f->tmp, f->cnt9[o], (long)f->cells * 9);
+  // (at least one workgroup per four rows of cells for the row tiles)
+  const unsigned nbs = std::max(nb, (unsigned)(f->rtiles ? (f->ncf + 3) / 4 : 0));
+  fr_scatter<S, SRK><<<nbs, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], f->mv, f->cnt9[c], f->tmp,
                                         f->rstate[c], f->rid[c], rnat, f->ssum, f->tsum, f->scnt, U,
                                         SRK ? compl_out : nullptr, f->rstate[o], f->rid[o], f->rnat[o], f->rsnap,
-                                        SRK ? f->romega : nullptr);
+                                        SRK ? f->romega : nullptr, f->rtiles, f->rt_wcap, f->rt_cmax);
   f->cur = o;
 }
 
@@ -521,8 +549,9 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
 
 // ---------------------------------------------------------------------------
 // graph-replayed fused steps of a single system: kGraphSteps steps (even, so that the double
-// buffers end where they started) captured once, each reading its Philox step from the device
-// counter and advancing it, then replayed; launch gaps between the six kernels of a step vanish
+// buffers end where they started) captured once, each advancing the device step counter in its
+// fr_move and reading it as its Philox step in its fr_scatter, then replayed; launch gaps
+// between the five kernels of a step vanish
 static constexpr int kGraphSteps = 10;
 
 __global__ void fs_set_counter(unsigned long long *c, unsigned long long v) {
@@ -553,12 +582,12 @@ long fast_run_graph(FastPath *f, hipStream_t st, const DevParams &P, int srk, co
     drop_step_graph(f);
     StepArgs U = step_args(dt, D_phi, friction, 3, nullptr, seed, 0);
     U.step_dev = f->step_dev;
+    U.step_inc = f->step_dev;  // each step's fr_move advances the counter before its fr_scatter
     if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return 0;
     const int cur0 = f->cur;
     int rc = SPSS_OK;
     for (int k = 0; k < kGraphSteps && rc == SPSS_OK; ++k) {
       rc = step_impl(f, st, P, do_srk, state, nat, compl_out, U);
-      fb_step_inc<<<1, 64, 0, st>>>(f->step_dev);
     }
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(st, &g);
@@ -587,7 +616,7 @@ long fast_run_graph(FastPath *f, hipStream_t st, const DevParams &P, int srk, co
     f->sg_nat = nat;
     f->sg_compl = compl_out;
   }
-  fs_set_counter<<<1, 64, 0, st>>>(f->step_dev, (unsigned long long)step);
+  fs_set_counter<<<1, 64, 0, st>>>(f->step_dev, (unsigned long long)step - 1ull);  // advanced before use
   for (long r = 0; r < reps; ++r) {
     const hipError_t el = hipGraphLaunch(f->sg_exec, st);
     if (el != hipSuccess) {

@@ -93,6 +93,28 @@ def test_resident_steps_track_fp64(spss, model, n, rho, sub, v0, stepper):
     assert g32.step == 30
 
 
+def test_resident_clustered_rows_track_fp64(spss):
+    """dense clusters in one row of cells: cells far above one workgroup's particles (dense-cell
+    chunks of the overflow pass), stages that overflow, and a row that needs more spans than
+    the per-step row tiles provide (the last spans widened past the particle bound)"""
+    n, rho, v0, dt = 20000, 0.02, 0.01, 0.005
+    print(f"[gpu] resident clustered rows n={n} rho={rho}", flush=True)
+    rng = np.random.default_rng(17)
+    st = _state(n, 4, 17).reshape(-1, 4)
+    for q, x0 in enumerate((0.1, 0.4, 0.7)):  # three clusters of 2500 in the cell row [0.40, 0.42)
+        sl = slice(q * 2500, (q + 1) * 2500)
+        st[sl, 0] = x0 + 0.03 * rng.random(2500)
+        st[sl, 1] = 0.401 + 0.018 * rng.random(2500)
+    natf = rng.normal(0, 0.2, n)
+    g64, g32 = _pair(spss, 4, n, rho, 1, v0, natf=natf)
+    for g in (g64, g32):
+        g.set_state(st.reshape(-1))
+    for chunk in range(2):
+        for g in (g64, g32):
+            g.run(spss.STEPPER_EULER, spss.METHOD_LINKED_LIST, 10, dt, spss.NOISE_PHILOX)
+        _close(spss, 4, g32.get_state(), g64.get_state(), f"after {10 * (chunk + 1)} steps", v0)
+
+
 def test_resident_reads_do_not_perturb(spss):
     """get_state / order parameter / averaging between steps leave the run bit-identical."""
     n, rho = 30000, 0.015
