@@ -36,6 +36,7 @@ struct spss_batch {
   long graph_every = -1;
   long graph_from = -1;
   double graph_dt = 0.0;
+  int graph_parity = -1;  // double-buffer parity of the resident order the graph was captured at
 };
 
 namespace {
@@ -91,6 +92,9 @@ int steps_and_observe(spss_batch *b, int count, double dt, bool observe, long fr
 // order-parameter reading after every `every` steps when every > 0
 int capture(spss_batch *b, int steps, long every, long from, double dt) {
   drop_graph(b);
+  int rp = fast_prepare_moves(b->fast, b->st, b->P, dt);
+  if (rp != SPSS_OK) return fast_fail(rp);
+  const int parity0 = fast_buffer_parity(b->fast);
   BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed));
   int rc = SPSS_OK;
   for (int k = 0; k < steps && rc == SPSS_OK; ++k) {
@@ -117,6 +121,7 @@ int capture(spss_batch *b, int steps, long every, long from, double dt) {
   b->graph_every = every;
   b->graph_from = from;
   b->graph_dt = dt;
+  b->graph_parity = parity0;
   return SPSS_OK;
 }
 
@@ -247,7 +252,7 @@ int spss_batch_run(spss_batch *b, long nsteps, double dt, long observe_every, lo
     int gs = observe_every > 0 ? (int)observe_every : 10;
     if (gs % 2) gs *= 2;
     if (b->graph_steps != gs || b->graph_every != observe_every || b->graph_from != observe_from ||
-        b->graph_dt != dt || !b->exec) {
+        b->graph_dt != dt || b->graph_parity != fast_buffer_parity(b->fast) || !b->exec) {
       // keep the observation phase: a graph starts right after a reading
       while (observe_every > 0 && done < nsteps && done % observe_every != 0) {
         BTRY(steps_and_observe(b, 1, dt, (done + 1) % observe_every == 0, observe_from));
@@ -256,6 +261,8 @@ int spss_batch_run(spss_batch *b, long nsteps, double dt, long observe_every, lo
       if (nsteps - done >= gs) BTRY(capture(b, gs, observe_every, observe_from, dt));
     }
     while (b->exec && nsteps - done >= b->graph_steps && (observe_every == 0 || done % observe_every == 0)) {
+      const int rp = fast_prepare_moves(b->fast, b->st, b->P, dt);
+      if (rp != SPSS_OK) return fast_fail(rp);
       BCHK(hipGraphLaunch(b->exec, b->st));
       done += b->graph_steps;
     }

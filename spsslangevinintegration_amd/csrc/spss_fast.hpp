@@ -33,6 +33,9 @@ int fast_replica_count(const FastPath *f);
 // inside the replica, step from the device counter, which the call advances)
 int fast_batch_step(FastPath *f, hipStream_t st, const DevParams &P, const double *state, const double *nat,
                     double dt);
+// before capturing or replaying Euler steps of size dt: the resident order's current moves in
+// place for dt (each step computes the next step's moves, so a captured step never needs fr_move)
+int fast_prepare_moves(FastPath *f, hipStream_t st, const DevParams &P, double dt);
 // per-replica order parameter of the current state; |Z|, |Z|^2 and a count are added to the
 // running sums once the step counter has reached `from`
 int fast_batch_observe(FastPath *f, hipStream_t st, uint64_t from);
@@ -55,6 +58,8 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
               double *compl_out, double dt, double D_phi, double friction, int noise_mode, const double *noise,
               uint64_t seed, uint64_t step);
 bool fast_is_resident(const FastPath *f);
+// which of its double buffers the resident order is in (a captured step graph replays only there)
+int fast_buffer_parity(const FastPath *f);
 // writes the resident state back into `state` in original order; the copy stays resident
 // Fused steps replayed from a HIP graph (Philox noise from the device step counter): runs the
 // largest multiple of the graph's step count not above nsteps, starting at Philox step `step`,

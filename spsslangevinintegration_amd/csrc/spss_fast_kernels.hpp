@@ -396,36 +396,37 @@ template <int S, bool SRK, int MODE, class GOF>
 __device__ __forceinline__ void scan_range(const DevParams &P, const PairBufs &B, const float4 *sdat,
                                            const float *somega, int lo, int hi, float xi_t, float syi,
                                            float r2c, float r2h, float r2k, const double *pi, int self_t, GOF gof,
-                                           float *ax, float *ay, float *an, float *bx, float *by, int &nnb,
+                                           float *ax, float *ay, float &an, float *bx, float *by, int &nnb,
                                            int &w, int wbase) {
   constexpr bool SUMS = MODE == 0 || MODE >= 3;
   const float4 *p = sdat + lo;
   const float4 *const pe = sdat + hi;
   for (; p < pe; p += 4) {
     float4 o[4];
-    float m[4];
-    bool band[4];
+    float m[4], dd[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       o[j] = p[j];
       const float dx = o[j].x - xi_t;
       const float dy = o[j].y - syi;
-      const float dd = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, -r2c));
-      band[j] = __builtin_fabsf(dd) <= r2h;
-      m[j] = __builtin_amdgcn_fmed3f(-r2k * dd, 0.f, 1.f);
+      dd[j] = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, -r2c));
+      m[j] = __builtin_amdgcn_fmed3f(-r2k * dd[j], 0.f, 1.f);
     }
-    if (band[0] || band[1] || band[2] || band[3]) {  // rare: FP64 decision inside the stencil
+    // one test for the four: the smallest |dd| inside the band
+    const float dmin = __builtin_fminf(__builtin_fminf(__builtin_fabsf(dd[0]), __builtin_fabsf(dd[1])),
+                                       __builtin_fminf(__builtin_fabsf(dd[2]), __builtin_fabsf(dd[3])));
+    if (dmin <= r2h) {  // rare: FP64 decision inside the stencil
       const int t = (int)(p - sdat);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (band[j]) m[j] = ((t + j < hi) && band_in<S, MODE>(P, B, pi, gof(t + j))) ? 1.f : 0.f;
+        if (__builtin_fabsf(dd[j]) <= r2h) m[j] = ((t + j < hi) && band_in<S, MODE>(P, B, pi, gof(t + j))) ? 1.f : 0.f;
     }
     if (SUMS) {
+      an += (m[0] + m[1]) + (m[2] + m[3]);  // a count: exact in any order
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         ax[j] = __builtin_fmaf(m[j], o[j].z, ax[j]);
         ay[j] = __builtin_fmaf(m[j], o[j].w, ay[j]);
-        an[j] += m[j];
         if (SRK) {
           const float mo = m[j] * somega[(p - sdat) + j];
           bx[j] = __builtin_fmaf(mo, o[j].z, bx[j]);
@@ -446,12 +447,12 @@ __device__ __forceinline__ void scan_range(const DevParams &P, const PairBufs &B
 }
 
 template <bool SRK>
-__device__ __forceinline__ void flush_row(const float *ax, const float *ay, const float *an, const float *bx,
+__device__ __forceinline__ void flush_row(const float *ax, const float *ay, float an, const float *bx,
                                           const float *by, double &sx, double &sy, double &tx, double &ty,
                                           int &nnb) {
   sx += ((double)ax[0] + (double)ax[1]) + ((double)ax[2] + (double)ax[3]);
   sy += ((double)ay[0] + (double)ay[1]) + ((double)ay[2] + (double)ay[3]);
-  nnb += (int)(an[0] + an[1] + an[2] + an[3]);
+  nnb += (int)an;
   if (SRK) {
     tx += ((double)bx[0] + (double)bx[1]) + ((double)bx[2] + (double)bx[3]);
     ty += ((double)by[0] + (double)by[1]) + ((double)by[2] + (double)by[3]);
@@ -637,7 +638,7 @@ __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A
         }
         return LIST ? L.sg[t] : cglob[lo2] + (t - cpos[lo2]);
       };
-      float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an[4] = {0.f, 0.f, 0.f, 0.f};
+      float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an = 0.f;
       float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
       scan_range<S, SRK, MODE>(P, B, sdat, L.somega, lo, hi, xi_t, syi, r2c, r2h, r2k, pi, li, gof, ax, ay, an, bx, by,
                                nnb, w, wbase);
@@ -722,7 +723,7 @@ __device__ __forceinline__ void pairs_dense_cell(const DevParams &P, const FastA
       int len = 0;
       for (int dc = 0; dc < nrows; ++dc) len += L.ccnt[dc];
       const float syi = (float)((double)yi - (double)(r - k) * H);
-      float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an[4] = {0.f, 0.f, 0.f, 0.f};
+      float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an = 0.f;
       float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
       for (int c0 = 0; c0 < len; c0 += kChunk) {
         const int clen = min(kChunk, len - c0);
@@ -892,24 +893,19 @@ __device__ __forceinline__ int cell_of_position(const DevParams &P, double x, do
 // ---------------------------------------------------------------------------
 // resident re-sort
 
-// the move of every resident particle this step: old cell * 16 + (dy+1)*3 + (dx+1), or kFar
-// with an unordered arrival slot in farcnt of the destination.  The near moves are counted per
-// (old cell, direction) into cnt9 (cleared by the previous step's fr_far_bucket) and per destination
-// into newcnt (zeroed by the scan): particles lie in cell order, so a wave's equal (cell,
-// direction) keys form runs and one lane per run adds the run's length.
+// The move of one resident particle from its state pi (at slot p of the cell order): old cell * 16
+// + (dy+1)*3 + (dx+1), or kFar with an unordered arrival slot in farcnt of the destination; then
+// the near moves counted per (old cell, direction) into cnt9 and per destination into newcnt.
+// Particles lie in cell order, so a wave's equal (cell, direction) keys form runs and one lane per
+// run adds the run's length.  Every lane of the wave calls this (valid = false for lanes past
+// the end): the run search reads its neighbours.
 template <int S, bool SRK>
-__global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int ncf,
-                                               const double *__restrict__ rstate, StepArgs U, int *__restrict__ mv,
+__device__ __forceinline__ void move_and_count(const DevParams &P, const Layout &L, int ncf, const double *pi,
+                                               const StepArgs &U, int p, bool valid, int *__restrict__ mv,
                                                int *__restrict__ farcnt, int *__restrict__ farslot,
                                                int *__restrict__ cnt9, int *__restrict__ newcnt) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  // graph-replayed steps: the counter read by this step's fr_scatter moves on here, one kernel
-  // after the previous step's fr_scatter read it
-  if (U.step_inc && p == 0) *U.step_inc += 1ull;
-  const bool valid = p < n;
   int cell = -1, d = kFar, dest = 0;  // invalid lanes: a key of their own that is never counted
   if (valid) {
-    const double *pi = rstate + (long)p * S;
     double sn, cn, x1, y1;
     step_position<S, SRK>(P, pi, U, sn, cn, x1, y1);
     double xw, yw;
@@ -933,7 +929,6 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int
     dest = base + ny * ncf + nx;
     mv[p] = (cell << 4) | d;
   }
-  // runs of equal keys inside the wave (every lane takes part: no early return above)
   const int lane = threadIdx.x & 63;
   const int pc = __shfl_up(cell, 1, 64), pd = __shfl_up(d, 1, 64);
   const bool head = lane == 0 || pc != cell || pd != d;
@@ -944,6 +939,19 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int
     atomicAdd(cnt9 + (long)cell * 9 + d, len);
     atomicAdd(newcnt + dest, len);
   }
+}
+
+// the moves of a resident order whose moves no fr_scatter has computed yet (entry into the
+// resident order, or a step with another dt or stepper); cnt9, farcnt and newcnt start at zero
+template <int S, bool SRK>
+__global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int ncf,
+                                               const double *__restrict__ rstate, StepArgs U, int *__restrict__ mv,
+                                               int *__restrict__ farcnt, int *__restrict__ farslot,
+                                               int *__restrict__ cnt9, int *__restrict__ newcnt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = p < n;
+  move_and_count<S, SRK>(P, L, ncf, valid ? rstate + (long)p * S : rstate, U, p, valid, mv, farcnt, farslot, cnt9,
+                         newcnt);
 }
 
 // The tiles of one row of cells for the next pair kernel, built by one wave from the row's cell
@@ -1002,6 +1010,9 @@ __global__ __launch_bounds__(256) void fr_far_bucket(DevParams P, int n, int ncf
                               long nclear) {
   constexpr int kBlock = 256 * kScanItems;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  // graph-replayed steps: the counter read by this step's fr_scatter moves on here, one kernel
+  // after the previous step's fr_scatter read it
+  if (U.step_inc && p == 0) *U.step_inc += 1ull;
   const long stride = (long)gridDim.x * blockDim.x;
   // the other move-count buffer (last read by the previous step) starts the next step at zero
   for (long e = p; e < nclear; e += stride) cnt9_next[e] = 0;
@@ -1021,11 +1032,13 @@ __global__ __launch_bounds__(256) void fr_far_bucket(DevParams P, int n, int ncf
   double sn, cn, x1, y1;
   step_position<S, SRK>(P, rstate + (long)p * S, U, sn, cn, x1, y1);
   const int cd = cell_base(mk >> 4, cpr) + cell_of_position<S>(P, x1, y1, ncf);
-  bucket[lstart[cd] + boff[cd / kBlock] + new_count_of(cnt9, cd, ncf, cpr) + farslot[p]] = p;
+  const int q = lstart[cd] + boff[cd / kBlock] + new_count_of(cnt9, cd, ncf, cpr) + farslot[p];
+  if (q < n) bucket[q] = p;  // always, when the counts match the moves
 }
 
 // the step of every particle, written to its new slot with its id, natural frequency and
-// snapshot
+// snapshot; then (mvn set) the move of the next step from the new state, counted into cnt9n,
+// farcnt and newcnt as fr_move would (the new position depends on phase and omega only)
 template <int S, bool SRK>
 __global__ __launch_bounds__(256) void fr_scatter(
     DevParams P, Layout L, int n, int ncf, double H, const int *__restrict__ start, const int *__restrict__ nstart,
@@ -1033,63 +1046,74 @@ __global__ __launch_bounds__(256) void fr_scatter(
     const double *__restrict__ rstate, const int *__restrict__ rid, const double *__restrict__ rnat,
     const double2 *__restrict__ ssum, const double2 *__restrict__ tsum, const int *__restrict__ scnt, StepArgs U,
     double *__restrict__ compl_out, double *__restrict__ nstate, int *__restrict__ nrid, double *__restrict__ nnat,
-    float4 *__restrict__ snap, float *__restrict__ omega_f, int4 *__restrict__ rtiles, int wcap, int cmax) {
+    float4 *__restrict__ snap, float *__restrict__ omega_f, int4 *__restrict__ rtiles, int wcap, int cmax,
+    int *__restrict__ mvn, int *__restrict__ farcnt, int *__restrict__ farslot, int *__restrict__ cnt9n,
+    int *__restrict__ newcnt) {
   // the next pair kernel's row tiles (single systems) from the new starts, in the first
   // workgroups: their serial span search hides behind the other workgroups' memory traffic
   if (rtiles && (int)blockIdx.x * 4 < ncf) rows_to_tiles([&](int c) { return nstart[c]; }, ncf, wcap, cmax, rtiles);
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int cpr = L.cpr;
-  const int m = mv[p];
-  const int cs = m >> 4, d = m & 15;
-  const int r = cs / cpr, base = r * cpr;
-  Coef K;
-  if (U.coef) {
-    K = U.coef[r];
-  } else {
-    K.xi = P.xi;
-    K.sigma = P.sigma;
-    K.int_phi = U.int_phi;
-    K.sq2d = U.sq2d;
-    K.seed = U.seed;
-  }
-  const uint64_t step = U.step_dev ? (uint64_t)*U.step_dev : U.step;
-  const int id = rid[p];
-  const double nat_i = rnat ? rnat[p] : 0.0;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = p < n;
   double out[S];
-  double C;
-  // noise and injected draws are indexed by the particle's index inside its replica
-  step_particle<S, SRK>(P, K, rstate + (long)p * S, id - L.rep_first[r], nat_i, ssum[p],
-                        SRK ? tsum[p] : make_double2(0.0, 0.0), scnt[p], U, step, out, C);
-  if (SRK && compl_out) compl_out[id] = C;
-  int cd, pos;
-  if (d == kFar) {
-    cd = base + cell_of_position<S>(P, out[0], out[1], ncf);
-    const int b = nstart[cd] + new_count_of(cnt9, cd, ncf, cpr);
-    const int e = nstart[cd + 1];
-    int rank = 0;
-    for (int q = b; q < e; ++q) rank += (bucket[q] < p) ? 1 : 0;
-    pos = b + rank;
-  } else {
-    int rank = 0;
-    for (int q = start[cs]; q < p; ++q) rank += ((mv[q] & 15) == d) ? 1 : 0;
-    const int ls = cs - base;
-    const int dcx = d % 3 - 1, dcy = d / 3 - 1;
-    cd = base + ((ls / ncf + dcy + ncf) % ncf) * ncf + (ls % ncf + dcx + ncf) % ncf;
-    int b0 = 0;
-    const int od = move_order(d);
-    for (int e = 0; e < 9; ++e)
-      if (move_order(e) < od) b0 += cnt9[(long)move_source(cd, e, ncf, cpr) * 9 + e];
-    pos = nstart[cd] + b0 + rank;
-  }
-  double *dst = nstate + (long)pos * S;
 #pragma unroll
-  for (int q = 0; q < S; ++q) dst[q] = out[q];
-  nrid[pos] = id;
-  if (rnat) nnat[pos] = nat_i;
-  const int ld = cd - base;
-  snap[pos] = snap_entry<S>(P, out, ld % ncf, ld / ncf, H);
-  if (omega_f) omega_f[pos] = (S == 4) ? (float)out[3] : 0.f;
+  for (int q = 0; q < S; ++q) out[q] = 0.0;
+  int pos = 0;
+  if (valid) {
+    const int cpr = L.cpr;
+    const int m = mv[p];
+    const int cs = m >> 4, d = m & 15;
+    const int r = cs / cpr, base = r * cpr;
+    Coef K;
+    if (U.coef) {
+      K = U.coef[r];
+    } else {
+      K.xi = P.xi;
+      K.sigma = P.sigma;
+      K.int_phi = U.int_phi;
+      K.sq2d = U.sq2d;
+      K.seed = U.seed;
+    }
+    const uint64_t step = U.step_dev ? (uint64_t)*U.step_dev : U.step;
+    const int id = rid[p];
+    const double nat_i = rnat ? rnat[p] : 0.0;
+    double C;
+    // noise and injected draws are indexed by the particle's index inside its replica
+    step_particle<S, SRK>(P, K, rstate + (long)p * S, id - L.rep_first[r], nat_i, ssum[p],
+                          SRK ? tsum[p] : make_double2(0.0, 0.0), scnt[p], U, step, out, C);
+    if (SRK && compl_out) compl_out[id] = C;
+    int cd;
+    if (d == kFar) {
+      cd = base + cell_of_position<S>(P, out[0], out[1], ncf);
+      const int b = nstart[cd] + new_count_of(cnt9, cd, ncf, cpr);
+      const int e = nstart[cd + 1];
+      int rank = 0;
+      for (int q = b; q < e; ++q) rank += (bucket[q] < p) ? 1 : 0;
+      pos = b + rank;
+    } else {
+      int rank = 0;
+      for (int q = start[cs]; q < p; ++q) rank += ((mv[q] & 15) == d) ? 1 : 0;
+      const int ls = cs - base;
+      const int dcx = d % 3 - 1, dcy = d / 3 - 1;
+      cd = base + ((ls / ncf + dcy + ncf) % ncf) * ncf + (ls % ncf + dcx + ncf) % ncf;
+      int b0 = 0;
+      const int od = move_order(d);
+      for (int e = 0; e < 9; ++e)
+        if (move_order(e) < od) b0 += cnt9[(long)move_source(cd, e, ncf, cpr) * 9 + e];
+      pos = nstart[cd] + b0 + rank;
+    }
+    valid = pos < n;  // always, when the counts match the moves
+    if (valid) {
+      double *dst = nstate + (long)pos * S;
+#pragma unroll
+      for (int q = 0; q < S; ++q) dst[q] = out[q];
+      nrid[pos] = id;
+      if (rnat) nnat[pos] = nat_i;
+      const int ld = cd - base;
+      snap[pos] = snap_entry<S>(P, out, ld % ncf, ld / ncf, H);
+      if (omega_f) omega_f[pos] = (S == 4) ? (float)out[3] : 0.f;
+    }
+  }
+  if (mvn) move_and_count<S, SRK>(P, L, ncf, out, U, pos, valid, mvn, farcnt, farslot, cnt9n, newcnt);
 }
 
 // advance the device step counter of graph-replayed runs

@@ -89,7 +89,15 @@ struct FastPath {
   double2 *ssum = nullptr, *tsum = nullptr;
   int *scnt = nullptr;
   int *cnt9[2] = {nullptr, nullptr};  // move counts per (cell, direction), by step parity (cur)
-  int *mv = nullptr, *newcnt = nullptr, *farcnt = nullptr;
+  int *newcnt = nullptr, *farcnt = nullptr;
+  // moves of the current step by slot, double-buffered (mvc): every fr_scatter computes the
+  // next step's moves and counts from the new state; mv_ready says they are there, for the
+  // dt, stepper and v0 recorded (another step size or stepper needs fr_move again)
+  int *mvb[2] = {nullptr, nullptr};
+  int mvc = 0;
+  bool mv_ready = false;
+  bool mv_srk = false;
+  double mv_dt = 0.0, mv_v0 = 0.0;
   hipStream_t st = nullptr;
   KernelTimer *timer = nullptr;
 };
@@ -248,7 +256,8 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->ssum, n));
   A(falloc(&f->tsum, n));
   A(falloc(&f->scnt, n));
-  A(falloc(&f->mv, n));
+  A(falloc(&f->mvb[0], n));
+  A(falloc(&f->mvb[1], n));
   A(falloc(&f->cnt9[0], (size_t)cells * 9));
   A(falloc(&f->cnt9[1], (size_t)cells * 9));
   A(falloc(&f->farcnt, cells));
@@ -304,7 +313,7 @@ void fast_destroy(FastPath *f) {
   if (f->sg_graph) hipGraphDestroy(f->sg_graph);
   void *b[] = {f->key,       f->slot,      f->count,     f->start,   f->tmp,       f->sorted,    f->blocksum,
                f->snap,      f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],  f->rid[1],    f->rnat[0],
-               f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mv,
+               f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mvb[0], f->mvb[1],
                f->cnt9[0],   f->cnt9[1],   f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
                f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc,      f->ovf_list,  f->ovf_count,
                f->lstart,    f->boff,      f->scan_done, f->rtiles};
@@ -430,6 +439,8 @@ int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stag
 
 bool fast_is_resident(const FastPath *f) { return f && f->resident; }
 
+int fast_buffer_parity(const FastPath *f) { return f ? 2 * f->cur + f->mvc : -1; }
+
 void fast_resident_drop(FastPath *f) {
   if (f) f->resident = false;
 }
@@ -445,6 +456,38 @@ int fast_resident_sync(FastPath *f, hipStream_t st, double *state) {
   return SPSS_OK;
 }
 
+// forget the moves the last fr_scatter computed: their counts go back to zero
+static void drop_moves(FastPath *f, hipStream_t st) {
+  if (!f->mv_ready) return;
+  launch_fill_int(st, f->cells, f->newcnt, 0);
+  launch_fill_int(st, f->cells, f->farcnt, 0);
+  launch_fill_int(st, (long)f->cells * 9, f->cnt9[f->cur], 0);
+  f->mv_ready = false;
+}
+
+// the moves and counts of the current step: from the last fr_scatter when it computed them
+// for this dt, stepper and v0, else from fr_move
+template <int S, bool SRK>
+static void ensure_moves(FastPath *f, hipStream_t st, const DevParams &P, const StepArgs &U) {
+  if (f->mv_ready && f->mv_dt == U.dt && f->mv_srk == SRK && f->mv_v0 == P.v0) return;
+  drop_moves(f, st);
+  fr_move<S, SRK><<<nblocks(f->n), 256, 0, st>>>(P, layout_of(f), f->n, f->ncf, f->rstate[f->cur], U,
+                                                 f->mvb[f->mvc], f->farcnt, f->slot, f->cnt9[f->cur], f->newcnt);
+  f->mv_ready = true;
+  f->mv_dt = U.dt;
+  f->mv_srk = SRK;
+  f->mv_v0 = P.v0;
+}
+
+static void ensure_moves_of(FastPath *f, hipStream_t st, const DevParams &P, bool do_srk, const StepArgs &U) {
+  if (do_srk)
+    ensure_moves<4, true>(f, st, P, U);
+  else if (P.s == 4)
+    ensure_moves<4, false>(f, st, P, U);
+  else
+    ensure_moves<3, false>(f, st, P, U);
+}
+
 // entry into resident mode: cell order of `state`, then the gather into the resident arrays
 static void resident_enter(FastPath *f, hipStream_t st, const DevParams &P, const double *state,
                            const double *nat, bool srk) {
@@ -457,6 +500,7 @@ static void resident_enter(FastPath *f, hipStream_t st, const DevParams &P, cons
     fr_gather<3><<<nb, 256, 0, st>>>(f->n, state, f->sorted, nat, f->rstate[c], f->rid[c], f->rnat[c]);
   if (f->rtiles)
     fk_row_tiles<<<(f->ncf + 3) / 4, 256, 0, st>>>(f->rstart[c], f->ncf, f->rt_wcap, f->rt_cmax, f->rtiles);
+  drop_moves(f, st);
   f->resident = true;
   f->resident_nat = nat != nullptr;
 }
@@ -471,21 +515,23 @@ static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const
   const unsigned nb = nblocks(n);
   const double *rnat = f->resident_nat ? f->rnat[c] : nullptr;
   const Layout L = layout_of(f);
-  fr_move<S, SRK><<<nb, 256, 0, st>>>(P, L, n, f->ncf, f->rstate[c], U, f->mv, f->farcnt, f->slot, f->cnt9[c],
-                                      f->newcnt);
+  ensure_moves<S, SRK>(f, st, P, U);
+  int *const mv = f->mvb[f->mvc], *const mvn = f->mvb[1 - f->mvc];
   fr_scan_overflow<S, SRK><<<f->nsb + kOverflowBlocks, kFB, 0, st>>>(P, args_of(f), B, f->newcnt, f->farcnt,
                                                                      f->cells, f->lstart, f->blocksum, f->nsb,
                                                                      f->scan_done, f->boff);
-  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, f->mv, f->rstate[c], U, f->lstart, f->boff,
+  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, mv, f->rstate[c], U, f->lstart, f->boff,
                                            f->cells, f->rstart[o], f->newcnt, f->farcnt, f->ovf_count, f->cnt9[c],
                                            f->slot, f->tmp, f->cnt9[o], (long)f->cells * 9);
   // (at least one workgroup per four rows of cells for the row tiles)
   const unsigned nbs = std::max(nb, (unsigned)(f->rtiles ? (f->ncf + 3) / 4 : 0));
-  fr_scatter<S, SRK><<<nbs, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], f->mv, f->cnt9[c], f->tmp,
+  fr_scatter<S, SRK><<<nbs, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], mv, f->cnt9[c], f->tmp,
                                         f->rstate[c], f->rid[c], rnat, f->ssum, f->tsum, f->scnt, U,
                                         SRK ? compl_out : nullptr, f->rstate[o], f->rid[o], f->rnat[o], f->rsnap,
-                                        SRK ? f->romega : nullptr, f->rtiles, f->rt_wcap, f->rt_cmax);
+                                        SRK ? f->romega : nullptr, f->rtiles, f->rt_wcap, f->rt_cmax, mvn, f->farcnt,
+                                        f->slot, f->cnt9[o], f->newcnt);
   f->cur = o;
+  f->mvc = 1 - f->mvc;  // the next step's moves (same dt, stepper and v0) are in place
 }
 
 static int step_impl(FastPath *f, hipStream_t st, const DevParams &P, bool do_srk, const double *state,
@@ -574,10 +620,14 @@ long fast_run_graph(FastPath *f, hipStream_t st, const DevParams &P, int srk, co
   // records events around the pair kernel
   if (reps < 1 || f->timer || !f->resident || (nat != nullptr) != f->resident_nat || (do_srk && !f->romega_valid))
     return 0;
-  const bool same = f->sg_exec && f->sg_srk == (int)do_srk && f->sg_cur == f->cur && f->sg_gen == f->gen &&
+  const bool same = f->sg_exec && f->sg_srk == (int)do_srk && f->sg_cur == fast_buffer_parity(f) &&
+                    f->sg_gen == f->gen &&
                     f->sg_higher == P.higher_order && f->sg_dt == dt && f->sg_dphi == D_phi &&
                     f->sg_friction == friction && f->sg_seed == seed && f->sg_state == state && f->sg_nat == nat &&
                     f->sg_compl == compl_out;
+  // the first step's moves come from the step before (or are computed here), never from a
+  // captured fr_move: a replay after a step that left its moves would count them twice
+  ensure_moves_of(f, st, P, do_srk, step_args(dt, D_phi, friction, 3, nullptr, seed, 0));
   if (!same) {
     drop_step_graph(f);
     StepArgs U = step_args(dt, D_phi, friction, 3, nullptr, seed, 0);
@@ -605,7 +655,7 @@ long fast_run_graph(FastPath *f, hipStream_t st, const DevParams &P, int srk, co
       return -1;
     }
     f->sg_srk = do_srk;
-    f->sg_cur = f->cur;
+    f->sg_cur = fast_buffer_parity(f);
     f->sg_gen = f->gen;
     f->sg_higher = P.higher_order;
     f->sg_dt = dt;
@@ -643,6 +693,13 @@ int fast_batch_step(FastPath *f, hipStream_t st, const DevParams &P, const doubl
   int rc = step_impl(f, st, P, false, state, nat, nullptr, U);
   if (rc != SPSS_OK) return rc;
   fb_step_inc<<<1, 64, 0, st>>>(f->step_dev);
+  FCHK(hipGetLastError());
+  return SPSS_OK;
+}
+
+int fast_prepare_moves(FastPath *f, hipStream_t st, const DevParams &P, double dt) {
+  if (!f->resident) return SPSS_OK;
+  ensure_moves_of(f, st, P, false, step_args(dt, 0.0, 0.0, 3, nullptr, 0, 0));
   FCHK(hipGetLastError());
   return SPSS_OK;
 }
