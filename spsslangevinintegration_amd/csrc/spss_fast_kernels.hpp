@@ -10,8 +10,8 @@
 //    phasor sums of every particle in the old order; fr_move decides each particle's
 //    move (its new position depends on phase and omega only) and counts the moves per
 //    (cell, direction) and the arrivals per cell, a two-level scan of the arrival counts
-//    gives the new cell starts (block-local in fr_scan_overflow, whose last scan block also
-//    scans the block totals; the totals are added in fr_far_bucket), and fr_scatter
+//    gives the new cell starts (block-local in fr_scan_overflow, the earlier blocks' totals
+//    added in fr_far_bucket), and fr_scatter
 //    applies the step and writes every particle to its new slot: inside a cell, particles that
 //    stayed come first in their previous order, then arrivals grouped by the direction
 //    they came from, each group in its previous order, then the rare particles that jumped
@@ -202,57 +202,13 @@ __global__ __launch_bounds__(256) void fk_scan_add(int *__restrict__ count, int 
   }
 }
 
-// One scan block of the resident step: block-local starts into lstart and the block total into
-// blocksum; the last block to finish (counted in *done, which it resets) writes the exclusive
-// prefix of the nsb block totals to boff[0..nsb].  The block totals are published by a
-// device-scope fence before the count and read after one, so no block waits for another.
-__device__ __forceinline__ void scan_local_last(const int *__restrict__ count, const int *__restrict__ count2, int m,
-                                                int *__restrict__ lstart, int *blocksum, int blk, int nsb, int *done,
-                                                int *__restrict__ boff) {
-  scan_local_block(count, count2, m, lstart, blocksum, blk);
-  __shared__ int last;
-  __shared__ int carry;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    last = atomicAdd(done, 1) == nsb - 1;
-    carry = 0;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  __shared__ int sh[256];
-  for (int b0 = 0; b0 < nsb; b0 += 256 * kScanItems) {
-    const int base = b0 + threadIdx.x * kScanItems;
-    int v[kScanItems];
-    int sum = 0;
+// the sum of the first b scan-block totals, by one wave (every lane gets it; all lanes call)
+__device__ __forceinline__ int wave_prefix_of(const int *__restrict__ blocksum, int b) {
+  int s = 0;
+  for (int j = threadIdx.x & 63; j < b; j += 64) s += blocksum[j];
 #pragma unroll
-    for (int q = 0; q < kScanItems; ++q) {
-      v[q] = base + q < nsb ? blocksum[base + q] : 0;
-      sum += v[q];
-    }
-    sh[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-      const int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    int run = carry + sh[threadIdx.x] - sum;
-#pragma unroll
-    for (int q = 0; q < kScanItems; ++q) {
-      if (base + q < nsb) boff[base + q] = run;
-      run += v[q];
-    }
-    __syncthreads();  // every thread has read carry and sh
-    if (threadIdx.x == 255) carry += sh[255];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    boff[nsb] = carry;
-    *done = 0;
-  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
 }
 
 __global__ void fk_scatter(int n, const int *__restrict__ key, const int *__restrict__ slot,
@@ -809,9 +765,9 @@ __global__ __launch_bounds__(kFB) void fk_pairs_ovf(DevParams P, FastArgs A, Pai
 template <int S, bool SRK>
 __global__ __launch_bounds__(kFB) void fr_scan_overflow(DevParams P, FastArgs A, PairBufs B, const int *count,
                                                         const int *count2, int m, int *lstart, int *blocksum,
-                                                        int nsb, int *done, int *boff) {
+                                                        int nsb) {
   if ((int)blockIdx.x < nsb)
-    scan_local_last(count, count2, m, lstart, blocksum, blockIdx.x, nsb, done, boff);
+    scan_local_block(count, count2, m, lstart, blocksum, blockIdx.x);
   else
     overflow_items<S, SRK, 3>(P, A, B, blockIdx.x - nsb, gridDim.x - nsb);
 }
@@ -998,16 +954,18 @@ __global__ __launch_bounds__(256) void fk_row_tiles(const int *__restrict__ star
 }
 
 // far arrivals: unordered bucket behind the near arrivals of their destination cell.  Also the
-// second scan pass of the new cell starts (nstart[c] = lstart[c] + boff[block of c], from the
-// block-local starts and the prefix of the block totals left by fr_scan_overflow), the clearing
-// of the arrival counts it summed, of the next step's move counters and of the overflow count.
+// second scan pass of the new cell starts (nstart[c] = lstart[c] + the totals of the scan blocks
+// before c's, from the block-local starts and totals of fr_scan_overflow), the clearing of the
+// arrival counts it summed, of the next step's move counters and of the overflow count.
 template <int S, bool SRK>
 __global__ __launch_bounds__(256) void fr_far_bucket(DevParams P, int n, int ncf, int cpr, const int *__restrict__ mv,
-                              const double *__restrict__ rstate, StepArgs U, const int *__restrict__ lstart,
-                              const int *__restrict__ boff, int m, int *__restrict__ nstart, int *__restrict__ count,
-                              int *__restrict__ count2, int *__restrict__ ovf_count, const int *__restrict__ cnt9,
-                              const int *__restrict__ farslot, int *__restrict__ bucket, int *__restrict__ cnt9_next,
-                              long nclear) {
+                                                     const double *__restrict__ rstate, StepArgs U,
+                                                     const int *__restrict__ lstart, const int *__restrict__ blocksum,
+                                                     int nsb, int m, int *__restrict__ nstart, int *__restrict__ count,
+                                                     int *__restrict__ count2, int *__restrict__ ovf_count,
+                                                     const int *__restrict__ cnt9, const int *__restrict__ farslot,
+                                                     int *__restrict__ bucket, int *__restrict__ cnt9_next,
+                                                     long nclear) {
   constexpr int kBlock = 256 * kScanItems;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   // graph-replayed steps: the counter read by this step's fr_scatter moves on here, one kernel
@@ -1016,13 +974,17 @@ __global__ __launch_bounds__(256) void fr_far_bucket(DevParams P, int n, int ncf
   const long stride = (long)gridDim.x * blockDim.x;
   // the other move-count buffer (last read by the previous step) starts the next step at zero
   for (long e = p; e < nclear; e += stride) cnt9_next[e] = 0;
-  for (long c = p; c <= m; c += stride) {
+  // a workgroup's 256 cells lie in one scan block: one offset per pass, summed by every wave
+  for (long c0 = (long)blockIdx.x * blockDim.x; c0 <= m; c0 += stride) {
+    const int b = (int)(c0 / kBlock);
+    const int off = wave_prefix_of(blocksum, b);
+    const long c = c0 + threadIdx.x;
     if (c < m) {
-      nstart[c] = lstart[c] + boff[c / kBlock];
+      nstart[c] = lstart[c] + off;
       count[c] = 0;
       count2[c] = 0;
-    } else {
-      nstart[m] = boff[(m + kBlock - 1) / kBlock];
+    } else if (c == m) {
+      nstart[m] = off + (b < nsb ? blocksum[b] : 0);
     }
   }
   if (p == 0) *ovf_count = 0;
@@ -1032,7 +994,9 @@ __global__ __launch_bounds__(256) void fr_far_bucket(DevParams P, int n, int ncf
   double sn, cn, x1, y1;
   step_position<S, SRK>(P, rstate + (long)p * S, U, sn, cn, x1, y1);
   const int cd = cell_base(mk >> 4, cpr) + cell_of_position<S>(P, x1, y1, ncf);
-  const int q = lstart[cd] + boff[cd / kBlock] + new_count_of(cnt9, cd, ncf, cpr) + farslot[p];
+  int off = 0;  // (rare) this lane's own sum of the earlier block totals
+  for (int j = 0; j < cd / kBlock; ++j) off += blocksum[j];
+  const int q = lstart[cd] + off + new_count_of(cnt9, cd, ncf, cpr) + farslot[p];
   if (q < n) bucket[q] = p;  // always, when the counts match the moves
 }
 

@@ -70,9 +70,7 @@ struct FastPath {
   // rebuilt cell list (RHS, pair export, entry into resident mode)
   int *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr, *sorted = nullptr;
   int *blocksum = nullptr;
-  // the resident step's scan: block-local starts, the prefix of the block totals, the count of
-  // finished scan blocks (zero between steps)
-  int *lstart = nullptr, *boff = nullptr, *scan_done = nullptr;
+  int *lstart = nullptr;  // block-local cell starts of the resident step's scan
   float4 *snap = nullptr;
   float *omega_f = nullptr;
   // resident copy of the fused steps (double-buffered where a step permutes)
@@ -232,8 +230,6 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->sorted, n));
   A(falloc(&f->blocksum, f->nsb + 1));
   A(falloc(&f->lstart, cells));
-  A(falloc(&f->boff, f->nsb + 1));
-  A(falloc(&f->scan_done, 1));
   if (R == 1 && f->ncf <= kRowTileCols) {
     // slots per row: the greedy spans of a row 1.5x the mean, at about 94% of kFB each, plus 2,
     // and at least enough spans of rt_wcap columns to cover a row
@@ -293,7 +289,6 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   launch_fill_int(st, (long)cells * 9, f->cnt9[1], 0);
   launch_fill_int(st, cells, f->newcnt, 0);
   launch_fill_int(st, 1, f->ovf_count, 0);
-  launch_fill_int(st, 1, f->scan_done, 0);
   if (coef) {
     launch_fill_double(st, 3L * R, f->racc, 0.0);
     launch_fill_double(st, 2L * R, f->zlast, 0.0);
@@ -316,7 +311,7 @@ void fast_destroy(FastPath *f) {
                f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mvb[0], f->mvb[1],
                f->cnt9[0],   f->cnt9[1],   f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
                f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc,      f->ovf_list,  f->ovf_count,
-               f->lstart,    f->boff,      f->scan_done, f->rtiles};
+               f->lstart,    f->rtiles};
   for (void *p : b) hipFree(p);
   delete f;
 }
@@ -518,11 +513,10 @@ static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const
   ensure_moves<S, SRK>(f, st, P, U);
   int *const mv = f->mvb[f->mvc], *const mvn = f->mvb[1 - f->mvc];
   fr_scan_overflow<S, SRK><<<f->nsb + kOverflowBlocks, kFB, 0, st>>>(P, args_of(f), B, f->newcnt, f->farcnt,
-                                                                     f->cells, f->lstart, f->blocksum, f->nsb,
-                                                                     f->scan_done, f->boff);
-  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, mv, f->rstate[c], U, f->lstart, f->boff,
-                                           f->cells, f->rstart[o], f->newcnt, f->farcnt, f->ovf_count, f->cnt9[c],
-                                           f->slot, f->tmp, f->cnt9[o], (long)f->cells * 9);
+                                                                     f->cells, f->lstart, f->blocksum, f->nsb);
+  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, mv, f->rstate[c], U, f->lstart, f->blocksum,
+                                           f->nsb, f->cells, f->rstart[o], f->newcnt, f->farcnt, f->ovf_count,
+                                           f->cnt9[c], f->slot, f->tmp, f->cnt9[o], (long)f->cells * 9);
   // (at least one workgroup per four rows of cells for the row tiles)
   const unsigned nbs = std::max(nb, (unsigned)(f->rtiles ? (f->ncf + 3) / 4 : 0));
   fr_scatter<S, SRK><<<nbs, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], mv, f->cnt9[c], f->tmp,
