@@ -777,9 +777,8 @@ __global__ __launch_bounds__(kFB) void fr_scan_overflow(DevParams P, FastArgs A,
 // StochasticEulerStepperPtr.hpp:42-47 would hold it, then the update.  The new position depends on
 // the phase and omega only, so step_position gives it bit-identically without the sums.
 template <int S, bool SRK>
-__device__ __forceinline__ void step_position(const DevParams &P, const double *pi, const StepArgs &U, double &sn,
-                                              double &cn, double &x1, double &y1) {
-  sincos(pi[2], &sn, &cn);
+__device__ __forceinline__ void position_from(const DevParams &P, const double *pi, const StepArgs &U, double sn,
+                                              double cn, double &x1, double &y1) {
   const double f0 = P.v0 * cn, f1 = P.v0 * sn;
   if (S == 4 && SRK) {
     const double f2 = pi[3], dt = U.dt;
@@ -789,6 +788,14 @@ __device__ __forceinline__ void step_position(const DevParams &P, const double *
     x1 = pi[0] + f0 * U.dt;
     y1 = pi[1] + f1 * U.dt;
   }
+}
+
+// the new position from the state pi; sn, cn = sin, cos of its phase
+template <int S, bool SRK>
+__device__ __forceinline__ void step_position(const DevParams &P, const double *pi, const StepArgs &U, double &sn,
+                                              double &cn, double &x1, double &y1) {
+  sincos(pi[2], &sn, &cn);
+  position_from<S, SRK>(P, pi, U, sn, cn, x1, y1);
 }
 
 template <int S, bool SRK>
@@ -855,17 +862,17 @@ __device__ __forceinline__ int cell_of_position(const DevParams &P, double x, do
 // Particles lie in cell order, so a wave's equal (cell, direction) keys form runs and one lane per
 // run adds the run's length.  Every lane of the wave calls this (valid = false for lanes past
 // the end): the run search reads its neighbours.
+// (sn, cn: sin and cos of the phase pi[2]; xw, yw: pi[0], pi[1] wrapped into the box)
 template <int S, bool SRK>
 __device__ __forceinline__ void move_and_count(const DevParams &P, const Layout &L, int ncf, const double *pi,
-                                               const StepArgs &U, int p, bool valid, int *__restrict__ mv,
-                                               int *__restrict__ farcnt, int *__restrict__ farslot,
-                                               int *__restrict__ cnt9, int *__restrict__ newcnt) {
+                                               double sn, double cn, double xw, double yw, const StepArgs &U, int p,
+                                               bool valid, int *__restrict__ mv, int *__restrict__ farcnt,
+                                               int *__restrict__ farslot, int *__restrict__ cnt9,
+                                               int *__restrict__ newcnt) {
   int cell = -1, d = kFar, dest = 0;  // invalid lanes: a key of their own that is never counted
   if (valid) {
-    double sn, cn, x1, y1;
-    step_position<S, SRK>(P, pi, U, sn, cn, x1, y1);
-    double xw, yw;
-    apply_pbc(P, pi[0], pi[1], xw, yw);
+    double x1, y1;
+    position_from<S, SRK>(P, pi, U, sn, cn, x1, y1);
     const int ox = cell_coord(xw, ncf), oy = cell_coord(yw, ncf);
     apply_pbc(P, x1, y1, xw, yw);
     const int nx = cell_coord(xw, ncf), ny = cell_coord(yw, ncf);
@@ -906,8 +913,13 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int
                                                int *__restrict__ cnt9, int *__restrict__ newcnt) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = p < n;
-  move_and_count<S, SRK>(P, L, ncf, valid ? rstate + (long)p * S : rstate, U, p, valid, mv, farcnt, farslot, cnt9,
-                         newcnt);
+  const double *pi = valid ? rstate + (long)p * S : rstate;
+  double sn = 0.0, cn = 1.0, xw = 0.0, yw = 0.0;
+  if (valid) {
+    sincos(pi[2], &sn, &cn);
+    apply_pbc(P, pi[0], pi[1], xw, yw);
+  }
+  move_and_count<S, SRK>(P, L, ncf, pi, sn, cn, xw, yw, U, p, valid, mv, farcnt, farslot, cnt9, newcnt);
 }
 
 // The tiles of one row of cells for the next pair kernel, built by one wave from the row's cell
@@ -1022,6 +1034,7 @@ __global__ __launch_bounds__(256) void fr_scatter(
 #pragma unroll
   for (int q = 0; q < S; ++q) out[q] = 0.0;
   int pos = 0;
+  double sn2 = 0.0, cn2 = 1.0, xw2 = 0.0, yw2 = 0.0;
   if (valid) {
     const int cpr = L.cpr;
     const int m = mv[p];
@@ -1072,12 +1085,18 @@ __global__ __launch_bounds__(256) void fr_scatter(
       for (int q = 0; q < S; ++q) dst[q] = out[q];
       nrid[pos] = id;
       if (rnat) nnat[pos] = nat_i;
+      // the snapshot (as snap_entry), keeping the phase's sine and cosine and the wrapped
+      // position for the next move
+      sincos(out[2], &sn2, &cn2);
+      apply_pbc(P, out[0], out[1], xw2, yw2);
       const int ld = cd - base;
-      snap[pos] = snap_entry<S>(P, out, ld % ncf, ld / ncf, H);
+      snap[pos] = make_float4((float)(xw2 - (double)(ld % ncf) * H), (float)(yw2 - (double)(ld / ncf) * H),
+                              (float)cn2, (float)sn2);
       if (omega_f) omega_f[pos] = (S == 4) ? (float)out[3] : 0.f;
     }
   }
-  if (mvn) move_and_count<S, SRK>(P, L, ncf, out, U, pos, valid, mvn, farcnt, farslot, cnt9n, newcnt);
+  if (mvn)
+    move_and_count<S, SRK>(P, L, ncf, out, sn2, cn2, xw2, yw2, U, pos, valid, mvn, farcnt, farslot, cnt9n, newcnt);
 }
 
 // advance the device step counter of graph-replayed runs
