@@ -43,12 +43,16 @@ def pair_kernel_bytes(n, cells):
 
 # HBM bytes per launch of that kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled
 # by the copy-kernel calibration, by cell subdivision (profiles/fk_pairs_euler_N1M_hbm_traffic.txt)
-PAIR_KERNEL_TRAFFIC = {1: 3.9923e7}
-# VALU issue utilisation of that kernel (SQ_INSTS_VALU per CU-cycle, profiles/fk_pairs_euler_N1M_valu.txt)
-PAIR_KERNEL_VALU_UTIL = 0.96
-# algorithmic bytes per particle-step of the whole step (state 64, snapshot w+r 24,
-# key and sorted index w+r 16, cell table w+r 8 * 62500 / N)
-STEP_BYTES = 64.0 + 24.0 + 16.0 + 8.0 * 62500.0 / 1048576.0
+PAIR_KERNEL_TRAFFIC = {1: 3.9937e7}
+# VALU instructions issued per CU-cycle by that kernel (SQ_INSTS_VALU / (256 CUs x cycles),
+# profiles/fk_pairs_euler_N1M_valu.txt): at or above the nominal one per CU-cycle, i.e. VALU-bound
+PAIR_KERNEL_VALU_PER_CU_CYCLE = 1.04
+# algorithmic bytes per particle-step of the whole resident Euler step: pair kernel 36 (snapshot
+# read, sums and count written), scatter 132 (move word, id, FP64 state, natural frequency, sums
+# read; state, id, natural frequency, snapshot, next move word written), and per cell 68
+# (scan: counts read, local start written; far pass: local start read, start written, counts
+# and the 9 move counters cleared), N = 1,048,576 and 62,500 cells
+STEP_BYTES = 36.0 + 132.0 + 68.0 * 62500.0 / 1048576.0
 
 
 def headline_system(spss, n=1 << 20, precision=None, subdiv=1, seed=0x5EED0003):
@@ -292,7 +296,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": PAIR_KERNEL_TRAFFIC.get(subdiv)},
         # the pair kernel is bound by VALU issue, not HBM (no MFMA work on this path)
-        "valu": {"kernel": "fk_pairs", "issue_util": PAIR_KERNEL_VALU_UTIL},
+        "valu": {"kernel": "fk_pairs", "valu_instr_per_cu_cycle": PAIR_KERNEL_VALU_PER_CU_CYCLE},
         "cpu_baseline": cpu,
         "pair_kernel_us": pair_ms * 1e3,
         "hbm_copy_gbps": copy_gbps,
