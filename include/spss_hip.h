@@ -302,6 +302,27 @@ int spss_averaging_reset(spss_system *sys);
 int spss_averaging_accumulate(spss_system *sys);
 int spss_averaging_get(spss_system *sys, double *host_sum, long *count);
 
+/* Adaptive Dormand-Prince 5(4) with dense output, the reference's Boost.Odeint calls
+ * (SimulationEngineFirstOrder.cpp:158-165: integrate_const(make_dense_output(1e-6, 1e-6,
+ * runge_kutta_dopri5<std::vector<Real>>()), particle_system, state, t_0, t_1, dt, observer);
+ * :273-281: integrate_adaptive(make_dense_output(1e-10, 1e-10, ...), particle_system, state, t_0,
+ * t_1, dt)).  The system is the reference's functor (ParticleSystemFirstOrder.cpp:114-120):
+ * EvaluateRhs(state, zeros, derivative, 0, t) with `method` (SPSS_METHOD_AUTO: the reference's
+ * dispatch).  SPSS_DOPRI5_CONST observes t_0 + j dt (j = 0, 1, ... while <= t_1) from the dense
+ * output and leaves the last observed state as the system's state; SPSS_DOPRI5_ADAPTIVE observes
+ * t_0 and every accepted step and leaves the state at t_1.  The step control is Boost's
+ * default_error_checker (a_x = a_dxdt = 1) and default_step_adjuster; 500 rejected trials in a
+ * row are an error, as Boost's failed_step_checker throws.  The observer gets the state
+ * (s*n doubles, AoS) as a host pointer (observe_on_host = 1) or a device pointer, valid for the
+ * call; a nonzero return stops the integration with SPSS_ERR_INVALID.  steps / rejected (may be
+ * NULL) count accepted steps and rejected trials. */
+#define SPSS_DOPRI5_CONST 0
+#define SPSS_DOPRI5_ADAPTIVE 1
+typedef int (*spss_observer_fn)(void *user, double t, const double *state, int n, int s);
+int spss_dopri5_integrate(spss_system *sys, int method, int mode, double t0, double t1, double dt, double eps_abs,
+                          double eps_rel, spss_observer_fn observer, void *user, int observe_on_host, long *steps,
+                          long *rejected);
+
 #ifdef __cplusplus
 }
 #endif

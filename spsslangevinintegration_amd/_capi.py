@@ -17,6 +17,9 @@ PRECISION_FP64, PRECISION_FP32C = 0, 1
 VERLET_COMPAT, VERLET_EXACT = 0, 1
 STEPPER_RK4, STEPPER_EULER, STEPPER_SRK = 0, 1, 2
 NOISE_NONE, NOISE_HOST, NOISE_DEVICE, NOISE_PHILOX = 0, 1, 2, 3
+DOPRI5_CONST, DOPRI5_ADAPTIVE = 0, 1
+# int observer(void *user, double t, const double *state, int n, int s)
+OBSERVER_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_int, C.c_int)
 
 # every entry point include/spss_hip.h declares (checked by tests/test_capi_symbols.py)
 EXPORTS = [
@@ -37,7 +40,7 @@ EXPORTS = [
     "spss_mlp_predict", "spss_mlp_gradients", "spss_mlp_set_data", "spss_mlp_train", "spss_mlp_reset_optimizer",
     "spss_mlp_adam_steps",
     "spss_frame", "spss_frame_append", "spss_summary", "spss_summary_append", "spss_read_last_frame",
-    "spss_load_last_frame",
+    "spss_load_last_frame", "spss_dopri5_integrate",
 ]
 
 
@@ -148,6 +151,7 @@ def load(path=None):
         "spss_summary_append": (i, [vp, C.c_char_p, d]),
         "spss_read_last_frame": (i, [C.c_char_p, i, i, C.POINTER(C.c_float), vp]),
         "spss_load_last_frame": (i, [vp, C.c_char_p, C.POINTER(d)]),
+        "spss_dopri5_integrate": (i, [vp, i, i, d, d, d, d, d, OBSERVER_FN, vp, i, C.POINTER(l), C.POINTER(l)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -79,6 +81,10 @@ struct spss_system {
   // is then behind it until the next sync_master
   bool master_stale = false;
   KernelTimer *timer = nullptr;  // around the pair kernel of the RHS
+  // Dormand-Prince 5(4) buffers (allocated on first use): two states and derivatives, k2..k6,
+  // a stage state, a zero vector (the RHS's k_prev) and a pinned host copy for observers
+  double *dp[11] = {};
+  double *dp_host = nullptr;
 };
 
 namespace {
@@ -490,6 +496,8 @@ int spss_destroy(spss_system *h) {
                   h->noise, h->partial, h->scal, h->acc, h->io[0], h->io[1], h->io[2], h->vcount, h->vstart,
                   h->vlist, h->pcount, h->pstart, h->plist};
   for (void *b : bufs) hipFree(b);
+  for (double *b : h->dp) hipFree(b);
+  if (h->dp_host) hipHostFree(h->dp_host);
   for (CellBuffers *cb : {&h->cells, &h->vcells}) {
     hipFree(cb->key);
     hipFree(cb->slot);
@@ -1010,3 +1018,298 @@ int spss_averaging_get(spss_system *h, double *sum, long *count) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Dormand-Prince 5(4) with dense output: the reference's Boost.Odeint integration
+// (SimulationEngineFirstOrder.cpp:158-165 integrate_const, :273-281 integrate_adaptive, both with
+// make_dense_output(eps_abs, eps_rel, runge_kutta_dopri5<vector<Real>>())).  Boost is not in the
+// reference tree; this restates the published algorithm of Boost.Odeint (1.6x): the dopri5 stepper
+// with first-same-as-last derivative, controlled_runge_kutta with default_error_checker (a_x =
+// a_dxdt = 1) and default_step_adjuster, dense_output_runge_kutta, and the dense-output versions of
+// integrate_const and integrate_adaptive.  The system is the reference's functor
+// (ParticleSystemFirstOrder.cpp:114-120): EvaluateRhs(state, zeros, derivative, 0, t).
+
+namespace {
+
+enum { DP_X0, DP_X1, DP_D0, DP_D1, DP_K2, DP_K3, DP_K4, DP_K5, DP_K6, DP_TMP, DP_ZERO };
+
+// the coefficients as Boost forms them: quotients of integers in double
+struct DpTableau {
+  double a21, a31, a32, a41, a42, a43, a51, a52, a53, a54, a61, a62, a63, a64, a65;
+  double b1, b3, b4, b5, b6;
+  double d1, d3, d4, d5, d6, d7;  // fifth- minus fourth-order weights (the error estimate)
+  DpTableau() {
+    a21 = 1.0 / 5.0;
+    a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
+    a41 = 44.0 / 45.0, a42 = -56.0 / 15.0, a43 = 32.0 / 9.0;
+    a51 = 19372.0 / 6561.0, a52 = -25360.0 / 2187.0, a53 = 64448.0 / 6561.0, a54 = -212.0 / 729.0;
+    a61 = 9017.0 / 3168.0, a62 = -355.0 / 33.0, a63 = 46732.0 / 5247.0, a64 = 49.0 / 176.0;
+    a65 = -5103.0 / 18656.0;
+    b1 = 35.0 / 384.0, b3 = 500.0 / 1113.0, b4 = 125.0 / 192.0, b5 = -2187.0 / 6784.0, b6 = 11.0 / 84.0;
+    d1 = b1 - 5179.0 / 57600.0;
+    d3 = b3 - 7571.0 / 16695.0;
+    d4 = b4 - 393.0 / 640.0;
+    d5 = b5 - (-92097.0 / 339200.0);
+    d6 = b6 - 187.0 / 2100.0;
+    d7 = -1.0 / 40.0;
+  }
+};
+
+struct DpRun {
+  spss_system *h;
+  int method;
+  long m;
+  double eps_abs, eps_rel;
+  int cur = 0;  // which of the two state/derivative buffers holds the current point
+  double t = 0.0, t_old = 0.0, dt = 0.0;
+  bool deriv_init = false;
+  long steps = 0, rejected = 0;
+  double *x(int q) const { return h->dp[q ? DP_X1 : DP_X0]; }
+  double *dx(int q) const { return h->dp[q ? DP_D1 : DP_D0]; }
+};
+
+DpTerms dp_terms(std::initializer_list<const double *> k, std::initializer_list<double> c) {
+  DpTerms T{};
+  int j = 0;
+  for (const double *p : k) T.k[j++] = p;
+  T.n = j;
+  j = 0;
+  for (double v : c) T.c[j++] = v;
+  return T;
+}
+
+int dp_rhs(DpRun &R, const double *x, double *out) {
+  return rhs_device(R.h, R.method, x, R.h->dp[DP_ZERO], out, 0.0, 0.0);
+}
+
+// controlled_runge_kutta::try_step (FSAL): one trial step from the current point; on success the
+// new point is in the other buffers and t has advanced; dt is the next (or the retried) step size
+int dp_try_step(DpRun &R, bool *ok) {
+  static const DpTableau A;
+  spss_system *h = R.h;
+  hipStream_t st = h->stream;
+  const double *in = R.x(R.cur), *din = R.dx(R.cur);
+  double *out = R.x(1 - R.cur), *dout = R.dx(1 - R.cur);
+  double *const *k = h->dp;
+  double *tmp = h->dp[DP_TMP];
+  const double dt = R.dt;
+  launch_dp_combine(st, R.m, in, dp_terms({din}, {dt * A.a21}), tmp);
+  TRY(dp_rhs(R, tmp, k[DP_K2]));
+  launch_dp_combine(st, R.m, in, dp_terms({din, k[DP_K2]}, {dt * A.a31, dt * A.a32}), tmp);
+  TRY(dp_rhs(R, tmp, k[DP_K3]));
+  launch_dp_combine(st, R.m, in, dp_terms({din, k[DP_K2], k[DP_K3]}, {dt * A.a41, dt * A.a42, dt * A.a43}), tmp);
+  TRY(dp_rhs(R, tmp, k[DP_K4]));
+  launch_dp_combine(st, R.m, in,
+                    dp_terms({din, k[DP_K2], k[DP_K3], k[DP_K4]}, {dt * A.a51, dt * A.a52, dt * A.a53, dt * A.a54}),
+                    tmp);
+  TRY(dp_rhs(R, tmp, k[DP_K5]));
+  launch_dp_combine(st, R.m, in,
+                    dp_terms({din, k[DP_K2], k[DP_K3], k[DP_K4], k[DP_K5]},
+                             {dt * A.a61, dt * A.a62, dt * A.a63, dt * A.a64, dt * A.a65}),
+                    tmp);
+  TRY(dp_rhs(R, tmp, k[DP_K6]));
+  launch_dp_combine(st, R.m, in,
+                    dp_terms({din, k[DP_K3], k[DP_K4], k[DP_K5], k[DP_K6]},
+                             {dt * A.b1, dt * A.b3, dt * A.b4, dt * A.b5, dt * A.b6}),
+                    out);
+  TRY(dp_rhs(R, out, dout));
+  launch_dp_error(st, h->n, h->s, in, din,
+                  dp_terms({din, k[DP_K3], k[DP_K4], k[DP_K5], k[DP_K6], dout},
+                           {dt * A.d1, dt * A.d3, dt * A.d4, dt * A.d5, dt * A.d6, dt * A.d7}),
+                  dt, R.eps_abs, R.eps_rel, h->partial, h->scal + 24);
+  HIPCHECK(hipGetLastError());
+  double err = 0.0;
+  HIPCHECK(hipMemcpyAsync(&err, h->scal + 24, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  if (err > 1.0) {  // default_step_adjuster::decrease_step, error order 4
+    R.dt = dt * std::max(0.9 * std::pow(err, -1.0 / 3.0), 0.2);
+    *ok = false;
+    return SPSS_OK;
+  }
+  R.t += dt;
+  if (err < 0.5) {  // default_step_adjuster::increase_step, stepper order 5
+    err = std::max(std::pow(5.0, -5.0), err);
+    R.dt = dt * (0.9 * std::pow(err, -1.0 / 5.0));
+  }
+  *ok = true;
+  return SPSS_OK;
+}
+
+// dense_output_runge_kutta::initialize
+int dp_initialize(DpRun &R, const double *x0, double t0, double dt0) {
+  if (x0 != R.x(R.cur))
+    HIPCHECK(hipMemcpyAsync(R.x(R.cur), x0, sizeof(double) * R.m, hipMemcpyDeviceToDevice, R.h->stream));
+  R.t = t0;
+  R.dt = dt0;
+  R.deriv_init = false;
+  return SPSS_OK;
+}
+
+// dense_output_runge_kutta::do_step (FSAL): trial steps until one is accepted (at most 500)
+int dp_do_step(DpRun &R) {
+  if (!R.deriv_init) {
+    TRY(dp_rhs(R, R.x(R.cur), R.dx(R.cur)));
+    R.deriv_init = true;
+  }
+  R.t_old = R.t;
+  for (int fails = 0;; ++fails) {
+    if (fails >= 500)
+      return fail(SPSS_ERR_INVALID, "dopri5: 500 failed step size adjustments (Boost's failed_step_checker)");
+    bool ok = false;
+    TRY(dp_try_step(R, &ok));
+    if (ok) break;
+    ++R.rejected;
+  }
+  R.cur = 1 - R.cur;
+  ++R.steps;
+  return SPSS_OK;
+}
+
+// runge_kutta_dopri5::calc_state: the fourth-order continuous extension of the last step (the
+// coefficient polynomials of Hairer's DOPRI5 dense output in Boost's form) at time t
+void dp_calc_state(DpRun &R, double t, double *out) {
+  static const DpTableau A;
+  double *const *k = R.h->dp;
+  const double dt = R.t - R.t_old;
+  const double theta = (t - R.t_old) / dt;
+  const double X1 = 5.0 * (2558722523.0 - 31403016.0 * theta) / 11282082432.0;
+  const double X3 = 100.0 * (882725551.0 - 15701508.0 * theta) / 32700410799.0;
+  const double X4 = 25.0 * (443332067.0 - 31403016.0 * theta) / 1880347072.0;
+  const double X5 = 32805.0 * (23143187.0 - 3489224.0 * theta) / 199316789632.0;
+  const double X6 = 55.0 * (29972135.0 - 7076736.0 * theta) / 822651844.0;
+  const double X7 = 10.0 * (7414447.0 - 829305.0 * theta) / 29380423.0;
+  const double theta_m_1 = theta - 1.0;
+  const double theta_sq = theta * theta;
+  const double Aq = theta_sq * (3.0 - 2.0 * theta);
+  const double Bq = theta_sq * theta_m_1;
+  const double Cq = theta_sq * theta_m_1 * theta_m_1;
+  const double Dq = theta * theta_m_1 * theta_m_1;
+  const double b1t = Aq * A.b1 - Cq * X1 + Dq;
+  const double b3t = Aq * A.b3 + Cq * X3;
+  const double b4t = Aq * A.b4 - Cq * X4;
+  const double b5t = Aq * A.b5 + Cq * X5;
+  const double b6t = Aq * A.b6 - Cq * X6;
+  const double b7t = Bq + Cq * X7;
+  launch_dp_combine(R.h->stream, R.m, R.x(1 - R.cur),
+                    dp_terms({R.dx(1 - R.cur), k[DP_K3], k[DP_K4], k[DP_K5], k[DP_K6], R.dx(R.cur)},
+                             {dt * b1t, dt * b3t, dt * b4t, dt * b5t, dt * b6t, dt * b7t}),
+                    out);
+}
+
+// odeint's comparisons of times with the sign of the step and a tolerance of one machine epsilon
+bool less_with_sign(double t1, double t2, double dt) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  return dt > 0 ? t2 - t1 > eps : t1 - t2 > eps;
+}
+bool less_eq_with_sign(double t1, double t2, double dt) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  return dt > 0 ? t1 - t2 <= eps : t2 - t1 <= eps;
+}
+
+struct DpObserver {
+  spss_observer_fn fn;
+  void *user;
+  int on_host;
+  int call(spss_system *h, const double *d_state, double t) {
+    if (!fn) return SPSS_OK;
+    const double *p = d_state;
+    if (on_host) {
+      HIPCHECK(hipMemcpyAsync(h->dp_host, d_state, sizeof(double) * h->s * h->n, hipMemcpyDeviceToHost, h->stream));
+      HIPCHECK(hipStreamSynchronize(h->stream));
+      p = h->dp_host;
+    } else {
+      HIPCHECK(hipStreamSynchronize(h->stream));
+    }
+    if (fn(user, t, p, h->n, h->s) != 0) return fail(SPSS_ERR_INVALID, "dopri5: the observer stopped the integration");
+    return SPSS_OK;
+  }
+};
+
+// integrate_const with a dense-output stepper: observations at t0 + j dt from the continuous
+// extension; the state ends as the last observed state
+int dp_integrate_const(DpRun &R, double t0, double t1, double dt, DpObserver &obs) {
+  spss_system *h = R.h;
+  if (!less_with_sign(t0, t1, dt)) return SPSS_OK;
+  TRY(dp_initialize(R, h->state, t0, dt));
+  TRY(obs.call(h, h->state, t0));
+  long obs_step = 1;
+  double time = t0 + (double)obs_step * dt;
+  while (less_eq_with_sign(time + dt, t1, dt)) {
+    while (less_eq_with_sign(time, R.t, dt)) {
+      dp_calc_state(R, time, h->state);
+      TRY(obs.call(h, h->state, time));
+      ++obs_step;
+      time = t0 + (double)obs_step * dt;
+    }
+    if (less_with_sign(R.t + R.dt, t1, R.dt)) {  // not at the end yet: real steps past the next observation
+      while (less_eq_with_sign(R.t, time, dt)) TRY(dp_do_step(R));
+    } else if (less_with_sign(R.t, t1, R.dt)) {  // the last step, ending on t1
+      TRY(dp_initialize(R, R.x(R.cur), R.t, t1 - R.t));
+      TRY(dp_do_step(R));
+    }
+  }
+  if (less_eq_with_sign(time, t1, dt)) {  // the last observation
+    dp_calc_state(R, time, h->state);
+    TRY(obs.call(h, h->state, time));
+  }
+  return SPSS_OK;
+}
+
+// integrate_adaptive with a dense-output stepper: observed after every step; the state ends at t1
+int dp_integrate_adaptive(DpRun &R, double t0, double t1, double dt, DpObserver &obs) {
+  spss_system *h = R.h;
+  TRY(dp_initialize(R, h->state, t0, dt));
+  TRY(obs.call(h, h->state, t0));
+  while (less_with_sign(R.t, t1, R.dt)) {
+    while (less_eq_with_sign(R.t + R.dt, t1, R.dt)) {
+      TRY(dp_do_step(R));
+      TRY(obs.call(h, R.x(R.cur), R.t));
+    }
+    // the step size that ends exactly on t1
+    TRY(dp_initialize(R, R.x(R.cur), R.t, t1 - R.t));
+  }
+  HIPCHECK(hipMemcpyAsync(h->state, R.x(R.cur), sizeof(double) * R.m, hipMemcpyDeviceToDevice, h->stream));
+  return SPSS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spss_dopri5_integrate(spss_system *h, int method, int mode, double t0, double t1, double dt, double eps_abs,
+                          double eps_rel, spss_observer_fn observer, void *user, int observe_on_host, long *steps,
+                          long *rejected) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (mode != SPSS_DOPRI5_CONST && mode != SPSS_DOPRI5_ADAPTIVE) return fail(SPSS_ERR_INVALID, "unknown mode");
+  if (!(dt > 0.0) || !(eps_abs >= 0.0) || !(eps_rel >= 0.0) || !(eps_abs + eps_rel > 0.0))
+    return fail(SPSS_ERR_INVALID, "dopri5: dt must be > 0 and the tolerances >= 0, not both 0");
+  TRY(check_method(h, resolve_method(h, method)));
+  TRY(master_write(h, true));
+  const long m = (long)h->s * h->n;
+  for (int q = 0; q < 11; ++q)
+    if (!h->dp[q]) {
+      // zeroed like odeint's value-initialised buffers: the RHS leaves k[3] of a particle without
+      // neighbours untouched, so such entries keep what the buffer last held, as in the reference
+      TRY(dalloc_n(&h->dp[q], (size_t)m));
+      launch_fill_double(h->stream, m, h->dp[q], 0.0);
+    }
+  if (observer && observe_on_host && !h->dp_host)
+    HIPCHECK(hipHostMalloc((void **)&h->dp_host, sizeof(double) * (m > 0 ? m : 1), hipHostMallocDefault));
+  DpRun R;
+  R.h = h;
+  R.method = method;
+  R.m = m;
+  R.eps_abs = eps_abs;
+  R.eps_rel = eps_rel;
+  DpObserver obs{observer, user, observe_on_host};
+  const int rc = mode == SPSS_DOPRI5_CONST ? dp_integrate_const(R, t0, t1, dt, obs)
+                                           : dp_integrate_adaptive(R, t0, t1, dt, obs);
+  if (steps) *steps = R.steps;
+  if (rejected) *rejected = R.rejected;
+  if (rc != SPSS_OK) return rc;
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return SPSS_OK;
+}
+
+}  // extern "C"
+

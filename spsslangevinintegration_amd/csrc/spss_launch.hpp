@@ -99,6 +99,20 @@ void launch_srk(hipStream_t st, int n, double *state, const double *f, const dou
                 double c15, double is3, double sq2d, double friction, int noise_mode, const double *noise,
                 uint64_t seed, uint64_t step, double *max_partial);
 void launch_max_final(hipStream_t st, int nb, const double *partial, double *out);
+
+// Dormand-Prince 5(4): out = 1.0 * x + c[0] * k[0] + ... + c[n-1] * k[n-1] over m values (a stage
+// state, the step's output or a dense-output state), and the step's error norm
+constexpr int kDpMaxTerms = 6;
+struct DpTerms {
+  const double *k[kDpMaxTerms];
+  double c[kDpMaxTerms];
+  int n;
+};
+void launch_dp_combine(hipStream_t st, long m, const double *x, const DpTerms &T, double *out);
+// max over the m = s * n values of |sum_j c_j k_j| / (eps_abs + eps_rel * (|x_old| + dt * |dxdt_old|))
+// into out[0] (partial: one double per 256 particles); returns the number of partials
+int launch_dp_error(hipStream_t st, int n, int s, const double *x_old, const double *dxdt_old, const DpTerms &T,
+                    double dt, double eps_abs, double eps_rel, double *partial, double *out);
 void launch_disp_max(hipStream_t st, int n, int s, const double *tot, double *partial);
 void launch_order_parameter(hipStream_t st, int n, int s, int slot, const double *state, double *partial,
                             double *out);

@@ -161,6 +161,29 @@ class ParticleSystem:
         A.check(self.lib.spss_load_last_frame(self.h, os.fsencode(path), C.byref(t)))
         return t.value
 
+    # ---- adaptive Dormand-Prince 5(4) with dense output
+    def integrate_dopri5(self, t0, t1, dt, eps_abs=1e-6, eps_rel=1e-6, method=None, mode=None, observer=None):
+        """the reference's Boost.Odeint integration (SimulationEngineFirstOrder.cpp:158-165 and
+        :273-281): mode DOPRI5_CONST observes t0 + j*dt from the dense output and ends at the last
+        observed state, DOPRI5_ADAPTIVE observes every accepted step and ends at t1.  observer(t,
+        state) gets a host copy of the s*n state; returning True stops the integration.
+        Returns (accepted steps, rejected trials)."""
+        method = A.METHOD_AUTO if method is None else method
+        mode = A.DOPRI5_CONST if mode is None else mode
+        cb = A.OBSERVER_FN(0)
+        if observer is not None:
+            def _obs(user, t, state, n, s):
+                try:
+                    return 1 if observer(t, np.ctypeslib.as_array(state, shape=(n * s,)).copy()) else 0
+                except Exception:
+                    return 1
+            cb = A.OBSERVER_FN(_obs)
+        steps, rejected = C.c_long(), C.c_long()
+        A.check(self.lib.spss_dopri5_integrate(self.h, int(method), int(mode), float(t0), float(t1), float(dt),
+                                               float(eps_abs), float(eps_rel), cb, None, 1, C.byref(steps),
+                                               C.byref(rejected)))
+        return steps.value, rejected.value
+
     # ---- cell-size autotuner (FP32C)
     def autotune_cells(self, stepper, dt, probe_steps=10):
         """times the FP32C step at cell subdivisions 1, 2, 3 and keeps the fastest; returns
