@@ -314,8 +314,9 @@ int spss_averaging_get(spss_system *sys, double *host_sum, long *count);
  * default_error_checker (a_x = a_dxdt = 1) and default_step_adjuster; 500 rejected trials in a
  * row are an error, as Boost's failed_step_checker throws.  The observer gets the state
  * (s*n doubles, AoS) as a host pointer (observe_on_host = 1) or a device pointer, valid for the
- * call; a nonzero return stops the integration with SPSS_ERR_INVALID.  steps / rejected (may be
- * NULL) count accepted steps and rejected trials. */
+ * call; during the call the system's own state is the observed state (so spss_frame_append and
+ * spss_summary_append write it); a nonzero return stops the integration with SPSS_ERR_INVALID.
+ * steps / rejected (may be NULL) count accepted steps and rejected trials. */
 #define SPSS_DOPRI5_CONST 0
 #define SPSS_DOPRI5_ADAPTIVE 1
 typedef int (*spss_observer_fn)(void *user, double t, const double *state, int n, int s);

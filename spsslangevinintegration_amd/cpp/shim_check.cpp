@@ -11,6 +11,9 @@
 //                                                              summary files into folder (BinaryObserverFirstOrderPtr),
 //                                                              then InitializeSystemStateFromPreviousSolution on them
 //   shim_check continue n folder in.traj out.bin               RunContinuationMethod from in.traj: 2 alphas, t_1 = 0.05
+//   shim_check dopri5  n folder          out.bin               SimulationEngineFirstOrder (Boost.Odeint dopri5 with dense
+//                                                              output, t_1 = 2) writing the reference's files into folder;
+//                                                              out.bin: initial state, natural frequencies, final state
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -53,6 +56,21 @@ int main(int argc, char **argv) {
       }
       write_file(argv[3], engine.system_state());
       write_file((std::string(argv[3]) + ".summary").c_str(), out);
+      return 0;
+    }
+    if (mode == "dopri5") {
+      spss_shim::default_options().output_folder = std::string(argv[3]) + "/";
+      SimulationEngineFirstOrder engine(&thread, n, s);
+      SimulationEngineFirstOrder::Run r;
+      r.t_1 = 2.0;
+      r.seed = 7;
+      engine.RunSimulation(r);
+      std::vector<Real> out(engine.initial_state());
+      out.insert(out.end(), engine.natural_frequencies().begin(), engine.natural_frequencies().end());
+      out.insert(out.end(), engine.system_state().begin(), engine.system_state().end());
+      write_file(argv[4], out);
+      std::ofstream f(std::string(argv[4]) + ".files");
+      f << engine.files()[0] << "\n" << engine.files()[1] << "\n" << engine.steps() << "\n" << engine.rejected() << "\n";
       return 0;
     }
     if (mode == "continue") {

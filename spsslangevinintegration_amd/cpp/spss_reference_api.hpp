@@ -33,6 +33,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <iomanip>
 #include <memory>
 #include <random>
 #include <sstream>
@@ -554,6 +555,121 @@ class BinaryObserverSecondOrderPtr {
     return b.str();
   }
   spss_shim::BinaryFiles files_;
+};
+
+// ---------------------------------------------------------------------------
+// BinaryObserverFirstOrder.cpp:18-82,101-148 (the serial observer that Boost.Odeint calls): the
+// same frames and summary lines as BinaryObserverFirstOrderPtr, every 100th call each, under the
+// serial class's file names (alpha and what follows it in the trajectory name printed with
+// setprecision(10)); operator()(state, t) saves both.  Frames and sums come from the device state
+// of the bound system, which during an integration's observer call is the observed state.
+class BinaryObserverFirstOrder {
+ public:
+  BinaryObserverFirstOrder(Thread *thread, Real v_0, Real xi, Real sigma, Real rho, Real alpha, Real D_phi,
+                           PeriodicBoundaryConditions &pbc_config, int number_of_particles,
+                           int number_of_state_variables, Real dt, int trial)
+      : files_(name(v_0, xi, sigma, rho, alpha, D_phi, number_of_particles, trial, true),
+               name(v_0, xi, sigma, rho, alpha, D_phi, number_of_particles, trial, false), 100) {
+    (void)thread, (void)pbc_config, (void)number_of_state_variables, (void)dt;
+  }
+  void Bind(spss_shim::DeviceSystem &dev) { files_.Bind(dev); }
+  void SaveSystemState(const std::vector<Real> & /*system_state*/, Real t) { files_.SaveSystemState(t); }
+  void SaveSummaryStatistics(const std::vector<Real> & /*system_state*/, Real t) { files_.SaveSummaryStatistics(t); }
+  void operator()(Real t) {
+    files_.SaveSystemState(t);
+    files_.SaveSummaryStatistics(t);
+  }
+  const std::string &simulation_file_name() const { return files_.simulation_file_name(); }
+  const std::string &summary_statistics_file_name() const { return files_.summary_statistics_file_name(); }
+
+ private:
+  static std::string name(Real v_0, Real xi, Real sigma, Real rho, Real alpha, Real D_phi, int n, int trial,
+                          bool trajectory) {
+    std::ostringstream b;  // BinaryObserverFirstOrder.cpp:50-60, 66-76
+    b << spss_shim::default_options().output_folder << "v0_" << v_0 << "_xi_" << xi << "_sigma_" << sigma
+      << "_rho_" << rho << "_alpha_";
+    if (trajectory) b << std::setprecision(10);
+    b << alpha << "_Dphi_" << D_phi << "_N_" << n << "_" << 0 << "_" << trial << (trajectory ? ".bin" : ".txt");
+    return b.str();
+  }
+  spss_shim::BinaryFiles files_;
+};
+
+// SimulationEngineFirstOrder.cpp:96-165 (the serial engine): random initial state and natural
+// frequencies ~ N(0, 0.5) (:35-64), then Boost.Odeint's integrate_const(make_dense_output(1e-6,
+// 1e-6, runge_kutta_dopri5<vector<Real>>()), particle_system, state, t_0, t_1, dt,
+// boost::ref(binary_observer)) as spss_dopri5_integrate(SPSS_DOPRI5_CONST) on the device, the
+// observer writing the reference's files at every 100th observation.
+class SimulationEngineFirstOrder {
+ public:
+  SimulationEngineFirstOrder(Thread *thread, int number_of_particles, int number_of_state_variables)
+      : thread_(thread), pbc_config_(thread, number_of_particles, number_of_state_variables, 1.0, 1.0),
+        n_(number_of_particles), s_(number_of_state_variables),
+        system_state_((size_t)number_of_particles * number_of_state_variables, 0.0),
+        natural_frequencies_(number_of_particles, 0.0) {}
+
+  struct Run {
+    Real v_0 = 1.0, xi = 0.1, sigma = 5.0, rho = 0.8, alpha = 0.0, D_phi = 0.0;  // :115-120
+    Real t_0 = 0.0, t_1 = 10000.0, dt = 0.01;                                   // :123-125
+    Real eps_abs = 1.0e-6, eps_rel = 1.0e-6;                                    // :159
+    unsigned seed = 0;
+    int trial = 0;
+    int method = SPSS_METHOD_AUTO;  // the reference's EvaluateRhs dispatch
+  };
+
+  void RunSimulation() { RunSimulation(Run{}); }
+  void RunSimulation(const Run &r) {
+    InitializeRandomSystemState(r.sigma, r.xi, r.seed);
+    initial_state_ = system_state_;
+    spss_shim::DeviceSystem dev(
+        spss_shim::make_params(s_, n_, r.v_0, r.xi, r.sigma, r.rho, r.alpha, r.D_phi, 1.0, 0.0, pbc_config_));
+    spss_shim::check(spss_set_natural_frequencies(dev.handle(), natural_frequencies_.data(), n_));
+    spss_shim::check(spss_set_state(dev.handle(), system_state_.data()));
+    BinaryObserverFirstOrder observer(thread_, r.v_0, r.xi, r.sigma, r.rho, r.alpha, r.D_phi, pbc_config_, n_, s_,
+                                      r.dt, r.trial);
+    observer.Bind(dev);
+    files_ = {observer.simulation_file_name(), observer.summary_statistics_file_name()};
+    spss_shim::check(spss_dopri5_integrate(dev.handle(), r.method, SPSS_DOPRI5_CONST, r.t_0, r.t_1, r.dt, r.eps_abs,
+                                           r.eps_rel, &observe, &observer, 0, &steps_, &rejected_));
+    dev.DownloadState(system_state_.data());
+  }
+  const std::vector<Real> &system_state() const { return system_state_; }
+  const std::vector<Real> &initial_state() const { return initial_state_; }
+  const std::vector<Real> &natural_frequencies() const { return natural_frequencies_; }
+  const std::vector<std::string> &files() const { return files_; }
+  long steps() const { return steps_; }
+  long rejected() const { return rejected_; }
+
+ private:
+  static int observe(void *user, double t, const double * /*state*/, int /*n*/, int /*s*/) {
+    try {
+      (*static_cast<BinaryObserverFirstOrder *>(user))(t);
+      return 0;
+    } catch (...) {
+      return 1;
+    }
+  }
+  void InitializeRandomSystemState(Real sigma, Real xi, unsigned seed) {
+    std::mt19937 gen(seed);
+    std::uniform_real_distribution<Real> unif(0.0, 1.0);
+    std::uniform_real_distribution<Real> ang(0.0, 2.0 * M_PI);
+    std::uniform_real_distribution<Real> om(-sigma / xi, sigma / xi);
+    for (int i = 0; i < n_; ++i) {
+      system_state_[(size_t)s_ * i] = unif(gen);
+      system_state_[(size_t)s_ * i + 1] = unif(gen);
+      system_state_[(size_t)s_ * i + 2] = ang(gen);
+      if (s_ == 4) system_state_[(size_t)s_ * i + 3] = om(gen);
+    }
+    std::normal_distribution<Real> nat(0.0, 0.5);
+    for (int i = 0; i < n_; ++i) natural_frequencies_[i] = nat(gen);
+  }
+
+  Thread *thread_;
+  PeriodicBoundaryConditions pbc_config_;
+  int n_, s_;
+  std::vector<Real> system_state_, initial_state_, natural_frequencies_;
+  std::vector<std::string> files_;
+  long steps_ = 0, rejected_ = 0;
 };
 
 // ---------------------------------------------------------------------------

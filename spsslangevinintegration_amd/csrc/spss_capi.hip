@@ -1263,7 +1263,10 @@ int dp_integrate_adaptive(DpRun &R, double t0, double t1, double dt, DpObserver 
   while (less_with_sign(R.t, t1, R.dt)) {
     while (less_eq_with_sign(R.t + R.dt, t1, R.dt)) {
       TRY(dp_do_step(R));
-      TRY(obs.call(h, R.x(R.cur), R.t));
+      if (obs.fn) {  // the system's state is the observed state during the call
+        HIPCHECK(hipMemcpyAsync(h->state, R.x(R.cur), sizeof(double) * R.m, hipMemcpyDeviceToDevice, h->stream));
+        TRY(obs.call(h, h->state, R.t));
+      }
     }
     // the step size that ends exactly on t1
     TRY(dp_initialize(R, R.x(R.cur), R.t, t1 - R.t));

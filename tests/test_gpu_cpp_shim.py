@@ -194,3 +194,47 @@ def test_shim_parameter_continuation_from_a_trajectory_file():
         assert fr.size == want.size and fr[0] == want[0]
         np.testing.assert_allclose(fr[1:], want[1:], rtol=3e-7, atol=3e-7)
     assert np.max(np.abs(got - st)) <= 1e-12
+
+
+def test_shim_serial_engine_dopri5_with_observer_files():
+    """SimulationEngineFirstOrder::RunSimulation (SimulationEngineFirstOrder.cpp:96-165): Boost.Odeint's
+    integrate_const(make_dense_output(1e-6, 1e-6, dopri5)) with BinaryObserverFirstOrder writing a
+    frame and a summary line every 100th observation, t_1 = 2, dt = 0.01; against the Dormand-Prince
+    restatement driven by the oracle's RHS from the same initial state and natural frequencies"""
+    from oracle import dopri5_oracle as D
+    from oracle import observer_oracle as OO
+    n = 300
+    with tempfile.TemporaryDirectory() as td:
+        fout = os.path.join(td, "out.bin")
+        res = subprocess.run([EXE, "dopri5", str(n), td, fout], capture_output=True, text=True, timeout=300)
+        assert res.returncode == 0, res.stderr
+        got = np.fromfile(fout)
+        sim, summ, steps, rejected = open(fout + ".files").read().split()
+        frames = np.fromfile(sim, dtype=np.float32).reshape(-1, 1 + 4 * n)
+        lines = open(summ).read().splitlines()
+    assert os.path.basename(sim) == "v0_1_xi_0.1_sigma_5_rho_0.8_alpha_0_Dphi_0_N_300_0_0.bin"
+    assert os.path.basename(summ) == "v0_1_xi_0.1_sigma_5_rho_0.8_alpha_0_Dphi_0_N_300_0_0.txt"
+    x0, natf, final = got[:4 * n], got[4 * n:5 * n], got[5 * n:]
+    o = oracle.Oracle(model=4, n=n, v0=1.0, xi=0.1, sigma=5.0, rho=0.8, alpha=0.0, D_phi=0.0)
+    o.set_natural_frequencies(natf)
+    zeros = np.zeros(4 * n)
+
+    def rhs(x):
+        out = np.zeros(4 * n)
+        o.evaluate_rhs(0, np.ascontiguousarray(x), zeros, out, 0.0, 0.0)
+        return out
+
+    want_final, seen, st = D.integrate_const(rhs, x0, 0.0, 2.0, 0.01, 1e-6, 1e-6)
+    print(f"  steps {steps}/{st.steps} rejected {rejected}/{st.rejected} observations {len(seen)}", flush=True)
+    assert (int(steps), int(rejected)) == (st.steps, st.rejected)
+    saved = seen[::100]  # the observer's counter: every 100th call
+    assert frames.shape[0] == len(saved) and len(lines) == len(saved)
+    for fr, line, (t, x) in zip(frames, lines, saved):
+        want = OO.frame(x, 4, t)
+        assert fr[0] == want[0]
+        np.testing.assert_allclose(fr[1:], want[1:], rtol=3e-7, atol=3e-7)
+        t_s, a_s, g_s = line.split("\t")
+        wa, wg = OO.summary(x, 4)
+        assert float(t_s) == float("%g" % np.float32(t))
+        assert abs(float(a_s) - wa) <= 1e-5 and abs(float(g_s) - wg) <= 1e-5
+    assert np.max(np.abs(final - want_final) / np.maximum(np.abs(want_final), 1.0)) <= 1e-10
