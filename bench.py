@@ -271,6 +271,11 @@ def main():
         print("[bench] batched ensemble", file=sys.stderr, flush=True)
         batched = batched_ensemble(spss, steps=args.batch_steps, plain_steps=args.plain_steps)
         print(f"[bench] batched {batched['speedup_vs_plain_loop']:.1f}x plain loop", file=sys.stderr, flush=True)
+    # secondary: the reference's deterministic production path, Boost.Odeint's dense-output
+    # dopri5 (eps 1e-6) over the global-field RHS of its default run (v0 = 1, xi = 0.1, sigma = 5,
+    # rho = 0.8, natural frequencies ~ N(0, 0.5)), observing every dt = 0.01 on the device
+    print("[bench] dopri5", file=sys.stderr, flush=True)
+    dopri5 = dopri5_run(spss)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -304,9 +309,34 @@ def main():
         "pair_interactions_per_s": pairs / n * value if pairs else None,
         "srk_particle_steps_per_s": n * srk_steps / wall2,
         "batched": batched,
+        "dopri5": dopri5,
     }
     g.close()
     print(json.dumps(line), flush=True)
+
+
+def dopri5_run(spss, n=1 << 20, t1=0.5, dt=0.01):
+    """accepted steps, rejected trials and particle-RHS evaluations per second of spss_dopri5_integrate
+    (integrate_const, eps_abs = eps_rel = 1e-6) at N = 1,048,576 over [0, t1], device observer"""
+    rng = np.random.default_rng(0x5EED0005)
+    st = np.empty((n, 4))
+    st[:, 0], st[:, 1] = rng.random(n), rng.random(n)
+    st[:, 2] = rng.uniform(0, 2 * np.pi, n)
+    st[:, 3] = rng.uniform(-50.0, 50.0, n)  # U[-sigma/xi, sigma/xi]
+    g = spss.ParticleSystem(model=4, n=n, v0=1.0, xi=0.1, sigma=5.0, rho=0.8, alpha=0.0, D_phi=0.0)
+    g.set_natural_frequencies(rng.normal(0.0, 0.5, n))
+    g.set_state(st.reshape(-1))
+    g.integrate_dopri5(0.0, 2 * dt, dt, 1e-6, 1e-6)  # warm-up
+    g.set_state(st.reshape(-1))
+    g.synchronize()
+    t = time.perf_counter()
+    steps, rejected = g.integrate_dopri5(0.0, t1, dt, 1e-6, 1e-6, method=spss.METHOD_GLOBAL_FIELD)
+    g.synchronize()
+    sec = time.perf_counter() - t
+    g.close()
+    evals = 6 * (steps + rejected) + 1  # six per trial step (first same as last) and the initial one
+    return {"n": n, "t1": t1, "dt": dt, "eps": 1e-6, "method": "global field", "accepted_steps": steps,
+            "rejected_trials": rejected, "seconds": sec, "particle_rhs_evals_per_s": evals * n / sec}
 
 
 def C_pairs(g, spss):

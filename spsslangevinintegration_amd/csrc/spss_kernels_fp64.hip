@@ -297,17 +297,33 @@ __global__ __launch_bounds__(kBlock) void k_polar_partial(int n, int s, int slot
 }
 
 // compensated sum of the block partials in block order; writes sums[4] (not yet divided by n)
-__global__ void k_polar_final(int nblocks, const double *__restrict__ partial, double *__restrict__ sums) {
-  if (threadIdx.x >= 4) return;
-  const int q = threadIdx.x;
+// the four block-partial columns summed with compensation (two_sum): 64 lanes per column each take
+// every 64th block, then one lane combines the 64 compensated pairs in lane order; the sums are
+// divided by n (the field Z and W).  The order is fixed, so the result does not vary between runs.
+__global__ __launch_bounds__(256) void k_polar_final(int nblocks, const double *__restrict__ partial,
+                                                     double *__restrict__ sums, double n) {
+  __shared__ double sh[2][4][64];
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double s = 0.0, c = 0.0;
-  for (int b = 0; b < nblocks; ++b) {
+  for (int b = lane; b < nblocks; b += 64) {
     double t, e;
     two_sum(s, partial[(long)b * 4 + q], t, e);
     s = t;
     c += e;
   }
-  sums[q] = s + c;
+  sh[0][q][lane] = s;
+  sh[1][q][lane] = c;
+  __syncthreads();
+  if (lane == 0) {
+    double ts = 0.0, tc = 0.0;
+    for (int l = 0; l < 64; ++l) {
+      double t, e;
+      two_sum(ts, sh[0][q][l], t, e);
+      ts = t;
+      tc += e + sh[1][q][l];
+    }
+    sums[q] = (ts + tc) / n;
+  }
 }
 
 // k of the global branch; fields = (Re Z, Im Z, Re W, Im W) already divided by N
@@ -341,9 +357,6 @@ __global__ void k_global_rhs(DevParams P, const double *__restrict__ stage, cons
   }
 }
 
-__global__ void k_divide(double *__restrict__ v, int cnt, double d) {
-  if ((int)threadIdx.x < cnt) v[threadIdx.x] = v[threadIdx.x] / d;
-}
 
 // ---------------------------------------------------------------------------
 // steppers.  Each writes the step's total displacement only as the per-block
@@ -527,17 +540,32 @@ __global__ __launch_bounds__(kBlock) void k_order_partial(int n, int s, int slot
   }
 }
 
-__global__ void k_order_final(int nb, const double *__restrict__ partial, double n, double *__restrict__ out) {
-  if (threadIdx.x >= 2) return;
-  const int q = threadIdx.x;
+// the (cos, sin) block partials summed with compensation: 64 lanes per column each take every
+// 64th block, then one lane combines the 64 compensated pairs in lane order (a fixed order)
+__global__ __launch_bounds__(128) void k_order_final(int nb, const double *__restrict__ partial, double n,
+                                                     double *__restrict__ out) {
+  __shared__ double sh[2][2][64];
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double s = 0.0, c = 0.0;
-  for (int b = 0; b < nb; ++b) {
+  for (int b = lane; b < nb; b += 64) {
     double t, e;
     two_sum(s, partial[(long)b * 4 + q], t, e);
     s = t;
     c += e + partial[(long)b * 4 + 2 + q];
   }
-  out[q] = (s + c) / n;
+  sh[0][q][lane] = s;
+  sh[1][q][lane] = c;
+  __syncthreads();
+  if (lane == 0) {
+    double ts = 0.0, tc = 0.0;
+    for (int l = 0; l < 64; ++l) {
+      double t, e;
+      two_sum(ts, sh[0][q][l], t, e);
+      ts = t;
+      tc += e + sh[1][q][l];
+    }
+    out[q] = (ts + tc) / n;
+  }
 }
 
 // ApplyPeriodicBoundaryConditions(Real*, long), PeriodicBoundaryConditions.cpp:147-155
@@ -636,8 +664,7 @@ void launch_global(hipStream_t st, const DevParams &P, const double *stage, doub
   const int s = P.s;
   const int nb = (int)nblk(P.n);
   k_polar_partial<<<dim3(nb), dim3(kBlock), 0, st>>>(P.n, s, Slots<4>::phi, (s == 4) ? 3 : -1, stage, partial);
-  k_polar_final<<<dim3(1), dim3(64), 0, st>>>(nb, partial, fields);
-  k_divide<<<dim3(1), dim3(64), 0, st>>>(fields, 4, (double)P.n);
+  k_polar_final<<<dim3(1), dim3(256), 0, st>>>(nb, partial, fields, (double)P.n);
   if (s == 4)
     k_global_rhs<4><<<dim3(nblk(P.n)), dim3(kBlock), 0, st>>>(P, stage, fields, nat, k_next, compl_out);
   else
@@ -673,7 +700,7 @@ void launch_order_parameter(hipStream_t st, int n, int s, int slot, const double
                             double *out) {
   const int nb = (int)nblk(n);
   k_order_partial<<<dim3(nb), dim3(kBlock), 0, st>>>(n, s, slot, state, partial);
-  k_order_final<<<dim3(1), dim3(64), 0, st>>>(nb, partial, (double)n, out);
+  k_order_final<<<dim3(1), dim3(128), 0, st>>>(nb, partial, (double)n, out);
 }
 
 void launch_apply_pbc(hipStream_t st, int n, int s, double *state, double xs, double ys, double xrs, double yrs) {
