@@ -234,8 +234,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see the sharding design in "
-              "DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the sharding by slabs of cell rows is "
+              "designed, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import spsslangevinintegration_amd as spss

@@ -10,7 +10,7 @@ reference, which sweeps parameters in a host loop, SimulationEngineFirstOrder.cp
   points, only those where the predicted |d mean|Z| / d sigma| is large or the predicted
   spread of |Z| exceeds a bound, and scores the network against those simulations;
   `screened_sweep` locates the crossing of a level of mean |Z| along a sigma sweep with few
-  simulations.  The simulation is both the fallback and the judge: every point the screen
+  simulations.  The simulation is both the fallback and the arbiter: every point the screen
   reports on is simulated, and a sweep whose simulated points do not bracket the crossing (or
   disagree with the prediction by more than `max_error`) is widened until they do, up to the
   plain sweep.

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 TOL_POS = 1e-12      # k slots 0..2: FP64 self terms
 TOL_COUPLING = 1e-6  # k slot 3: sigma * (phasor pair sum) / n, absolute
-TOL_R = 1e-4         # order parameter vs FP64 oracle
+TOL_R = 1e-4         # order parameter vs FP64 oracle (the stated accuracy target)
 
 
 @pytest.fixture(scope="module")

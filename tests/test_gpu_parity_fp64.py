@@ -2,7 +2,7 @@
 
 1. Every golden fixture (the reference's own outputs, tests/golden) is replayed
    through libspss_hip.so: trajectories within 1e-10 relative per state slot
-   (the north-star FP64 bound; device sin/cos differ from glibc by ulps only),
+   (the FP64 trajectory bound; device sin/cos differ from glibc by ulps only),
    Verlet lists bit-exact in content and order.
 2. Random inputs against the oracle for every method and model: one RHS call
    within 1e-12 relative, neighbour pair sets bit-exact against brute force.
