@@ -123,8 +123,9 @@ def ensemble_config(R=512, seed=0x5EED0004):
 
 
 def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0.03, every=10):
-    """512-replica ensemble: batched run (autotuned tiles, graph replays) against the plain loop
-    (the same kernels through one single-system handle per replica, one after another)."""
+    """512-replica ensemble: batched run (autotuned cell size and tiles, graph replays) against the
+    plain loop (the same kernels at the same cell size through one single-system handle per
+    replica, one after another)."""
     groups = ensemble_config()
     total_n = sum(int(g["ns"].sum()) for g in groups.values())
     batches, tune = {}, {}
@@ -132,8 +133,10 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
         b = spss.ReplicaBatch(model, g["ns"], g["xi"], g["sigma"], g["D"], g["seeds"], v0=v0, rho=rho)
         b.set_natural_frequencies(np.concatenate(g["nats"]))
         b.set_state(np.concatenate(g["states"]))
+        ctable, k = b.autotune_cells(dt, probe_steps=10)
         table, chosen = b.autotune(dt, probe_steps=10)
-        tune[model] = {"ms_per_step": [round(float(t), 4) for t in table], "chosen": int(chosen)}
+        tune[model] = {"cell_subdivision_ms_per_step": [round(float(t), 4) for t in ctable], "cell_subdivision": k,
+                       "ms_per_step": [round(float(t), 4) for t in table], "chosen": int(chosen)}
         batches[model] = b
     # sample for the plain-loop comparison: plain_steps steps of every replica, both ways
     t0 = time.perf_counter()
@@ -146,7 +149,7 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
         for k in range(len(g["ns"])):
             h = spss.ParticleSystem(model=model, n=int(g["ns"][k]), v0=v0, xi=g["xi"][k], sigma=g["sigma"][k],
                                     rho=rho, D_phi=g["D"][k], natfreq_local=1, precision=spss.PRECISION_FP32C,
-                                    seed=int(g["seeds"][k]))
+                                    seed=int(g["seeds"][k]), cell_subdivision=batches[model].cell_subdivision)
             h.set_natural_frequencies(g["nats"][k])
             h.set_state(g["states"][k])
             h.synchronize()

@@ -243,6 +243,14 @@ int spss_batch_set_tile_widths(spss_batch *b, const int *W);
  * particles per workgroup), keeps the fastest; ms per step into table_ms[<= table_cap], the
  * chosen index into *chosen.  State and step counter are restored, the averages cleared. */
 int spss_batch_autotune(spss_batch *b, double dt, int probe_steps, double *table_ms, int table_cap, int *chosen);
+/* cell-size autotuner: times `probe_steps` steps at cell subdivisions 1, 2, 3 (default tile widths),
+ * keeps the fastest; ms per step into table_ms[<= table_cap] (-1 where a subdivision does not
+ * build), the chosen subdivision into *chosen.  State and step counter are restored, the averages
+ * cleared.  Later results depend on the cell size to the FP32C accuracy; a replica stays
+ * bit-identical to a single FP32C system of the same subdivision. */
+int spss_batch_autotune_cells(spss_batch *b, double dt, int probe_steps, double *table_ms, int table_cap,
+                              int *chosen);
+int spss_batch_get_cell_subdivision(spss_batch *b, int *k);
 
 /* ---- the network of the learned sweep screen (no reference counterpart).  A fully connected
  * FP32 network of widths dims[0] -> ... -> dims[ndims-1] (at most 8 layers, widths <= 4096):

@@ -36,6 +36,7 @@ EXPORTS = [
     "spss_batch_create", "spss_batch_destroy", "spss_batch_set_state", "spss_batch_get_state",
     "spss_batch_set_natural_frequencies", "spss_batch_reset", "spss_batch_run", "spss_batch_order_stats",
     "spss_batch_get_tile_widths", "spss_batch_set_tile_widths", "spss_batch_autotune",
+    "spss_batch_autotune_cells", "spss_batch_get_cell_subdivision",
     "spss_mlp_create", "spss_mlp_destroy", "spss_mlp_param_count", "spss_mlp_set_params", "spss_mlp_get_params",
     "spss_mlp_predict", "spss_mlp_gradients", "spss_mlp_set_data", "spss_mlp_train", "spss_mlp_reset_optimizer",
     "spss_mlp_adam_steps",
@@ -134,6 +135,8 @@ def load(path=None):
         "spss_batch_get_tile_widths": (i, [vp, vp]),
         "spss_batch_set_tile_widths": (i, [vp, vp]),
         "spss_batch_autotune": (i, [vp, d, i, vp, i, C.POINTER(i)]),
+        "spss_batch_autotune_cells": (i, [vp, d, i, vp, i, C.POINTER(i)]),
+        "spss_batch_get_cell_subdivision": (i, [vp, C.POINTER(i)]),
         "spss_mlp_create": (i, [C.POINTER(vp), vp, i, i, C.c_uint64]),
         "spss_mlp_destroy": (i, [vp]),
         "spss_mlp_param_count": (i, [vp, C.POINTER(l)]),

@@ -83,6 +83,21 @@ class ReplicaBatch:
         w = np.ascontiguousarray(w, dtype=np.int32)
         A.check(self.lib.spss_batch_set_tile_widths(self.h, w.ctypes.data))
 
+    def autotune_cells(self, dt, probe_steps=20):
+        """times cell subdivisions 1, 2, 3 and keeps the fastest; returns (ms per step for each,
+        -1 where one does not build; the chosen subdivision)"""
+        table = np.zeros(3)
+        chosen = C.c_int()
+        A.check(self.lib.spss_batch_autotune_cells(self.h, float(dt), int(probe_steps), A.ptr(table), 3,
+                                                   C.byref(chosen)))
+        return table, chosen.value
+
+    @property
+    def cell_subdivision(self):
+        k = C.c_int()
+        A.check(self.lib.spss_batch_get_cell_subdivision(self.h, C.byref(k)))
+        return k.value
+
     def autotune(self, dt, probe_steps=20):
         """times the candidate tile widths; returns (ms per step for each, chosen index)"""
         table = np.zeros(6)

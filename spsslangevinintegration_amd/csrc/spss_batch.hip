@@ -30,6 +30,8 @@ struct spss_batch {
   bool have_nat = false;
   bool master_stale = false;  // the resident copy is ahead of `state`
   std::vector<int> rep_n;
+  std::vector<Coef> coef;  // per-replica coefficients (to rebuild the cell path at another cell size)
+  int nc = 1, k = 1;       // int(1/rho) and the cell subdivision of the current cell path
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
@@ -176,6 +178,9 @@ int spss_batch_create(const spss_batch_params *bp, int nrep, const spss_replica 
   }
   const int nc = std::max(1, (int)(1.0 / bp->rho));
   const int k = bp->cell_subdivision > 0 ? bp->cell_subdivision : 1;
+  b->coef = coef;
+  b->nc = nc;
+  b->k = k;
   int rc = fast_create_replicas(&b->fast, b->st, P, bp->rho, nc, k, nrep, b->rep_n.data(), coef.data());
   if (rc != SPSS_OK) {
     spss_batch_destroy(b);
@@ -315,6 +320,75 @@ int spss_batch_set_tile_widths(spss_batch *b, const int *W) {
 // 0.5x ... 2x the default particles-per-workgroup of every replica and keeps the fastest.  The
 // state, step counter and averages are restored afterwards; the tile width does not change
 // the results (a lane's candidates and their order depend on the cells, not on the tile).
+// the cell path rebuilt at cell subdivision k (state, natural frequencies and step counter are
+// the caller's to restore)
+static int rebuild_cells(spss_batch *b, int k) {
+  drop_graph(b);
+  FastPath *f = nullptr;
+  int rc = fast_create_replicas(&f, b->st, b->P, b->bp.rho, b->nc, k, b->R, b->rep_n.data(), b->coef.data());
+  if (rc != SPSS_OK) return rc;
+  fast_destroy(b->fast);
+  b->fast = f;
+  b->k = k;
+  return SPSS_OK;
+}
+
+// Cell size of the batch: times probe_steps graph-replayed steps at cell subdivisions 1, 2 and 3
+// (cells of side 1/(k int(1/rho)), each at its default tile widths), keeps the fastest and
+// restores the state and the step counter; the time averages start again.  The cell size
+// changes the FP32 candidate order, so the results of later steps depend on the choice (to
+// the FP32C accuracy); a replica stays bit-identical to a single FP32C system of the same cell
+// subdivision.
+int spss_batch_autotune_cells(spss_batch *b, double dt, int probe_steps, double *table_ms, int table_cap,
+                              int *chosen) {
+  if (!b || probe_steps < 2 || !(dt > 0.0)) return set_error(SPSS_ERR_INVALID, "bad argument");
+  BTRY(sync_master(b));
+  std::vector<double> saved((size_t)b->s * b->n), acc(3 * b->R), z(2 * b->R);
+  BCHK(hipMemcpyAsync(saved.data(), b->state, sizeof(double) * saved.size(), hipMemcpyDeviceToHost, b->st));
+  uint64_t step0 = 0;
+  int rc = fast_batch_stats(b->fast, b->st, acc.data(), z.data(), &step0);
+  if (rc != SPSS_OK) return fast_fail(rc);
+  double best = 1e300;
+  int best_k = b->k;
+  for (int k = 1; k <= 3; ++k) {
+    double ms_step = -1.0;
+    if (rebuild_cells(b, k) == SPSS_OK) {
+      BTRY(spss_batch_set_state(b, saved.data()));
+      BTRY(spss_batch_run(b, 2 + probe_steps % 2, dt, 0, 0, 1));  // enter, warm up
+      hipEvent_t e0, e1;
+      BCHK(hipEventCreate(&e0));
+      BCHK(hipEventCreate(&e1));
+      BCHK(hipEventRecord(e0, b->st));
+      BTRY(spss_batch_run(b, probe_steps, dt, 0, 0, 1));
+      BCHK(hipEventRecord(e1, b->st));
+      BCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      BCHK(hipEventElapsedTime(&ms, e0, e1));
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      ms_step = ms / probe_steps;
+      if (ms_step < best) {
+        best = ms_step;
+        best_k = k;
+      }
+    }
+    if (table_ms && k - 1 < table_cap) table_ms[k - 1] = ms_step;  // -1: this subdivision does not build
+  }
+  rc = rebuild_cells(b, best_k);
+  if (rc != SPSS_OK) return fast_fail(rc);
+  BTRY(spss_batch_set_state(b, saved.data()));
+  rc = fast_batch_reset(b->fast, b->st, step0);
+  if (rc != SPSS_OK) return fast_fail(rc);
+  if (chosen) *chosen = best_k;
+  return SPSS_OK;
+}
+
+int spss_batch_get_cell_subdivision(spss_batch *b, int *k) {
+  if (!b || !k) return set_error(SPSS_ERR_INVALID, "NULL argument");
+  *k = b->k;
+  return SPSS_OK;
+}
+
 int spss_batch_autotune(spss_batch *b, double dt, int probe_steps, double *table_ms, int table_cap, int *chosen) {
   if (!b || probe_steps < 2 || !(dt > 0.0)) return set_error(SPSS_ERR_INVALID, "bad argument");
   BTRY(sync_master(b));

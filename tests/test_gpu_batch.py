@@ -43,11 +43,12 @@ def _replicas(model, R, seed):
     return ns, xi, sig, Dp, seeds, states, nats
 
 
-def _singles(spss, model, ns, xi, sig, Dp, seeds, states, nats, K, dt, v0, rho):
+def _singles(spss, model, ns, xi, sig, Dp, seeds, states, nats, K, dt, v0, rho, k=1):
     out = []
     for r in range(len(ns)):
         g = spss.ParticleSystem(model=model, n=int(ns[r]), v0=v0, xi=xi[r], sigma=sig[r], rho=rho, D_phi=Dp[r],
-                                natfreq_local=1, precision=spss.PRECISION_FP32C, seed=int(seeds[r]))
+                                natfreq_local=1, precision=spss.PRECISION_FP32C, seed=int(seeds[r]),
+                                cell_subdivision=k)
         g.set_natural_frequencies(nats[r])
         g.set_state(states[r])
         g.run(spss.STEPPER_EULER, spss.METHOD_LINKED_LIST, K, dt, spss.NOISE_PHILOX)
@@ -130,3 +131,29 @@ def test_batch_tile_widths_and_autotune_do_not_change_results(spss):
         outs.append(b.get_state())
         b.close()
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_batch_cell_autotune_keeps_replicas_equal_to_single_systems(spss):
+    """the cell-size autotuner rebuilds the batch at subdivisions 1, 2, 3, restores state and step
+    counter, and leaves a batch whose replicas are bit-identical to single FP32C systems of the
+    chosen subdivision"""
+    model, R, K, dt, v0, rho = 4, 4, 12, 0.005, 0.01, 0.03
+    print("[gpu] batch cell autotune", flush=True)
+    ns, xi, sig, Dp, seeds, states, nats = _replicas(model, R, 77)
+    b = spss.ReplicaBatch(model, ns, xi, sig, Dp, seeds, v0=v0, rho=rho)
+    b.set_natural_frequencies(np.concatenate(nats))
+    b.set_state(np.concatenate(states))
+    table, k = b.autotune_cells(dt, probe_steps=4)
+    print(f"  ms/step by subdivision {np.round(table, 4)} chosen {k}", flush=True)
+    assert k in (1, 2, 3) and b.cell_subdivision == k and table[k - 1] == table[table > 0].min()
+    assert b.order_stats()[4] == 0  # step counter restored
+    np.testing.assert_array_equal(b.get_state(), np.concatenate(states))
+    b.run(K, dt, observe_every=0, graph=True)
+    got = b.get_state()
+    b.close()
+    want = _singles(spss, model, ns, xi, sig, Dp, seeds, states, nats, K, dt, v0, rho, k=k)
+    off = 0
+    for r in range(R):
+        m = model * int(ns[r])
+        np.testing.assert_array_equal(got[off:off + m], want[r])
+        off += m
