@@ -82,8 +82,8 @@ struct spss_system {
   bool master_stale = false;
   KernelTimer *timer = nullptr;  // around the pair kernel of the RHS
   // Dormand-Prince 5(4) buffers (allocated on first use): two states and derivatives, k2..k6,
-  // a stage state, a zero vector (the RHS's k_prev) and a pinned host copy for observers
-  double *dp[11] = {};
+  // a stage state and a pinned host copy for observers
+  double *dp[10] = {};
   double *dp_host = nullptr;
 };
 
@@ -1031,7 +1031,7 @@ int spss_averaging_get(spss_system *h, double *sum, long *count) {
 
 namespace {
 
-enum { DP_X0, DP_X1, DP_D0, DP_D1, DP_K2, DP_K3, DP_K4, DP_K5, DP_K6, DP_TMP, DP_ZERO };
+enum { DP_X0, DP_X1, DP_D0, DP_D1, DP_K2, DP_K3, DP_K4, DP_K5, DP_K6, DP_TMP, DP_COUNT };
 
 // the coefficients as Boost forms them: quotients of integers in double
 struct DpTableau {
@@ -1079,7 +1079,8 @@ DpTerms dp_terms(std::initializer_list<const double *> k, std::initializer_list<
 }
 
 int dp_rhs(DpRun &R, const double *x, double *out) {
-  return rhs_device(R.h, R.method, x, R.h->dp[DP_ZERO], out, 0.0, 0.0);
+  // the functor's EvaluateRhs(x, zeros, dxdt, 0, t): k_prev all zeros (no buffer read)
+  return rhs_device(R.h, R.method, x, nullptr, out, 0.0, 0.0);
 }
 
 // controlled_runge_kutta::try_step (FSAL): one trial step from the current point; on success the
@@ -1289,7 +1290,7 @@ int spss_dopri5_integrate(spss_system *h, int method, int mode, double t0, doubl
   TRY(check_method(h, resolve_method(h, method)));
   TRY(master_write(h, true));
   const long m = (long)h->s * h->n;
-  for (int q = 0; q < 11; ++q)
+  for (int q = 0; q < DP_COUNT; ++q)
     if (!h->dp[q]) {
       // zeroed like odeint's value-initialised buffers: the RHS leaves k[3] of a particle without
       // neighbours untouched, so such entries keep what the buffer last held, as in the reference

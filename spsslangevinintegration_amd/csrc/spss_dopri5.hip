@@ -31,17 +31,18 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_combine(long m, const double *_
   out[e] = v;
 }
 
-// per particle: the largest relative error of its s values; per block: the largest of those
-__global__ __launch_bounds__(kDpBlock) void k_dp_error(int n, int s, const double *__restrict__ x_old,
+// per block: the largest relative error over the s values of its 256 particles, read as s
+// coalesced passes of 256 consecutive values (the largest value does not depend on the order)
+__global__ __launch_bounds__(kDpBlock) void k_dp_error(long m, int s, const double *__restrict__ x_old,
                                                        const double *__restrict__ dxdt_old, DpTerms T, double dt,
                                                        double eps_abs, double eps_rel,
                                                        double *__restrict__ partial) {
   __shared__ double red[kDpBlock];
-  const int i = blockIdx.x * kDpBlock + threadIdx.x;
+  const long e0 = (long)blockIdx.x * kDpBlock * s + threadIdx.x;
   double worst = 0.0;
-  if (i < n) {
-    for (int q = 0; q < s; ++q) {
-      const long e = (long)i * s + q;
+  for (int q = 0; q < s; ++q) {
+    const long e = e0 + (long)q * kDpBlock;
+    if (e < m) {
       double err = T.c[0] * T.k[0][e];
 #pragma unroll
       for (int j = 1; j < kDpMaxTerms; ++j)
@@ -68,8 +69,9 @@ void launch_dp_combine(hipStream_t st, long m, const double *x, const DpTerms &T
 
 int launch_dp_error(hipStream_t st, int n, int s, const double *x_old, const double *dxdt_old, const DpTerms &T,
                     double dt, double eps_abs, double eps_rel, double *partial, double *out) {
-  const int nb = (n + kDpBlock - 1) / kDpBlock;
-  k_dp_error<<<dim3(nb), dim3(kDpBlock), 0, st>>>(n, s, x_old, dxdt_old, T, dt, eps_abs, eps_rel, partial);
+  const int nb = (n + kDpBlock - 1) / kDpBlock;  // partial holds at least one value per 256 particles
+  k_dp_error<<<dim3(nb), dim3(kDpBlock), 0, st>>>((long)n * s, s, x_old, dxdt_old, T, dt, eps_abs, eps_rel,
+                                                  partial);
   launch_max_final(st, nb, partial, out);
   return nb;
 }

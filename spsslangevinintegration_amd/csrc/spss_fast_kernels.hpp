@@ -116,6 +116,9 @@ __device__ __forceinline__ int move_source(int c, int d, int ncf, int cpr) {
   return base + sy * ncf + sx;
 }
 
+// a cell coordinate at most one cell outside [0, ncf) wrapped into it
+__device__ __forceinline__ int wrap_cell(int c, int ncf) { return c < 0 ? c + ncf : (c >= ncf ? c - ncf : c); }
+
 __device__ __forceinline__ int new_count_of(const int *cnt9, int c, int ncf, int cpr) {
   int s = 0;
 #pragma unroll
@@ -1030,6 +1033,33 @@ __global__ __launch_bounds__(256) void fr_scatter(
   if (rtiles && (int)blockIdx.x * 4 < ncf) rows_to_tiles([&](int c) { return nstart[c]; }, ncf, wcap, cmax, rtiles);
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = p < n;
+  const int lane = threadIdx.x & 63;
+  const int m = valid ? mv[p] : -1;  // lanes past the end: cell -1, no move
+  const int cs = m >> 4, d = m & 15;
+  // rank of p among the earlier particles of its old cell with the same near move: the wave's
+  // lanes from the cell's first lane in the wave (one ballot per move), plus, for a cell begun in
+  // an earlier wave, the slots before the wave's first (four independent loads per pass)
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int pcs = __shfl_up(cs, 1, 64);  // with every lane active: the shuffle reads lane - 1
+  const unsigned long long heads = __ballot(lane == 0 || pcs != cs);
+  const int seg = 63 - __clzll((long long)(heads & (below | (1ull << lane))));
+  unsigned long long same = 0ull;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const unsigned long long b = __ballot(d == k);
+    if (d == k) same = b;
+  }
+  int rank = __popcll(same & below & ~((1ull << seg) - 1ull));
+  if (valid && seg == 0 && d != kFar) {
+    const int p0 = p - lane;
+    for (int q = start[cs]; q < p0; q += 4) {
+      const int a0 = mv[q];
+      const int a1 = q + 1 < p0 ? mv[q + 1] : -1;
+      const int a2 = q + 2 < p0 ? mv[q + 2] : -1;
+      const int a3 = q + 3 < p0 ? mv[q + 3] : -1;
+      rank += (((a0 & 15) == d) + ((a1 & 15) == d)) + (((a2 & 15) == d) + ((a3 & 15) == d));
+    }
+  }
   double out[S];
 #pragma unroll
   for (int q = 0; q < S; ++q) out[q] = 0.0;
@@ -1037,8 +1067,6 @@ __global__ __launch_bounds__(256) void fr_scatter(
   double sn2 = 0.0, cn2 = 1.0, xw2 = 0.0, yw2 = 0.0;
   if (valid) {
     const int cpr = L.cpr;
-    const int m = mv[p];
-    const int cs = m >> 4, d = m & 15;
     const int r = cs / cpr, base = r * cpr;
     Coef K;
     if (U.coef) {
@@ -1067,15 +1095,20 @@ __global__ __launch_bounds__(256) void fr_scatter(
       for (int q = b; q < e; ++q) rank += (bucket[q] < p) ? 1 : 0;
       pos = b + rank;
     } else {
-      int rank = 0;
-      for (int q = start[cs]; q < p; ++q) rank += ((mv[q] & 15) == d) ? 1 : 0;
+      // destination cell, then the arrivals ordered before move d: the counts of the moves e
+      // with move_order(e) < move_order(d) out of their source cells (ncf >= 3: one wrap each)
       const int ls = cs - base;
-      const int dcx = d % 3 - 1, dcy = d / 3 - 1;
-      cd = base + ((ls / ncf + dcy + ncf) % ncf) * ncf + (ls % ncf + dcx + ncf) % ncf;
+      const int cx = ls % ncf, cy = ls / ncf;
+      const int tx = wrap_cell(cx + d % 3 - 1, ncf), ty = wrap_cell(cy + d / 3 - 1, ncf);
+      cd = base + ty * ncf + tx;
       int b0 = 0;
       const int od = move_order(d);
-      for (int e = 0; e < 9; ++e)
-        if (move_order(e) < od) b0 += cnt9[(long)move_source(cd, e, ncf, cpr) * 9 + e];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) {
+        const int sx = wrap_cell(tx - (e % 3 - 1), ncf), sy = wrap_cell(ty - (e / 3 - 1), ncf);
+        const int v = cnt9[(long)(base + sy * ncf + sx) * 9 + e];
+        b0 += move_order(e) < od ? v : 0;
+      }
       pos = nstart[cd] + b0 + rank;
     }
     valid = pos < n;  // always, when the counts match the moves

@@ -19,6 +19,13 @@ __global__ void k_stage(long m, const double *__restrict__ state, const double *
   if (k < m) stage[k] = state[k] + kc_dt * kprev[k];
 }
 
+// the stage of an all-zero k_prev: state + 0.0 is what state + kc_dt * 0.0 gives for any finite
+// kc_dt (-0.0 becomes +0.0), without reading a zero vector
+__global__ void k_stage_zero(long m, const double *__restrict__ state, double *__restrict__ stage) {
+  long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) stage[k] = state[k] + 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // all-pairs, ParticleSystemFirstOrderPtr.cpp:225-316 / ParticleSystemSecondOrderPtr.cpp:155-230.
 // One thread per i; j-tiles of the stage state are staged through LDS and
@@ -258,6 +265,23 @@ __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e
   e = (a - (s - bb)) + (b - bb);
 }
 
+// the final reductions: loads issued together per lane
+constexpr int kFinalBatch = 16;
+
+// the 64 compensated lane pairs (sum, correction) of a wave combined pairwise in a fixed tree
+// (lane l with lane l + 32, then + 16, ...); lane 0 returns sum + correction
+__device__ __forceinline__ double wave_combine(double s, double c) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double s2 = __shfl_down(s, off, 64), c2 = __shfl_down(c, off, 64);
+    double t, e;
+    two_sum(s, s2, t, e);
+    s = t;
+    c = c + c2 + e;
+  }
+  return s + c;
+}
+
 struct Sum4 {
   double v[4];
   double c[4];
@@ -296,34 +320,35 @@ __global__ __launch_bounds__(kBlock) void k_polar_partial(int n, int s, int slot
   }
 }
 
-// compensated sum of the block partials in block order; writes sums[4] (not yet divided by n)
-// the four block-partial columns summed with compensation (two_sum): 64 lanes per column each take
-// every 64th block, then one lane combines the 64 compensated pairs in lane order; the sums are
-// divided by n (the field Z and W).  The order is fixed, so the result does not vary between runs.
+// the four block-partial columns summed with compensation (two_sum), one wave per column: each
+// lane takes every 64th block in block order, then the wave combines its 64 compensated pairs in
+// a fixed tree; the sums are divided by n (the field Z and W).  The order is fixed, so the result
+// does not vary between runs.
 __global__ __launch_bounds__(256) void k_polar_final(int nblocks, const double *__restrict__ partial,
                                                      double *__restrict__ sums, double n) {
-  __shared__ double sh[2][4][64];
   const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double s = 0.0, c = 0.0;
-  for (int b = lane; b < nblocks; b += 64) {
-    double t, e;
-    two_sum(s, partial[(long)b * 4 + q], t, e);
-    s = t;
-    c += e;
-  }
-  sh[0][q][lane] = s;
-  sh[1][q][lane] = c;
-  __syncthreads();
-  if (lane == 0) {
-    double ts = 0.0, tc = 0.0;
-    for (int l = 0; l < 64; ++l) {
-      double t, e;
-      two_sum(ts, sh[0][q][l], t, e);
-      ts = t;
-      tc += e + sh[1][q][l];
+  // eight loads in flight per lane, then the sums in block order (the partials were written by
+  // other CUs: one load at a time would wait out that latency once per block)
+  for (int b0 = lane; b0 < nblocks; b0 += 64 * kFinalBatch) {
+    double x[kFinalBatch];
+#pragma unroll
+    for (int k = 0; k < kFinalBatch; ++k) {
+      const int b = b0 + 64 * k;
+      x[k] = b < nblocks ? partial[(long)b * 4 + q] : 0.0;
     }
-    sums[q] = (ts + tc) / n;
+#pragma unroll
+    for (int k = 0; k < kFinalBatch; ++k) {
+      if (b0 + 64 * k < nblocks) {
+        double t, e;
+        two_sum(s, x[k], t, e);
+        s = t;
+        c += e;
+      }
+    }
   }
+  const double total = wave_combine(s, c);
+  if (lane == 0) sums[q] = total / n;
 }
 
 // k of the global branch; fields = (Re Z, Im Z, Re W, Im W) already divided by N
@@ -478,7 +503,13 @@ __global__ __launch_bounds__(kBlock) void k_srk(int n, double *__restrict__ stat
 __global__ void k_max_final(int nb, const double *__restrict__ partial, double *__restrict__ out) {
   __shared__ double sh[kBlock];
   double m = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kBlock) m = fmax(m, partial[b]);
+  for (int b0 = threadIdx.x; b0 < nb; b0 += kBlock * kFinalBatch) {
+    double x[kFinalBatch];
+#pragma unroll
+    for (int k = 0; k < kFinalBatch; ++k) x[k] = b0 + kBlock * k < nb ? partial[b0 + kBlock * k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < kFinalBatch; ++k) m = fmax(m, x[k]);
+  }
   sh[threadIdx.x] = m;
   __syncthreads();
   for (int off = kBlock / 2; off > 0; off >>= 1) {
@@ -540,32 +571,32 @@ __global__ __launch_bounds__(kBlock) void k_order_partial(int n, int s, int slot
   }
 }
 
-// the (cos, sin) block partials summed with compensation: 64 lanes per column each take every
-// 64th block, then one lane combines the 64 compensated pairs in lane order (a fixed order)
+// the (cos, sin) block partials summed with compensation, one wave per column: each lane takes
+// every 64th block, then the wave combines its 64 compensated pairs in a fixed tree
 __global__ __launch_bounds__(128) void k_order_final(int nb, const double *__restrict__ partial, double n,
                                                      double *__restrict__ out) {
-  __shared__ double sh[2][2][64];
   const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double s = 0.0, c = 0.0;
-  for (int b = lane; b < nb; b += 64) {
-    double t, e;
-    two_sum(s, partial[(long)b * 4 + q], t, e);
-    s = t;
-    c += e + partial[(long)b * 4 + 2 + q];
-  }
-  sh[0][q][lane] = s;
-  sh[1][q][lane] = c;
-  __syncthreads();
-  if (lane == 0) {
-    double ts = 0.0, tc = 0.0;
-    for (int l = 0; l < 64; ++l) {
-      double t, e;
-      two_sum(ts, sh[0][q][l], t, e);
-      ts = t;
-      tc += e + sh[1][q][l];
+  for (int b0 = lane; b0 < nb; b0 += 64 * kFinalBatch) {
+    double x[kFinalBatch], y[kFinalBatch];
+#pragma unroll
+    for (int k = 0; k < kFinalBatch; ++k) {
+      const int b = b0 + 64 * k;
+      x[k] = b < nb ? partial[(long)b * 4 + q] : 0.0;
+      y[k] = b < nb ? partial[(long)b * 4 + 2 + q] : 0.0;
     }
-    out[q] = (ts + tc) / n;
+#pragma unroll
+    for (int k = 0; k < kFinalBatch; ++k) {
+      if (b0 + 64 * k < nb) {
+        double t, e;
+        two_sum(s, x[k], t, e);
+        s = t;
+        c += e + y[k];
+      }
+    }
   }
+  const double total = wave_combine(s, c);
+  if (lane == 0) out[q] = total / n;
 }
 
 // ApplyPeriodicBoundaryConditions(Real*, long), PeriodicBoundaryConditions.cpp:147-155
@@ -606,7 +637,10 @@ __global__ void k_fill_int(long m, int *__restrict__ p, int v) {
 static inline unsigned nblk(long m, int b = kBlock) { return (unsigned)((m + b - 1) / b); }
 
 void launch_stage(hipStream_t st, long m, const double *state, const double *kprev, double kc_dt, double *stage) {
-  k_stage<<<dim3(nblk(m)), dim3(kBlock), 0, st>>>(m, state, kprev, kc_dt, stage);
+  if (kprev)
+    k_stage<<<dim3(nblk(m)), dim3(kBlock), 0, st>>>(m, state, kprev, kc_dt, stage);
+  else
+    k_stage_zero<<<dim3(nblk(m)), dim3(kBlock), 0, st>>>(m, state, stage);
 }
 
 #define SPSS_DISPATCH_S(S, ...) \

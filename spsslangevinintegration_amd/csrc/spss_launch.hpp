@@ -72,6 +72,7 @@ struct CellBuffers {
   int cells = 0;
 };
 
+// stage = state + kc_dt * kprev; kprev null: an all-zero k_prev
 void launch_stage(hipStream_t st, long m, const double *state, const double *kprev, double kc_dt, double *stage);
 void launch_all_pairs(hipStream_t st, const DevParams &P, const double *stage, const double *nat, double *k_next,
                       double *compl_out);
