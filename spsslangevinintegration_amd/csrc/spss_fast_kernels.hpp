@@ -532,25 +532,29 @@ __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A
     return false;
   }
 
-  // staged entry e belongs to the last cell whose LDS start is <= e
-  for (int e = threadIdx.x; e < total; e += kFB) {
-    int lo = 0, hi = ncells - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (cpos[mid] <= e) lo = mid; else hi = mid - 1;
-    }
-    const int j = e - cpos[lo];
-    if (j < ccnt[lo]) {
-      const int q = lo % ncols;
-      const int g = cglob[lo] + j;
-      const float4 v = B.snap[g];
-      sdat[e] = make_float4((float)((double)q * H + (double)v.x), v.y, v.z, v.w);
-      if (SRK) L.somega[e] = B.omega_f[g];
-      if (LIST) L.sg[e] = g;
-    } else {  // row padding: never within range
-      sdat[e] = make_float4(1e30f, 0.f, 0.f, 0.f);
-      if (SRK) L.somega[e] = 0.f;
-      if (LIST) L.sg[e] = -1;
+  // the staged cells, G lanes per cell (as many as fit the workgroup: no search for the cell of
+  // an entry); the last cell of every staged row is followed by its row's padding
+  {
+    const int G = ncells <= kFB / 8 ? 8 : (ncells <= kFB / 4 ? 4 : (ncells <= kFB / 2 ? 2 : 1));
+    const int sub = (int)threadIdx.x % G;
+    for (int c = (int)threadIdx.x / G; c < ncells; c += kFB / G) {
+      const int q = c % ncols;
+      const int e0 = cpos[c], n0 = ccnt[c], g0 = cglob[c];
+      const double xq = (double)q * H;
+      for (int j = sub; j < n0; j += G) {
+        const int g = g0 + j;
+        const float4 v = B.snap[g];
+        sdat[e0 + j] = make_float4((float)(xq + (double)v.x), v.y, v.z, v.w);
+        if (SRK) L.somega[e0 + j] = B.omega_f[g];
+        if (LIST) L.sg[e0 + j] = g;
+      }
+      if (q == ncols - 1) {
+        for (int j = sub; j < kPad; j += G) {  // row padding: never within range
+          sdat[e0 + n0 + j] = make_float4(1e30f, 0.f, 0.f, 0.f);
+          if (SRK) L.somega[e0 + n0 + j] = 0.f;
+          if (LIST) L.sg[e0 + n0 + j] = -1;
+        }
+      }
     }
   }
   __syncthreads();
