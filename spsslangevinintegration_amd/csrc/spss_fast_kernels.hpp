@@ -496,8 +496,10 @@ __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A
   // ---- staging plan: global extent of every staged cell, then the LDS layout
   for (int e = threadIdx.x; e < ncells; e += kFB) {
     const int r = e / ncols, q = e % ncols;
-    const int rrow = ((row + r - k) % ncf + ncf) % ncf;
-    const int col = ((ca - k + q) % ncf + ncf) % ncf;
+    // at most k < ncf cells outside [0, ncf): one wrap each
+    int rrow = row + r - k, col = ca - k + q;
+    rrow += rrow < 0 ? ncf : (rrow >= ncf ? -ncf : 0);
+    col += col < 0 ? ncf : (col >= ncf ? -ncf : 0);
     const int c = cbase + rrow * ncf + col;
     const int g0 = start[c];
     cglob[e] = g0;
@@ -506,24 +508,19 @@ __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A
   __syncthreads();
   if (threadIdx.x < 64) {  // one wave: exclusive prefix of the LDS sizes (count + row padding)
     const int lane = threadIdx.x;
-    constexpr int kPer = (kMaxRows * kMaxCols + 63) / 64;
-    int v[kPer];
-    int sum = 0;
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int e = lane * kPer + q;
-      v[q] = (e < ncells) ? ccnt[e] + ((e % ncols == ncols - 1) ? kPad : 0) : 0;
-      sum += v[q];
+    const int step = 64 % ncols;
+    int q = lane % ncols;  // staged column of cell e = e0 + lane
+    int run = 0;
+    for (int e0 = 0; e0 < ncells; e0 += 64) {  // 64 cells per pass
+      const int e = e0 + lane;
+      const int v = (e < ncells) ? ccnt[e] + (q == ncols - 1 ? kPad : 0) : 0;
+      const int incl = wave_incl_scan(v, lane);
+      if (e < ncells) cpos[e] = run + incl - v;
+      run += __shfl(incl, 63, 64);
+      q += step;
+      if (q >= ncols) q -= ncols;
     }
-    const int incl = wave_incl_scan(sum, lane);
-    int run = incl - sum;
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int e = lane * kPer + q;
-      if (e < ncells) cpos[e] = run;
-      run += v[q];
-    }
-    if (lane == 63) cpos[ncells] = incl;
+    if (lane == 0) cpos[ncells] = run;
   }
   __syncthreads();
   const int total = cpos[ncells];
