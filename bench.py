@@ -43,10 +43,10 @@ def pair_kernel_bytes(n, cells):
 
 # HBM bytes per launch of that kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled
 # by the copy-kernel calibration, by cell subdivision (profiles/fk_pairs_euler_N1M_hbm_traffic.txt)
-PAIR_KERNEL_TRAFFIC = {1: 3.9937e7}
+PAIR_KERNEL_TRAFFIC = {1: 3.9915e7}
 # VALU instructions issued per CU-cycle by that kernel (SQ_INSTS_VALU / (256 CUs x cycles),
 # profiles/fk_pairs_euler_N1M_valu.txt): at or above the nominal one per CU-cycle, i.e. VALU-bound
-PAIR_KERNEL_VALU_PER_CU_CYCLE = 1.04
+PAIR_KERNEL_VALU_PER_CU_CYCLE = 1.01
 # algorithmic bytes per particle-step of the whole resident Euler step: pair kernel 36 (snapshot
 # read, sums and count written), scatter 132 (move word, id, FP64 state, natural frequency, sums
 # read; state, id, natural frequency, snapshot, next move word written), and per cell 68
