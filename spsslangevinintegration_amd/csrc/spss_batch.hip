@@ -355,17 +355,13 @@ int spss_batch_autotune_cells(spss_batch *b, double dt, int probe_steps, double 
     if (rebuild_cells(b, k) == SPSS_OK) {
       BTRY(spss_batch_set_state(b, saved.data()));
       BTRY(spss_batch_run(b, 2 + probe_steps % 2, dt, 0, 0, 1));  // enter, warm up
-      hipEvent_t e0, e1;
-      BCHK(hipEventCreate(&e0));
-      BCHK(hipEventCreate(&e1));
-      BCHK(hipEventRecord(e0, b->st));
+      EventPair ev;
+      BCHK(ev.create());
+      BCHK(hipEventRecord(ev.start, b->st));
       BTRY(spss_batch_run(b, probe_steps, dt, 0, 0, 1));
-      BCHK(hipEventRecord(e1, b->st));
-      BCHK(hipEventSynchronize(e1));
+      BCHK(hipEventRecord(ev.stop, b->st));
       float ms = 0.f;
-      BCHK(hipEventElapsedTime(&ms, e0, e1));
-      hipEventDestroy(e0);
-      hipEventDestroy(e1);
+      BCHK(ev.elapsed(&ms));
       ms_step = ms / probe_steps;
       if (ms_step < best) {
         best = ms_step;
@@ -409,17 +405,13 @@ int spss_batch_autotune(spss_batch *b, double dt, int probe_steps, double *table
     BTRY(spss_batch_set_tile_widths(b, W.data()));
     BTRY(spss_batch_set_state(b, saved.data()));
     BTRY(spss_batch_run(b, 2 + probe_steps % 2, dt, 0, 0, 1));  // enter, warm up
-    hipEvent_t e0, e1;
-    BCHK(hipEventCreate(&e0));
-    BCHK(hipEventCreate(&e1));
-    BCHK(hipEventRecord(e0, b->st));
+    EventPair ev;
+    BCHK(ev.create());
+    BCHK(hipEventRecord(ev.start, b->st));
     BTRY(spss_batch_run(b, probe_steps, dt, 0, 0, 1));
-    BCHK(hipEventRecord(e1, b->st));
-    BCHK(hipEventSynchronize(e1));
+    BCHK(hipEventRecord(ev.stop, b->st));
     float ms = 0.f;
-    BCHK(hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    BCHK(ev.elapsed(&ms));
     if (table_ms && kc < table_cap) table_ms[kc] = ms / probe_steps;
     if (ms < best) {
       best = ms;

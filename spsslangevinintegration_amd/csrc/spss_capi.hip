@@ -912,28 +912,29 @@ int spss_apply_periodic_boundary_conditions(spss_system *h) {
 int spss_device_copy_bandwidth(long bytes, int repeats, double *gbps) {
   if (bytes <= 0 || repeats <= 0 || !gbps) return fail(SPSS_ERR_INVALID, "bad argument");
   bytes = (bytes / 16) * 16;
-  void *a = nullptr, *b = nullptr;
-  hipStream_t st;
-  HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  HIPCHECK(hipMalloc(&a, bytes));
-  HIPCHECK(hipMalloc(&b, bytes));
-  HIPCHECK(hipMemsetAsync(a, 1, bytes, st));
-  hipEvent_t e0, e1;
-  HIPCHECK(hipEventCreate(&e0));
-  HIPCHECK(hipEventCreate(&e1));
-  launch_copy16(st, bytes / 16, a, b);  // warm-up
-  HIPCHECK(hipEventRecord(e0, st));
-  for (int r = 0; r < repeats; ++r) launch_copy16(st, bytes / 16, a, b);
-  HIPCHECK(hipEventRecord(e1, st));
-  HIPCHECK(hipEventSynchronize(e1));
+  struct Probe {  // released on every return
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    ~Probe() {
+      if (st) (void)hipStreamSynchronize(st);
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+      if (st) (void)hipStreamDestroy(st);
+    }
+  } P;
+  HIPCHECK(hipStreamCreateWithFlags(&P.st, hipStreamNonBlocking));
+  HIPCHECK(hipMalloc(&P.a, bytes));
+  HIPCHECK(hipMalloc(&P.b, bytes));
+  HIPCHECK(hipMemsetAsync(P.a, 1, bytes, P.st));
+  EventPair ev;
+  HIPCHECK(ev.create());
+  launch_copy16(P.st, bytes / 16, P.a, P.b);  // warm-up
+  HIPCHECK(hipEventRecord(ev.start, P.st));
+  for (int r = 0; r < repeats; ++r) launch_copy16(P.st, bytes / 16, P.a, P.b);
+  HIPCHECK(hipEventRecord(ev.stop, P.st));
   float ms = 0.f;
-  HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+  HIPCHECK(ev.elapsed(&ms));
   *gbps = 2.0 * (double)bytes * repeats / (ms * 1e-3) / 1e9;  // read + write
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(a);
-  hipFree(b);
-  hipStreamDestroy(st);
   return SPSS_OK;
 }
 

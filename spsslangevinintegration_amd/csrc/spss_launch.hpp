@@ -25,6 +25,28 @@ struct spss_system;
 namespace spss {
 int system_view(spss_system *h, SystemView *v);
 
+// two HIP events timing a stretch of one stream, destroyed with the object (also on the early
+// returns of an error path)
+struct EventPair {
+  hipEvent_t start = nullptr, stop = nullptr;
+  hipError_t create() {
+    const hipError_t e = hipEventCreate(&start);
+    return e == hipSuccess ? hipEventCreate(&stop) : e;
+  }
+  // milliseconds between the two records, after waiting for the second
+  hipError_t elapsed(float *ms) {
+    const hipError_t e = hipEventSynchronize(stop);
+    return e == hipSuccess ? hipEventElapsedTime(ms, start, stop) : e;
+  }
+  EventPair() = default;
+  EventPair(const EventPair &) = delete;
+  EventPair &operator=(const EventPair &) = delete;
+  ~EventPair() {
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+  }
+};
+
 // HIP-event timer around one kernel family, enabled on request (bench instrumentation):
 // records an event pair per launch on the launching stream, sums the durations on read.
 struct KernelTimer {
