@@ -1004,16 +1004,33 @@ __global__ __launch_bounds__(256) void fr_far_bucket(DevParams P, int n, int ncf
     }
   }
   if (p == 0) *ovf_count = 0;
-  if (p >= n) return;
-  const int mk = mv[p];
-  if ((mk & 15) != kFar) return;
-  double sn, cn, x1, y1;
-  step_position<S, SRK>(P, rstate + (long)p * S, U, sn, cn, x1, y1);
-  const int cd = cell_base(mk >> 4, cpr) + cell_of_position<S>(P, x1, y1, ncf);
-  int off = 0;  // (rare) this lane's own sum of the earlier block totals
-  for (int j = 0; j < cd / kBlock; ++j) off += blocksum[j];
-  const int q = lstart[cd] + off + new_count_of(cnt9, cd, ncf, cpr) + farslot[p];
-  if (q < n) bucket[q] = p;  // always, when the counts match the moves
+  // the moves of particles 4p .. 4p+3 (one 16-byte load where all four exist)
+  const long p4 = 4L * p;
+  if (p4 >= n) return;
+  int mk4[4];
+  if (p4 + 3 < n) {
+    const int4 v = reinterpret_cast<const int4 *>(mv)[p];
+    mk4[0] = v.x;
+    mk4[1] = v.y;
+    mk4[2] = v.z;
+    mk4[3] = v.w;
+  } else {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) mk4[u] = p4 + u < n ? mv[p4 + u] : 0;  // 0: not a far move
+  }
+#pragma unroll 1
+  for (int u = 0; u < 4; ++u) {
+    const int mk = mk4[u];
+    if ((mk & 15) != kFar) continue;
+    const int pf = (int)p4 + u;
+    double sn, cn, x1, y1;
+    step_position<S, SRK>(P, rstate + (long)pf * S, U, sn, cn, x1, y1);
+    const int cd = cell_base(mk >> 4, cpr) + cell_of_position<S>(P, x1, y1, ncf);
+    int off = 0;  // (rare) this lane's own sum of the earlier block totals
+    for (int j = 0; j < cd / kBlock; ++j) off += blocksum[j];
+    const int q = lstart[cd] + off + new_count_of(cnt9, cd, ncf, cpr) + farslot[pf];
+    if (q < n) bucket[q] = pf;  // always, when the counts match the moves
+  }
 }
 
 // the step of every particle, written to its new slot with its id, natural frequency and

@@ -514,9 +514,10 @@ static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const
   int *const mv = f->mvb[f->mvc], *const mvn = f->mvb[1 - f->mvc];
   fr_scan_overflow<S, SRK><<<f->nsb + kOverflowBlocks, kFB, 0, st>>>(P, args_of(f), B, f->newcnt, f->farcnt,
                                                                      f->cells, f->lstart, f->blocksum, f->nsb);
-  fr_far_bucket<S, SRK><<<nb, 256, 0, st>>>(P, n, f->ncf, f->cpr, mv, f->rstate[c], U, f->lstart, f->blocksum,
-                                           f->nsb, f->cells, f->rstart[o], f->newcnt, f->farcnt, f->ovf_count,
-                                           f->cnt9[c], f->slot, f->tmp, f->cnt9[o], (long)f->cells * 9);
+  const unsigned nb4 = (unsigned)((n + 1023) / 1024);  // four move words per thread
+  fr_far_bucket<S, SRK><<<nb4, 256, 0, st>>>(P, n, f->ncf, f->cpr, mv, f->rstate[c], U, f->lstart, f->blocksum,
+                                            f->nsb, f->cells, f->rstart[o], f->newcnt, f->farcnt, f->ovf_count,
+                                            f->cnt9[c], f->slot, f->tmp, f->cnt9[o], (long)f->cells * 9);
   // (at least one workgroup per four rows of cells for the row tiles)
   const unsigned nbs = std::max(nb, (unsigned)(f->rtiles ? (f->ncf + 3) / 4 : 0));
   fr_scatter<S, SRK><<<nbs, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], mv, f->cnt9[c], f->tmp,

@@ -75,6 +75,7 @@ RESIDENT_CASES = [
     (4, 2000, 0.3, 1, 0.5, "euler"),     # 3 x 3 cells
     (4, 20000, 0.02, 1, 1.0, "euler"),   # v0*dt = cell side / 4: many near moves, all eight directions
     (4, 20000, 0.1, 1, 2.0, "srk"),      # ~200 per cell: cells span several waves, 10% move per step
+    (4, 20001, 0.02, 2, 3.0, "euler"),   # far jumps with N not a multiple of 4 (the far pass's tail)
 ]
 
 
