@@ -82,6 +82,7 @@ CASES = [
     (3, 700, 0.75, 1, 0.4, 0),
     (6, 1100, 0.06, 3, 1.5, 0),
     (6, 400, 0.3, 2, 1.54, 0),
+    (4, 300_001, 0.8, 1, 0.5, 1),   # global field over 1,172 blocks: several loads per lane, a partial batch
 ]
 
 
