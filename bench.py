@@ -9,13 +9,16 @@ omega_i/m with omega_i ~ Lorentzian(0, 0.05) clipped at +-10), phi_dot(0) = 0,
 v0 = 0.005, dt = 0.002, cell membership refreshed every step, stochastic Euler with
 Philox noise (seed 0x5EED0003), FP32 pair arithmetic (blocked sums: FP32 partials, FP64 totals), FP64 state.
 
-Prints one JSON line.  `value` is whole-job particle-steps/s with the state
-resident in HBM (the library replays the steps from a HIP graph).  `roofline` is the
-pair kernel (fk_pairs, phasor sums of the resident state) timed with HIP events on
-its stream over a separate run of eager steps after the timed ones;
-`cpu_baseline` is the reference's own serial linked-list RHS + Euler update
-(compiled from its sources into oracle/_ref) or, without that build, the
-oracle restatement, on a bounded sample.
+Prints one JSON line.  `value` is whole-job particle-steps/s with the state resident in HBM:
+after the warm-up steps the step graphs are captured (spss_prepare_run) and the timed steps are
+replayed from them; the line reports how many timed steps were replayed and how many ran eagerly.
+`roofline` is the whole step against the algorithmic bytes per particle-step of the cell-list step
+(104.47 B: FP64 state read + write, FP32 snapshot write + read, cell key and sorted index write +
+read, cell table), with the pair kernel (fk_pairs, timed with HIP events on its stream in a
+separate run of eager steps) as a sub-field.  `cpu_baseline` is the reference algorithm on the
+host: all cores through the oracle's threaded restatement (contiguous particle blocks per thread,
+as the reference's ranks own them), and one core through the reference's own serial linked-list
+RHS + Euler update compiled from its sources with its own -O3 -flto (oracle/_ref).
 """
 import argparse
 import json
@@ -44,15 +47,25 @@ def pair_kernel_bytes(n, cells):
 # HBM bytes per launch of that kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled
 # by the copy-kernel calibration, by cell subdivision (profiles/fk_pairs_euler_N1M_hbm_traffic.txt)
 PAIR_KERNEL_TRAFFIC = {1: 3.9915e7}
+# HBM bytes of the four kernels of one resident step, the same way (filled in from the profiles)
+STEP_TRAFFIC = {}
 # VALU instructions issued per CU-cycle by that kernel (SQ_INSTS_VALU / (256 CUs x cycles),
 # profiles/fk_pairs_euler_N1M_valu.txt): at or above the nominal one per CU-cycle, i.e. VALU-bound
 PAIR_KERNEL_VALU_PER_CU_CYCLE = 1.01
-# algorithmic bytes per particle-step of the whole resident Euler step: pair kernel 36 (snapshot
-# read, sums and count written), scatter 132 (move word, id, FP64 state, natural frequency, sums
-# read; state, id, natural frequency, snapshot, next move word written), and per cell 68
-# (scan: counts read, local start written; far pass: local start read, start written, counts
-# and the 9 move counters cleared), N = 1,048,576 and 62,500 cells
-STEP_BYTES = 36.0 + 132.0 + 68.0 * 62500.0 / 1048576.0
+# algorithmic HBM bytes per particle-step of the cell-list step (s = 4 state, one RHS): FP64 state
+# read + write 16*s, FP32 snapshot x, y, phi written and read 8*3, cell key and sorted index written
+# and read 16, cell start/count table 8 per cell -- the compulsory traffic of the reference
+# algorithm, whatever the implementation moves
+def step_bytes(n, cells, s=4):
+    return 16.0 * s + 8.0 * 3 + 16.0 + 8.0 * cells / n
+
+
+# bytes the implementation's resident step moves per particle (its own design, for comparison):
+# pair kernel 36 (snapshot read, sums and count written), scatter 132 (move word, id, FP64 state,
+# natural frequency, sums read; state, id, natural frequency, snapshot, next move word written),
+# and per cell 68 (scan: counts read, local start written; far pass: local start read, start
+# written, counts and the 9 move counters cleared)
+IMPL_STEP_BYTES = 36.0 + 132.0 + 68.0 * 62500.0 / 1048576.0
 
 
 def headline_system(spss, n=1 << 20, precision=None, subdiv=1, seed=0x5EED0003):
@@ -73,13 +86,18 @@ def headline_system(spss, n=1 << 20, precision=None, subdiv=1, seed=0x5EED0003):
 
 
 def time_steps(g, spss, stepper, steps, warmup, dt=0.002):
-    """wall time of `steps` steps after `warmup`: spss_run replays the steps from a HIP graph"""
+    """wall time of `steps` steps after `warmup` warm-up steps and the capture of the step graphs
+    (outside the timed region), and how the timed steps ran: (seconds, eager, replayed, captured)"""
     g.run(stepper, spss.METHOD_LINKED_LIST, warmup, dt, spss.NOISE_PHILOX)
+    g.prepare_run(stepper, spss.METHOD_LINKED_LIST, dt, spss.NOISE_PHILOX)
     g.synchronize()
+    g.run_stats(reset=True)
     t0 = time.perf_counter()
     g.run(stepper, spss.METHOD_LINKED_LIST, steps, dt, spss.NOISE_PHILOX)
     g.synchronize()
-    return time.perf_counter() - t0
+    sec = time.perf_counter() - t0
+    eager, replayed, captured = g.run_stats(reset=True)
+    return sec, eager, replayed, captured
 
 
 def time_pair_kernel(g, spss, stepper, steps, dt=0.002):
@@ -190,29 +208,62 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
     }
 
 
-def cpu_baseline(n=1 << 20, steps=3, warmup=1):
-    """The reference's serial linked-list RHS + Euler update on the host (1 core)."""
-    m, sigma, D = 0.5, 1.5, 0.02
-    ref = os.path.join(ROOT, "oracle", "_ref", "ref_serial")
+def host_cpu():
+    """CPU model name and the cores this process may use (the GPU host's share)"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return {"model": model, "nproc": cores, "os_cpu_count": os.cpu_count()}
+
+
+def _cpu_state(n):
     rng = np.random.default_rng(1)
     st = np.zeros((n, 4))
     st[:, 0], st[:, 1], st[:, 2] = rng.random(n), rng.random(n), rng.random(n) * 2 * np.pi
-    sample = f"N={n} rho=0.004 cell list + Euler, {steps} timed steps after {warmup}"
-    if os.path.exists(ref):
-        with tempfile.TemporaryDirectory() as td:
-            cfg, inp = os.path.join(td, "c.txt"), os.path.join(td, "in.bin")
-            with open(cfg, "w") as f:
-                f.write(f"model=4\nn={n}\nv0=0.005\nxi={1 / m!r}\nsigma={sigma / m!r}\nrho=0.004\nalpha=0.0\n"
-                        f"D_phi={D / m ** 2!r}\ndt=0.002\nsteps={steps}\nwarmup={warmup}\nstepper=bench\n")
-            st.reshape(-1).tofile(inp)
-            out = subprocess.run([ref, cfg, inp, os.path.join(td, "o.bin")], capture_output=True, text=True,
-                                 check=True, timeout=600).stdout
-        res = json.loads(out.strip().splitlines()[-1])
-        return {"value": res["particle_steps_per_s"], "unit": "particle-steps/s", "cores": 1, "kind": "reference",
-                "sample": sample + " (reference sources, serial class family, g++ -O2)"}
+    return st.reshape(-1)
+
+
+def cpu_single_core(n=1 << 20, steps=3, warmup=1):
+    """the reference's serial linked-list RHS + Euler update, compiled from its sources with its own
+    -O3 -flto (oracle/_ref/ref_serial_o3), on one core; None without that build"""
+    m, sigma, D = 0.5, 1.5, 0.02
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_serial_o3")
+    if not os.path.exists(ref):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        cfg, inp = os.path.join(td, "c.txt"), os.path.join(td, "in.bin")
+        with open(cfg, "w") as f:
+            f.write(f"model=4\nn={n}\nv0=0.005\nxi={1 / m!r}\nsigma={sigma / m!r}\nrho=0.004\nalpha=0.0\n"
+                    f"D_phi={D / m ** 2!r}\ndt=0.002\nsteps={steps}\nwarmup={warmup}\nstepper=bench\n")
+        _cpu_state(n).tofile(inp)
+        cpu = sorted(os.sched_getaffinity(0))[0]
+        out = subprocess.run([ref, cfg, inp, os.path.join(td, "o.bin")], capture_output=True, text=True,
+                             check=True, timeout=900, preexec_fn=lambda: os.sched_setaffinity(0, {cpu})).stdout
+    res = json.loads(out.strip().splitlines()[-1])
+    return {"value": res["particle_steps_per_s"], "unit": "particle-steps/s", "cores": 1, "kind": "reference",
+            "sample": f"N={n} rho=0.004 cell list + Euler, {steps} timed steps after {warmup} (the reference's "
+                      "serial class family from its sources, g++ -O3 -flto -march=x86-64-v3)"}
+
+
+def cpu_all_cores(n=1 << 20, steps=20, warmup=2, threads=None):
+    """the oracle's restatement of the reference's linked-list RHS + Euler update with its particle
+    loop over contiguous blocks, one per thread (ThreadSharedMemory.cpp:20-23), on the host's cores"""
     import oracle
+    m, sigma, D = 0.5, 1.5, 0.02
+    threads = threads or min(16, host_cpu()["nproc"])
     o = oracle.Oracle(model=4, n=n, v0=0.005, xi=1 / m, sigma=sigma / m, rho=0.004, D_phi=D / m ** 2)
-    s = st.reshape(-1).copy()
+    o.set_threads(threads)
+    s = _cpu_state(n)
     z = np.zeros(n)
     for _ in range(warmup):
         o.euler_step(3, s, 0.002, z)
@@ -220,8 +271,30 @@ def cpu_baseline(n=1 << 20, steps=3, warmup=1):
     for _ in range(steps):
         o.euler_step(3, s, 0.002, z)
     sec = time.perf_counter() - t0
-    return {"value": n * steps / sec, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": sample + " (oracle restatement)"}
+    return {"value": n * steps / sec, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"N={n} rho=0.004 cell list + Euler, {steps} timed steps after {warmup}, {threads} threads "
+                      "(the oracle's C restatement, gcc -O2, particle blocks per thread)"}
+
+
+def cpu_baseline(single_steps=3, all_steps=20):
+    line = cpu_all_cores(steps=all_steps)
+    line["single_core"] = cpu_single_core(steps=single_steps)
+    line["host"] = host_cpu()
+    return line
+
+
+def fp64_step_rate(spss, steps=10, warmup=2):
+    """the reference-precision (FP64, the reference's operation order) cell-list Euler step of the
+    headline system: device counting sort and 9-cell pair loop every step"""
+    g, kw, _, _ = headline_system(spss, precision=spss.PRECISION_FP64)
+    g.run(spss.STEPPER_EULER, spss.METHOD_LINKED_LIST, warmup, 0.002, spss.NOISE_PHILOX)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run(spss.STEPPER_EULER, spss.METHOD_LINKED_LIST, steps, 0.002, spss.NOISE_PHILOX)
+    g.synchronize()
+    sec = time.perf_counter() - t0
+    g.close()
+    return {"particle_steps_per_s": kw["n"] * steps / sec, "ms_per_step": sec / steps * 1e3, "steps": steps}
 
 
 def main():
@@ -231,10 +304,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--subdiv", type=int, default=0, help="cell subdivision 1-3; 0: the autotuner picks it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=20, help="timed steps of the all-core CPU baseline")
+    ap.add_argument("--cpu-single-steps", type=int, default=3, help="timed steps of the one-core CPU baseline")
     ap.add_argument("--batch-steps", type=int, default=3000)
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the sharding by slabs of cell rows is "
@@ -256,19 +331,25 @@ def main():
     subdiv = g.cell_subdivision
     cells = (250 * subdiv) ** 2
     print(f"[bench] headline n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
-    wall = time_steps(g, spss, spss.STEPPER_EULER, args.steps, args.warmup)
+    wall, eager, replayed, captured = time_steps(g, spss, spss.STEPPER_EULER, args.steps, args.warmup)
     pair_ms = time_pair_kernel(g, spss, spss.STEPPER_EULER, max(args.steps // 5, 20))
     value = n * args.steps / wall
     ms_step = wall / args.steps * 1e3
     # neighbour count of the final state (pair-interactions/s = sum_i n_i * steps / s)
     pairs = C_pairs(g, spss)
-    achieved = pair_kernel_bytes(n, cells) * n / (pair_ms * 1e-3)
+    pair_achieved = pair_kernel_bytes(n, cells) * n / (pair_ms * 1e-3)
+    step_achieved = step_bytes(n, 250 * 250) * value
     # secondary: the SRK stepper (complementary force, 2 draws per particle)
     g2, _, _, _ = headline_system(spss, subdiv=subdiv)
     srk_steps = max(args.steps // 5, 10)
-    wall2 = time_steps(g2, spss, spss.STEPPER_SRK, srk_steps, 10)
+    wall2, _, _, _ = time_steps(g2, spss, spss.STEPPER_SRK, srk_steps, 10)
     g2.close()
-    print(f"[bench] {ms_step:.4f} ms/step, pair kernel {pair_ms * 1e3:.1f} us", file=sys.stderr, flush=True)
+    print(f"[bench] {ms_step:.4f} ms/step ({replayed} replayed, {eager} eager, {captured} graphs captured in the "
+          f"timed region), pair kernel {pair_ms * 1e3:.1f} us", file=sys.stderr, flush=True)
+    fp64 = None
+    if not args.no_fp64:
+        print("[bench] fp64 cell-list step", file=sys.stderr, flush=True)
+        fp64 = fp64_step_rate(spss)
     batched = None
     if not args.no_batch:
         print("[bench] batched ensemble", file=sys.stderr, flush=True)
@@ -282,7 +363,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(steps=args.cpu_steps)
+        cpu = cpu_baseline(single_steps=args.cpu_single_steps, all_steps=args.cpu_steps)
     line = {
         "metric": METRIC,
         "value": value,
@@ -301,16 +382,23 @@ def main():
                                "FP32 distance tests, phasor sums in FP32 partials combined in FP64, FP64 state",
                    "n": n, "rho": 0.004, "cell_subdivision": subdiv, "cell_autotune": tune,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": PAIR_KERNEL_TRAFFIC.get(subdiv)},
+        "timed_steps": {"graph_replayed": replayed, "eager": eager, "graphs_captured": captured},
+        "roofline": {"bound": "hbm", "achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": step_achieved / HBM_PEAK, "traffic": STEP_TRAFFIC.get(subdiv),
+                     "kernel": "resident cell-list step (fk_pairs, fr_scan_overflow, fr_far_bucket, fr_scatter)",
+                     "algorithmic_bytes_per_particle_step": step_bytes(n, 250 * 250),
+                     "pair_kernel": {"achieved": pair_achieved / 1e9, "frac": pair_achieved / HBM_PEAK,
+                                     "us": pair_ms * 1e3, "bytes_per_particle": pair_kernel_bytes(n, cells),
+                                     "traffic": PAIR_KERNEL_TRAFFIC.get(subdiv)}},
         # the pair kernel is bound by VALU issue, not HBM (no MFMA work on this path)
         "valu": {"kernel": "fk_pairs", "valu_instr_per_cu_cycle": PAIR_KERNEL_VALU_PER_CU_CYCLE},
         "cpu_baseline": cpu,
         "pair_kernel_us": pair_ms * 1e3,
         "hbm_copy_gbps": copy_gbps,
-        "step_roofline_frac": STEP_BYTES * value / HBM_PEAK,
+        "impl_step_bytes_frac": IMPL_STEP_BYTES * value / HBM_PEAK,
         "pair_interactions_per_s": pairs / n * value if pairs else None,
         "srk_particle_steps_per_s": n * srk_steps / wall2,
+        "fp64_cell_list": fp64,
         "batched": batched,
         "dopri5": dopri5,
     }

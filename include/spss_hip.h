@@ -143,6 +143,16 @@ int spss_srk_step(spss_system *sys, int method, double dt, int noise_mode, const
 /* `nsteps` steps of one stepper (the time loop of SimulationEngineFirstOrderPtr::RunSimulation,
  * SimulationEngineFirstOrderPtr.cpp:128-137, without observers).  Noise: NONE or PHILOX. */
 int spss_run(spss_system *sys, int stepper, int method, long nsteps, double dt, int noise_mode);
+/* No reference counterpart (the reference has no graphs): readies spss_run of the FP32C linked-list
+ * Euler/SRK steps with Philox noise -- enters the cell-ordered resident copy of the state and
+ * captures the HIP graphs every later run of this stepper and dt replays, from both buffer
+ * parities, without taking a step.  A no-op for runs that step eagerly (other precisions, methods
+ * or noise).  spss_run then replays every step from graphs (the first step of a run enters the
+ * resident order eagerly when it is not there yet). */
+int spss_prepare_run(spss_system *sys, int stepper, int method, double dt, int noise_mode);
+/* No reference counterpart: steps launched one by one and steps replayed from graphs since the
+ * last reset, and graphs captured since then; reset != 0 clears the counts after reading */
+int spss_run_stats(spss_system *sys, long *eager_steps, long *replayed_steps, long *graphs_captured, int reset);
 /* number of steps taken so far (the Philox step counter) */
 int spss_get_step(spss_system *sys, long *step);
 int spss_set_step(spss_system *sys, long step);

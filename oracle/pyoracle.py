@@ -44,6 +44,7 @@ def load_library(build=True):
     lib.orc_destroy.argtypes = [C.c_void_p]
     lib.orc_set_natural_frequencies.argtypes = [C.c_void_p, dp]
     lib.orc_set_higher_order.argtypes = [C.c_void_p, C.c_int]
+    lib.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
     lib.orc_uses_verlet.argtypes = [C.c_void_p]
     lib.orc_should_update_lists.argtypes = [C.c_void_p]
     lib.orc_accumulated_displacement.argtypes = [C.c_void_p]
@@ -138,6 +139,10 @@ class Oracle:
 
     def set_higher_order(self, on):
         self.lib.orc_set_higher_order(self.h, int(on))
+
+    def set_threads(self, threads):
+        """worker threads of the linked-list particle loop (results equal the serial loop's)"""
+        self.lib.orc_set_threads(self.h, int(threads))
 
     @property
     def should_update_lists(self):

@@ -16,6 +16,7 @@
 #include "spss_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -47,6 +48,7 @@ struct orc_system {
   double last_step_max;
   int uses_verlet;
   int higher_order;
+  int threads;                /* worker threads of the linked-list particle loop (1: serial) */
   double *nat;                /* natural frequencies [n] */
   double *stage;              /* stage state [s*n] */
   double *align;              /* alignment force [n] */
@@ -62,7 +64,7 @@ struct orc_system {
  * Periodic boundary conditions: BoundaryConfiguration/PeriodicBoundaryConditions.cpp
  * -------------------------------------------------------------------------*/
 
-/* PeriodicBoundaryConditions.cpp:76-88 (ClassAEffectiveParticleDistance) */
+/* PeriodicBoundaryConditions.cpp:26-38 (ClassAEffectiveParticleDistance) */
 void orc_pbc_class_a(double xs, double ys, double xi, double yi, double xj, double yj, double *dx,
                      double *dy) {
   double xr = 1.0 / xs, yr = 1.0 / ys;
@@ -74,7 +76,7 @@ void orc_pbc_class_a(double xs, double ys, double xi, double yi, double xj, doub
   *dy = ddy;
 }
 
-/* PeriodicBoundaryConditions.cpp:91-115 */
+/* PeriodicBoundaryConditions.cpp:41-65 (ClassBEffectiveParticleDistanceSigned) */
 void orc_pbc_class_b_signed(double xs, double ys, double xi, double yi, double xj, double yj,
                             double *dx, double *dy) {
   double ddx = xj - xi;
@@ -91,7 +93,7 @@ void orc_pbc_class_b_signed(double xs, double ys, double xi, double yi, double x
   *dy = ddy;
 }
 
-/* PeriodicBoundaryConditions.cpp:118-136 */
+/* PeriodicBoundaryConditions.cpp:68-86 (ClassBEffectiveParticleDistanceUnsigned) */
 void orc_pbc_class_b_unsigned(double xs, double ys, double xi, double yi, double xj, double yj,
                               double *dx, double *dy) {
   double ddx = fabs(xj - xi);
@@ -102,7 +104,7 @@ void orc_pbc_class_b_unsigned(double xs, double ys, double xi, double yi, double
   *dy = ddy;
 }
 
-/* PeriodicBoundaryConditions.cpp:140-152 (std::nearbyint: round-half-even in the default mode) */
+/* PeriodicBoundaryConditions.cpp:90-102 (ClassCEffectiveParticleDistanceSigned; std::nearbyint: round-half-even in the default mode) */
 void orc_pbc_class_c_signed(double xs, double ys, double xi, double yi, double xj, double yj,
                             double *dx, double *dy) {
   double xr = 1.0 / xs, yr = 1.0 / ys;
@@ -114,7 +116,7 @@ void orc_pbc_class_c_signed(double xs, double ys, double xi, double yi, double x
   *dy = ddy;
 }
 
-/* PeriodicBoundaryConditions.cpp:157-169 (ClassCEffectiveParticleDistanceUnsigned) */
+/* PeriodicBoundaryConditions.cpp:107-119 (ClassCEffectiveParticleDistanceUnsigned) */
 void orc_pbc_class_c_unsigned(double xs, double ys, double xi, double yi, double xj, double yj,
                               double *dx, double *dy) {
   double xr = 1.0 / xs, yr = 1.0 / ys;
@@ -126,14 +128,14 @@ void orc_pbc_class_c_unsigned(double xs, double ys, double xi, double yi, double
   *dy = ddy;
 }
 
-/* PeriodicBoundaryConditions.cpp:171-175 */
+/* PeriodicBoundaryConditions.cpp:121-125 */
 void orc_pbc_apply(double xs, double ys, double x, double y, double *xp, double *yp) {
   double xr = 1.0 / xs, yr = 1.0 / ys;
   *xp = x - floor(x * xr) * xs;
   *yp = y - floor(y * yr) * ys;
 }
 
-/* PeriodicBoundaryConditions.cpp:197-205 (the size argument is unused there) */
+/* PeriodicBoundaryConditions.cpp:147-155 (the size argument is unused there) */
 void orc_pbc_apply_state(double xs, double ys, double *state, int n, int s) {
   double xr = 1.0 / xs, yr = 1.0 / ys;
   for (int i = 0; i < n; ++i) {
@@ -185,6 +187,7 @@ orc_system *orc_create(const orc_params *p) {
   }
   sys->should_update = 1;
   sys->acc_disp = 0.0;
+  sys->threads = 1;
   sys->nat = (double *)calloc((size_t)n, sizeof(double));
   sys->stage = (double *)calloc((size_t)n * (size_t)s, sizeof(double));
   sys->align = (double *)calloc((size_t)n, sizeof(double));
@@ -407,15 +410,11 @@ static const double *s6_wrapped_stage(orc_system *sys, const double *stage, doub
   return w;
 }
 
-/* ParticleSystemFirstOrderPtr.cpp:318-468 (s=4), ParticleSystemSecondOrderPtr.cpp:240-383 (s=6) */
-static void rhs_linked_list(orc_system *sys, const double *stage, double *k_next) {
-  int n = sys->p.n, s = sys->s;
-  clear_accumulators(sys);
-  build_buckets(sys, stage, sys->buckets, sys->ncx, sys->ncy);
-  double *tmp = NULL;
-  const double *jstage = stage;
-  if (s == 6) jstage = s6_wrapped_stage(sys, stage, &tmp);
-  for (int i = 0; i < n; ++i) {
+/* the particle loop of the linked-list branch over particles [i0, i1): ParticleSystemFirstOrderPtr.cpp:398-466 */
+static void linked_list_particles(orc_system *sys, const double *stage, const double *jstage, double *k_next,
+                                  int i0, int i1) {
+  int s = sys->s;
+  for (int i = i0; i < i1; ++i) {
     int ii = i * s;
     double x_i, y_i;
     key_wrap(sys, stage[ii], stage[ii + 1], &x_i, &y_i);
@@ -433,8 +432,60 @@ static void rhs_linked_list(orc_system *sys, const double *stage, double *k_next
     }
     write_k(sys, i, stage, k_next);
   }
+}
+
+struct ll_task {
+  orc_system *sys;
+  const double *stage, *jstage;
+  double *k_next;
+  int i0, i1;
+};
+
+static void *ll_worker(void *arg) {
+  struct ll_task *t = (struct ll_task *)arg;
+  linked_list_particles(t->sys, t->stage, t->jstage, t->k_next, t->i0, t->i1);
+  return NULL;
+}
+
+/* ParticleSystemFirstOrderPtr.cpp:318-468 (s=4), ParticleSystemSecondOrderPtr.cpp:240-383 (s=6).
+ * With threads > 1 the particle loop runs over contiguous blocks of particles, one per thread, as
+ * the reference's ranks own them (Parallelization/ThreadSharedMemory.cpp:20-23, with the last block
+ * taking the remainder); every particle's sums are the serial ones, so results do not change. */
+static void rhs_linked_list(orc_system *sys, const double *stage, double *k_next) {
+  int n = sys->p.n, s = sys->s;
+  clear_accumulators(sys);
+  build_buckets(sys, stage, sys->buckets, sys->ncx, sys->ncy);
+  double *tmp = NULL;
+  const double *jstage = stage;
+  if (s == 6) jstage = s6_wrapped_stage(sys, stage, &tmp);
+  int T = sys->threads;
+  if (T > n) T = n;
+  if (T <= 1) {
+    linked_list_particles(sys, stage, jstage, k_next, 0, n);
+  } else {
+    pthread_t th[256];
+    struct ll_task task[256];
+    int running[256];
+    if (T > 256) T = 256;
+    int per = n / T;
+    for (int t = 0; t < T; ++t) {
+      task[t].sys = sys;
+      task[t].stage = stage;
+      task[t].jstage = jstage;
+      task[t].k_next = k_next;
+      task[t].i0 = t * per;
+      task[t].i1 = (t == T - 1) ? n : (t + 1) * per;
+      running[t] = t > 0 && pthread_create(&th[t], NULL, ll_worker, &task[t]) == 0;
+      if (t > 0 && !running[t]) ll_worker(&task[t]); /* no thread: the block runs here */
+    }
+    ll_worker(&task[0]);
+    for (int t = 1; t < T; ++t)
+      if (running[t]) pthread_join(th[t], NULL);
+  }
   free(tmp);
 }
+
+void orc_set_threads(orc_system *sys, int threads) { sys->threads = threads < 1 ? 1 : threads; }
 
 /* Verlet list build over all pairs: ParticleSystemFirstOrderPtr.cpp:497-543 (x_i not wrapped) */
 static void build_verlet_all_pairs(orc_system *sys, const double *stage) {
@@ -730,7 +781,7 @@ static int cmp_pair(const void *a, const void *b) {
 }
 
 /* the ClassC in-range test of the pair loops over all j (the set a correct cell or Verlet
- * search must reproduce): PeriodicBoundaryConditions.cpp:157-169 + ParticleSystemFirstOrderPtr.cpp:285-288 */
+ * search must reproduce): PeriodicBoundaryConditions.cpp:107-119 + ParticleSystemFirstOrderPtr.cpp:285-288 */
 long orc_pairs_bruteforce(const double *stage, int n, int s, double rho, double xs, double ys,
                           int32_t *out, long cap) {
   double rho2 = rho * rho;
@@ -793,7 +844,23 @@ long orc_pairs_linked_list(orc_system *sys, const double *stage, int32_t *out, l
     }
   }
   free(tmp);
-  if (cnt <= cap) qsort(out, (size_t)cnt, 2 * sizeof(int32_t), cmp_pair);
+  if (cnt <= cap) { /* pairs come out by ascending i: sort each particle's j in place */
+    long a = 0;
+    while (a < cnt) {
+      long b = a + 1;
+      while (b < cnt && out[2 * b] == out[2 * a]) ++b;
+      for (long u = a + 1; u < b; ++u) { /* insertion sort of a few dozen entries */
+        int32_t j = out[2 * u + 1];
+        long v = u;
+        while (v > a && out[2 * (v - 1) + 1] > j) {
+          out[2 * v + 1] = out[2 * (v - 1) + 1];
+          --v;
+        }
+        out[2 * v + 1] = j;
+      }
+      a = b;
+    }
+  }
   return cnt;
 }
 

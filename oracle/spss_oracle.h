@@ -60,6 +60,9 @@ orc_system *orc_create(const orc_params *p);
 void orc_destroy(orc_system *sys);
 void orc_set_natural_frequencies(orc_system *sys, const double *w);
 void orc_set_higher_order(orc_system *sys, int on);
+/* worker threads of the linked-list particle loop (contiguous particle blocks, as the reference's
+ * ranks own them, Parallelization/ThreadSharedMemory.cpp:20-23); results equal the serial ones */
+void orc_set_threads(orc_system *sys, int threads);
 int orc_uses_verlet(const orc_system *sys);
 int orc_should_update_lists(const orc_system *sys);
 double orc_accumulated_displacement(const orc_system *sys);

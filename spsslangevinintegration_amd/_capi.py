@@ -27,7 +27,8 @@ EXPORTS = [
     "spss_set_stream", "spss_synchronize", "spss_set_state", "spss_get_state", "spss_set_state_device",
     "spss_state_device", "spss_set_natural_frequencies", "spss_set_higher_order",
     "spss_get_complementary_force", "spss_evaluate_rhs", "spss_evaluate_rhs_device", "spss_rk4_step",
-    "spss_euler_step", "spss_srk_step", "spss_run", "spss_get_step", "spss_set_step",
+    "spss_euler_step", "spss_srk_step", "spss_run", "spss_prepare_run", "spss_run_stats", "spss_get_step",
+    "spss_set_step",
     "spss_calculate_max_displacement", "spss_verlet_status", "spss_get_verlet_lists",
     "spss_neighbour_pairs", "spss_order_parameter", "spss_apply_periodic_boundary_conditions",
     "spss_averaging_reset", "spss_averaging_accumulate", "spss_averaging_get",
@@ -106,6 +107,8 @@ def load(path=None):
         "spss_euler_step": (i, [vp, i, d, i, vp]),
         "spss_srk_step": (i, [vp, i, d, i, vp]),
         "spss_run": (i, [vp, i, i, l, d, i]),
+        "spss_prepare_run": (i, [vp, i, i, d, i]),
+        "spss_run_stats": (i, [vp, C.POINTER(l), C.POINTER(l), C.POINTER(l), i]),
         "spss_get_step": (i, [vp, C.POINTER(l)]),
         "spss_set_step": (i, [vp, l]),
         "spss_calculate_max_displacement": (i, [vp, vp]),

@@ -57,7 +57,10 @@ namespace {
 
 int fast_fail(int rc) { return set_error(rc, fast_last_error()); }
 
+// the stream is drained first: a graph exec is never destroyed while a replay of it may run
 void drop_graph(spss_batch *b) {
+  if (!b->exec && !b->graph) return;
+  if (b->st) (void)hipStreamSynchronize(b->st);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   b->exec = nullptr;
@@ -97,6 +100,9 @@ int capture(spss_batch *b, int steps, long every, long from, double dt) {
   int rp = fast_prepare_moves(b->fast, b->st, b->P, dt);
   if (rp != SPSS_OK) return fast_fail(rp);
   const int parity0 = fast_buffer_parity(b->fast);
+  // nothing runs during the capture: the host bookkeeping the recorded steps change is put back
+  const FastHostMark mark = fast_host_mark(b->fast);
+  const bool stale0 = b->master_stale;
   BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed));
   int rc = SPSS_OK;
   for (int k = 0; k < steps && rc == SPSS_OK; ++k) {
@@ -108,6 +114,8 @@ int capture(spss_batch *b, int steps, long every, long from, double dt) {
   }
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->st, &g);
+  fast_restore_host_mark(b->fast, mark);
+  b->master_stale = stale0;
   if (rc != SPSS_OK) {
     if (g) hipGraphDestroy(g);
     return rc;
@@ -124,6 +132,39 @@ int capture(spss_batch *b, int steps, long every, long from, double dt) {
   b->graph_from = from;
   b->graph_dt = dt;
   b->graph_parity = parity0;
+  return SPSS_OK;
+}
+
+static int batch_run(spss_batch *b, long nsteps, double dt, long observe_every, long observe_from, int use_graph) {
+  long done = 0;
+  // the first step after new inputs enters the resident cell order; it runs eagerly
+  if (!fast_is_resident(b->fast)) {
+    BTRY(steps_and_observe(b, 1, dt, observe_every == 1, observe_from));
+    ++done;
+  }
+  if (use_graph) {
+    int gs = observe_every > 0 ? (int)observe_every : 10;
+    if (gs % 2) gs *= 2;
+    if (b->graph_steps != gs || b->graph_every != observe_every || b->graph_from != observe_from ||
+        b->graph_dt != dt || b->graph_parity != fast_buffer_parity(b->fast) || !b->exec) {
+      // keep the observation phase: a graph starts right after a reading
+      while (observe_every > 0 && done < nsteps && done % observe_every != 0) {
+        BTRY(steps_and_observe(b, 1, dt, (done + 1) % observe_every == 0, observe_from));
+        ++done;
+      }
+      if (nsteps - done >= gs) BTRY(capture(b, gs, observe_every, observe_from, dt));
+    }
+    while (b->exec && nsteps - done >= b->graph_steps && (observe_every == 0 || done % observe_every == 0)) {
+      const int rp = fast_prepare_moves(b->fast, b->st, b->P, dt);
+      if (rp != SPSS_OK) return fast_fail(rp);
+      BCHK(hipGraphLaunch(b->exec, b->st));
+      done += b->graph_steps;
+    }
+  }
+  while (done < nsteps) {
+    ++done;
+    BTRY(steps_and_observe(b, 1, dt, observe_every > 0 && done % observe_every == 0, observe_from));
+  }
   return SPSS_OK;
 }
 
@@ -244,39 +285,14 @@ int spss_batch_reset(spss_batch *b, long step) {
   return rc == SPSS_OK ? SPSS_OK : fast_fail(rc);
 }
 
+// the run, then the stream drained on every return (error returns included)
 int spss_batch_run(spss_batch *b, long nsteps, double dt, long observe_every, long observe_from, int use_graph) {
   if (!b || nsteps < 0 || observe_every < 0 || !(dt > 0.0)) return set_error(SPSS_ERR_INVALID, "bad argument");
   if (nsteps == 0) return SPSS_OK;
-  long done = 0;
-  // the first step after new inputs enters the resident cell order; it runs eagerly
-  if (!fast_is_resident(b->fast)) {
-    BTRY(steps_and_observe(b, 1, dt, observe_every == 1, observe_from));
-    ++done;
-  }
-  if (use_graph) {
-    int gs = observe_every > 0 ? (int)observe_every : 10;
-    if (gs % 2) gs *= 2;
-    if (b->graph_steps != gs || b->graph_every != observe_every || b->graph_from != observe_from ||
-        b->graph_dt != dt || b->graph_parity != fast_buffer_parity(b->fast) || !b->exec) {
-      // keep the observation phase: a graph starts right after a reading
-      while (observe_every > 0 && done < nsteps && done % observe_every != 0) {
-        BTRY(steps_and_observe(b, 1, dt, (done + 1) % observe_every == 0, observe_from));
-        ++done;
-      }
-      if (nsteps - done >= gs) BTRY(capture(b, gs, observe_every, observe_from, dt));
-    }
-    while (b->exec && nsteps - done >= b->graph_steps && (observe_every == 0 || done % observe_every == 0)) {
-      const int rp = fast_prepare_moves(b->fast, b->st, b->P, dt);
-      if (rp != SPSS_OK) return fast_fail(rp);
-      BCHK(hipGraphLaunch(b->exec, b->st));
-      done += b->graph_steps;
-    }
-  }
-  while (done < nsteps) {
-    ++done;
-    BTRY(steps_and_observe(b, 1, dt, observe_every > 0 && done % observe_every == 0, observe_from));
-  }
-  BCHK(hipStreamSynchronize(b->st));
+  const int rc = batch_run(b, nsteps, dt, observe_every, observe_from, use_graph);
+  const hipError_t e = hipStreamSynchronize(b->st);
+  if (rc != SPSS_OK) return rc;
+  if (e != hipSuccess) return set_error(SPSS_ERR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
   return SPSS_OK;
 }
 
@@ -316,10 +332,6 @@ int spss_batch_set_tile_widths(spss_batch *b, const int *W) {
   return rc == SPSS_OK ? SPSS_OK : fast_fail(rc);
 }
 
-// The autotuner: times `probe_steps` steps (graph replays, no readings) for tile widths of
-// 0.5x ... 2x the default particles-per-workgroup of every replica and keeps the fastest.  The
-// state, step counter and averages are restored afterwards; the tile width does not change
-// the results (a lane's candidates and their order depend on the cells, not on the tile).
 // the cell path rebuilt at cell subdivision k (state, natural frequencies and step counter are
 // the caller's to restore)
 static int rebuild_cells(spss_batch *b, int k) {
@@ -343,10 +355,10 @@ int spss_batch_autotune_cells(spss_batch *b, double dt, int probe_steps, double 
                               int *chosen) {
   if (!b || probe_steps < 2 || !(dt > 0.0)) return set_error(SPSS_ERR_INVALID, "bad argument");
   BTRY(sync_master(b));
-  std::vector<double> saved((size_t)b->s * b->n), acc(3 * b->R), z(2 * b->R);
+  std::vector<double> saved((size_t)b->s * b->n);
   BCHK(hipMemcpyAsync(saved.data(), b->state, sizeof(double) * saved.size(), hipMemcpyDeviceToHost, b->st));
   uint64_t step0 = 0;
-  int rc = fast_batch_stats(b->fast, b->st, acc.data(), z.data(), &step0);
+  int rc = fast_batch_stats(b->fast, b->st, nullptr, nullptr, &step0);
   if (rc != SPSS_OK) return fast_fail(rc);
   double best = 1e300;
   int best_k = b->k;
@@ -385,13 +397,18 @@ int spss_batch_get_cell_subdivision(spss_batch *b, int *k) {
   return SPSS_OK;
 }
 
+// Tile-width autotuner: times `probe_steps` steps (graph replays, no readings) for tile widths of
+// 0.5x ... 2x the default particles-per-workgroup of every replica and keeps the fastest.  The
+// state and step counter are restored afterwards and the running averages cleared; the tile
+// width does not change the results (a lane's candidates and their order depend on the cells,
+// not on the tile).
 int spss_batch_autotune(spss_batch *b, double dt, int probe_steps, double *table_ms, int table_cap, int *chosen) {
   if (!b || probe_steps < 2 || !(dt > 0.0)) return set_error(SPSS_ERR_INVALID, "bad argument");
   BTRY(sync_master(b));
-  std::vector<double> saved((size_t)b->s * b->n), acc(3 * b->R), z(2 * b->R);
+  std::vector<double> saved((size_t)b->s * b->n);
   BCHK(hipMemcpyAsync(saved.data(), b->state, sizeof(double) * saved.size(), hipMemcpyDeviceToHost, b->st));
   uint64_t step0 = 0;
-  int rc = fast_batch_stats(b->fast, b->st, acc.data(), z.data(), &step0);
+  int rc = fast_batch_stats(b->fast, b->st, nullptr, nullptr, &step0);
   if (rc != SPSS_OK) return fast_fail(rc);
   std::vector<int> W0;
   fast_tile_widths(b->fast, &W0);

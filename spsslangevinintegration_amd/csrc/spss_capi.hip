@@ -19,6 +19,7 @@
 #include "../../include/spss_hip.h"
 #include "spss_fast.hpp"
 #include "spss_launch.hpp"
+#include "spss_scan.hpp"
 
 using namespace spss;
 
@@ -67,6 +68,7 @@ struct spss_system {
   int *vcount = nullptr, *vstart = nullptr, *vlist = nullptr;
   long vcap = 0;
   int *pcount = nullptr, *pstart = nullptr, *plist = nullptr;
+  int *scan_blk = nullptr;  // scratch of the scans over n counts (Verlet lists, pair export)
   long pcap = 0;
   // host bookkeeping mirroring the reference members
   bool should_update = true;      // should_update_lists_
@@ -76,6 +78,8 @@ struct spss_system {
   bool have_nat = false;
   long step = 0;
   long avg_count = 0;
+  // steps run since the last spss_run_stats reset: launched one by one, and replayed from graphs
+  long eager_steps = 0, replayed_steps = 0, captures0 = 0;
   FastPath *fast = nullptr;
   // the fused FP32C steps advance a resident copy of the state kept in cell order; `state`
   // is then behind it until the next sync_master
@@ -127,6 +131,7 @@ int ensure_cells(spss_system *h, CellBuffers &cb, int ncx, int ncy) {
   hipFree(cb.start);
   hipFree(cb.tmp);
   hipFree(cb.sorted);
+  hipFree(cb.blocksum);
   cb = CellBuffers{};
   TRY(dalloc_n(&cb.key, h->n));
   TRY(dalloc_n(&cb.slot, h->n));
@@ -134,9 +139,16 @@ int ensure_cells(spss_system *h, CellBuffers &cb, int ncx, int ncy) {
   TRY(dalloc_n(&cb.start, cells + 1));
   TRY(dalloc_n(&cb.tmp, h->n));
   TRY(dalloc_n(&cb.sorted, h->n));
+  TRY(dalloc_n(&cb.blocksum, scan_blocks(cells) + 1));
   HIPCHECK(hipMemsetAsync(cb.count, 0, sizeof(int) * cells, h->stream));
   cb.cells = cells;
   return SPSS_OK;
+}
+
+// scratch of the scans over n counts
+int ensure_scan(spss_system *h) {
+  if (h->scan_blk) return SPSS_OK;
+  return dalloc_n(&h->scan_blk, scan_blocks(h->n) + 1);
 }
 
 // EvaluateRhs's dispatch on rho (ParticleSystemFirstOrderPtr.cpp:131-153; model 6:
@@ -179,7 +191,8 @@ int build_verlet(spss_system *h, int m) {
     launch_cell_build(st, h->P, h->stage, h->ncvx, h->ncvy, h->vcells);
     launch_verlet_count_cells(st, h->P, h->stage, h->ncvx, h->ncvy, h->vcells, rmax2, h->vcount);
   }
-  launch_scan(st, h->vcount, h->vstart, h->n);
+  TRY(ensure_scan(h));
+  launch_scan(st, h->vcount, h->vstart, h->n, h->scan_blk);
   int total = 0;
   HIPCHECK(hipMemcpyAsync(&total, h->vstart + h->n, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
@@ -300,6 +313,7 @@ int rk4_step(spss_system *h, int method, double dt) {
                      track ? h->partial : nullptr);
   HIPCHECK(hipGetLastError());
   ++h->step;
+  ++h->eager_steps;
   return finish_step_displacement(h);
 }
 
@@ -315,6 +329,7 @@ int fast_fused_step(spss_system *h, int srk, double dt, int noise_mode, const do
   if (rc != SPSS_OK) return fail(rc, fast_last_error());
   h->master_stale = true;
   ++h->step;
+  ++h->eager_steps;
   return SPSS_OK;
 }
 
@@ -338,6 +353,7 @@ int euler_step(spss_system *h, int method, double dt, int noise_mode, const doub
                track ? h->partial : nullptr);
   HIPCHECK(hipGetLastError());
   ++h->step;
+  ++h->eager_steps;
   return finish_step_displacement(h);
 }
 
@@ -361,6 +377,7 @@ int srk_step(spss_system *h, int method, double dt, int noise_mode, const double
              track ? h->partial : nullptr);
   HIPCHECK(hipGetLastError());
   ++h->step;
+  ++h->eager_steps;
   return finish_step_displacement(h);
 }
 
@@ -494,7 +511,7 @@ int spss_destroy(spss_system *h) {
   delete h->timer;
   void *bufs[] = {h->state, h->stage, h->k[0], h->k[1], h->k[2], h->k[3], h->deriv, h->compl_f, h->nat,
                   h->noise, h->partial, h->scal, h->acc, h->io[0], h->io[1], h->io[2], h->vcount, h->vstart,
-                  h->vlist, h->pcount, h->pstart, h->plist};
+                  h->vlist, h->pcount, h->pstart, h->plist, h->scan_blk};
   for (void *b : bufs) hipFree(b);
   for (double *b : h->dp) hipFree(b);
   if (h->dp_host) hipHostFree(h->dp_host);
@@ -505,6 +522,7 @@ int spss_destroy(spss_system *h) {
     hipFree(cb->start);
     hipFree(cb->tmp);
     hipFree(cb->sorted);
+    hipFree(cb->blocksum);
   }
   if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -623,23 +641,34 @@ int spss_srk_step(spss_system *h, int method, double dt, int noise_mode, const d
   return srk_step(h, method, dt, noise_mode, noise);
 }
 
+// the fused FP32C stochastic steps with Philox noise that spss_run can replay from graphs
+bool graph_run_applies(const spss_system *h, int stepper, int method, int noise_mode) {
+  const bool srk = stepper == SPSS_STEPPER_SRK && h->s == 4;
+  return (stepper == SPSS_STEPPER_EULER || srk) && noise_mode == SPSS_NOISE_PHILOX && use_fused(h, method);
+}
+
 int spss_run(spss_system *h, int stepper, int method, long nsteps, double dt, int noise_mode) {
   if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
   if (noise_mode != SPSS_NOISE_NONE && noise_mode != SPSS_NOISE_PHILOX)
     return fail(SPSS_ERR_INVALID, "spss_run draws noise from NONE or PHILOX only");
   long k = 0;
-  // fused FP32C stochastic steps with Philox noise: the first step (which may enter the resident
-  // order) runs eagerly, the bulk is replayed from a HIP graph, the remainder runs eagerly
+  // fused FP32C stochastic steps with Philox noise: a step that enters the resident order runs
+  // eagerly, every other step is replayed from HIP graphs
   const bool srk = stepper == SPSS_STEPPER_SRK && h->s == 4;
-  if ((stepper == SPSS_STEPPER_EULER || srk) && noise_mode == SPSS_NOISE_PHILOX && use_fused(h, method) &&
-      nsteps > 1) {
-    TRY(srk ? srk_step(h, method, dt, noise_mode, nullptr) : euler_step(h, method, dt, noise_mode, nullptr));
-    k = 1;
-    const long g = fast_run_graph(h->fast, h->stream, h->P, srk ? 1 : 0, h->state, h->have_nat ? h->nat : nullptr,
-                                  srk ? h->compl_f : nullptr, dt, h->p.D_phi, h->p.xi, h->p.seed,
-                                  (uint64_t)h->step, nsteps - k);
+  const double *nat = h->have_nat ? h->nat : nullptr;
+  if (graph_run_applies(h, stepper, method, noise_mode) && nsteps > 0) {
+    if (!fast_graph_ready(h->fast, nat, srk ? 1 : 0)) {
+      TRY(srk ? srk_step(h, method, dt, noise_mode, nullptr) : euler_step(h, method, dt, noise_mode, nullptr));
+      k = 1;
+    }
+    h->P.higher_order = srk ? 1 : 0;
+    if (srk) h->higher_order = true;  // StochasticRungeKuttaStepperPtr.hpp:44
+    const long g = fast_run_graph(h->fast, h->stream, h->P, srk ? 1 : 0, h->state, nat, srk ? h->compl_f : nullptr,
+                                  dt, h->p.D_phi, h->p.xi, h->p.seed, (uint64_t)h->step, nsteps - k);
     if (g < 0) return fail(SPSS_ERR_DEVICE, fast_last_error());
+    if (g > 0) h->master_stale = true;
     h->step += g;
+    h->replayed_steps += g;
     k += g;
   }
   for (; k < nsteps; ++k) {
@@ -649,6 +678,39 @@ int spss_run(spss_system *h, int stepper, int method, long nsteps, double dt, in
       case SPSS_STEPPER_SRK: TRY(srk_step(h, method, dt, noise_mode, nullptr)); break;
       default: return fail(SPSS_ERR_INVALID, "unknown stepper");
     }
+  }
+  return SPSS_OK;
+}
+
+int spss_prepare_run(spss_system *h, int stepper, int method, double dt, int noise_mode) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (!graph_run_applies(h, stepper, method, noise_mode)) return SPSS_OK;  // spss_run steps eagerly here
+  const bool srk = stepper == SPSS_STEPPER_SRK && h->s == 4;
+  const double *nat = h->have_nat ? h->nat : nullptr;
+  if (!fast_graph_ready(h->fast, nat, srk ? 1 : 0)) {
+    TRY(master_write(h, true));  // the resident order is entered from the master state
+    int rc = fast_enter(h->fast, h->stream, h->P, srk ? 1 : 0, h->state, nat);
+    if (rc != SPSS_OK) return fail(rc, fast_last_error());
+  }
+  h->P.higher_order = srk ? 1 : 0;
+  if (srk) h->higher_order = true;
+  int rc = fast_prepare_graphs(h->fast, h->stream, h->P, srk ? 1 : 0, h->state, nat, srk ? h->compl_f : nullptr, dt,
+                               h->p.D_phi, h->p.xi, h->p.seed);
+  if (rc != SPSS_OK) return fail(rc, fast_last_error());
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return SPSS_OK;
+}
+
+int spss_run_stats(spss_system *h, long *eager_steps, long *replayed_steps, long *graphs_captured, int reset) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  const long cap = h->fast ? fast_graph_captures(h->fast) : 0;
+  if (eager_steps) *eager_steps = h->eager_steps;
+  if (replayed_steps) *replayed_steps = h->replayed_steps;
+  if (graphs_captured) *graphs_captured = cap - h->captures0;
+  if (reset) {
+    h->eager_steps = 0;
+    h->replayed_steps = 0;
+    h->captures0 = cap;
   }
   return SPSS_OK;
 }
@@ -827,7 +889,8 @@ int spss_neighbour_pairs(spss_system *h, int method, const double *host_stage, i
   } else {
     return fail(SPSS_ERR_UNSUPPORTED, "pair export is implemented for the all-pairs and linked-list methods");
   }
-  launch_scan(st, h->pcount, h->pstart, h->n);
+  TRY(ensure_scan(h));
+  launch_scan(st, h->pcount, h->pstart, h->n, h->scan_blk);
   HIPCHECK(hipMemcpyAsync(&total, h->pstart + h->n, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   if (!pairs) {  // count only

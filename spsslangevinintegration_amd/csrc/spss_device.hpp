@@ -25,14 +25,14 @@ struct DevParams {
   double sin_a;    // std::sin(alpha)
   double xi_r;     // model 6
   double xs, ys;   // PeriodicBoundaryConditions x_size_, y_size_
-  double xrs, yrs; // 1.0 / x_size (PeriodicBoundaryConditions.cpp:67-68)
+  double xrs, yrs; // 1.0 / x_size (the constructor, PeriodicBoundaryConditions.cpp:7-21)
   int natfreq_local;
   int higher_order;
 };
 
 // ---- periodic boundary conditions (BoundaryConfiguration/PeriodicBoundaryConditions.cpp)
 
-// ClassCEffectiveParticleDistanceUnsigned, PeriodicBoundaryConditions.cpp:157-169
+// ClassCEffectiveParticleDistanceUnsigned, PeriodicBoundaryConditions.cpp:107-119
 __device__ __forceinline__ void class_c_unsigned(const DevParams &P, double xi, double yi, double xj,
                                                  double yj, double &dx, double &dy) {
   dx = fabs(xj - xi);
@@ -41,7 +41,7 @@ __device__ __forceinline__ void class_c_unsigned(const DevParams &P, double xi, 
   dy -= (double)((int)(dy * P.yrs + 0.5)) * P.ys;
 }
 
-// ClassAEffectiveParticleDistance, PeriodicBoundaryConditions.cpp:76-88
+// ClassAEffectiveParticleDistance, PeriodicBoundaryConditions.cpp:26-38
 __device__ __forceinline__ void class_a(const DevParams &P, double xi, double yi, double xj, double yj,
                                         double &dx, double &dy) {
   dx = xj - xi;
@@ -50,7 +50,7 @@ __device__ __forceinline__ void class_a(const DevParams &P, double xi, double yi
   dy -= (double)((int)(dy * 2.0 * P.yrs)) * P.ys;
 }
 
-// ApplyPeriodicBoundaryConditions(x, y, x_pbc, y_pbc), PeriodicBoundaryConditions.cpp:171-175
+// ApplyPeriodicBoundaryConditions(x, y, x_pbc, y_pbc), PeriodicBoundaryConditions.cpp:121-125
 __device__ __forceinline__ void apply_pbc(const DevParams &P, double x, double y, double &xw, double &yw) {
   xw = x - floor(x * P.xrs) * P.xs;
   yw = y - floor(y * P.yrs) * P.ys;

@@ -60,14 +60,35 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
 bool fast_is_resident(const FastPath *f);
 // which of its double buffers the resident order is in (a captured step graph replays only there)
 int fast_buffer_parity(const FastPath *f);
-// writes the resident state back into `state` in original order; the copy stays resident
-// Fused steps replayed from a HIP graph (Philox noise from the device step counter): runs the
-// largest multiple of the graph's step count not above nsteps, starting at Philox step `step`,
-// and returns how many steps it ran (0: not applicable now, e.g. not resident yet or the kernel
-// timer is on; -1: a HIP error, see fast_last_error).  The caller runs the rest with fast_step.
+// Fused steps replayed from HIP graphs (Philox noise from the device step counter): runs all
+// nsteps from graphs of kGraphSteps steps and of one step, starting at Philox step `step`, and
+// returns nsteps (0: not applicable now, see fast_graph_ready; -1: a HIP error, see
+// fast_last_error).
 long fast_run_graph(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat,
                     double *compl_out, double dt, double D_phi, double friction, uint64_t seed, uint64_t step,
                     long nsteps);
+// whether fast_run_graph can run now: the resident order exists in the form the stepper needs
+// (natural frequencies, FP32 omega for SRK) and the kernel timer is off
+bool fast_graph_ready(const FastPath *f, const double *nat, int srk);
+// enters the resident order from `state` without stepping (no-op when already there)
+int fast_enter(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat);
+// captures and instantiates the step graphs a run of any length needs, from both buffer parities,
+// without running a step (the resident order must exist: fast_enter)
+int fast_prepare_graphs(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state,
+                        const double *nat, double *compl_out, double dt, double D_phi, double friction,
+                        uint64_t seed);
+// number of step graphs captured since creation
+long fast_graph_captures(const FastPath *f);
+// the host bookkeeping a step changes (buffer parity, move and resident flags): a stream capture
+// records steps without running them, so a capturer saves it before and puts it back after
+struct FastHostMark {
+  bool resident, resident_nat, romega_valid, mv_ready, mv_srk;
+  int cur, mvc;
+  double mv_dt, mv_v0;
+};
+FastHostMark fast_host_mark(const FastPath *f);
+void fast_restore_host_mark(FastPath *f, const FastHostMark &m);
+// writes the resident state back into `state` in original order; the copy stays resident
 int fast_resident_sync(FastPath *f, hipStream_t st, double *state);
 // forgets the resident copy (the next fused step re-enters from the caller's state)
 void fast_resident_drop(FastPath *f);

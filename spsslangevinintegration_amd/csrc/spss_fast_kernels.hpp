@@ -23,6 +23,7 @@
 
 #include "spss_device.hpp"
 #include "spss_fast.hpp"
+#include "spss_scan.hpp"
 
 namespace spss {
 
@@ -32,7 +33,6 @@ constexpr int kStageMax = 1024; // staged entries per workgroup (particles + row
 constexpr int kPad = 4;         // padding entries after each staged row (>= the unroll - 1)
 constexpr int kMaxRows = 7;     // 2k+1 with k <= 3
 constexpr int kMaxCols = 48;    // W + 2k
-constexpr int kScanItems = 8;   // counts per thread in the two-level scans
 constexpr int kRowTileCols = 1024;  // rows of at most this many cells get per-step tiles
 }  // namespace
 
@@ -42,7 +42,7 @@ struct FastArgs {
   int4 *ovf_list;              // work of tiles whose staged cells overflow the LDS stage, for fk_pairs_ovf:
                                // {first cell of the replica, row, ca, cb} a column span that fits, or
                                // {.., .., column, -1 - j} particles j*kFB.. of a cell that does not
-  int *ovf_count;              // their number, zeroed by fk_scan_add or fr_far_bucket
+  int *ovf_count;              // their number, zeroed by scan_add or fr_far_bucket
   double H;
   // guard band of the FP32 range test: |d^2 - r2c| <= r2h is decided in FP64; r2k is a power of
   // two with r2k * r2h > 4, so clamp(-r2k * (d^2 - r2c), 0, 1) is exactly 1 below the band, 0 above
@@ -139,79 +139,6 @@ __global__ void fk_keys(DevParams P, Layout L, const double *__restrict__ stage,
   const int c = replica_of(L, i) * L.cpr + cell_coord(yw, ncf) * ncf + cell_coord(xw, ncf);
   key[i] = c;
   slot[i] = atomicAdd(count + c, 1);
-}
-
-// block-local exclusive scan of 2048 values per block; a value is count[c] + count2[c]
-// (count2 may be null)
-__device__ __forceinline__ void scan_local_block(const int *__restrict__ count, const int *__restrict__ count2, int m,
-                                                 int *__restrict__ start, int *__restrict__ blocksum, int blk) {
-  __shared__ int sh[256];
-  const int base = blk * 256 * kScanItems + threadIdx.x * kScanItems;
-  int v[kScanItems];
-  int s = 0;
-#pragma unroll
-  for (int q = 0; q < kScanItems; ++q) {
-    v[q] = 0;
-    if (base + q < m) {
-      v[q] = count[base + q];
-      if (count2) v[q] += count2[base + q];
-    }
-    s += v[q];
-  }
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    int t = (threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += t;
-    __syncthreads();
-  }
-  int run = sh[threadIdx.x] - s;
-#pragma unroll
-  for (int q = 0; q < kScanItems; ++q) {
-    if (base + q < m) start[base + q] = run;
-    run += v[q];
-  }
-  if (threadIdx.x == 255) blocksum[blk] = sh[255];
-}
-
-__global__ __launch_bounds__(256) void fk_scan_local(const int *__restrict__ count, const int *__restrict__ count2,
-                                                     int m, int *__restrict__ start, int *__restrict__ blocksum) {
-  scan_local_block(count, count2, m, start, blocksum, blockIdx.x);
-}
-
-// adds the scanned block totals, writes start[m], zeroes count[] and count2[] for the next use
-// (and the overflow-tile counter of the pair kernel)
-__global__ __launch_bounds__(256) void fk_scan_add(int *__restrict__ count, int *__restrict__ count2, int m,
-                                                   int *__restrict__ start, const int *__restrict__ blocksum,
-                                                   int nblocks, int *__restrict__ ovf_count) {
-  __shared__ int off;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && ovf_count) *ovf_count = 0;
-  if (threadIdx.x == 0) {
-    int s = 0;
-    for (int b = 0; b < (int)blockIdx.x; ++b) s += blocksum[b];
-    off = s;
-    if (blockIdx.x == nblocks - 1) start[m] = s + blocksum[blockIdx.x];
-  }
-  __syncthreads();
-  const int base = blockIdx.x * 256 * kScanItems;
-  for (int q = threadIdx.x; q < 256 * kScanItems; q += 256) {
-    const int c = base + q;
-    if (c < m) {
-      start[c] += off;
-      count[c] = 0;
-      if (count2) count2[c] = 0;
-    }
-  }
-}
-
-// the sum of the first b scan-block totals, by one wave (every lane gets it; all lanes call)
-__device__ __forceinline__ int wave_prefix_of(const int *__restrict__ blocksum, int b) {
-  int s = 0;
-  for (int j = threadIdx.x & 63; j < b; j += 64) s += blocksum[j];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  return s;
 }
 
 __global__ void fk_scatter(int n, const int *__restrict__ key, const int *__restrict__ slot,

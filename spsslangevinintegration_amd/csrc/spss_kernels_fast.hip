@@ -52,20 +52,26 @@ struct FastPath {
   int4 *ovf_list = nullptr;  // work split off overflowing tiles
   int *ovf_count = nullptr;
   int ntiles = 0, tiles_cap = 0;
-  // single systems: per-step row tiles of the resident order (cmax slots per row of cells)
-  int4 *rtiles = nullptr;
+  // single systems: per-step row tiles of the resident order (cmax slots per row of cells), and
+  // the row tiles of the rebuilt cell list (RHS, pair export), so that every pair-kernel launch
+  // of a single system stages the same kind of tile
+  int4 *rtiles = nullptr, *btiles = nullptr;
   int rt_cmax = 0, rt_wcap = 0;
   Coef *coef = nullptr;                    // per-replica coefficients (batches)
   unsigned long long *step_dev = nullptr;  // device step counter (batches, graph-replayed steps)
   int gen = 0;                             // bumped whenever launch shapes change (tile table)
-  // graph of kGraphSteps fused steps of a single system and what it was captured with
-  hipGraph_t sg_graph = nullptr;
-  hipGraphExec_t sg_exec = nullptr;
-  int sg_srk = -1, sg_cur = -1, sg_gen = -1, sg_higher = -1;
+  // graphs of fused steps of a single system: sg[0][p] runs kGraphSteps steps, sg[1][p] one
+  // step, from buffer parity p (fast_buffer_parity); all were captured with the key below
+  struct StepGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } sg[2][4];
+  int sg_srk = -1, sg_gen = -1, sg_higher = -1;
   double sg_dt = 0.0, sg_dphi = 0.0, sg_friction = 0.0;
   uint64_t sg_seed = 0;
   const double *sg_state = nullptr, *sg_nat = nullptr;
   double *sg_compl = nullptr;
+  long sg_captures = 0;  // step graphs captured since creation
   double *zlast = nullptr, *racc = nullptr;
   // rebuilt cell list (RHS, pair export, entry into resident mode)
   int *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr, *sorted = nullptr;
@@ -133,12 +139,13 @@ static int default_tile_width(double per_cell, int k, int ncf) {
   return std::max(1, std::min(W, std::min(kMaxCols - 2 * k, ncf)));
 }
 
-// tile table for per-replica tile widths, and the guard band of the widest tile: the FP32
-// distance error bound (coordinates <= (W+2k+1) cells from the staged origin, rounded once;
-// offsets < one cell) times a safety factor of 8
+// tile table for per-replica tile widths, and the guard band of the widest span any launch can
+// stage: the FP32 distance error bound (coordinates <= (W+2k+1) cells from the staged origin,
+// rounded once; offsets < one cell) times a safety factor of 8.  A single system's resident
+// steps stage per-step row tiles of up to rt_wcap columns, so their width counts too.
 static int set_tiles(FastPath *f, const std::vector<int> &W) {
   std::vector<int4> t;
-  int wmax = 1;
+  int wmax = f->rtiles ? f->rt_wcap : 1;
   for (int r = 0; r < f->R; ++r) {
     const int w = std::max(1, std::min(W[r], std::min(kMaxCols - 2 * f->k, f->ncf)));
     wmax = std::max(wmax, w);
@@ -238,6 +245,7 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
     f->rt_cmax = std::max((f->ncf + f->rt_wcap - 1) / f->rt_wcap,
                           std::min(f->ncf, (int)std::ceil(1.5 * row_mean / (0.94 * kFB)) + 2));
     A(falloc(&f->rtiles, (size_t)f->ncf * f->rt_cmax));
+    A(falloc(&f->btiles, (size_t)f->ncf * f->rt_cmax));
   }
   A(falloc(&f->snap, n));
   A(falloc(&f->omega_f, n));
@@ -289,6 +297,7 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   launch_fill_int(st, (long)cells * 9, f->cnt9[1], 0);
   launch_fill_int(st, cells, f->newcnt, 0);
   launch_fill_int(st, 1, f->ovf_count, 0);
+  launch_fill_int(st, 2, reinterpret_cast<int *>(f->step_dev), 0);  // the 64-bit step counter
   if (coef) {
     launch_fill_double(st, 3L * R, f->racc, 0.0);
     launch_fill_double(st, 2L * R, f->zlast, 0.0);
@@ -304,14 +313,17 @@ int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, 
 
 void fast_destroy(FastPath *f) {
   if (!f) return;
-  if (f->sg_exec) hipGraphExecDestroy(f->sg_exec);
-  if (f->sg_graph) hipGraphDestroy(f->sg_graph);
+  for (auto &row : f->sg)
+    for (auto &g : row) {
+      if (g.exec) hipGraphExecDestroy(g.exec);
+      if (g.graph) hipGraphDestroy(g.graph);
+    }
   void *b[] = {f->key,       f->slot,      f->count,     f->start,   f->tmp,       f->sorted,    f->blocksum,
                f->snap,      f->omega_f,   f->rstate[0], f->rstate[1], f->rid[0],  f->rid[1],    f->rnat[0],
                f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mvb[0], f->mvb[1],
                f->cnt9[0],   f->cnt9[1],   f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
                f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc,      f->ovf_list,  f->ovf_count,
-               f->lstart,    f->rtiles};
+               f->lstart,    f->rtiles,    f->btiles};
   for (void *p : b) hipFree(p);
   delete f;
 }
@@ -382,13 +394,18 @@ static void build(FastPath *f, hipStream_t st, const DevParams &P, const double 
     fk_keys<4><<<nb, 256, 0, st>>>(P, L, stage, f->ncf, f->key, f->slot, f->count);
   else
     fk_keys<3><<<nb, 256, 0, st>>>(P, L, stage, f->ncf, f->key, f->slot, f->count);
-  fk_scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum);
-  fk_scan_add<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum, f->nsb, f->ovf_count);
+  scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum);
+  scan_add<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum, f->nsb, f->ovf_count);
   fk_scatter<<<nb, 256, 0, st>>>(n, f->key, f->slot, start, f->tmp);
   if (P.s == 4)
     fk_rank<4><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->cpr, f->H, f->key, start, f->tmp, f->sorted, snap, omega);
   else
     fk_rank<3><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->cpr, f->H, f->key, start, f->tmp, f->sorted, snap, nullptr);
+}
+
+// row tiles of the rebuilt cell list (single systems), from the starts build() left in f->start
+static void rebuilt_row_tiles(FastPath *f, hipStream_t st) {
+  if (f->btiles) fk_row_tiles<<<(f->ncf + 3) / 4, 256, 0, st>>>(f->start, f->ncf, f->rt_wcap, f->rt_cmax, f->btiles);
 }
 
 static PairBufs rebuilt_bufs(const FastPath *f, const double *stage) {
@@ -408,6 +425,9 @@ static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, 
   if (MODE == 3 && f->rtiles) {  // the resident order's row tiles, rebuilt every step
     a.tiles = f->rtiles;
     a.ntiles = f->ncf * f->rt_cmax;
+  } else if (MODE != 3 && f->btiles) {  // the rebuilt cell list's row tiles (rebuilt_row_tiles)
+    a.tiles = f->btiles;
+    a.ntiles = f->ncf * f->rt_cmax;
   }
   fk_pairs<S, SRK, MODE><<<tile_grid(a.ntiles), kFB, 0, st>>>(P, a, B);
   // the resident step (MODE 3) runs the overflow work inside its first scan launch (fr_scan_overflow)
@@ -418,6 +438,7 @@ int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stag
              double *k_next, double *compl_out) {
   const bool srk = P.higher_order && compl_out && P.s == 4;
   build(f, st, P, stage, f->start, f->snap, srk ? f->omega_f : nullptr);
+  rebuilt_row_tiles(f, st);
   PairBufs B = rebuilt_bufs(f, stage);
   B.nat = nat;
   B.k_next = k_next;
@@ -591,68 +612,98 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
 }
 
 // ---------------------------------------------------------------------------
-// graph-replayed fused steps of a single system: kGraphSteps steps (even, so that the double
-// buffers end where they started) captured once, each advancing the device step counter in its
-// fr_move and reading it as its Philox step in its fr_scatter, then replayed; launch gaps
-// between the five kernels of a step vanish
+// graph-replayed fused steps of a single system.  Each captured step advances the device step
+// counter in its fr_far_bucket and reads it as its Philox step in its fr_scatter, so one graph
+// serves every replay.  Graphs of kGraphSteps steps (even: the double buffers end where they
+// started) carry the bulk of a run, graphs of one step the remainder; a graph is replayed only
+// from the buffer parity it was captured at.  Capture records launches without running them, so
+// the host bookkeeping (buffer parity, move flags) is put back after every capture and advanced
+// by each replay instead.
 static constexpr int kGraphSteps = 10;
 
 __global__ void fs_set_counter(unsigned long long *c, unsigned long long v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *c = v;
 }
 
-static void drop_step_graph(FastPath *f) {
-  if (f->sg_exec) hipGraphExecDestroy(f->sg_exec);
-  if (f->sg_graph) hipGraphDestroy(f->sg_graph);
-  f->sg_exec = nullptr;
-  f->sg_graph = nullptr;
+static void drop_step_graphs(FastPath *f) {
+  for (auto &row : f->sg)
+    for (auto &g : row) {
+      if (g.exec) hipGraphExecDestroy(g.exec);
+      if (g.graph) hipGraphDestroy(g.graph);
+      g.exec = nullptr;
+      g.graph = nullptr;
+    }
 }
 
-long fast_run_graph(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat,
-                    double *compl_out, double dt, double D_phi, double friction, uint64_t seed, uint64_t step,
-                    long nsteps) {
-  const bool do_srk = srk && P.s == 4;
-  const long reps = nsteps / kGraphSteps;
-  // only from the resident order in the form this stepper needs, and not while the kernel timer
-  // records events around the pair kernel
-  if (reps < 1 || f->timer || !f->resident || (nat != nullptr) != f->resident_nat || (do_srk && !f->romega_valid))
-    return 0;
-  const bool same = f->sg_exec && f->sg_srk == (int)do_srk && f->sg_cur == fast_buffer_parity(f) &&
-                    f->sg_gen == f->gen &&
-                    f->sg_higher == P.higher_order && f->sg_dt == dt && f->sg_dphi == D_phi &&
-                    f->sg_friction == friction && f->sg_seed == seed && f->sg_state == state && f->sg_nat == nat &&
-                    f->sg_compl == compl_out;
-  // the first step's moves come from the step before (or are computed here), never from a
-  // captured fr_move: a replay after a step that left its moves would count them twice
+FastHostMark fast_host_mark(const FastPath *f) {
+  return FastHostMark{f->resident, f->resident_nat, f->romega_valid, f->mv_ready, f->mv_srk,
+                      f->cur,      f->mvc,          f->mv_dt,        f->mv_v0};
+}
+
+void fast_restore_host_mark(FastPath *f, const FastHostMark &m) {
+  f->resident = m.resident;
+  f->resident_nat = m.resident_nat;
+  f->romega_valid = m.romega_valid;
+  f->mv_ready = m.mv_ready;
+  f->mv_srk = m.mv_srk;
+  f->cur = m.cur;
+  f->mvc = m.mvc;
+  f->mv_dt = m.mv_dt;
+  f->mv_v0 = m.mv_v0;
+}
+
+// one replayed (or skipped-over) step moves both double buffers
+static void flip_parity(FastPath *f) {
+  f->cur = 1 - f->cur;
+  f->mvc = 1 - f->mvc;
+}
+
+bool fast_graph_ready(const FastPath *f, const double *nat, int srk) {
+  return f && !f->timer && f->resident && (nat != nullptr) == f->resident_nat && (!srk || f->romega_valid);
+}
+
+// records `steps` fused steps from the current parity into g (host bookkeeping restored after)
+static int capture_steps(FastPath *f, hipStream_t st, const DevParams &P, bool do_srk, const double *state,
+                         const double *nat, double *compl_out, const StepArgs &U, int steps,
+                         FastPath::StepGraph &g) {
+  const FastHostMark m = fast_host_mark(f);
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) return ffail(std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  int rc = SPSS_OK;
+  for (int k = 0; k < steps && rc == SPSS_OK; ++k) rc = step_impl(f, st, P, do_srk, state, nat, compl_out, U);
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(st, &graph);
+  fast_restore_host_mark(f, m);  // nothing ran
+  if (rc != SPSS_OK || e != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    if (rc == SPSS_OK) return ffail(std::string("capture of the step graph failed: ") + hipGetErrorString(e));
+    return rc;
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(graph);
+    return ffail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  g.graph = graph;
+  g.exec = exec;
+  ++f->sg_captures;
+  return SPSS_OK;
+}
+
+// the moves of the first step in place (never from a captured fr_move: a replay after a step that
+// left its moves would count them twice), and the graphs dropped when the key changed
+static StepArgs graph_setup(FastPath *f, hipStream_t st, const DevParams &P, bool do_srk, const double *state,
+                            const double *nat, double *compl_out, double dt, double D_phi, double friction,
+                            uint64_t seed) {
   ensure_moves_of(f, st, P, do_srk, step_args(dt, D_phi, friction, 3, nullptr, seed, 0));
+  const bool same = f->sg_srk == (int)do_srk && f->sg_gen == f->gen && f->sg_higher == P.higher_order &&
+                    f->sg_dt == dt && f->sg_dphi == D_phi && f->sg_friction == friction && f->sg_seed == seed &&
+                    f->sg_state == state && f->sg_nat == nat && f->sg_compl == compl_out;
   if (!same) {
-    drop_step_graph(f);
-    StepArgs U = step_args(dt, D_phi, friction, 3, nullptr, seed, 0);
-    U.step_dev = f->step_dev;
-    U.step_inc = f->step_dev;  // each step's fr_move advances the counter before its fr_scatter
-    if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return 0;
-    const int cur0 = f->cur;
-    int rc = SPSS_OK;
-    for (int k = 0; k < kGraphSteps && rc == SPSS_OK; ++k) {
-      rc = step_impl(f, st, P, do_srk, state, nat, compl_out, U);
-    }
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(st, &g);
-    if (rc != SPSS_OK || e != hipSuccess || f->cur != cur0) {
-      if (g) hipGraphDestroy(g);
-      (void)hipGetLastError();
-      if (rc == SPSS_OK) g_fast_err = std::string("capture of the step graph failed: ") + hipGetErrorString(e);
-      return -1;
-    }
-    f->sg_graph = g;
-    const hipError_t ei = hipGraphInstantiate(&f->sg_exec, g, nullptr, nullptr, 0);
-    if (ei != hipSuccess) {
-      drop_step_graph(f);
-      g_fast_err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
-      return -1;
-    }
+    drop_step_graphs(f);
     f->sg_srk = do_srk;
-    f->sg_cur = fast_buffer_parity(f);
     f->sg_gen = f->gen;
     f->sg_higher = P.higher_order;
     f->sg_dt = dt;
@@ -663,17 +714,91 @@ long fast_run_graph(FastPath *f, hipStream_t st, const DevParams &P, int srk, co
     f->sg_nat = nat;
     f->sg_compl = compl_out;
   }
-  fs_set_counter<<<1, 64, 0, st>>>(f->step_dev, (unsigned long long)step - 1ull);  // advanced before use
-  for (long r = 0; r < reps; ++r) {
-    const hipError_t el = hipGraphLaunch(f->sg_exec, st);
-    if (el != hipSuccess) {
-      g_fast_err = std::string("hipGraphLaunch: ") + hipGetErrorString(el);
-      return -1;
+  StepArgs U = step_args(dt, D_phi, friction, 3, nullptr, seed, 0);
+  U.step_dev = f->step_dev;
+  U.step_inc = f->step_dev;
+  return U;
+}
+
+static int ensure_graph(FastPath *f, hipStream_t st, const DevParams &P, bool do_srk, const double *state,
+                        const double *nat, double *compl_out, const StepArgs &U, int size) {
+  FastPath::StepGraph &g = f->sg[size == 1 ? 1 : 0][fast_buffer_parity(f)];
+  if (g.exec) return SPSS_OK;
+  return capture_steps(f, st, P, do_srk, state, nat, compl_out, U, size, g);
+}
+
+int fast_enter(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat) {
+  const bool do_srk = srk && P.s == 4;
+  if (fast_graph_ready(f, nat, do_srk)) return SPSS_OK;
+  if (f->timer) {
+    g_fast_err = "the step graphs are not captured while the kernel timer is on";
+    return SPSS_ERR_INVALID;
+  }
+  resident_enter(f, st, P, state, nat, do_srk);
+  f->romega_valid = do_srk;
+  FCHK(hipGetLastError());
+  return SPSS_OK;
+}
+
+int fast_prepare_graphs(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state,
+                        const double *nat, double *compl_out, double dt, double D_phi, double friction,
+                        uint64_t seed) {
+  const bool do_srk = srk && P.s == 4;
+  if (!fast_graph_ready(f, nat, do_srk)) {
+    g_fast_err = "the step graphs need the resident order in this stepper's form (fast_enter)";
+    return SPSS_ERR_INVALID;
+  }
+  const StepArgs U = graph_setup(f, st, P, do_srk, state, nat, compl_out, dt, D_phi, friction, seed);
+  // from this parity and from the other one (the host is set there for the capture only)
+  for (int side = 0; side < 2; ++side) {
+    for (int size : {kGraphSteps, 1}) {
+      const int rc = ensure_graph(f, st, P, do_srk, state, nat, compl_out, U, size);
+      if (rc != SPSS_OK) {
+        if (side) flip_parity(f);
+        return rc;
+      }
     }
+    flip_parity(f);
+  }
+  return SPSS_OK;
+}
+
+long fast_run_graph(FastPath *f, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat,
+                    double *compl_out, double dt, double D_phi, double friction, uint64_t seed, uint64_t step,
+                    long nsteps) {
+  const bool do_srk = srk && P.s == 4;
+  // only from the resident order in the form this stepper needs, and not while the kernel timer
+  // records events around the pair kernel
+  if (nsteps < 1 || !fast_graph_ready(f, nat, do_srk)) return 0;
+  const StepArgs U = graph_setup(f, st, P, do_srk, state, nat, compl_out, dt, D_phi, friction, seed);
+  const long reps = nsteps / kGraphSteps, rest = nsteps % kGraphSteps;
+  // every graph this run replays, captured before anything is launched
+  int rc = SPSS_OK;
+  if (reps > 0) rc = ensure_graph(f, st, P, do_srk, state, nat, compl_out, U, kGraphSteps);
+  for (int side = 0; side < 2 && side < rest && rc == SPSS_OK; ++side) {
+    if (side) flip_parity(f);
+    rc = ensure_graph(f, st, P, do_srk, state, nat, compl_out, U, 1);
+    if (side) flip_parity(f);
+  }
+  if (rc != SPSS_OK) return -1;
+  auto failed = [&](const char *what, hipError_t e) -> long {
+    g_fast_err = std::string(what) + ": " + hipGetErrorString(e);
+    return -1;
+  };
+  fs_set_counter<<<1, 64, 0, st>>>(f->step_dev, (unsigned long long)step - 1ull);  // advanced before use
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return failed("fs_set_counter", e);
+  for (long r = 0; r < reps; ++r)
+    if ((e = hipGraphLaunch(f->sg[0][fast_buffer_parity(f)].exec, st)) != hipSuccess) return failed("hipGraphLaunch", e);
+  for (long r = 0; r < rest; ++r) {
+    if ((e = hipGraphLaunch(f->sg[1][fast_buffer_parity(f)].exec, st)) != hipSuccess) return failed("hipGraphLaunch", e);
+    flip_parity(f);
   }
   f->romega_valid = do_srk;
-  return reps * kGraphSteps;
+  return nsteps;
 }
+
+long fast_graph_captures(const FastPath *f) { return f ? f->sg_captures : 0; }
 
 // ---------------------------------------------------------------------------
 // replica batches: per-replica coefficients, Philox step from the device counter
@@ -730,6 +855,7 @@ int fast_replica_count(const FastPath *f) { return f->R; }
 
 int fast_pairs_count(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, int *count) {
   build(f, st, P, stage, f->start, f->snap, nullptr);
+  rebuilt_row_tiles(f, st);
   PairBufs B = rebuilt_bufs(f, stage);
   B.pcount = count;
   if (P.s == 4)

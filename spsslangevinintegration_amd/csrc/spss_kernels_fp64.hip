@@ -8,6 +8,7 @@
 
 #include "spss_device.hpp"
 #include "spss_launch.hpp"
+#include "spss_scan.hpp"
 
 namespace spss {
 
@@ -82,33 +83,6 @@ __global__ void k_cell_keys(DevParams P, const double *__restrict__ stage, int n
   int c = cell_coord(yw, ncy) * ncx + cell_coord(xw, ncx);
   key[i] = c;
   slot[i] = atomicAdd(count + c, 1);
-}
-
-// exclusive scan of count[0..m) into start[0..m], then count[] = 0 for the next build.
-// One workgroup of 1024 threads; each thread scans a contiguous chunk.
-__global__ __launch_bounds__(1024) void k_scan_counts(int *__restrict__ count, int *__restrict__ start, int m) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int chunk = (m + 1023) / 1024;
-  const int b = t * chunk, e = min(m, b + chunk);
-  int s = 0;
-  for (int k = b; k < e; ++k) s += count[k];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = (t >= off) ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = part[t] - s;
-  for (int k = b; k < e; ++k) {
-    int c = count[k];
-    start[k] = run;
-    run += c;
-    count[k] = 0;
-  }
-  if (t == 1023) start[m] = part[1023];
 }
 
 __global__ void k_scatter(int n, const int *__restrict__ key, const int *__restrict__ slot,
@@ -662,7 +636,7 @@ void launch_all_pairs(hipStream_t st, const DevParams &P, const double *stage, c
 void launch_cell_build(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy, CellBuffers &cb) {
   const int m = ncx * ncy;
   SPSS_DISPATCH_S(P.s, k_cell_keys<SS><<<dim3(nblk(P.n)), dim3(kBlock), 0, st>>>(P, stage, ncx, ncy, cb.key, cb.slot, cb.count));
-  k_scan_counts<<<dim3(1), dim3(1024), 0, st>>>(cb.count, cb.start, m);
+  launch_scan(st, cb.count, cb.start, m, cb.blocksum);
   k_scatter<<<dim3(nblk(P.n)), dim3(kBlock), 0, st>>>(P.n, cb.key, cb.slot, cb.start, cb.tmp);
   k_stable_rank<<<dim3(nblk(P.n)), dim3(kBlock), 0, st>>>(P.n, cb.key, cb.start, cb.tmp, cb.sorted);
 }
@@ -672,8 +646,10 @@ void launch_linked_list(hipStream_t st, const DevParams &P, const double *stage,
   SPSS_DISPATCH_S(P.s, k_linked_list<SS><<<dim3(nblk(P.n)), dim3(kBlock), 0, st>>>(P, stage, ncx, ncy, cb.start, cb.sorted, nat, k_next, compl_out));
 }
 
-void launch_scan(hipStream_t st, int *count, int *start, int m) {
-  k_scan_counts<<<dim3(1), dim3(1024), 0, st>>>(count, start, m);
+void launch_scan(hipStream_t st, int *count, int *start, int m, int *blocksum) {
+  const int nb = scan_blocks(m);
+  scan_local<<<dim3(nb), dim3(256), 0, st>>>(count, nullptr, m, start, blocksum);
+  scan_add<<<dim3(nb), dim3(256), 0, st>>>(count, nullptr, m, start, blocksum, nb, nullptr);
 }
 
 void launch_verlet_count_all(hipStream_t st, const DevParams &P, const double *stage, double rmax2, int *vcount) {

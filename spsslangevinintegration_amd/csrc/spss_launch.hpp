@@ -91,6 +91,7 @@ struct CellBuffers {
   int *start = nullptr;   // [cells + 1] exclusive scan of count
   int *tmp = nullptr;     // [n] unstable bucket fill
   int *sorted = nullptr;  // [n] particle indices, cell-major, ascending index inside a cell
+  int *blocksum = nullptr;  // [scan_blocks(cells) + 1] totals of the scan's workgroups
   int cells = 0;
 };
 
@@ -101,7 +102,9 @@ void launch_all_pairs(hipStream_t st, const DevParams &P, const double *stage, c
 void launch_cell_build(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy, CellBuffers &cb);
 void launch_linked_list(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy,
                         const CellBuffers &cb, const double *nat, double *k_next, double *compl_out);
-void launch_scan(hipStream_t st, int *count, int *start, int m);
+// exclusive scan of count[0..m) into start[0..m]; count is left zeroed; blocksum: scratch of
+// scan_blocks(m) + 1 ints (spss_scan.hpp)
+void launch_scan(hipStream_t st, int *count, int *start, int m, int *blocksum);
 void launch_verlet_count_all(hipStream_t st, const DevParams &P, const double *stage, double rmax2, int *vcount);
 void launch_verlet_fill_all(hipStream_t st, const DevParams &P, const double *stage, double rmax2,
                             const int *vstart, int *vlist);

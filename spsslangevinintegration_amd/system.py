@@ -91,6 +91,16 @@ class ParticleSystem:
     def run(self, stepper, method, nsteps, dt, noise_mode=A.NOISE_NONE):
         A.check(self.lib.spss_run(self.h, stepper, method, int(nsteps), dt, noise_mode))
 
+    def prepare_run(self, stepper, method, dt, noise_mode=A.NOISE_NONE):
+        """capture the step graphs later runs of this stepper replay (FP32C Euler/SRK, Philox)"""
+        A.check(self.lib.spss_prepare_run(self.h, stepper, method, dt, noise_mode))
+
+    def run_stats(self, reset=False):
+        """(eager steps, graph-replayed steps, graphs captured) since the last reset"""
+        e, r, c = C.c_long(), C.c_long(), C.c_long()
+        A.check(self.lib.spss_run_stats(self.h, C.byref(e), C.byref(r), C.byref(c), int(bool(reset))))
+        return e.value, r.value, c.value
+
     @property
     def step(self):
         v = C.c_long()
