@@ -68,11 +68,14 @@ def step_bytes(n, cells, s=4):
 IMPL_STEP_BYTES = 36.0 + 132.0 + 68.0 * 62500.0 / 1048576.0
 
 
-def headline_system(spss, n=1 << 20, precision=None, subdiv=1, seed=0x5EED0003):
+def headline_system(spss, n=1 << 20, precision=None, subdiv=1, seed=0x5EED0003, verlet=False):
+    """the large configuration's system; verlet=True: its Verlet lists (exact mode, skin 0.2 rho)"""
     m, sigma, D = 0.5, 1.5, 0.02
     kw = dict(model=4, n=n, v0=0.005, xi=1.0 / m, sigma=sigma / m, rho=0.004, alpha=0.0, D_phi=D / m ** 2,
               natfreq_local=1, seed=seed, cell_subdivision=subdiv,
               precision=spss.PRECISION_FP32C if precision is None else precision)
+    if verlet:
+        kw.update(verlet_mode=spss.VERLET_EXACT, verlet_skin=0.2 * 0.004)
     rng = np.random.default_rng(seed)
     st = np.zeros((n, 4))
     st[:, 0] = rng.random(n)
@@ -140,10 +143,12 @@ def ensemble_config(R=512, seed=0x5EED0004):
     return groups
 
 
-def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0.03, every=10):
+def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0.03, every=10, graph=True,
+                     tune_it=True):
     """512-replica ensemble: batched run (autotuned cell size and tiles, graph replays) against the
     plain loop (the same kernels at the same cell size through one single-system handle per
-    replica, one after another)."""
+    replica, one after another).  graph=False, tune_it=False: eager launches, default tiles (a
+    kernel trace of the batch's kernels)."""
     groups = ensemble_config()
     total_n = sum(int(g["ns"].sum()) for g in groups.values())
     batches, tune = {}, {}
@@ -151,19 +156,23 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
         b = spss.ReplicaBatch(model, g["ns"], g["xi"], g["sigma"], g["D"], g["seeds"], v0=v0, rho=rho)
         b.set_natural_frequencies(np.concatenate(g["nats"]))
         b.set_state(np.concatenate(g["states"]))
-        ctable, k = b.autotune_cells(dt, probe_steps=10)
-        table, chosen = b.autotune(dt, probe_steps=10)
-        tune[model] = {"cell_subdivision_ms_per_step": [round(float(t), 4) for t in ctable], "cell_subdivision": k,
-                       "ms_per_step": [round(float(t), 4) for t in table], "chosen": int(chosen)}
+        if tune_it:
+            ctable, k = b.autotune_cells(dt, probe_steps=10)
+            table, chosen = b.autotune(dt, probe_steps=10)
+            tune[model] = {"cell_subdivision_ms_per_step": [round(float(t), 4) for t in ctable],
+                           "cell_subdivision": k, "ms_per_step": [round(float(t), 4) for t in table],
+                           "chosen": int(chosen)}
         batches[model] = b
     # sample for the plain-loop comparison: plain_steps steps of every replica, both ways
     t0 = time.perf_counter()
     for b in batches.values():
-        b.run(plain_steps, dt, observe_every=every, observe_from=0, graph=True)
+        b.run(plain_steps, dt, observe_every=every, observe_from=0, graph=graph)
     t_batch_sample = time.perf_counter() - t0
     zb = {m: b.order_stats() for m, b in batches.items()}
     t_plain, max_err = 0.0, 0.0
     for model, g in groups.items():
+        if plain_steps <= 0:
+            break
         for k in range(len(g["ns"])):
             h = spss.ParticleSystem(model=model, n=int(g["ns"][k]), v0=v0, xi=g["xi"][k], sigma=g["sigma"][k],
                                     rho=rho, D_phi=g["D"][k], natfreq_local=1, precision=spss.PRECISION_FP32C,
@@ -186,7 +195,7 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
         b.set_state(np.concatenate(groups[model]["states"]))
     t0 = time.perf_counter()
     for b in batches.values():
-        b.run(steps, dt, observe_every=every, observe_from=steps // 2, graph=True)
+        b.run(steps, dt, observe_every=every, observe_from=steps // 2, graph=graph)
     t_batch = time.perf_counter() - t0
     R_mean = np.zeros(sum(len(g["ns"]) for g in groups.values()))
     R_var = np.zeros_like(R_mean)
@@ -200,8 +209,9 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
         "particle_steps_per_s": total_n * steps / t_batch,
         "sample_steps": plain_steps,
         "sample_batched_particle_steps_per_s": total_n * plain_steps / t_batch_sample,
-        "sample_plain_particle_steps_per_s": total_n * plain_steps / t_plain,
-        "speedup_vs_plain_loop": t_plain / t_batch_sample,
+        "sample_plain_particle_steps_per_s": total_n * plain_steps / t_plain if t_plain else None,
+        "speedup_vs_plain_loop": t_plain / t_batch_sample if t_plain else None,
+        "graph_replays": graph,
         "order_param_max_abs_err_vs_plain": max_err,
         "mean_R_over_replicas": float(R_mean.mean()),
         "autotune": tune,
@@ -283,6 +293,82 @@ def cpu_baseline(single_steps=3, all_steps=20):
     return line
 
 
+def verlet_srk_rate(spss, steps=200, warmup=20):
+    """the large configuration as specified: N = 1,048,576, Verlet lists with skin 0.2 rho rebuilt on
+    the largest displacement (decided on the device), stochastic Runge-Kutta dt = 0.002, FP32 pair
+    work and FP64 stage combination; steps replayed from graphs after a warm-up"""
+    g, kw, _, _ = headline_system(spss, verlet=True)
+    method = spss.METHOD_LINKED_LIST_VERLET
+    g.run(spss.STEPPER_SRK, method, warmup, 0.002, spss.NOISE_PHILOX)
+    g.prepare_run(spss.STEPPER_SRK, method, 0.002, spss.NOISE_PHILOX)
+    g.synchronize()
+    g.run_stats(reset=True)
+    t0 = time.perf_counter()
+    g.run(spss.STEPPER_SRK, method, steps, 0.002, spss.NOISE_PHILOX)
+    g.synchronize()
+    sec = time.perf_counter() - t0
+    eager, replayed, captured = g.run_stats(reset=True)
+    g.close()
+    return {"particle_steps_per_s": kw["n"] * steps / sec, "ms_per_step": sec / steps * 1e3, "steps": steps,
+            "graph_replayed": replayed, "eager": eager, "skin": 0.2 * 0.004, "stepper": "SRK"}
+
+
+def screen_at_scale(spss, n_train=20000, n_screen=100000, n_lines=16, steps=2000, seed=0x5C4EE):
+    """the learned sweep screen at its stated scale: a network trained on the GPU (the library's
+    forward/backward/Adam kernels) on n_train batched-run results of random points (sigma, D, m, v0,
+    rho, N; N log-uniform in [2048, 4096]), then n_screen points screened (simulated only where the
+    predicted |d mean|Z| / d sigma| > 0.5 or the predicted spread of |Z| > 0.05, scored against
+    those simulations), and n_lines sigma sweeps of 40 points located with and without the screen"""
+    from spsslangevinintegration_amd import screen as S
+    sim = dict(steps=steps, dt=0.005, observe_every=10, observe_from=steps // 2)
+    rng = np.random.default_rng(seed)
+    sc = spss.SweepScreen(seed=0x5EED0005, sim_kwargs=sim)
+    train = S.random_points(n_train, rng, N=(2048, 4096), seed0=0x5EED6000)
+    t0 = time.perf_counter()
+    y = sc.simulate(train)
+    t_sim = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = sc.fit(train, y, steps=8000, batch=256, lr=2e-3)
+    t_fit = time.perf_counter() - t0
+    pts = S.random_points(n_screen, np.random.default_rng(seed + 1), N=(2048, 4096), seed0=0x5EED7000)
+    t0 = time.perf_counter()
+    out = sc.screen(pts, slope_min=0.5, spread_max=0.05, simulate=True)
+    t_screen = time.perf_counter() - t0
+    sel = out["selected"]
+    err = out.get("error", np.zeros((0, 2)))
+    G = 40
+    sig = np.linspace(0.0, 3.0, G)
+    plain_sims = screened_sims = 0
+    worst, found, missed = 0.0, 0, 0
+    lr = np.random.default_rng(seed + 2)
+    for k in range(n_lines):
+        D, m = lr.uniform(0.0, 0.15), lr.uniform(0.3, 1.0)
+        v0, rho = lr.choice([0.005, 0.01, 0.02]), lr.choice([0.02, 0.03, 0.04])
+        N = int(2 ** lr.uniform(11.0, 12.0))
+        line = S.points(sig, D, m, v0, rho, N, 0x5EED8000 + 100 * k + np.arange(G, dtype=np.uint64))
+        before = sc.simulations
+        x_plain, _ = sc.plain_sweep(line)
+        plain_sims += sc.simulations - before
+        before = sc.simulations
+        x_scr, _, _, _ = sc.screened_sweep(line, window=6)
+        screened_sims += sc.simulations - before
+        if (x_plain is None) != (x_scr is None):
+            missed += 1
+        elif x_plain is not None:
+            found += 1
+            worst = max(worst, abs(x_scr - x_plain))
+    return {"training_points": int(n_train), "steps_per_simulation": steps,
+            "training_simulation_seconds": t_sim, "training_particle_steps_per_s": float(train["N"].sum()) * steps / t_sim,
+            "fit_seconds": t_fit, "adam_steps": 8000, "loss_first_last": [float(losses[:50].mean()), float(losses[-50:].mean())],
+            "screened_points": int(n_screen), "simulated": int(sel.size), "simulations_saved": int(n_screen - sel.size),
+            "screen_seconds": t_screen,
+            "mean_abs_error_on_simulated": [float(err[:, 0].mean()), float(err[:, 1].mean())] if sel.size else None,
+            "sweeps": {"lines": n_lines, "points_per_line": G, "plain_simulations": plain_sims,
+                       "screened_simulations": screened_sims, "crossings_found": found,
+                       "crossing_disagreements": missed, "max_abs_crossing_error": worst,
+                       "grid_step": float(sig[1] - sig[0])}}
+
+
 def fp64_step_rate(spss, steps=10, warmup=2):
     """the reference-precision (FP64, the reference's operation order) cell-list Euler step of the
     headline system: device counting sort and 9-cell pair loop every step"""
@@ -310,6 +396,11 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-verlet", action="store_true")
+    ap.add_argument("--no-screen", action="store_true")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run the batched ensemble alone (with --batch-eager: eager launches, default tiles)")
+    ap.add_argument("--batch-eager", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the sharding by slabs of cell rows is "
@@ -318,6 +409,12 @@ def main():
 
     import spsslangevinintegration_amd as spss
     from spsslangevinintegration_amd import _capi
+
+    if args.only_batch:
+        res = batched_ensemble(spss, steps=args.batch_steps, plain_steps=args.plain_steps,
+                               graph=not args.batch_eager, tune_it=not args.batch_eager)
+        print(json.dumps({"batched": res}), flush=True)
+        return
 
     # attainable HBM bandwidth of the build's own streaming copy kernel (1 GiB, read + write)
     copy_gbps = _capi.copy_bandwidth(1 << 30, 10)
@@ -346,6 +443,11 @@ def main():
     g2.close()
     print(f"[bench] {ms_step:.4f} ms/step ({replayed} replayed, {eager} eager, {captured} graphs captured in the "
           f"timed region), pair kernel {pair_ms * 1e3:.1f} us", file=sys.stderr, flush=True)
+    verlet = None
+    if not args.no_verlet:
+        print("[bench] verlet SRK (large configuration)", file=sys.stderr, flush=True)
+        verlet = verlet_srk_rate(spss)
+        print(f"[bench] verlet SRK {verlet['ms_per_step']:.4f} ms/step", file=sys.stderr, flush=True)
     fp64 = None
     if not args.no_fp64:
         print("[bench] fp64 cell-list step", file=sys.stderr, flush=True)
@@ -355,6 +457,12 @@ def main():
         print("[bench] batched ensemble", file=sys.stderr, flush=True)
         batched = batched_ensemble(spss, steps=args.batch_steps, plain_steps=args.plain_steps)
         print(f"[bench] batched {batched['speedup_vs_plain_loop']:.1f}x plain loop", file=sys.stderr, flush=True)
+    screen = None
+    if not args.no_screen:
+        print("[bench] learned sweep screen at scale", file=sys.stderr, flush=True)
+        screen = screen_at_scale(spss)
+        print(f"[bench] screen: {screen['simulated']} of {screen['screened_points']} simulated", file=sys.stderr,
+              flush=True)
     # secondary: the reference's deterministic production path, Boost.Odeint's dense-output
     # dopri5 (eps 1e-6) over the global-field RHS of its default run (v0 = 1, xi = 0.1, sigma = 5,
     # rho = 0.8, natural frequencies ~ N(0, 0.5)), observing every dt = 0.01 on the device
@@ -398,8 +506,10 @@ def main():
         "impl_step_bytes_frac": IMPL_STEP_BYTES * value / HBM_PEAK,
         "pair_interactions_per_s": pairs / n * value if pairs else None,
         "srk_particle_steps_per_s": n * srk_steps / wall2,
+        "verlet_srk": verlet,
         "fp64_cell_list": fp64,
         "batched": batched,
+        "screen": screen,
         "dopri5": dopri5,
     }
     g.close()

@@ -48,11 +48,12 @@ extern "C" {
 
 /* arithmetic of the pair sums */
 #define SPSS_PRECISION_FP64 0  /* FP64, the reference's operation order */
-#define SPSS_PRECISION_FP32C 1 /* linked-list method, models 3 and 4: FP32 distance tests on
-                                  cell-relative offsets (guard band decided in FP64, so the
-                                  neighbour sets equal the FP64 ones), coupling sums through the
-                                  neighbours' unit phasors in FP32 partial sums combined in FP64,
-                                  FP64 state and stage combination */
+#define SPSS_PRECISION_FP32C 1 /* models 3 and 4, the linked-list method (RHS and fused stochastic
+                                  steps) and exact-mode Verlet lists (fused stochastic steps): FP32
+                                  distance tests (guard band decided in FP64, so the neighbour sets
+                                  equal the FP64 ones), coupling sums through the neighbours' unit
+                                  phasors in FP32 partial sums combined in FP64, FP64 state and
+                                  stage combination */
 
 /* Verlet semantics */
 #define SPSS_VERLET_COMPAT 0 /* reference: r_max = rho + 0.05, candidates from the 9 cells */
@@ -166,6 +167,16 @@ int spss_set_step(spss_system *sys, long step);
 int spss_autotune_cells(spss_system *sys, int stepper, double dt, int probe_steps, double *table_ms, int table_cap,
                         int *chosen);
 int spss_get_cell_subdivision(spss_system *sys, int *k);
+/* all-pairs autotuner (no reference counterpart): times `probe_evals` RHS evaluations of the current
+ * state by EvaluateInteractionsWithAllPairs (ParticleSystemFirstOrderPtr.cpp:225-316) for each of
+ * the kernel's workgroup / j-tile shapes {64,64}, {64,256}, {128,128}, {128,512}, {256,256},
+ * {256,512} (particles per workgroup, j per LDS tile) and keeps the fastest for the handle's
+ * all-pairs evaluations; ms per evaluation into table_ms (at most table_cap entries), the chosen
+ * index into *chosen.  Every shape visits j in ascending order, so results do not depend on it. */
+int spss_autotune_all_pairs(spss_system *sys, int probe_evals, double *table_ms, int table_cap, int *chosen);
+int spss_get_all_pairs_config(spss_system *sys, int *block, int *jtile);
+/* selects shape `config` (the index into the list above) by hand */
+int spss_set_all_pairs_config(spss_system *sys, int config);
 
 /* CalculateMaxDisplacement (ParticleSystemFirstOrderPtr.cpp:836-861) on a host total-displacement
  * vector, and the Verlet bookkeeping it updates */

@@ -34,6 +34,7 @@ EXPORTS = [
     "spss_averaging_reset", "spss_averaging_accumulate", "spss_averaging_get",
     "spss_kernel_timer_start", "spss_kernel_timer_stop", "spss_device_copy_bandwidth",
     "spss_philox4x32_10", "spss_philox_normals", "spss_autotune_cells", "spss_get_cell_subdivision",
+    "spss_autotune_all_pairs", "spss_get_all_pairs_config", "spss_set_all_pairs_config",
     "spss_batch_create", "spss_batch_destroy", "spss_batch_set_state", "spss_batch_get_state",
     "spss_batch_set_natural_frequencies", "spss_batch_reset", "spss_batch_run", "spss_batch_order_stats",
     "spss_batch_get_tile_widths", "spss_batch_set_tile_widths", "spss_batch_autotune",
@@ -127,6 +128,9 @@ def load(path=None):
         "spss_philox_normals": (i, [C.c_uint64, C.c_uint64, C.c_uint32, l, l, vp, vp]),
         "spss_autotune_cells": (i, [vp, i, d, i, vp, i, C.POINTER(i)]),
         "spss_get_cell_subdivision": (i, [vp, C.POINTER(i)]),
+        "spss_autotune_all_pairs": (i, [vp, i, vp, i, C.POINTER(i)]),
+        "spss_get_all_pairs_config": (i, [vp, C.POINTER(i), C.POINTER(i)]),
+        "spss_set_all_pairs_config": (i, [vp, i]),
         "spss_batch_create": (i, [C.POINTER(SpssBatchParams), i, vp, C.POINTER(vp)]),
         "spss_batch_destroy": (i, [vp]),
         "spss_batch_set_state": (i, [vp, vp]),

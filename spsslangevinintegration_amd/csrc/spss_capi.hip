@@ -20,6 +20,7 @@
 #include "spss_fast.hpp"
 #include "spss_launch.hpp"
 #include "spss_scan.hpp"
+#include "spss_verlet_fast.hpp"
 
 using namespace spss;
 
@@ -81,9 +82,13 @@ struct spss_system {
   // steps run since the last spss_run_stats reset: launched one by one, and replayed from graphs
   long eager_steps = 0, replayed_steps = 0, captures0 = 0;
   FastPath *fast = nullptr;
-  // the fused FP32C steps advance a resident copy of the state kept in cell order; `state`
-  // is then behind it until the next sync_master
+  VerletFast *vfast = nullptr;  // FP32C Verlet lists (exact mode), created on the first such step
+  // the fused FP32C steps advance a resident copy of the state kept in cell order (of the cell
+  // list: kind 1, of the Verlet lists' last rebuild: kind 2); `state` is then behind it until
+  // the next sync_master
   bool master_stale = false;
+  int resident_kind = 0;
+  int ap_config = kAllPairsDefault;  // all-pairs workgroup and j-tile sizes (spss_autotune_all_pairs)
   KernelTimer *timer = nullptr;  // around the pair kernel of the RHS
   // Dormand-Prince 5(4) buffers (allocated on first use): two states and derivatives, k2..k6,
   // a stage state and a pinned host copy for observers
@@ -96,17 +101,32 @@ namespace {
 // brings `state` (original order) up to date with the resident copy of the fused steps
 int sync_master(spss_system *h) {
   if (!h->master_stale) return SPSS_OK;
-  int rc = fast_resident_sync(h->fast, h->stream, h->state);
-  if (rc != SPSS_OK) return fail(rc, fast_last_error());
+  if (h->resident_kind == 2) {
+    int rc = vf_sync(h->vfast, h->stream, h->state);
+    if (rc != SPSS_OK) return fail(rc, vf_last_error());
+  } else {
+    int rc = fast_resident_sync(h->fast, h->stream, h->state);
+    if (rc == SPSS_OK) rc = fast_check(h->fast, h->stream);
+    if (rc != SPSS_OK) return fail(rc, fast_last_error());
+  }
   h->master_stale = false;
   return SPSS_OK;
 }
 
-// before `state` changes by any other means than a fused step: the resident copy is dropped
+// before `state` changes by any other means than a fused step: the resident copies are dropped
 int master_write(spss_system *h, bool keep_values) {
   if (keep_values) TRY(sync_master(h));
   h->master_stale = false;
+  h->resident_kind = 0;
   if (h->fast) fast_resident_drop(h->fast);
+  if (h->vfast) vf_drop(h->vfast);
+  return SPSS_OK;
+}
+
+// a fused step of the other kind follows: the resident copy of the current kind goes back into
+// the master state first
+int switch_resident(spss_system *h, int kind) {
+  if (h->resident_kind != 0 && h->resident_kind != kind) TRY(master_write(h, true));
   return SPSS_OK;
 }
 
@@ -170,7 +190,8 @@ int check_method(const spss_system *h, int m) {
     return fail(SPSS_ERR_UNSUPPORTED,
                 "model 6 has no global-field or Verlet branch (ParticleSystemSecondOrderPtr.cpp:393-400 is a stub)");
   if (h->p.precision == SPSS_PRECISION_FP32C && m != SPSS_METHOD_LINKED_LIST)
-    return fail(SPSS_ERR_UNSUPPORTED, "FP32C precision is implemented for the linked-list method");
+    return fail(SPSS_ERR_UNSUPPORTED, "FP32C precision evaluates a single RHS with the linked-list method only "
+                                      "(Verlet lists run as fused stochastic steps)");
   if (h->p.precision == SPSS_PRECISION_FP32C && h->s == 6)
     return fail(SPSS_ERR_UNSUPPORTED, "FP32C precision is implemented for models 3 and 4");
   return SPSS_OK;
@@ -224,7 +245,7 @@ int rhs_device(spss_system *h, int method, const double *d_state, const double *
       launch_global(st, h->P, h->stage, h->partial, h->scal, nat, d_knext, compl_out);
       break;
     case SPSS_METHOD_ALL_PAIRS:
-      launch_all_pairs(st, h->P, h->stage, nat, d_knext, compl_out);
+      launch_all_pairs(st, h->P, h->stage, nat, d_knext, compl_out, h->ap_config);
       break;
     case SPSS_METHOD_LINKED_LIST:
       if (h->p.precision == SPSS_PRECISION_FP32C) {
@@ -319,6 +340,7 @@ int rk4_step(spss_system *h, int method, double dt) {
 
 // FP32C linked list: the fused step of the fast path on its resident copy of the state
 int fast_fused_step(spss_system *h, int srk, double dt, int noise_mode, const double *noise) {
+  TRY(switch_resident(h, 1));
   int kmode;
   const double *kptr;
   TRY(noise_source(h, noise_mode, noise, (size_t)(srk ? 2 : 1) * h->n, &kmode, &kptr));
@@ -328,6 +350,7 @@ int fast_fused_step(spss_system *h, int srk, double dt, int noise_mode, const do
                      (uint64_t)h->step);
   if (rc != SPSS_OK) return fail(rc, fast_last_error());
   h->master_stale = true;
+  h->resident_kind = 1;
   ++h->step;
   ++h->eager_steps;
   return SPSS_OK;
@@ -338,8 +361,44 @@ bool use_fused(const spss_system *h, int method) {
          h->s != 6;
 }
 
+// FP32C with Verlet lists: fused stochastic steps on the Verlet path's resident copy
+bool use_fused_verlet(const spss_system *h, int method) {
+  const int m = resolve_method(h, method);
+  return h->p.precision == SPSS_PRECISION_FP32C && (m == SPSS_METHOD_VERLET || m == SPSS_METHOD_LINKED_LIST_VERLET) &&
+         h->s != 6;
+}
+
+int ensure_vfast(spss_system *h) {
+  if (h->vfast) return SPSS_OK;
+  if (h->p.verlet_mode != SPSS_VERLET_EXACT)
+    return fail(SPSS_ERR_UNSUPPORTED, "FP32C Verlet lists are implemented in the exact mode (SPSS_VERLET_EXACT); "
+                                      "the reference-compatible mode runs in FP64");
+  int rc = vf_create(&h->vfast, h->stream, h->P, h->p.rho, h->r_max);
+  if (rc != SPSS_OK) return fail(rc, vf_last_error());
+  return SPSS_OK;
+}
+
+int verlet_fused_step(spss_system *h, int srk, double dt, int noise_mode, const double *noise) {
+  TRY(ensure_vfast(h));
+  TRY(switch_resident(h, 2));
+  int kmode;
+  const double *kptr;
+  TRY(noise_source(h, noise_mode, noise, (size_t)(srk ? 2 : 1) * h->n, &kmode, &kptr));
+  h->P.higher_order = srk ? 1 : 0;
+  int rc = vf_step(h->vfast, h->stream, h->P, srk, h->state, h->have_nat ? h->nat : nullptr,
+                   srk ? h->compl_f : nullptr, dt, h->p.D_phi, h->p.xi, kmode, kptr, h->p.seed, (uint64_t)h->step);
+  if (rc != SPSS_OK) return fail(rc, vf_last_error());
+  h->master_stale = true;
+  h->resident_kind = 2;
+  h->uses_verlet = true;
+  ++h->step;
+  ++h->eager_steps;
+  return SPSS_OK;
+}
+
 int euler_step(spss_system *h, int method, double dt, int noise_mode, const double *noise) {
   if (use_fused(h, method)) return fast_fused_step(h, 0, dt, noise_mode, noise);
+  if (use_fused_verlet(h, method)) return verlet_fused_step(h, 0, dt, noise_mode, noise);
   TRY(master_write(h, true));
   const size_t m = (size_t)h->s * h->n;
   launch_fill_double(h->stream, (long)m, h->deriv, 0.0);  // std::fill(derivative, 0), :43
@@ -362,6 +421,10 @@ int srk_step(spss_system *h, int method, double dt, int noise_mode, const double
   if (use_fused(h, method)) {
     h->higher_order = true;  // StochasticRungeKuttaStepperPtr.hpp:44
     return fast_fused_step(h, 1, dt, noise_mode, noise);
+  }
+  if (use_fused_verlet(h, method)) {
+    h->higher_order = true;
+    return verlet_fused_step(h, 1, dt, noise_mode, noise);
   }
   TRY(master_write(h, true));
   const size_t m = (size_t)h->s * h->n;
@@ -508,6 +571,7 @@ int spss_destroy(spss_system *h) {
   if (!h) return SPSS_OK;
   if (h->stream) hipStreamSynchronize(h->stream);
   fast_destroy(h->fast);
+  vf_destroy(h->vfast);
   delete h->timer;
   void *bufs[] = {h->state, h->stage, h->k[0], h->k[1], h->k[2], h->k[3], h->deriv, h->compl_f, h->nat,
                   h->noise, h->partial, h->scal, h->acc, h->io[0], h->io[1], h->io[2], h->vcount, h->vstart,
@@ -641,10 +705,19 @@ int spss_srk_step(spss_system *h, int method, double dt, int noise_mode, const d
   return srk_step(h, method, dt, noise_mode, noise);
 }
 
-// the fused FP32C stochastic steps with Philox noise that spss_run can replay from graphs
-bool graph_run_applies(const spss_system *h, int stepper, int method, int noise_mode) {
+// the fused FP32C stochastic steps with Philox noise that spss_run can replay from graphs:
+// 1 on the cell-list path, 2 on the Verlet-list path, 0 none
+int graph_run_kind(const spss_system *h, int stepper, int method, int noise_mode) {
   const bool srk = stepper == SPSS_STEPPER_SRK && h->s == 4;
-  return (stepper == SPSS_STEPPER_EULER || srk) && noise_mode == SPSS_NOISE_PHILOX && use_fused(h, method);
+  if (!(stepper == SPSS_STEPPER_EULER || srk) || noise_mode != SPSS_NOISE_PHILOX) return 0;
+  if (use_fused(h, method)) return 1;
+  if (use_fused_verlet(h, method) && h->p.verlet_mode == SPSS_VERLET_EXACT) return 2;
+  return 0;
+}
+
+bool kind_ready(const spss_system *h, int kind, const double *nat, int srk) {
+  if (h->resident_kind != kind) return false;
+  return kind == 1 ? fast_graph_ready(h->fast, nat, srk) : vf_graph_ready(h->vfast, nat, srk);
 }
 
 int spss_run(spss_system *h, int stepper, int method, long nsteps, double dt, int noise_mode) {
@@ -656,16 +729,20 @@ int spss_run(spss_system *h, int stepper, int method, long nsteps, double dt, in
   // eagerly, every other step is replayed from HIP graphs
   const bool srk = stepper == SPSS_STEPPER_SRK && h->s == 4;
   const double *nat = h->have_nat ? h->nat : nullptr;
-  if (graph_run_applies(h, stepper, method, noise_mode) && nsteps > 0) {
-    if (!fast_graph_ready(h->fast, nat, srk ? 1 : 0)) {
+  const int kind = graph_run_kind(h, stepper, method, noise_mode);
+  if (kind && nsteps > 0) {
+    if (!kind_ready(h, kind, nat, srk ? 1 : 0)) {
       TRY(srk ? srk_step(h, method, dt, noise_mode, nullptr) : euler_step(h, method, dt, noise_mode, nullptr));
       k = 1;
     }
     h->P.higher_order = srk ? 1 : 0;
     if (srk) h->higher_order = true;  // StochasticRungeKuttaStepperPtr.hpp:44
-    const long g = fast_run_graph(h->fast, h->stream, h->P, srk ? 1 : 0, h->state, nat, srk ? h->compl_f : nullptr,
-                                  dt, h->p.D_phi, h->p.xi, h->p.seed, (uint64_t)h->step, nsteps - k);
-    if (g < 0) return fail(SPSS_ERR_DEVICE, fast_last_error());
+    const long g =
+        kind == 1 ? fast_run_graph(h->fast, h->stream, h->P, srk ? 1 : 0, h->state, nat, srk ? h->compl_f : nullptr,
+                                   dt, h->p.D_phi, h->p.xi, h->p.seed, (uint64_t)h->step, nsteps - k)
+                  : vf_run_graph(h->vfast, h->stream, h->P, srk ? 1 : 0, h->state, nat, srk ? h->compl_f : nullptr,
+                                 dt, h->p.D_phi, h->p.xi, h->p.seed, (uint64_t)h->step, nsteps - k);
+    if (g < 0) return fail(SPSS_ERR_DEVICE, kind == 1 ? fast_last_error() : vf_last_error());
     if (g > 0) h->master_stale = true;
     h->step += g;
     h->replayed_steps += g;
@@ -684,26 +761,33 @@ int spss_run(spss_system *h, int stepper, int method, long nsteps, double dt, in
 
 int spss_prepare_run(spss_system *h, int stepper, int method, double dt, int noise_mode) {
   if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
-  if (!graph_run_applies(h, stepper, method, noise_mode)) return SPSS_OK;  // spss_run steps eagerly here
+  const int kind = graph_run_kind(h, stepper, method, noise_mode);
+  if (!kind) return SPSS_OK;  // spss_run steps eagerly here
   const bool srk = stepper == SPSS_STEPPER_SRK && h->s == 4;
   const double *nat = h->have_nat ? h->nat : nullptr;
-  if (!fast_graph_ready(h->fast, nat, srk ? 1 : 0)) {
+  if (kind == 2) TRY(ensure_vfast(h));
+  if (!kind_ready(h, kind, nat, srk ? 1 : 0)) {
     TRY(master_write(h, true));  // the resident order is entered from the master state
-    int rc = fast_enter(h->fast, h->stream, h->P, srk ? 1 : 0, h->state, nat);
-    if (rc != SPSS_OK) return fail(rc, fast_last_error());
+    int rc = kind == 1 ? fast_enter(h->fast, h->stream, h->P, srk ? 1 : 0, h->state, nat)
+                       : vf_enter(h->vfast, h->stream, h->P, srk ? 1 : 0, h->state, nat);
+    if (rc != SPSS_OK) return fail(rc, kind == 1 ? fast_last_error() : vf_last_error());
+    h->resident_kind = kind;
+    if (kind == 2) h->uses_verlet = true;
   }
   h->P.higher_order = srk ? 1 : 0;
   if (srk) h->higher_order = true;
-  int rc = fast_prepare_graphs(h->fast, h->stream, h->P, srk ? 1 : 0, h->state, nat, srk ? h->compl_f : nullptr, dt,
-                               h->p.D_phi, h->p.xi, h->p.seed);
-  if (rc != SPSS_OK) return fail(rc, fast_last_error());
+  int rc = kind == 1 ? fast_prepare_graphs(h->fast, h->stream, h->P, srk ? 1 : 0, h->state, nat,
+                                           srk ? h->compl_f : nullptr, dt, h->p.D_phi, h->p.xi, h->p.seed)
+                     : vf_prepare_graphs(h->vfast, h->stream, h->P, srk ? 1 : 0, h->state, nat,
+                                         srk ? h->compl_f : nullptr, dt, h->p.D_phi, h->p.xi, h->p.seed);
+  if (rc != SPSS_OK) return fail(rc, kind == 1 ? fast_last_error() : vf_last_error());
   HIPCHECK(hipStreamSynchronize(h->stream));
   return SPSS_OK;
 }
 
 int spss_run_stats(spss_system *h, long *eager_steps, long *replayed_steps, long *graphs_captured, int reset) {
   if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
-  const long cap = h->fast ? fast_graph_captures(h->fast) : 0;
+  const long cap = (h->fast ? fast_graph_captures(h->fast) : 0) + (h->vfast ? vf_graph_captures(h->vfast) : 0);
   if (eager_steps) *eager_steps = h->eager_steps;
   if (replayed_steps) *replayed_steps = h->replayed_steps;
   if (graphs_captured) *graphs_captured = cap - h->captures0;
@@ -815,6 +899,64 @@ int spss_autotune_cells(spss_system *h, int stepper, double dt, int probe_steps,
   return SPSS_OK;
 }
 
+int spss_autotune_all_pairs(spss_system *h, int probe_evals, double *table_ms, int table_cap, int *chosen) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (probe_evals < 1) return fail(SPSS_ERR_INVALID, "probe_evals >= 1 required");
+  TRY(sync_master(h));
+  const size_t m = (size_t)h->s * h->n;
+  double *k = nullptr;
+  TRY(dalloc_n(&k, m));
+  const double *nat = h->have_nat ? h->nat : nullptr;
+  double *compl_out = h->higher_order ? h->compl_f : nullptr;
+  h->P.higher_order = h->higher_order ? 1 : 0;
+  launch_stage(h->stream, (long)m, h->state, nullptr, 0.0, h->stage);
+  double best = 1e300;
+  int best_c = h->ap_config;
+  int rc = SPSS_OK;
+  for (int c = 0; c < kAllPairsConfigCount && rc == SPSS_OK; ++c) {
+    launch_all_pairs(h->stream, h->P, h->stage, nat, k, compl_out, c);  // warm-up
+    EventPair ev;
+    hipError_t e = ev.create();
+    if (e == hipSuccess) e = hipEventRecord(ev.start, h->stream);
+    for (int r = 0; r < probe_evals && e == hipSuccess; ++r)
+      launch_all_pairs(h->stream, h->P, h->stage, nat, k, compl_out, c);
+    if (e == hipSuccess) e = hipEventRecord(ev.stop, h->stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = ev.elapsed(&ms);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+      rc = fail(SPSS_ERR_DEVICE, std::string("all-pairs autotuner: ") + hipGetErrorString(e));
+      break;
+    }
+    const double per = ms / probe_evals;
+    if (table_ms && c < table_cap) table_ms[c] = per;
+    if (per < best) {
+      best = per;
+      best_c = c;
+    }
+  }
+  hipStreamSynchronize(h->stream);
+  hipFree(k);
+  if (rc != SPSS_OK) return rc;
+  h->ap_config = best_c;
+  if (chosen) *chosen = best_c;
+  return SPSS_OK;
+}
+
+int spss_set_all_pairs_config(spss_system *h, int config) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (config < 0 || config >= kAllPairsConfigCount) return fail(SPSS_ERR_INVALID, "all-pairs shape index out of range");
+  h->ap_config = config;
+  return SPSS_OK;
+}
+
+int spss_get_all_pairs_config(spss_system *h, int *block, int *jtile) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (block) *block = kAllPairsConfigs[h->ap_config][0];
+  if (jtile) *jtile = kAllPairsConfigs[h->ap_config][1];
+  return SPSS_OK;
+}
+
 int spss_get_cell_subdivision(spss_system *h, int *k) {
   if (!h || !k) return fail(SPSS_ERR_INVALID, "NULL argument");
   *k = h->p.cell_subdivision;
@@ -838,6 +980,12 @@ int spss_calculate_max_displacement(spss_system *h, const double *tot) {
 
 int spss_verlet_status(spss_system *h, int *should_update, double *acc, int *uses) {
   if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (h->vfast && h->resident_kind == 2) {  // FP32C Verlet lists: the flag lives on the device
+    int rc = vf_status(h->vfast, h->stream, should_update, acc, nullptr);
+    if (rc != SPSS_OK) return fail(rc, vf_last_error());
+    if (uses) *uses = 1;
+    return SPSS_OK;
+  }
   if (should_update) *should_update = h->should_update ? 1 : 0;
   if (acc) *acc = h->acc_disp;
   if (uses) *uses = h->uses_verlet ? 1 : 0;
@@ -846,6 +994,15 @@ int spss_verlet_status(spss_system *h, int *should_update, double *acc, int *use
 
 int spss_get_verlet_lists(spss_system *h, int *counts, int *flat, long cap, long *total) {
   if (!h || !counts || !total) return fail(SPSS_ERR_INVALID, "NULL argument");
+  if (h->vfast && h->resident_kind == 2) {  // FP32C: the lists of the last rebuild, by particle id
+    std::vector<int> c, f;
+    int rc = vf_lists(h->vfast, h->stream, &c, &f);
+    if (rc != SPSS_OK) return fail(rc, vf_last_error());
+    std::copy(c.begin(), c.end(), counts);
+    *total = (long)f.size();
+    if (flat && cap > 0) std::copy(f.begin(), f.begin() + std::min<long>(cap, (long)f.size()), flat);
+    return SPSS_OK;
+  }
   if (!h->vstart) {
     std::fill(counts, counts + h->n, 0);
     *total = 0;

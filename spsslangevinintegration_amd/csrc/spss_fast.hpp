@@ -58,6 +58,8 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
               double *compl_out, double dt, double D_phi, double friction, int noise_mode, const double *noise,
               uint64_t seed, uint64_t step);
 bool fast_is_resident(const FastPath *f);
+// the arrivals scan's failure flag (a look-back wait that ran out): SPSS_OK or an error (waits)
+int fast_check(FastPath *f, hipStream_t st);
 // which of its double buffers the resident order is in (a captured step graph replays only there)
 int fast_buffer_parity(const FastPath *f);
 // Fused steps replayed from HIP graphs (Philox noise from the device step counter): runs all

@@ -9,13 +9,12 @@
 //    natural frequencies stay in cell order between steps.  The pair kernel writes the
 //    phasor sums of every particle in the old order; fr_move decides each particle's
 //    move (its new position depends on phase and omega only) and counts the moves per
-//    (cell, direction) and the arrivals per cell, a two-level scan of the arrival counts
-//    gives the new cell starts (block-local in fr_scan_overflow, the earlier blocks' totals
-//    added in fr_far_bucket), and fr_scatter
+//    (cell, direction) and the arrivals per cell, a scan of the arrival counts with a decoupled
+//    look-back gives the new cell starts (fr_arrivals), and fr_scatter
 //    applies the step and writes every particle to its new slot: inside a cell, particles that
 //    stayed come first in their previous order, then arrivals grouped by the direction
 //    they came from, each group in its previous order, then the rare particles that jumped
-//    farther than the 3x3 block around their cell, by previous position (fr_far_bucket).
+//    farther than the 3x3 block around their cell, by previous position (placed by fr_arrivals).
 //    No atomic decides an order, so results are deterministic.
 #pragma once
 
@@ -24,6 +23,7 @@
 #include "spss_device.hpp"
 #include "spss_fast.hpp"
 #include "spss_scan.hpp"
+#include "spss_step.hpp"
 
 namespace spss {
 
@@ -42,7 +42,7 @@ struct FastArgs {
   int4 *ovf_list;              // work of tiles whose staged cells overflow the LDS stage, for fk_pairs_ovf:
                                // {first cell of the replica, row, ca, cb} a column span that fits, or
                                // {.., .., column, -1 - j} particles j*kFB.. of a cell that does not
-  int *ovf_count;              // their number, zeroed by scan_add or fr_far_bucket
+  int *ovf_count;              // their number, zeroed by scan_add or fr_scatter
   double H;
   // guard band of the FP32 range test: |d^2 - r2c| <= r2h is decided in FP64; r2k is a power of
   // two with r2k * r2h > 4, so clamp(-r2k * (d^2 - r2c), 0, 1) is exactly 1 below the band, 0 above
@@ -88,16 +88,6 @@ struct PairBufs {
   int *scnt;             // MODE 3: neighbour count
 };
 
-// the stochastic update of fk_update (Euler or SRK)
-struct StepArgs {
-  double dt, sqrt_dt, int_phi, c15, is3, sq2d, friction;
-  int noise_mode;  // 0 none, 1 device array indexed by particle id, 3 Philox
-  const double *noise;
-  uint64_t seed, step;
-  const unsigned long long *step_dev;  // when set, the Philox step is read here (graph replays)
-  unsigned long long *step_inc;        // when set, fr_move advances this counter by one first
-  const Coef *coef;                    // when set, per-replica xi, sigma, noise intensities, seed
-};
 
 // ---------------------------------------------------------------------------
 // moves of the resident re-sort: d = (dy+1)*3 + (dx+1) for a move into one of the 3x3 cells
@@ -180,11 +170,6 @@ __global__ void fk_rank(DevParams P, const double *__restrict__ stage, int ncf, 
 // ---------------------------------------------------------------------------
 // the pair kernel
 
-__device__ __forceinline__ double philox_z(uint64_t seed, int i, uint32_t block, uint64_t step) {
-  double z0, z1;
-  philox_normal2(seed, (uint32_t)i, block, step, z0, z1);
-  return z0;
-}
 
 // FP64 re-test of a guard-band pair with the reference's formula
 // (x_i wrapped, x_j as stored, ClassCEffectiveParticleDistanceUnsigned, dr2 <= rho^2)
@@ -691,92 +676,6 @@ __global__ __launch_bounds__(kFB) void fk_pairs_ovf(DevParams P, FastArgs A, Pai
   overflow_items<S, SRK, MODE>(P, A, B, blockIdx.x, gridDim.x);
 }
 
-// the resident step's first scan pass and, in the blocks after the first nsb, the overflow work of
-// the pair kernel (its sums are needed only by fr_scatter), saving a dependent launch per step
-template <int S, bool SRK>
-__global__ __launch_bounds__(kFB) void fr_scan_overflow(DevParams P, FastArgs A, PairBufs B, const int *count,
-                                                        const int *count2, int m, int *lstart, int *blocksum,
-                                                        int nsb) {
-  if ((int)blockIdx.x < nsb)
-    scan_local_block(count, count2, m, lstart, blocksum, blockIdx.x);
-  else
-    overflow_items<S, SRK, 3>(P, A, B, blockIdx.x - nsb, gridDim.x - nsb);
-}
-
-// The stochastic step of one particle from its phasor sums (StochasticEulerStepperPtr.hpp:50-67,
-// StochasticRungeKuttaStepperPtr.hpp:50-71): the derivative as the zero-filled derivative vector of
-// StochasticEulerStepperPtr.hpp:42-47 would hold it, then the update.  The new position depends on
-// the phase and omega only, so step_position gives it bit-identically without the sums.
-template <int S, bool SRK>
-__device__ __forceinline__ void position_from(const DevParams &P, const double *pi, const StepArgs &U, double sn,
-                                              double cn, double &x1, double &y1) {
-  const double f0 = P.v0 * cn, f1 = P.v0 * sn;
-  if (S == 4 && SRK) {
-    const double f2 = pi[3], dt = U.dt;
-    x1 = pi[0] + (f0 * dt - 0.5 * f2 * f1 * dt * dt);
-    y1 = pi[1] + (f1 * dt + 0.5 * f2 * f0 * dt * dt);
-  } else {
-    x1 = pi[0] + f0 * U.dt;
-    y1 = pi[1] + f1 * U.dt;
-  }
-}
-
-// the new position from the state pi; sn, cn = sin, cos of its phase
-template <int S, bool SRK>
-__device__ __forceinline__ void step_position(const DevParams &P, const double *pi, const StepArgs &U, double &sn,
-                                              double &cn, double &x1, double &y1) {
-  sincos(pi[2], &sn, &cn);
-  position_from<S, SRK>(P, pi, U, sn, cn, x1, y1);
-}
-
-template <int S, bool SRK>
-__device__ __forceinline__ void step_particle(const DevParams &P, const Coef &K, const double *pi, int i,
-                                              double nat_i, double2 sm, double2 tm, int nnb, const StepArgs &U,
-                                              uint64_t step, double *out, double &C) {
-  double sn, cn;
-  step_position<S, SRK>(P, pi, U, sn, cn, out[0], out[1]);
-  const double st = sn * P.cos_a + cn * P.sin_a;  // sin(phi_i + alpha)
-  const double ct = cn * P.cos_a - sn * P.sin_a;  // cos(phi_i + alpha)
-  const double A64 = ct * sm.y - st * sm.x;
-  const double card = (double)nnb;
-  C = 0.0;
-  if (S == 4) {
-    const double omega_i = pi[3];
-    const double f2 = omega_i;
-    double f3 = 0.0;
-    if (P.natfreq_local) {
-      f3 = nnb ? -K.xi * omega_i + nat_i + K.sigma * A64 / card : -K.xi * omega_i + nat_i;
-    } else if (nnb) {
-      f3 = -K.xi * omega_i + K.sigma * A64 / card;
-    }
-    if (!SRK) {
-      const double z = (U.noise_mode == 1) ? U.noise[i] : (U.noise_mode == 3 ? philox_z(K.seed, i, 0, step) : 0.0);
-      const double dW = U.sqrt_dt * z;
-      out[2] = pi[2] + f2 * U.dt;
-      out[3] = pi[3] + (f3 * U.dt + K.int_phi * dW);
-    } else {
-      const double C64 = ct * (tm.x - omega_i * sm.x) + st * (tm.y - omega_i * sm.y);
-      C = nnb ? C64 * (K.sigma / card) : C64;
-      double U1 = 0.0, U2 = 0.0;
-      if (U.noise_mode == 1) {
-        U1 = U.noise[2L * i];
-        U2 = U.noise[2L * i + 1];
-      } else if (U.noise_mode == 3) {
-        philox_normal2(K.seed, (uint32_t)i, 0, step, U1, U2);
-      }
-      const double dt = U.dt;
-      const double dW = U1 * U.sqrt_dt;
-      const double dZ = U.c15 * (U1 + U.is3 * U2);
-      out[2] = pi[2] + (f2 * dt + 0.5 * f3 * dt * dt + K.sq2d * dZ);
-      out[3] = pi[3] + (f3 * dt + 0.5 * (C - K.xi * f3) * dt * dt + K.sq2d * (dW - K.xi * dZ));
-    }
-  } else {  // model 3 Euler
-    const double f2 = nnb ? nat_i + K.sigma * A64 / card : nat_i;
-    const double z = (U.noise_mode == 1) ? U.noise[i] : (U.noise_mode == 3 ? philox_z(K.seed, i, 0, step) : 0.0);
-    out[2] = pi[2] + (f2 * U.dt + K.int_phi * (U.sqrt_dt * z));
-  }
-}
-
 template <int S>
 __device__ __forceinline__ int cell_of_position(const DevParams &P, double x, double y, int ncf) {
   double xw, yw;
@@ -799,7 +698,8 @@ __device__ __forceinline__ void move_and_count(const DevParams &P, const Layout 
                                                double sn, double cn, double xw, double yw, const StepArgs &U, int p,
                                                bool valid, int *__restrict__ mv, int *__restrict__ farcnt,
                                                int *__restrict__ farslot, int *__restrict__ cnt9,
-                                               int *__restrict__ newcnt) {
+                                               int *__restrict__ newcnt, int *__restrict__ farlist,
+                                               int *__restrict__ farlist_n) {
   int cell = -1, d = kFar, dest = 0;  // invalid lanes: a key of their own that is never counted
   if (valid) {
     double x1, y1;
@@ -818,6 +718,7 @@ __device__ __forceinline__ void move_and_count(const DevParams &P, const Layout 
     } else {
       d = kFar;
       farslot[p] = atomicAdd(farcnt + base + ny * ncf + nx, 1);
+      farlist[atomicAdd(farlist_n, 1)] = p;  // rare: the arrivals kernel places these
     }
     cell = base + oy * ncf + ox;
     dest = base + ny * ncf + nx;
@@ -841,7 +742,8 @@ template <int S, bool SRK>
 __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int ncf,
                                                const double *__restrict__ rstate, StepArgs U, int *__restrict__ mv,
                                                int *__restrict__ farcnt, int *__restrict__ farslot,
-                                               int *__restrict__ cnt9, int *__restrict__ newcnt) {
+                                               int *__restrict__ cnt9, int *__restrict__ newcnt,
+                                               int *__restrict__ farlist, int *__restrict__ farlist_n) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = p < n;
   const double *pi = valid ? rstate + (long)p * S : rstate;
@@ -850,7 +752,8 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int
     sincos(pi[2], &sn, &cn);
     apply_pbc(P, pi[0], pi[1], xw, yw);
   }
-  move_and_count<S, SRK>(P, L, ncf, pi, sn, cn, xw, yw, U, p, valid, mv, farcnt, farslot, cnt9, newcnt);
+  move_and_count<S, SRK>(P, L, ncf, pi, sn, cn, xw, yw, U, p, valid, mv, farcnt, farslot, cnt9, newcnt, farlist,
+                         farlist_n);
 }
 
 // The tiles of one row of cells for the next pair kernel, built by one wave from the row's cell
@@ -896,67 +799,130 @@ __global__ __launch_bounds__(256) void fk_row_tiles(const int *__restrict__ star
   rows_to_tiles([&](int c) { return start[c]; }, ncf, wcap, cmax, tiles);
 }
 
-// far arrivals: unordered bucket behind the near arrivals of their destination cell.  Also the
-// second scan pass of the new cell starts (nstart[c] = lstart[c] + the totals of the scan blocks
-// before c's, from the block-local starts and totals of fr_scan_overflow), the clearing of the
-// arrival counts it summed, of the next step's move counters and of the overflow count.
+// The arrivals of a resident step, one launch: the new cell starts and the far arrivals' slots,
+// next to the pair kernel's overflow work.
+//  * Workgroups [0, nsb) scan 2048 cells each (the arrival counts newcnt + farcnt, cleared as they
+//    are read) and find the total of the cells before theirs by a decoupled look-back over the
+//    earlier workgroups' status words (aggregate, then inclusive prefix; workgroups are dispatched
+//    in index order and a workgroup only waits on lower ones, so every wait ends; the waits are
+//    also bounded), then write the final starts.  The last of them to finish places the step's far
+//    movers (a compact list filled where the moves were computed) behind the near arrivals of their
+//    destination cells, and resets the list, the status words and its own counter.
+//  * Workgroups [nsb, ...) clear the next step's move counters and run the overflow work of the
+//    pair kernel (its sums are needed only by fr_scatter).
+struct Arrivals {
+  int *newcnt, *farcnt;        // arrival counts per cell (read, then cleared)
+  int m, nsb;                  // cells, scan workgroups
+  int *nstart;                 // [m + 1] new cell starts
+  unsigned long long *status;  // [nsb] look-back words: kLbAgg | total, or kLbPre | inclusive prefix
+  unsigned int *done;          // scan workgroups finished
+  int *farlist, *farlist_n;    // slots of this step's far movers and their number
+  const int *mv;               // this step's move words
+  const int *cnt9;             // this step's move counts per (cell, direction)
+  const int *farslot;          // each far mover's slot among its destination's far arrivals
+  int *bucket;                 // far arrivals by new position
+  int *cnt9_next;              // the other move-count buffer, cleared for the next step
+  long nclear;
+  const double *rstate;
+  int ncf, cpr, n;
+  int *error;                  // set when a look-back wait ran out (never, when dispatch is ordered)
+};
+
+constexpr unsigned long long kLbAgg = 1ull << 62, kLbPre = 2ull << 62, kLbVal = (1ull << 62) - 1;
+
 template <int S, bool SRK>
-__global__ __launch_bounds__(256) void fr_far_bucket(DevParams P, int n, int ncf, int cpr, const int *__restrict__ mv,
-                                                     const double *__restrict__ rstate, StepArgs U,
-                                                     const int *__restrict__ lstart, const int *__restrict__ blocksum,
-                                                     int nsb, int m, int *__restrict__ nstart, int *__restrict__ count,
-                                                     int *__restrict__ count2, int *__restrict__ ovf_count,
-                                                     const int *__restrict__ cnt9, const int *__restrict__ farslot,
-                                                     int *__restrict__ bucket, int *__restrict__ cnt9_next,
-                                                     long nclear) {
-  constexpr int kBlock = 256 * kScanItems;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kFB) void fr_arrivals(DevParams P, FastArgs A, PairBufs B, Arrivals Q, StepArgs U) {
+  const int b = blockIdx.x;
+  if (b >= Q.nsb) {
+    const long stride = (long)(gridDim.x - Q.nsb) * kFB;
+    for (long e = (long)(b - Q.nsb) * kFB + threadIdx.x; e < Q.nclear; e += stride) Q.cnt9_next[e] = 0;
+    overflow_items<S, SRK, 3>(P, A, B, b - Q.nsb, gridDim.x - Q.nsb);
+    return;
+  }
   // graph-replayed steps: the counter read by this step's fr_scatter moves on here, one kernel
   // after the previous step's fr_scatter read it
-  if (U.step_inc && p == 0) *U.step_inc += 1ull;
-  const long stride = (long)gridDim.x * blockDim.x;
-  // the other move-count buffer (last read by the previous step) starts the next step at zero
-  for (long e = p; e < nclear; e += stride) cnt9_next[e] = 0;
-  // a workgroup's 256 cells lie in one scan block: one offset per pass, summed by every wave
-  for (long c0 = (long)blockIdx.x * blockDim.x; c0 <= m; c0 += stride) {
-    const int b = (int)(c0 / kBlock);
-    const int off = wave_prefix_of(blocksum, b);
-    const long c = c0 + threadIdx.x;
-    if (c < m) {
-      nstart[c] = lstart[c] + off;
-      count[c] = 0;
-      count2[c] = 0;
-    } else if (c == m) {
-      nstart[m] = off + (b < nsb ? blocksum[b] : 0);
-    }
-  }
-  if (p == 0) *ovf_count = 0;
-  // the moves of particles 4p .. 4p+3 (one 16-byte load where all four exist)
-  const long p4 = 4L * p;
-  if (p4 >= n) return;
-  int mk4[4];
-  if (p4 + 3 < n) {
-    const int4 v = reinterpret_cast<const int4 *>(mv)[p];
-    mk4[0] = v.x;
-    mk4[1] = v.y;
-    mk4[2] = v.z;
-    mk4[3] = v.w;
-  } else {
+  if (U.step_inc && b == 0 && threadIdx.x == 0) *U.step_inc += 1ull;
+  __shared__ int sh[kFB];
+  __shared__ long long sprefix;
+  __shared__ int slast;
+  const long base = (long)b * kFB * kScanItems + (long)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) mk4[u] = p4 + u < n ? mv[p4 + u] : 0;  // 0: not a far move
+  for (int q = 0; q < kScanItems; ++q) {
+    v[q] = 0;
+    if (base + q < Q.m) {
+      v[q] = Q.newcnt[base + q] + Q.farcnt[base + q];
+      Q.newcnt[base + q] = 0;
+      Q.farcnt[base + q] = 0;
+    }
+    s += v[q];
   }
-#pragma unroll 1
-  for (int u = 0; u < 4; ++u) {
-    const int mk = mk4[u];
-    if ((mk & 15) != kFar) continue;
-    const int pf = (int)p4 + u;
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < kFB; off <<= 1) {
+    const int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const unsigned long long total = (unsigned long long)sh[kFB - 1];
+    long long excl = 0;
+    if (b == 0) {
+      __hip_atomic_store(&Q.status[0], kLbPre | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&Q.status[b], kLbAgg | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      for (int j = b - 1; j >= 0; --j) {
+        unsigned long long w = 0;
+        for (long spin = 0; spin < (1L << 26); ++spin) {
+          w = __hip_atomic_load(&Q.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          if (w) break;
+          __builtin_amdgcn_s_sleep(1);
+        }
+        if (!w) {  // a wait that ran out: flag it and go on (the starts are then wrong)
+          atomicOr(Q.error, 1);
+          break;
+        }
+        excl += (long long)(w & kLbVal);
+        if (w & kLbPre) break;
+      }
+      __hip_atomic_store(&Q.status[b], kLbPre | (unsigned long long)(excl + (long long)total), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    sprefix = excl;
+  }
+  __syncthreads();
+  int run = (int)sprefix + sh[threadIdx.x] - s;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    if (base + q < Q.m) Q.nstart[base + q] = run;
+    run += v[q];
+  }
+  if (b == Q.nsb - 1 && threadIdx.x == kFB - 1) Q.nstart[Q.m] = run;  // every cell's particles: n
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) slast = atomicAdd(Q.done, 1u) == (unsigned)(Q.nsb - 1);
+  __syncthreads();
+  if (!slast) return;
+  __threadfence();
+  // the last scan workgroup: every start is final.  Far movers behind the near arrivals of their
+  // destination cells, in the unordered slots their atomics gave them (fr_scatter orders them).
+  const int nfar = __hip_atomic_load(Q.farlist_n, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  for (int k = threadIdx.x; k < nfar; k += kFB) {
+    const int pf = Q.farlist[k];
+    const int mk = Q.mv[pf];
     double sn, cn, x1, y1;
-    step_position<S, SRK>(P, rstate + (long)pf * S, U, sn, cn, x1, y1);
-    const int cd = cell_base(mk >> 4, cpr) + cell_of_position<S>(P, x1, y1, ncf);
-    int off = 0;  // (rare) this lane's own sum of the earlier block totals
-    for (int j = 0; j < cd / kBlock; ++j) off += blocksum[j];
-    const int q = lstart[cd] + off + new_count_of(cnt9, cd, ncf, cpr) + farslot[pf];
-    if (q < n) bucket[q] = pf;  // always, when the counts match the moves
+    step_position<S, SRK>(P, Q.rstate + (long)pf * S, U, sn, cn, x1, y1);
+    const int cd = cell_base(mk >> 4, Q.cpr) + cell_of_position<S>(P, x1, y1, Q.ncf);
+    const int q = Q.nstart[cd] + new_count_of(Q.cnt9, cd, Q.ncf, Q.cpr) + Q.farslot[pf];
+    if (q < Q.n) Q.bucket[q] = pf;  // always, when the counts match the moves
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < Q.nsb; j += kFB) Q.status[j] = 0ull;
+  if (threadIdx.x == 0) {
+    *Q.farlist_n = 0;
+    *Q.done = 0u;
   }
 }
 
@@ -972,7 +938,9 @@ __global__ __launch_bounds__(256) void fr_scatter(
     double *__restrict__ compl_out, double *__restrict__ nstate, int *__restrict__ nrid, double *__restrict__ nnat,
     float4 *__restrict__ snap, float *__restrict__ omega_f, int4 *__restrict__ rtiles, int wcap, int cmax,
     int *__restrict__ mvn, int *__restrict__ farcnt, int *__restrict__ farslot, int *__restrict__ cnt9n,
-    int *__restrict__ newcnt) {
+    int *__restrict__ newcnt, int *__restrict__ farlist, int *__restrict__ farlist_n, int *__restrict__ ovf_count) {
+  // the overflow work of this step was read by the arrivals kernel: the next pair kernel starts at zero
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ovf_count = 0;
   // the next pair kernel's row tiles (single systems) from the new starts, in the first
   // workgroups: their serial span search hides behind the other workgroups' memory traffic
   if (rtiles && (int)blockIdx.x * 4 < ncf) rows_to_tiles([&](int c) { return nstart[c]; }, ncf, wcap, cmax, rtiles);
@@ -1074,7 +1042,8 @@ __global__ __launch_bounds__(256) void fr_scatter(
     }
   }
   if (mvn)
-    move_and_count<S, SRK>(P, L, ncf, out, sn2, cn2, xw2, yw2, U, pos, valid, mvn, farcnt, farslot, cnt9n, newcnt);
+    move_and_count<S, SRK>(P, L, ncf, out, sn2, cn2, xw2, yw2, U, pos, valid, mvn, farcnt, farslot, cnt9n, newcnt,
+                           farlist, farlist_n);
 }
 
 // advance the device step counter of graph-replayed runs

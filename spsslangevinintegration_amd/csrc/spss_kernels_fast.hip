@@ -18,9 +18,10 @@
 //   self terms (v0 cos phi, v0 sin phi, omega, -xi omega + sigma A / n) in FP64.
 //
 // Fused stochastic steps keep the state resident in cell order (see spss_fast_kernels.hpp):
-//   fk_pairs (MODE 3: phasor sums) -> fr_move (moves and their counts) -> scan of the arrivals
-//   -> fr_far_bucket -> fr_scatter (FP64 Euler/SRK update written to the new slots),
-// seven launches per step and no host synchronisation.
+//   fk_pairs (MODE 3: phasor sums) -> fr_arrivals (new cell starts by a scan with a decoupled
+//   look-back, far arrivals, the pair kernel's overflow work) -> fr_scatter (FP64 Euler/SRK update
+//   written to the new slots, the next step's moves), three launches per step and no host
+//   synchronisation.
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -76,7 +77,6 @@ struct FastPath {
   // rebuilt cell list (RHS, pair export, entry into resident mode)
   int *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr, *sorted = nullptr;
   int *blocksum = nullptr;
-  int *lstart = nullptr;  // block-local cell starts of the resident step's scan
   float4 *snap = nullptr;
   float *omega_f = nullptr;
   // resident copy of the fused steps (double-buffered where a step permutes)
@@ -99,6 +99,12 @@ struct FastPath {
   // dt, stepper and v0 recorded (another step size or stepper needs fr_move again)
   int *mvb[2] = {nullptr, nullptr};
   int mvc = 0;
+  // the arrivals kernel: look-back status words per scan workgroup, its finished count, the
+  // far movers of the step (slots) and their number, a flag for a look-back wait that ran out
+  unsigned long long *lb_status = nullptr;
+  unsigned int *lb_done = nullptr;
+  int *farlist = nullptr, *farlist_n = nullptr, *lb_error = nullptr;
+  int *farslot = nullptr;  // each far mover's slot among its destination's far arrivals
   bool mv_ready = false;
   bool mv_srk = false;
   double mv_dt = 0.0, mv_v0 = 0.0;
@@ -236,7 +242,6 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->tmp, n));
   A(falloc(&f->sorted, n));
   A(falloc(&f->blocksum, f->nsb + 1));
-  A(falloc(&f->lstart, cells));
   if (R == 1 && f->ncf <= kRowTileCols) {
     // slots per row: the greedy spans of a row 1.5x the mean, at about 94% of kFB each, plus 2,
     // and at least enough spans of rt_wcap columns to cover a row
@@ -272,6 +277,12 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->ovf_list, (size_t)cells + n / kFB + 1));
   A(falloc(&f->rep_chunk, nchunk));
   A(falloc(&f->step_dev, 1));
+  A(falloc(&f->lb_status, f->nsb));
+  A(falloc(&f->lb_done, 1));
+  A(falloc(&f->farlist, n));
+  A(falloc(&f->farslot, n));
+  A(falloc(&f->farlist_n, 1));
+  A(falloc(&f->lb_error, 1));
   if (coef) {
     A(falloc(&f->coef, R));
     A(falloc(&f->zlast, 2 * R));
@@ -298,6 +309,10 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   launch_fill_int(st, cells, f->newcnt, 0);
   launch_fill_int(st, 1, f->ovf_count, 0);
   launch_fill_int(st, 2, reinterpret_cast<int *>(f->step_dev), 0);  // the 64-bit step counter
+  launch_fill_int(st, 2L * f->nsb, reinterpret_cast<int *>(f->lb_status), 0);
+  launch_fill_int(st, 1, reinterpret_cast<int *>(f->lb_done), 0);
+  launch_fill_int(st, 1, f->farlist_n, 0);
+  launch_fill_int(st, 1, f->lb_error, 0);
   if (coef) {
     launch_fill_double(st, 3L * R, f->racc, 0.0);
     launch_fill_double(st, 2L * R, f->zlast, 0.0);
@@ -323,7 +338,8 @@ void fast_destroy(FastPath *f) {
                f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mvb[0], f->mvb[1],
                f->cnt9[0],   f->cnt9[1],   f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
                f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc,      f->ovf_list,  f->ovf_count,
-               f->lstart,    f->rtiles,    f->btiles};
+               f->rtiles,    f->btiles,    f->lb_status, f->lb_done,  f->farlist,  f->farlist_n, f->lb_error,
+               f->farslot};
   for (void *p : b) hipFree(p);
   delete f;
 }
@@ -430,7 +446,7 @@ static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, 
     a.ntiles = f->ncf * f->rt_cmax;
   }
   fk_pairs<S, SRK, MODE><<<tile_grid(a.ntiles), kFB, 0, st>>>(P, a, B);
-  // the resident step (MODE 3) runs the overflow work inside its first scan launch (fr_scan_overflow)
+  // the resident step (MODE 3) runs the overflow work inside its arrivals launch (fr_arrivals)
   if (MODE != 3) fk_pairs_ovf<S, SRK, MODE><<<kOverflowBlocks, kFB, 0, st>>>(P, a, B);
 }
 
@@ -457,6 +473,14 @@ int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stag
 
 bool fast_is_resident(const FastPath *f) { return f && f->resident; }
 
+int fast_check(FastPath *f, hipStream_t st) {
+  int err = 0;
+  FCHK(hipMemcpyAsync(&err, f->lb_error, sizeof(int), hipMemcpyDeviceToHost, st));
+  FCHK(hipStreamSynchronize(st));
+  if (err) return ffail("a look-back wait of the arrivals scan ran out: the cell starts of some step are not valid");
+  return SPSS_OK;
+}
+
 int fast_buffer_parity(const FastPath *f) { return f ? 2 * f->cur + f->mvc : -1; }
 
 void fast_resident_drop(FastPath *f) {
@@ -480,6 +504,7 @@ static void drop_moves(FastPath *f, hipStream_t st) {
   launch_fill_int(st, f->cells, f->newcnt, 0);
   launch_fill_int(st, f->cells, f->farcnt, 0);
   launch_fill_int(st, (long)f->cells * 9, f->cnt9[f->cur], 0);
+  launch_fill_int(st, 1, f->farlist_n, 0);
   f->mv_ready = false;
 }
 
@@ -490,7 +515,8 @@ static void ensure_moves(FastPath *f, hipStream_t st, const DevParams &P, const 
   if (f->mv_ready && f->mv_dt == U.dt && f->mv_srk == SRK && f->mv_v0 == P.v0) return;
   drop_moves(f, st);
   fr_move<S, SRK><<<nblocks(f->n), 256, 0, st>>>(P, layout_of(f), f->n, f->ncf, f->rstate[f->cur], U,
-                                                 f->mvb[f->mvc], f->farcnt, f->slot, f->cnt9[f->cur], f->newcnt);
+                                                 f->mvb[f->mvc], f->farcnt, f->farslot, f->cnt9[f->cur], f->newcnt,
+                                                 f->farlist, f->farlist_n);
   f->mv_ready = true;
   f->mv_dt = U.dt;
   f->mv_srk = SRK;
@@ -535,19 +561,35 @@ static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const
   const Layout L = layout_of(f);
   ensure_moves<S, SRK>(f, st, P, U);
   int *const mv = f->mvb[f->mvc], *const mvn = f->mvb[1 - f->mvc];
-  fr_scan_overflow<S, SRK><<<f->nsb + kOverflowBlocks, kFB, 0, st>>>(P, args_of(f), B, f->newcnt, f->farcnt,
-                                                                     f->cells, f->lstart, f->blocksum, f->nsb);
-  const unsigned nb4 = (unsigned)((n + 1023) / 1024);  // four move words per thread
-  fr_far_bucket<S, SRK><<<nb4, 256, 0, st>>>(P, n, f->ncf, f->cpr, mv, f->rstate[c], U, f->lstart, f->blocksum,
-                                            f->nsb, f->cells, f->rstart[o], f->newcnt, f->farcnt, f->ovf_count,
-                                            f->cnt9[c], f->slot, f->tmp, f->cnt9[o], (long)f->cells * 9);
+  Arrivals Q;
+  Q.newcnt = f->newcnt;
+  Q.farcnt = f->farcnt;
+  Q.m = f->cells;
+  Q.nsb = f->nsb;
+  Q.nstart = f->rstart[o];
+  Q.status = f->lb_status;
+  Q.done = f->lb_done;
+  Q.farlist = f->farlist;
+  Q.farlist_n = f->farlist_n;
+  Q.mv = mv;
+  Q.cnt9 = f->cnt9[c];
+  Q.farslot = f->farslot;
+  Q.bucket = f->tmp;
+  Q.cnt9_next = f->cnt9[o];
+  Q.nclear = (long)f->cells * 9;
+  Q.rstate = f->rstate[c];
+  Q.ncf = f->ncf;
+  Q.cpr = f->cpr;
+  Q.n = n;
+  Q.error = f->lb_error;
+  fr_arrivals<S, SRK><<<f->nsb + kOverflowBlocks, kFB, 0, st>>>(P, args_of(f), B, Q, U);
   // (at least one workgroup per four rows of cells for the row tiles)
   const unsigned nbs = std::max(nb, (unsigned)(f->rtiles ? (f->ncf + 3) / 4 : 0));
   fr_scatter<S, SRK><<<nbs, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], mv, f->cnt9[c], f->tmp,
                                         f->rstate[c], f->rid[c], rnat, f->ssum, f->tsum, f->scnt, U,
                                         SRK ? compl_out : nullptr, f->rstate[o], f->rid[o], f->rnat[o], f->rsnap,
                                         SRK ? f->romega : nullptr, f->rtiles, f->rt_wcap, f->rt_cmax, mvn, f->farcnt,
-                                        f->slot, f->cnt9[o], f->newcnt);
+                                        f->farslot, f->cnt9[o], f->newcnt, f->farlist, f->farlist_n, f->ovf_count);
   f->cur = o;
   f->mvc = 1 - f->mvc;  // the next step's moves (same dt, stepper and v0) are in place
 }
@@ -613,7 +655,7 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
 
 // ---------------------------------------------------------------------------
 // graph-replayed fused steps of a single system.  Each captured step advances the device step
-// counter in its fr_far_bucket and reads it as its Philox step in its fr_scatter, so one graph
+// counter in its fr_arrivals and reads it as its Philox step in its fr_scatter, so one graph
 // serves every replay.  Graphs of kGraphSteps steps (even: the double buffers end where they
 // started) carry the bulk of a run, graphs of one step the remainder; a graph is replayed only
 // from the buffer parity it was captured at.  Capture records launches without running them, so

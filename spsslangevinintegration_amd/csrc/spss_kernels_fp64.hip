@@ -29,15 +29,16 @@ __global__ void k_stage_zero(long m, const double *__restrict__ state, double *_
 
 // ---------------------------------------------------------------------------
 // all-pairs, ParticleSystemFirstOrderPtr.cpp:225-316 / ParticleSystemSecondOrderPtr.cpp:155-230.
-// One thread per i; j-tiles of the stage state are staged through LDS and
-// visited in ascending j, the reference's summation order.
-template <int S>
-__global__ __launch_bounds__(kBlock) void k_all_pairs(DevParams P, const double *__restrict__ stage,
-                                                      const double *__restrict__ nat, double *__restrict__ k_next,
-                                                      double *__restrict__ compl_out) {
-  __shared__ double tile[kBlock * S];
+// One thread per i (BLOCK particles per workgroup); j-tiles of JT particles of the stage state are
+// staged through LDS and visited in ascending j, the reference's summation order, so every
+// (BLOCK, JT) gives the same sums bit for bit: the autotuner picks the fastest.
+template <int S, int BLOCK, int JT>
+__global__ __launch_bounds__(BLOCK) void k_all_pairs(DevParams P, const double *__restrict__ stage,
+                                                     const double *__restrict__ nat, double *__restrict__ k_next,
+                                                     double *__restrict__ compl_out) {
+  __shared__ double tile[JT * S];
   const int n = P.n;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
   double st[S];
   double x_i = 0.0, y_i = 0.0, phi_i = 0.0, omega_i = 0.0;
   if (i < n) {
@@ -50,10 +51,10 @@ __global__ __launch_bounds__(kBlock) void k_all_pairs(DevParams P, const double 
     if (S != 6) apply_pbc(P, x_i, y_i, x_i, y_i);  // :270 (model 6 wraps nothing, :188-193)
   }
   PairAcc a{0.0, 0.0, 0.0};
-  for (int j0 = 0; j0 < n; j0 += kBlock) {
-    const int cnt = min(kBlock, n - j0);
+  for (int j0 = 0; j0 < n; j0 += JT) {
+    const int cnt = min(JT, n - j0);
     __syncthreads();
-    for (int q = threadIdx.x; q < cnt * S; q += kBlock) tile[q] = stage[(long)j0 * S + q];
+    for (int q = threadIdx.x; q < cnt * S; q += BLOCK) tile[q] = stage[(long)j0 * S + q];
     __syncthreads();
     if (i < n) {
       for (int t = 0; t < cnt; ++t) {
@@ -628,9 +629,26 @@ void launch_stage(hipStream_t st, long m, const double *state, const double *kpr
     default: break;              \
   }
 
+const int kAllPairsConfigs[kAllPairsConfigCount][2] = {{64, 64}, {64, 256}, {128, 128}, {128, 512},
+                                                        {256, 256}, {256, 512}};
+
+template <int BLOCK, int JT>
+static void all_pairs_as(hipStream_t st, const DevParams &P, const double *stage, const double *nat, double *k_next,
+                         double *compl_out) {
+  const unsigned nb = (unsigned)((P.n + BLOCK - 1) / BLOCK);
+  SPSS_DISPATCH_S(P.s, k_all_pairs<SS, BLOCK, JT><<<dim3(nb), dim3(BLOCK), 0, st>>>(P, stage, nat, k_next, compl_out));
+}
+
 void launch_all_pairs(hipStream_t st, const DevParams &P, const double *stage, const double *nat, double *k_next,
-                      double *compl_out) {
-  SPSS_DISPATCH_S(P.s, k_all_pairs<SS><<<dim3(nblk(P.n)), dim3(kBlock), 0, st>>>(P, stage, nat, k_next, compl_out));
+                      double *compl_out, int config) {
+  switch (config) {
+    case 0: all_pairs_as<64, 64>(st, P, stage, nat, k_next, compl_out); break;
+    case 1: all_pairs_as<64, 256>(st, P, stage, nat, k_next, compl_out); break;
+    case 2: all_pairs_as<128, 128>(st, P, stage, nat, k_next, compl_out); break;
+    case 3: all_pairs_as<128, 512>(st, P, stage, nat, k_next, compl_out); break;
+    case 5: all_pairs_as<256, 512>(st, P, stage, nat, k_next, compl_out); break;
+    default: all_pairs_as<256, 256>(st, P, stage, nat, k_next, compl_out); break;
+  }
 }
 
 void launch_cell_build(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy, CellBuffers &cb) {

@@ -97,8 +97,13 @@ struct CellBuffers {
 
 // stage = state + kc_dt * kprev; kprev null: an all-zero k_prev
 void launch_stage(hipStream_t st, long m, const double *state, const double *kprev, double kc_dt, double *stage);
+// all-pairs RHS with workgroups of kAllPairsConfigs[config][0] particles and j-tiles of
+// kAllPairsConfigs[config][1] (identical sums for every config; kAllPairsDefault is {256, 256})
+constexpr int kAllPairsConfigCount = 6;
+constexpr int kAllPairsDefault = 4;
+extern const int kAllPairsConfigs[kAllPairsConfigCount][2];
 void launch_all_pairs(hipStream_t st, const DevParams &P, const double *stage, const double *nat, double *k_next,
-                      double *compl_out);
+                      double *compl_out, int config = kAllPairsDefault);
 void launch_cell_build(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy, CellBuffers &cb);
 void launch_linked_list(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy,
                         const CellBuffers &cb, const double *nat, double *k_next, double *compl_out);

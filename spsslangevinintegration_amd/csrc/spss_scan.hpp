@@ -57,10 +57,13 @@ __device__ __forceinline__ int wave_prefix_of(const int *__restrict__ blocksum, 
   return s;
 }
 
+// gate (may be null): when *gate == 0 the launch does nothing (work decided on the device)
 template <int I = kScanItems>
 __global__ __launch_bounds__(256) void scan_local(const int *__restrict__ count, const int *__restrict__ count2, int m,
-                                                  int *__restrict__ start, int *__restrict__ blocksum) {
+                                                  int *__restrict__ start, int *__restrict__ blocksum,
+                                                  const int *__restrict__ gate = nullptr) {
   static_assert(I == kScanItems, "one block size");
+  if (gate && *gate == 0) return;
   scan_local_block(count, count2, m, start, blocksum, blockIdx.x);
 }
 
@@ -69,8 +72,10 @@ __global__ __launch_bounds__(256) void scan_local(const int *__restrict__ count,
 template <int I = kScanItems>
 __global__ __launch_bounds__(256) void scan_add(int *__restrict__ count, int *__restrict__ count2, int m,
                                                 int *__restrict__ start, const int *__restrict__ blocksum,
-                                                int nblocks, int *__restrict__ zero) {
+                                                int nblocks, int *__restrict__ zero,
+                                                const int *__restrict__ gate = nullptr) {
   static_assert(I == kScanItems, "one block size");
+  if (gate && *gate == 0) return;
   __shared__ int off;
   if (blockIdx.x == 0 && threadIdx.x == 0 && zero) *zero = 0;
   if (threadIdx.x < 64) {

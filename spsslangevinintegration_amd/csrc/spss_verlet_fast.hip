@@ -1,0 +1,831 @@
+// spss_verlet_fast.hip -- Verlet neighbour lists in FP32C precision (exact mode): models 3 and 4,
+// stochastic Euler and Runge-Kutta steps, rebuilds decided on the device.
+//
+// Reference: EvaluateInteractionsWithVerletNeighborList and ...WithLinkedListAndVerletNeighborList
+// (ParticleSystemFirstOrderPtr.cpp:470-815) with CalculateMaxDisplacement (:836-861), in this
+// library's exact mode: r_max = rho + skin, candidates from cells of side >= r_max, so every list
+// holds every j within r_max and each step's neighbour set is {j : |r_ij| <= rho} whatever the
+// rebuild schedule.  The schedule itself follows the exact mode of the FP64 path: after every
+// step acc += sqrt(max_i |dr_i|^2), rebuild when acc + that step's value > (r_max - rho) / 2.
+//
+// Layout (the resident copy): particles in the cell order of the last rebuild on a grid of
+// side Hv = 1/int(1/r_max); between rebuilds they keep their slots.  Per slot: FP64 state
+// (double-buffered by step), id, natural frequency, FP32 snapshot {x, y wrapped into the box,
+// cos phi, sin phi} and FP32 omega (double-buffered with the state), and an ELL list of width K
+// stored slot-major (entry t of slot p at list[t * n + p], so a wave's loads of one entry are
+// contiguous) with its length.
+//
+// One step is a rebuild -- seven launches that return at once unless the device flag asks for
+// one: cell keys and histogram, the two-level scan, bucket fill, stable rank and gather into
+// scratch, back into the resident arrays with the new snapshot, list build -- and the step kernel:
+// each lane walks its list four entries at a time: FP32 minimum-image distance (x_j - x_i minus
+// its nearest integer), a guard band around rho^2 decided with the reference's FP64 ClassC formula
+// on the stored positions (so neighbour sets are exact), phasor sums in four FP32 partials flushed
+// into FP64 every 16 entries; then the Euler/SRK update of step_particle, the new snapshot and the
+// step's largest squared displacement.  The last workgroup to finish turns that maximum into the
+// next step's flag on the device.  No host synchronisation: steps replay from HIP graphs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "../../include/spss_hip.h"
+#include "spss_launch.hpp"
+#include "spss_scan.hpp"
+#include "spss_step.hpp"
+#include "spss_verlet_fast.hpp"
+
+namespace spss {
+
+namespace {
+thread_local std::string g_verr;
+constexpr int kVB = 256;         // threads per workgroup
+constexpr int kVGraphSteps = 10;  // steps of the bulk graph (even: the state buffers end where they started)
+}  // namespace
+
+// device control block
+struct VCtl {
+  int flag;                      // rebuild before the next step (should_update_lists_)
+  int pad0;
+  double acc;                    // accumulated_displacement_
+  unsigned long long max2_bits;  // the step's largest |dr|^2 as the bits of a non-negative double
+  unsigned int done;             // step workgroups finished
+  unsigned int pad1;
+  unsigned long long overflow;   // lists longer than the width K since entry (walked through their cells)
+  unsigned long long step;       // Philox step of graph-replayed steps
+};
+
+struct VArgs {
+  int n, ncv, cells, K;
+  double rmax2;           // FP64 list radius squared
+  double half_skin;       // (r_max - rho) / 2
+  float r2c, r2h, r2k;    // guard band around rho^2: |d^2 - r2c| <= r2h is decided in FP64
+  float m2c, m2h;         // guard band around r_max^2 (list build)
+  VCtl *ctl;
+};
+
+__device__ __forceinline__ bool gate_off(const VArgs &A) { return A.ctl->flag == 0; }
+
+// device addresses of fields of the control block, formed on the host without dereferencing it
+static int *ctl_flag(VCtl *c) { return reinterpret_cast<int *>(reinterpret_cast<char *>(c) + offsetof(VCtl, flag)); }
+static unsigned long long *ctl_step(VCtl *c) {
+  return reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(c) + offsetof(VCtl, step));
+}
+
+// snapshot entry: wrapped position and unit phasor, each computed in FP64 and rounded once
+__device__ __forceinline__ float4 vsnap_of(const DevParams &P, const double *st) {
+  double xw, yw, sn, cn;
+  apply_pbc(P, st[0], st[1], xw, yw);
+  sincos(st[2], &sn, &cn);
+  return make_float4((float)xw, (float)yw, (float)cn, (float)sn);
+}
+
+// FP32 squared minimum-image distance minus c2
+__device__ __forceinline__ float dd_of(float4 o, float xi, float yi, float c2) {
+  float dx = o.x - xi;
+  dx -= __builtin_rintf(dx);
+  float dy = o.y - yi;
+  dy -= __builtin_rintf(dy);
+  return __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, -c2));
+}
+
+// ---------------------------------------------------------------------------
+// rebuild (every launch returns at once when the flag is clear)
+
+template <int S>
+__global__ __launch_bounds__(kVB) void vr_keys(DevParams P, VArgs A, const double *__restrict__ state,
+                                               int *__restrict__ key, int *__restrict__ slotc,
+                                               int *__restrict__ count) {
+  if (gate_off(A)) return;
+  const int p = blockIdx.x * kVB + threadIdx.x;
+  if (p >= A.n) return;
+  double xw, yw;
+  apply_pbc(P, state[(long)p * S], state[(long)p * S + 1], xw, yw);
+  const int c = cell_coord(yw, A.ncv) * A.ncv + cell_coord(xw, A.ncv);
+  key[p] = c;
+  slotc[p] = atomicAdd(count + c, 1);
+}
+
+__global__ __launch_bounds__(kVB) void vr_scatter(VArgs A, const int *__restrict__ key, const int *__restrict__ slotc,
+                                                  const int *__restrict__ start, int *__restrict__ tmp) {
+  if (gate_off(A)) return;
+  const int p = blockIdx.x * kVB + threadIdx.x;
+  if (p < A.n) tmp[start[key[p]] + slotc[p]] = p;
+}
+
+// stable rank inside the cell (by previous slot) and the gather into the new order, in scratch
+template <int S>
+__global__ __launch_bounds__(kVB) void vr_gather(VArgs A, const int *__restrict__ key, const int *__restrict__ start,
+                                                 const int *__restrict__ tmp, const double *__restrict__ state,
+                                                 const int *__restrict__ id, const double *__restrict__ nat,
+                                                 double *__restrict__ sstate, int *__restrict__ sid,
+                                                 double *__restrict__ snat, int *__restrict__ scell) {
+  if (gate_off(A)) return;
+  const int p = blockIdx.x * kVB + threadIdx.x;
+  if (p >= A.n) return;
+  const int c = key[p];
+  const int b = start[c], e = start[c + 1];
+  int r = 0;
+  for (int t = b; t < e; ++t) r += tmp[t] < p ? 1 : 0;
+  const int q = b + r;
+#pragma unroll
+  for (int k = 0; k < S; ++k) sstate[(long)q * S + k] = state[(long)p * S + k];
+  sid[q] = id[p];
+  if (nat) snat[q] = nat[p];
+  scell[q] = c;
+}
+
+// back into the resident arrays, with the snapshot of the new order
+template <int S>
+__global__ __launch_bounds__(kVB) void vr_settle(DevParams P, VArgs A, const double *__restrict__ sstate,
+                                                 const int *__restrict__ sid, const double *__restrict__ snat,
+                                                 const int *__restrict__ scell, double *__restrict__ state,
+                                                 int *__restrict__ id, double *__restrict__ nat,
+                                                 int *__restrict__ cell, float4 *__restrict__ snap,
+                                                 float *__restrict__ omega) {
+  if (gate_off(A)) return;
+  const int q = blockIdx.x * kVB + threadIdx.x;
+  if (q >= A.n) return;
+  double st[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    st[k] = sstate[(long)q * S + k];
+    state[(long)q * S + k] = st[k];
+  }
+  id[q] = sid[q];
+  if (nat) nat[q] = snat[q];
+  cell[q] = scell[q];
+  snap[q] = vsnap_of(P, st);
+  omega[q] = S == 4 ? (float)st[3] : 0.f;
+}
+
+// The lists: the candidates of the 3 x 3 cells around the particle's cell in the stencil order of
+// ParticleSystemFirstOrderPtr.cpp:83-88, ascending slot inside a cell; j is listed when its
+// distance from the wrapped x_i to the stored x_j is within r_max (:694-739).  FP32 test outside a
+// guard band, the reference's FP64 ClassC formula inside it.
+template <int S>
+__global__ __launch_bounds__(kVB) void vr_lists(DevParams P, VArgs A, const int *__restrict__ start,
+                                                const int *__restrict__ cell, const double *__restrict__ state,
+                                                const float4 *__restrict__ snap, int *__restrict__ list,
+                                                int *__restrict__ len) {
+  if (gate_off(A)) return;
+  const int q = blockIdx.x * kVB + threadIdx.x;
+  if (q >= A.n) return;
+  const int c = cell[q];
+  const int cx = c % A.ncv, cy = c / A.ncv;
+  const float4 me = snap[q];
+  double xi, yi;
+  apply_pbc(P, state[(long)q * S], state[(long)q * S + 1], xi, yi);
+  int w = 0;
+  for (int dyc = -1; dyc <= 1; ++dyc) {
+    const int jy = adjust_cell(cy + dyc, A.ncv);
+    for (int dxc = -1; dxc <= 1; ++dxc) {
+      const int cj = jy * A.ncv + adjust_cell(cx + dxc, A.ncv);
+      const int e = start[cj + 1];
+      for (int u = start[cj]; u < e; ++u) {
+        if (u == q) continue;
+        const float dd = dd_of(snap[u], me.x, me.y, A.m2c);
+        bool in = dd < 0.f;
+        if (__builtin_fabsf(dd) <= A.m2h) {
+          double dx, dy;
+          class_c_unsigned(P, xi, yi, state[(long)u * S], state[(long)u * S + 1], dx, dy);
+          in = dx * dx + dy * dy <= A.rmax2;
+        }
+        if (in) {
+          if (w < A.K) list[(long)w * A.n + q] = u;
+          ++w;
+        }
+      }
+    }
+  }
+  // a list longer than the width is not stored: the step walks that particle's 3 x 3 cells instead
+  len[q] = w <= A.K ? w : -1;
+  if (w > A.K) atomicAdd(&A.ctl->overflow, 1ull);
+}
+
+// ---------------------------------------------------------------------------
+// the step
+
+// Four candidates against particle p: weights, phasor partials (and omega-weighted for SRK).  The
+// candidates are list entries t .. t+3 of slot p (CELLS = false) or the slots t .. t+3 of a cell of
+// the last rebuild (CELLS = true: a particle whose list outgrew the width; p itself is skipped).
+// TAIL: only those below `end` count.
+template <int S, bool SRK, bool TAIL, bool CELLS>
+__device__ __forceinline__ void walk4(const DevParams &P, const VArgs &A, const int *__restrict__ list,
+                                      const float4 *__restrict__ snap, const float *__restrict__ omega,
+                                      const double *__restrict__ state, int p, int t, int end, float xi, float yi,
+                                      const double *pi, float *ax, float *ay, float &an, float *bx, float *by) {
+  int j[4];
+  float4 o[4];
+  float dd[4], m[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const bool in = !TAIL || t + k < end;
+    j[k] = CELLS ? (in ? t + k : p) : (in ? list[(long)(t + k) * A.n + p] : p);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    o[k] = snap[j[k]];
+    dd[k] = dd_of(o[k], xi, yi, A.r2c);
+    m[k] = __builtin_amdgcn_fmed3f(-A.r2k * dd[k], 0.f, 1.f);
+  }
+  const float dmin = __builtin_fminf(__builtin_fminf(__builtin_fabsf(dd[0]), __builtin_fabsf(dd[1])),
+                                     __builtin_fminf(__builtin_fabsf(dd[2]), __builtin_fabsf(dd[3])));
+  if (dmin <= A.r2h) {  // rare: the reference's FP64 test (x_i as stored, rhs_verlet's :566-581)
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (__builtin_fabsf(dd[k]) <= A.r2h) {
+        double dx, dy;
+        class_c_unsigned(P, pi[0], pi[1], state[(long)j[k] * S], state[(long)j[k] * S + 1], dx, dy);
+        m[k] = dx * dx + dy * dy <= P.rho2 ? 1.f : 0.f;
+      }
+  }
+  if (TAIL || CELLS) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) m[k] = ((!TAIL || t + k < end) && (!CELLS || j[k] != p)) ? m[k] : 0.f;
+  }
+  an += (m[0] + m[1]) + (m[2] + m[3]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    ax[k] = __builtin_fmaf(m[k], o[k].z, ax[k]);
+    ay[k] = __builtin_fmaf(m[k], o[k].w, ay[k]);
+    if (SRK) {
+      const float mo = m[k] * omega[j[k]];
+      bx[k] = __builtin_fmaf(mo, o[k].z, bx[k]);
+      by[k] = __builtin_fmaf(mo, o[k].w, by[k]);
+    }
+  }
+}
+
+// candidates [a, b) (list entries, or cell slots with CELLS) in groups of four, four FP32 partials per
+// sum flushed into FP64 every 16 candidates
+template <int S, bool SRK, bool CELLS>
+__device__ __forceinline__ void walk_range(const DevParams &P, const VArgs &A, const int *__restrict__ list,
+                                           const float4 *__restrict__ snap, const float *__restrict__ omega,
+                                           const double *__restrict__ state, int p, int a, int b, float xi, float yi,
+                                           const double *pi, double &sx, double &sy, double &tx, double &ty,
+                                           int &nnb) {
+  for (int t0 = a; t0 < b; t0 += 16) {
+    float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an = 0.f;
+    float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
+    const int te = min(t0 + 16, b);
+    int t = t0;
+    for (; t + 4 <= te; t += 4)
+      walk4<S, SRK, false, CELLS>(P, A, list, snap, omega, state, p, t, te, xi, yi, pi, ax, ay, an, bx, by);
+    if (t < te) walk4<S, SRK, true, CELLS>(P, A, list, snap, omega, state, p, t, te, xi, yi, pi, ax, ay, an, bx, by);
+    sx += ((double)ax[0] + (double)ax[1]) + ((double)ax[2] + (double)ax[3]);
+    sy += ((double)ay[0] + (double)ay[1]) + ((double)ay[2] + (double)ay[3]);
+    nnb += (int)an;
+    if (SRK) {
+      tx += ((double)bx[0] + (double)bx[1]) + ((double)bx[2] + (double)bx[3]);
+      ty += ((double)by[0] + (double)by[1]) + ((double)by[2] + (double)by[3]);
+    }
+  }
+}
+
+template <int S, bool SRK>
+__global__ __launch_bounds__(kVB) void vs_step(DevParams P, VArgs A, StepArgs U, const int *__restrict__ list,
+                                               const int *__restrict__ len, const int *__restrict__ start,
+                                               const int *__restrict__ cell, const double *__restrict__ state,
+                                               const int *__restrict__ id, const double *__restrict__ nat,
+                                               const float4 *__restrict__ snap, const float *__restrict__ omega,
+                                               double *__restrict__ nstate, float4 *__restrict__ nsnap,
+                                               float *__restrict__ nomega, double *__restrict__ compl_out) {
+  __shared__ double wmax[kVB / 64];
+  __shared__ int last;
+  const int p = blockIdx.x * kVB + threadIdx.x;
+  double d2 = 0.0;
+  if (p < A.n) {
+    const double *pi = state + (long)p * S;
+    const float4 me = snap[p];
+    const int L = len[p];
+    double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
+    int nnb = 0;
+    if (L >= 0) {
+      walk_range<S, SRK, false>(P, A, list, snap, omega, state, p, 0, L, me.x, me.y, pi, sx, sy, tx, ty, nnb);
+    } else {  // (rare) the list outgrew the width: the 3 x 3 cells of the last rebuild hold every candidate
+      const int c = cell[p];
+      const int cx = c % A.ncv, cy = c / A.ncv;
+      for (int dyc = -1; dyc <= 1; ++dyc)
+        for (int dxc = -1; dxc <= 1; ++dxc) {
+          const int cj = adjust_cell(cy + dyc, A.ncv) * A.ncv + adjust_cell(cx + dxc, A.ncv);
+          walk_range<S, SRK, true>(P, A, list, snap, omega, state, p, start[cj], start[cj + 1], me.x, me.y, pi, sx,
+                                   sy, tx, ty, nnb);
+        }
+    }
+    Coef K;
+    K.xi = P.xi;
+    K.sigma = P.sigma;
+    K.int_phi = U.int_phi;
+    K.sq2d = U.sq2d;
+    K.seed = U.seed;
+    const uint64_t step = U.step_dev ? (uint64_t)*U.step_dev : U.step;
+    const int i = id[p];
+    double out[S], C;
+    step_particle<S, SRK>(P, K, pi, i, nat ? nat[p] : 0.0, make_double2(sx, sy), make_double2(tx, ty), nnb, U,
+                          step, out, C, &d2);
+#pragma unroll
+    for (int k = 0; k < S; ++k) nstate[(long)p * S + k] = out[k];
+    nsnap[p] = vsnap_of(P, out);
+    nomega[p] = S == 4 ? (float)out[3] : 0.f;
+    if (SRK && compl_out) compl_out[i] = C;
+  }
+  // the step's largest squared displacement: wave, workgroup, then one atomic per workgroup
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = wmax[0];
+    for (int w = 1; w < kVB / 64; ++w) bm = fmax(bm, wmax[w]);
+    atomicMax(&A.ctl->max2_bits, (unsigned long long)__double_as_longlong(bm));
+    __threadfence();
+    last = atomicAdd(&A.ctl->done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    // every other workgroup has added its maximum: CalculateMaxDisplacement's bookkeeping
+    // (ParticleSystemFirstOrderPtr.cpp:851-860) in the exact mode (one step of margin)
+    const double max2 = __longlong_as_double((long long)atomicAdd(&A.ctl->max2_bits, 0ull));
+    const double stepd = sqrt(max2);
+    double acc = A.ctl->acc + stepd;
+    const int flag = acc + stepd > A.half_skin ? 1 : 0;
+    if (flag) acc = 0.0;
+    A.ctl->acc = acc;
+    A.ctl->flag = flag;
+    A.ctl->max2_bits = 0ull;
+    A.ctl->done = 0u;
+    if (U.step_inc) *U.step_inc += 1ull;
+    __threadfence();
+  }
+}
+
+// the largest particle count of any 3 x 3 block of cells (the candidates of one list)
+__global__ __launch_bounds__(kVB) void vc_block_max(VArgs A, const int *__restrict__ count, int *__restrict__ out) {
+  const int c = blockIdx.x * kVB + threadIdx.x;
+  if (c >= A.cells) return;
+  const int cx = c % A.ncv, cy = c / A.ncv;
+  int s = 0;
+  for (int dy = -1; dy <= 1; ++dy)
+    for (int dx = -1; dx <= 1; ++dx) s += count[adjust_cell(cy + dy, A.ncv) * A.ncv + adjust_cell(cx + dx, A.ncv)];
+  atomicMax(out, s);
+}
+
+__global__ void vc_enter(VCtl *c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    c->flag = 1;  // should_update_lists_ starts true (ParticleSystemFirstOrderPtr.cpp:99)
+    c->acc = 0.0;
+    c->max2_bits = 0ull;
+    c->done = 0u;
+    c->overflow = 0ull;
+  }
+}
+
+__global__ void vc_set_step(VCtl *c, unsigned long long v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) c->step = v;
+}
+
+__global__ __launch_bounds__(kVB) void vc_iota(int n, int *__restrict__ id) {
+  const int p = blockIdx.x * kVB + threadIdx.x;
+  if (p < n) id[p] = p;
+}
+
+template <int S>
+__global__ __launch_bounds__(kVB) void vc_unpermute(int n, const double *__restrict__ vstate, const int *__restrict__ id,
+                                                    double *__restrict__ state) {
+  const int p = blockIdx.x * kVB + threadIdx.x;
+  if (p >= n) return;
+  const int i = id[p];
+#pragma unroll
+  for (int k = 0; k < S; ++k) state[(long)i * S + k] = vstate[(long)p * S + k];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct VerletFast {
+  int n = 0, s = 4, ncv = 1, cells = 1, K = 0, nsb = 1;
+  double rho = 0.0, r_max = 0.0;
+  VArgs A{};
+  double *vstate[2] = {nullptr, nullptr};
+  float4 *vsnap[2] = {nullptr, nullptr};
+  float *vomega[2] = {nullptr, nullptr};
+  int *vid = nullptr, *vcell = nullptr;
+  double *vnat = nullptr;
+  int *list = nullptr, *len = nullptr;
+  int *key = nullptr, *slotc = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr, *blocksum = nullptr;
+  double *sstate = nullptr, *snat = nullptr;
+  int *sid = nullptr, *scell = nullptr;
+  VCtl *ctl = nullptr;
+  int *bmax = nullptr;
+  size_t list_cap = 0;  // ints allocated for the lists
+  bool resident = false, resident_nat = false;
+  int cur = 0;
+  struct StepGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } sg[2][2];  // [bulk, single step][parity]
+  int sg_srk = -1, sg_higher = -1;
+  double sg_dt = 0.0, sg_dphi = 0.0, sg_friction = 0.0;
+  uint64_t sg_seed = 0;
+  const double *sg_nat = nullptr;
+  double *sg_compl = nullptr;
+  long captures = 0;
+};
+
+const char *vf_last_error() { return g_verr.c_str(); }
+
+static int vfail(int code, const std::string &m) {
+  g_verr = m;
+  return code;
+}
+
+#define VCHK(call)                                                                                      \
+  do {                                                                                                  \
+    hipError_t e_ = (call);                                                                             \
+    if (e_ != hipSuccess) return vfail(SPSS_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static unsigned vblocks(long n) { return (unsigned)((n + kVB - 1) / kVB); }
+
+static void drop_graphs(VerletFast *v) {
+  for (auto &row : v->sg)
+    for (auto &g : row) {
+      if (g.exec) hipGraphExecDestroy(g.exec);
+      if (g.graph) hipGraphDestroy(g.graph);
+      g.exec = nullptr;
+      g.graph = nullptr;
+    }
+}
+
+void vf_destroy(VerletFast *v) {
+  if (!v) return;
+  drop_graphs(v);
+  void *b[] = {v->vstate[0], v->vstate[1], v->vsnap[0], v->vsnap[1], v->vomega[0], v->vomega[1], v->vid,
+               v->vcell,     v->vnat,      v->list,     v->len,      v->key,       v->slotc,     v->count,
+               v->start,     v->tmp,       v->blocksum, v->sstate,   v->snat,      v->sid,       v->scell,
+               v->ctl,       v->bmax};
+  for (void *p : b) hipFree(p);
+  delete v;
+}
+
+int vf_create(VerletFast **out, hipStream_t st, const DevParams &P, double rho, double r_max) {
+  *out = nullptr;
+  if (P.s != 3 && P.s != 4) return vfail(SPSS_ERR_UNSUPPORTED, "FP32C Verlet lists are implemented for models 3 and 4");
+  if (P.xs != 1.0 || P.ys != 1.0)
+    return vfail(SPSS_ERR_UNSUPPORTED, "FP32C Verlet lists assume the unit box of the reference's cell grid");
+  if (!(r_max > rho)) return vfail(SPSS_ERR_INVALID, "the Verlet skin must be positive");
+  const int ncv = (int)(1.0 / r_max);
+  if (ncv < 3)
+    return vfail(SPSS_ERR_UNSUPPORTED, "FP32C Verlet lists need int(1/r_max) >= 3 (use the FP64 precision there)");
+  VerletFast *v = new VerletFast();
+  const int n = P.n;
+  v->n = n;
+  v->s = P.s;
+  v->ncv = ncv;
+  v->cells = ncv * ncv;
+  v->nsb = scan_blocks(v->cells);
+  v->rho = rho;
+  v->r_max = r_max;
+  hipError_t e = hipSuccess;
+  auto A = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  const size_t nn = (size_t)n;
+  for (int b = 0; b < 2; ++b) {
+    A(hipMalloc(&v->vstate[b], sizeof(double) * nn * P.s));
+    A(hipMalloc(&v->vsnap[b], sizeof(float4) * nn));
+    A(hipMalloc(&v->vomega[b], sizeof(float) * nn));
+  }
+  A(hipMalloc(&v->vid, sizeof(int) * nn));
+  A(hipMalloc(&v->vcell, sizeof(int) * nn));
+  A(hipMalloc(&v->vnat, sizeof(double) * nn));
+  A(hipMalloc(&v->len, sizeof(int) * nn));
+  A(hipMalloc(&v->key, sizeof(int) * nn));
+  A(hipMalloc(&v->slotc, sizeof(int) * nn));
+  A(hipMalloc(&v->count, sizeof(int) * (size_t)v->cells));
+  A(hipMalloc(&v->start, sizeof(int) * ((size_t)v->cells + 1)));
+  A(hipMalloc(&v->tmp, sizeof(int) * nn));
+  A(hipMalloc(&v->blocksum, sizeof(int) * ((size_t)v->nsb + 1)));
+  A(hipMalloc(&v->sstate, sizeof(double) * nn * P.s));
+  A(hipMalloc(&v->snat, sizeof(double) * nn));
+  A(hipMalloc(&v->sid, sizeof(int) * nn));
+  A(hipMalloc(&v->scell, sizeof(int) * nn));
+  A(hipMalloc(&v->ctl, sizeof(VCtl)));
+  A(hipMalloc(&v->bmax, sizeof(int)));
+  if (e != hipSuccess) {
+    vf_destroy(v);
+    return vfail(SPSS_ERR_DEVICE, std::string("Verlet list buffers: ") + hipGetErrorString(e));
+  }
+  launch_fill_int(st, v->cells, v->count, 0);
+  launch_fill_int(st, (long)(sizeof(VCtl) / sizeof(int)), reinterpret_cast<int *>(v->ctl), 0);
+  // guard bands: coordinates in [0, 1] rounded once to FP32 (<= 2^-25 each), the difference and
+  // the nearest-integer shift exact, so |dx| errs by <= 2^-24; the squared distance then by
+  // <= 2 sqrt(2) r 2^-24 plus a few ulps -- times a safety factor of 8
+  const double eps = std::ldexp(1.0, -24);
+  auto band = [&](double r) { return r * r * std::max(8.0 * (2.0 * std::sqrt(2.0) * eps / r + 4.0 * eps), 64.0 * eps); };
+  VArgs &a = v->A;
+  a.n = n;
+  a.ncv = ncv;
+  a.cells = v->cells;
+  a.rmax2 = r_max * r_max;
+  a.half_skin = 0.5 * (r_max - rho);
+  a.r2c = (float)(rho * rho);
+  a.r2h = (float)band(rho);
+  a.r2k = (float)std::ldexp(1.0, std::ilogb(4.0 / (double)a.r2h) + 1);  // power of two, r2k * r2h > 4
+  a.m2c = (float)(r_max * r_max);
+  a.m2h = (float)band(r_max);
+  a.ctl = v->ctl;
+  *out = v;
+  return SPSS_OK;
+}
+
+int vf_list_width(const VerletFast *v) { return v ? v->K : 0; }
+bool vf_is_resident(const VerletFast *v) { return v && v->resident; }
+void vf_drop(VerletFast *v) {
+  if (v) v->resident = false;
+}
+long vf_graph_captures(const VerletFast *v) { return v ? v->captures : 0; }
+
+bool vf_graph_ready(const VerletFast *v, const double *nat, int srk) {
+  (void)srk;  // omega is kept for every model-4 step, so either stepper may follow the other
+  return v && v->resident && (nat != nullptr) == v->resident_nat;
+}
+
+// Entry into the resident copy.  The list width K is sized here from the state entered: the
+// densest 3 x 3 block of cells holds bmax candidates, i.e. a local density of bmax / (9 Hv^2), so a
+// list holds about lam = bmax pi r_max^2 / (9 Hv^2); K = 1.5 lam + 6 sqrt(lam) + 32 (at most bmax
+// + 16) leaves room for the density to grow.  A list that outgrows K later is counted
+// (vf_status); entering again from a fresh state sizes K anew.
+int vf_enter(VerletFast *v, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat) {
+  (void)srk;
+  const unsigned nb = vblocks(v->n);
+  VCHK(hipMemcpyAsync(v->vstate[0], state, sizeof(double) * (size_t)v->n * v->s, hipMemcpyDeviceToDevice, st));
+  vc_iota<<<nb, kVB, 0, st>>>(v->n, v->vid);
+  if (nat) VCHK(hipMemcpyAsync(v->vnat, nat, sizeof(double) * (size_t)v->n, hipMemcpyDeviceToDevice, st));
+  vc_enter<<<1, 64, 0, st>>>(v->ctl);
+  // the cell counts of the state entered (the flag is set: vr_keys runs), their densest 3 x 3
+  // block, and the counts cleared again for the first rebuild
+  launch_fill_int(st, 1, v->bmax, 0);
+  if (v->s == 4)
+    vr_keys<4><<<nb, kVB, 0, st>>>(P, v->A, v->vstate[0], v->key, v->slotc, v->count);
+  else
+    vr_keys<3><<<nb, kVB, 0, st>>>(P, v->A, v->vstate[0], v->key, v->slotc, v->count);
+  vc_block_max<<<vblocks(v->cells), kVB, 0, st>>>(v->A, v->count, v->bmax);
+  int bmax = 0;
+  VCHK(hipMemcpyAsync(&bmax, v->bmax, sizeof(int), hipMemcpyDeviceToHost, st));
+  launch_fill_int(st, v->cells, v->count, 0);
+  VCHK(hipStreamSynchronize(st));
+  const double Hv = 1.0 / (double)v->ncv;
+  const double lam = (double)bmax * M_PI * v->r_max * v->r_max / (9.0 * Hv * Hv);
+  int K = (int)std::ceil(1.5 * lam + 6.0 * std::sqrt(lam) + 32.0);
+  K = std::max(16, std::min(K, bmax + 16));
+  K = (K + 15) / 16 * 16;
+  // an upper bound on the width from the environment (memory cap; lists beyond it walk their cells)
+  if (const char *cap = std::getenv("SPSS_VERLET_WIDTH_CAP")) K = std::max(4, std::min(K, std::atoi(cap)));
+  const size_t need = (size_t)v->n * (size_t)K;
+  if (need > v->list_cap) {
+    drop_graphs(v);  // captured launches hold the old list pointer
+    hipFree(v->list);
+    v->list = nullptr;
+    v->list_cap = 0;
+    VCHK(hipMalloc(&v->list, sizeof(int) * need));
+    v->list_cap = need;
+  }
+  if (K != v->K) drop_graphs(v);  // the width is a launch argument of the captured steps
+  v->K = K;
+  v->A.K = K;
+  VCHK(hipGetLastError());
+  v->cur = 0;
+  v->resident = true;
+  v->resident_nat = nat != nullptr;
+  return SPSS_OK;
+}
+
+// the rebuild launches (gated on the device flag) and the step, from buffer v->cur
+template <int S, bool SRK>
+static void launch_step(VerletFast *v, hipStream_t st, const DevParams &P, const StepArgs &U, double *compl_out) {
+  const int c = v->cur, o = 1 - c;
+  const unsigned nb = vblocks(v->n);
+  const VArgs &A = v->A;
+  const double *nat = v->resident_nat ? v->vnat : nullptr;
+  vr_keys<S><<<nb, kVB, 0, st>>>(P, A, v->vstate[c], v->key, v->slotc, v->count);
+  scan_local<<<v->nsb, 256, 0, st>>>(v->count, nullptr, v->cells, v->start, v->blocksum, ctl_flag(A.ctl));
+  scan_add<<<v->nsb, 256, 0, st>>>(v->count, nullptr, v->cells, v->start, v->blocksum, v->nsb, nullptr,
+                                   ctl_flag(A.ctl));
+  vr_scatter<<<nb, kVB, 0, st>>>(A, v->key, v->slotc, v->start, v->tmp);
+  vr_gather<S><<<nb, kVB, 0, st>>>(A, v->key, v->start, v->tmp, v->vstate[c], v->vid, nat, v->sstate, v->sid,
+                                   v->snat, v->scell);
+  vr_settle<S><<<nb, kVB, 0, st>>>(P, A, v->sstate, v->sid, v->snat, v->scell, v->vstate[c], v->vid,
+                                   nat ? v->vnat : nullptr, v->vcell, v->vsnap[c], v->vomega[c]);
+  vr_lists<S><<<nb, kVB, 0, st>>>(P, A, v->start, v->vcell, v->vstate[c], v->vsnap[c], v->list, v->len);
+  vs_step<S, SRK><<<nb, kVB, 0, st>>>(P, A, U, v->list, v->len, v->start, v->vcell, v->vstate[c], v->vid, nat, v->vsnap[c],
+                                      v->vomega[c], v->vstate[o], v->vsnap[o], v->vomega[o], SRK ? compl_out : nullptr);
+}
+
+static StepArgs vstep_args(double dt, double D_phi, double friction, int noise_mode, const double *noise,
+                           uint64_t seed, uint64_t step) {
+  StepArgs U{};
+  U.dt = dt;
+  U.sqrt_dt = std::sqrt(dt);
+  U.int_phi = std::sqrt(2.0 * D_phi);
+  U.c15 = 0.5 * std::pow(dt, 1.5);
+  U.is3 = 1.0 / std::sqrt(3.0);
+  U.sq2d = std::sqrt(2.0 * D_phi);
+  U.friction = friction;
+  U.noise_mode = noise_mode;
+  U.noise = noise;
+  U.seed = seed;
+  U.step = step;
+  return U;
+}
+
+static void step_impl(VerletFast *v, hipStream_t st, const DevParams &P, bool do_srk, const StepArgs &U,
+                      double *compl_out) {
+  if (do_srk)
+    launch_step<4, true>(v, st, P, U, compl_out);
+  else if (v->s == 4)
+    launch_step<4, false>(v, st, P, U, compl_out);
+  else
+    launch_step<3, false>(v, st, P, U, compl_out);
+  v->cur = 1 - v->cur;
+}
+
+int vf_step(VerletFast *v, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat,
+            double *compl_out, double dt, double D_phi, double friction, int noise_mode, const double *noise,
+            uint64_t seed, uint64_t step) {
+  const bool do_srk = srk && v->s == 4;
+  if (!vf_graph_ready(v, nat, do_srk)) {
+    const int rc = vf_enter(v, st, P, do_srk, state, nat);
+    if (rc != SPSS_OK) return rc;
+  }
+  step_impl(v, st, P, do_srk, vstep_args(dt, D_phi, friction, noise_mode, noise, seed, step), compl_out);
+  VCHK(hipGetLastError());
+  return SPSS_OK;
+}
+
+static int capture(VerletFast *v, hipStream_t st, const DevParams &P, bool do_srk, const StepArgs &U,
+                   double *compl_out, int steps, VerletFast::StepGraph &g) {
+  const int cur0 = v->cur;
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) return vfail(SPSS_ERR_DEVICE, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  for (int k = 0; k < steps; ++k) step_impl(v, st, P, do_srk, U, compl_out);
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(st, &graph);
+  v->cur = cur0;  // nothing ran
+  if (e != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return vfail(SPSS_ERR_DEVICE, std::string("capture of the Verlet step graph failed: ") + hipGetErrorString(e));
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(graph);
+    return vfail(SPSS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  g.graph = graph;
+  g.exec = exec;
+  ++v->captures;
+  return SPSS_OK;
+}
+
+static StepArgs graph_setup(VerletFast *v, const DevParams &P, bool do_srk, const double *nat, double *compl_out,
+                            double dt, double D_phi, double friction, uint64_t seed) {
+  const bool same = v->sg_srk == (int)do_srk && v->sg_higher == P.higher_order && v->sg_dt == dt &&
+                    v->sg_dphi == D_phi && v->sg_friction == friction && v->sg_seed == seed && v->sg_nat == nat &&
+                    v->sg_compl == compl_out;
+  if (!same) {
+    drop_graphs(v);
+    v->sg_srk = do_srk;
+    v->sg_higher = P.higher_order;
+    v->sg_dt = dt;
+    v->sg_dphi = D_phi;
+    v->sg_friction = friction;
+    v->sg_seed = seed;
+    v->sg_nat = nat;
+    v->sg_compl = compl_out;
+  }
+  StepArgs U = vstep_args(dt, D_phi, friction, 3, nullptr, seed, 0);
+  U.step_dev = ctl_step(v->ctl);
+  U.step_inc = ctl_step(v->ctl);
+  return U;
+}
+
+static int ensure_graph(VerletFast *v, hipStream_t st, const DevParams &P, bool do_srk, const StepArgs &U,
+                        double *compl_out, int size) {
+  VerletFast::StepGraph &g = v->sg[size == 1 ? 1 : 0][v->cur];
+  if (g.exec) return SPSS_OK;
+  return capture(v, st, P, do_srk, U, compl_out, size, g);
+}
+
+int vf_prepare_graphs(VerletFast *v, hipStream_t st, const DevParams &P, int srk, const double *state,
+                      const double *nat, double *compl_out, double dt, double D_phi, double friction, uint64_t seed) {
+  (void)state;
+  const bool do_srk = srk && v->s == 4;
+  if (!vf_graph_ready(v, nat, do_srk)) return vfail(SPSS_ERR_INVALID, "no resident copy to capture steps of");
+  const StepArgs U = graph_setup(v, P, do_srk, nat, compl_out, dt, D_phi, friction, seed);
+  for (int side = 0; side < 2; ++side) {
+    for (int size : {kVGraphSteps, 1}) {
+      const int rc = ensure_graph(v, st, P, do_srk, U, compl_out, size);
+      if (rc != SPSS_OK) {
+        if (side) v->cur = 1 - v->cur;
+        return rc;
+      }
+    }
+    v->cur = 1 - v->cur;
+  }
+  return SPSS_OK;
+}
+
+long vf_run_graph(VerletFast *v, hipStream_t st, const DevParams &P, int srk, const double *state, const double *nat,
+                  double *compl_out, double dt, double D_phi, double friction, uint64_t seed, uint64_t step,
+                  long nsteps) {
+  (void)state;
+  const bool do_srk = srk && v->s == 4;
+  if (nsteps < 1 || !vf_graph_ready(v, nat, do_srk)) return 0;
+  const StepArgs U = graph_setup(v, P, do_srk, nat, compl_out, dt, D_phi, friction, seed);
+  const long reps = nsteps / kVGraphSteps, rest = nsteps % kVGraphSteps;
+  int rc = SPSS_OK;
+  if (reps > 0) rc = ensure_graph(v, st, P, do_srk, U, compl_out, kVGraphSteps);
+  for (int side = 0; side < 2 && side < rest && rc == SPSS_OK; ++side) {
+    if (side) v->cur = 1 - v->cur;
+    rc = ensure_graph(v, st, P, do_srk, U, compl_out, 1);
+    if (side) v->cur = 1 - v->cur;
+  }
+  if (rc != SPSS_OK) return -1;
+  vc_set_step<<<1, 64, 0, st>>>(v->ctl, (unsigned long long)step);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    g_verr = std::string("vc_set_step: ") + hipGetErrorString(e);
+    return -1;
+  }
+  for (long r = 0; r < reps; ++r)
+    if ((e = hipGraphLaunch(v->sg[0][v->cur].exec, st)) != hipSuccess) {
+      g_verr = std::string("hipGraphLaunch: ") + hipGetErrorString(e);
+      return -1;
+    }
+  for (long r = 0; r < rest; ++r) {
+    if ((e = hipGraphLaunch(v->sg[1][v->cur].exec, st)) != hipSuccess) {
+      g_verr = std::string("hipGraphLaunch: ") + hipGetErrorString(e);
+      return -1;
+    }
+    v->cur = 1 - v->cur;
+  }
+  return nsteps;
+}
+
+int vf_sync(VerletFast *v, hipStream_t st, double *state) {
+  if (!v->resident) return SPSS_OK;
+  const unsigned nb = vblocks(v->n);
+  if (v->s == 4)
+    vc_unpermute<4><<<nb, kVB, 0, st>>>(v->n, v->vstate[v->cur], v->vid, state);
+  else
+    vc_unpermute<3><<<nb, kVB, 0, st>>>(v->n, v->vstate[v->cur], v->vid, state);
+  VCHK(hipGetLastError());
+  return SPSS_OK;
+}
+
+int vf_status(VerletFast *v, hipStream_t st, int *should_update, double *acc, long *overflow) {
+  VCtl c{};
+  if (v->resident) {
+    VCHK(hipMemcpyAsync(&c, v->ctl, sizeof(VCtl), hipMemcpyDeviceToHost, st));
+    VCHK(hipStreamSynchronize(st));
+  } else {
+    c.flag = 1;  // the next step enters and rebuilds
+  }
+  if (should_update) *should_update = c.flag;
+  if (acc) *acc = c.acc;
+  if (overflow) *overflow = (long)c.overflow;
+  return SPSS_OK;
+}
+
+int vf_lists(VerletFast *v, hipStream_t st, std::vector<int> *counts, std::vector<int> *flat) {
+  const int n = v->n;
+  counts->assign(n, 0);
+  flat->clear();
+  if (!v->resident) return SPSS_OK;
+  std::vector<int> id(n), len(n), list((size_t)n * v->K);
+  VCHK(hipMemcpyAsync(id.data(), v->vid, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  VCHK(hipMemcpyAsync(len.data(), v->len, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  VCHK(hipMemcpyAsync(list.data(), v->list, sizeof(int) * list.size(), hipMemcpyDeviceToHost, st));
+  VCHK(hipStreamSynchronize(st));
+  for (int p = 0; p < n; ++p)
+    if (len[p] < 0)
+      return vfail(SPSS_ERR_CAPACITY, "some Verlet lists outgrew the width " + std::to_string(v->K) +
+                                          " and are not stored (those particles walk their 3 x 3 cells)");
+  std::vector<int> slot_of(n);
+  for (int p = 0; p < n; ++p) slot_of[id[p]] = p;
+  for (int i = 0; i < n; ++i) {
+    const int p = slot_of[i];
+    (*counts)[i] = len[p];
+    for (int t = 0; t < len[p]; ++t) flat->push_back(id[list[(size_t)t * n + p]]);
+  }
+  return SPSS_OK;
+}
+
+}  // namespace spss

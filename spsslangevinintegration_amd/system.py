@@ -204,6 +204,24 @@ class ParticleSystem:
                                              C.byref(k)))
         return table, k.value
 
+    def autotune_all_pairs(self, probe_evals=3):
+        """times the all-pairs kernel's workgroup / j-tile shapes on the current state and keeps the
+        fastest; returns (ms per evaluation for each shape, chosen index)"""
+        table = np.zeros(6)
+        chosen = C.c_int()
+        A.check(self.lib.spss_autotune_all_pairs(self.h, int(probe_evals), A.ptr(table), 6, C.byref(chosen)))
+        return table, chosen.value
+
+    def set_all_pairs_config(self, index):
+        A.check(self.lib.spss_set_all_pairs_config(self.h, int(index)))
+
+    @property
+    def all_pairs_config(self):
+        """(particles per workgroup, j per LDS tile) of the all-pairs kernel"""
+        b, j = C.c_int(), C.c_int()
+        A.check(self.lib.spss_get_all_pairs_config(self.h, C.byref(b), C.byref(j)))
+        return b.value, j.value
+
     @property
     def cell_subdivision(self):
         k = C.c_int()

@@ -1,0 +1,108 @@
+"""Replica batches of the ensemble configuration against the FP64 oracle, and the full-size
+512-replica ensemble against the plain per-replica loop.
+
+* Batch vs oracle: replicas with m in {0, 0.5, 1} (m = 0 is the first-order model, s = 3; m > 0
+  maps to the reference's s = 4 model with xi = 1/m, sigma/m, D/m^2, omega/m) and N from 2,048 to
+  65,536 at rho = 0.03, v0 = 0.01, dt = 0.005, Philox seed 0x5EED0004 + r.  Every replica is
+  stepped by the oracle (ParticleSystemFirstOrderPtr's linked-list RHS + StochasticEulerStepperPtr,
+  FP64) fed the draws the device makes for it, regenerated on the host (spss_philox_normals with
+  the replica's seed, the step and the particle's index inside its replica).  After 12 steps, model 4:
+  positions within 1e-12, phases within 1e-9, omega within 1e-7 (the FP32C pair-sum accuracy);
+  model 3, whose phase takes the coupling directly: positions within 1e-11, phases within 1e-7.
+* Full size: the 512 replicas of the ensemble configuration (10.1M particles) for 20 graph-replayed
+  steps; every replica equals a single FP32C system stepped alone by spss_run, bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle.pyoracle import METHOD_LINKED_LIST
+from spsslangevinintegration_amd import _capi
+
+pytestmark = pytest.mark.gpu
+
+RHO, V0, DT = 0.03, 0.01, 0.005
+
+
+@pytest.fixture(scope="module")
+def spss():
+    import spsslangevinintegration_amd as m
+    return m
+
+
+def _group(model, ns, ms, seed):
+    rng = np.random.default_rng(seed)
+    R = len(ns)
+    sigma = rng.uniform(0.5, 4.0, R)
+    D = rng.uniform(0.0, 0.2, R)
+    mm = np.where(np.asarray(ms) > 0, ms, 1.0)
+    xi = 1.0 / mm if model == 4 else np.zeros(R)
+    sig = sigma / mm if model == 4 else sigma
+    Dp = D / mm ** 2 if model == 4 else D
+    seeds = 0x5EED0004 + np.arange(R) + (0 if model == 4 else 1000)
+    states, nats = [], []
+    for r in range(R):
+        st = np.zeros((ns[r], model))
+        st[:, 0], st[:, 1] = rng.random(ns[r]), rng.random(ns[r])
+        st[:, 2] = rng.random(ns[r]) * 2 * np.pi
+        states.append(st.reshape(-1))
+        w = rng.normal(0, 0.2, ns[r])
+        nats.append(w / mm[r] if model == 4 else w)
+    return xi, sig, Dp, seeds, states, nats
+
+
+@pytest.mark.parametrize("model,ns,ms", [
+    (4, [2048, 65536, 9001, 30000], [0.5, 1.0, 1.0, 0.5]),
+    (3, [2048, 65536, 12345], [0.0, 0.0, 0.0]),
+])
+def test_batch_replicas_vs_oracle(spss, model, ns, ms):
+    steps = 12
+    print(f"[gpu] batch vs oracle model={model} ns={ns}", flush=True)
+    xi, sig, Dp, seeds, states, nats = _group(model, ns, ms, 40 + model)
+    b = spss.ReplicaBatch(model, ns, xi, sig, Dp, seeds, v0=V0, rho=RHO)
+    b.set_natural_frequencies(np.concatenate(nats))
+    b.set_state(np.concatenate(states))
+    b.run(steps, DT, observe_every=0, graph=True)
+    got = b.get_state()
+    for r, n in enumerate(ns):
+        o = oracle.Oracle(model=model, n=n, v0=V0, xi=xi[r], sigma=sig[r], rho=RHO, D_phi=Dp[r], natfreq_local=1)
+        o.set_natural_frequencies(nats[r])
+        so = states[r].copy()
+        for k in range(steps):
+            z0, _ = _capi.philox_normals(int(seeds[r]), k, n)
+            o.euler_step(METHOD_LINKED_LIST, so, DT, z0)
+        d = np.abs(b.replica_state(got, r) - so).reshape(-1, model)
+        print(f"  replica {r} n={n}: max |batch - oracle| pos {d[:, :2].max():.2e} phase {d[:, 2].max():.2e}"
+              + (f" omega {d[:, 3].max():.2e}" if model == 4 else ""), flush=True)
+        if model == 4:
+            assert d[:, :2].max() <= 1e-12 and d[:, 2].max() <= 1e-9 and d[:, 3].max() <= 1e-7
+        else:  # the first-order model's phase takes the coupling directly (model 4's omega does)
+            assert d[:, :2].max() <= 1e-11 and d[:, 2].max() <= 1e-7
+    b.close()
+
+
+@pytest.mark.timeout(600)
+def test_full_ensemble_equals_plain_loop(spss):
+    import bench
+    steps = 20
+    groups = bench.ensemble_config()
+    for model, g in groups.items():
+        print(f"[gpu] 512-replica ensemble, model {model}: {len(g['ns'])} replicas, {int(g['ns'].sum())} particles",
+              flush=True)
+        b = spss.ReplicaBatch(model, g["ns"], g["xi"], g["sigma"], g["D"], g["seeds"], v0=V0, rho=RHO)
+        b.set_natural_frequencies(np.concatenate(g["nats"]))
+        b.set_state(np.concatenate(g["states"]))
+        b.run(steps, DT, observe_every=10, observe_from=0, graph=True)
+        got = b.get_state()
+        b.close()
+        for k in range(len(g["ns"])):
+            h = spss.ParticleSystem(model=model, n=int(g["ns"][k]), v0=V0, xi=g["xi"][k], sigma=g["sigma"][k],
+                                    rho=RHO, D_phi=g["D"][k], natfreq_local=1, precision=spss.PRECISION_FP32C,
+                                    seed=int(g["seeds"][k]))
+            h.set_natural_frequencies(g["nats"][k])
+            h.set_state(g["states"][k])
+            h.run(spss.STEPPER_EULER, spss.METHOD_LINKED_LIST, steps, DT, spss.NOISE_PHILOX)
+            want = h.get_state()
+            h.close()
+            first = int(np.sum(g["ns"][:k])) * model
+            assert np.array_equal(got[first:first + want.size], want), (model, k)
