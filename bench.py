@@ -47,7 +47,7 @@ def pair_kernel_bytes(n, cells):
 # HBM bytes per launch of that kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled
 # by the copy-kernel calibration, by cell subdivision (profiles/fk_pairs_euler_N1M_hbm_traffic.txt)
 PAIR_KERNEL_TRAFFIC = {1: 3.9915e7}
-# HBM bytes of the four kernels of one resident step, the same way (filled in from the profiles)
+# HBM bytes of the three kernels of one resident step, the same way (filled in from the profiles)
 STEP_TRAFFIC = {}
 # VALU instructions issued per CU-cycle by that kernel (SQ_INSTS_VALU / (256 CUs x cycles),
 # profiles/fk_pairs_euler_N1M_valu.txt): at or above the nominal one per CU-cycle, i.e. VALU-bound
@@ -493,7 +493,7 @@ def main():
         "timed_steps": {"graph_replayed": replayed, "eager": eager, "graphs_captured": captured},
         "roofline": {"bound": "hbm", "achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": step_achieved / HBM_PEAK, "traffic": STEP_TRAFFIC.get(subdiv),
-                     "kernel": "resident cell-list step (fk_pairs, fr_scan_overflow, fr_far_bucket, fr_scatter)",
+                     "kernel": "resident cell-list step (fk_pairs, fr_arrivals, fr_scatter)",
                      "algorithmic_bytes_per_particle_step": step_bytes(n, 250 * 250),
                      "pair_kernel": {"achieved": pair_achieved / 1e9, "frac": pair_achieved / HBM_PEAK,
                                      "us": pair_ms * 1e3, "bytes_per_particle": pair_kernel_bytes(n, cells),

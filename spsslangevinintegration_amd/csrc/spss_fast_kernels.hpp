@@ -10,11 +10,11 @@
 //    phasor sums of every particle in the old order; fr_move decides each particle's
 //    move (its new position depends on phase and omega only) and counts the moves per
 //    (cell, direction) and the arrivals per cell, a scan of the arrival counts with a decoupled
-//    look-back gives the new cell starts (fr_arrivals), and fr_scatter
-//    applies the step and writes every particle to its new slot: inside a cell, particles that
-//    stayed come first in their previous order, then arrivals grouped by the direction
-//    they came from, each group in its previous order, then the rare particles that jumped
-//    farther than the 3x3 block around their cell, by previous position (placed by fr_arrivals).
+//    look-back gives the new cell starts (extra workgroups of the pair kernel, fr_pairs), and
+//    fr_scatter applies the step and writes every particle to its new slot: inside a cell,
+//    particles that stayed come first in their previous order, then arrivals grouped by the
+//    direction they came from, each group in its previous order, then the rare particles that
+//    jumped farther than the 3x3 block around their cell, by previous position (placed by the scan).
 //    No atomic decides an order, so results are deterministic.
 #pragma once
 
@@ -718,7 +718,9 @@ __device__ __forceinline__ void move_and_count(const DevParams &P, const Layout 
     } else {
       d = kFar;
       farslot[p] = atomicAdd(farcnt + base + ny * ncf + nx, 1);
-      farlist[atomicAdd(farlist_n, 1)] = p;  // rare: the arrivals kernel places these
+      const int k = atomicAdd(farlist_n, 1);  // rare: the arrivals scan places these
+      farlist[2 * k] = p;
+      farlist[2 * k + 1] = base + ny * ncf + nx;
     }
     cell = base + oy * ncf + ox;
     dest = base + ny * ncf + nx;
@@ -799,52 +801,48 @@ __global__ __launch_bounds__(256) void fk_row_tiles(const int *__restrict__ star
   rows_to_tiles([&](int c) { return start[c]; }, ncf, wcap, cmax, tiles);
 }
 
-// The arrivals of a resident step, one launch: the new cell starts and the far arrivals' slots,
-// next to the pair kernel's overflow work.
+// The arrivals of a resident step: the new cell starts and the far arrivals' slots, computed by
+// extra workgroups of the pair kernel (fr_pairs).  The move words and arrival counts they scan
+// were written by the previous step's fr_scatter (or fr_move on entry), so this work needs
+// nothing from the pair sums and runs beside the tiles instead of after them.
 //  * Workgroups [0, nsb) scan 2048 cells each (the arrival counts newcnt + farcnt, cleared as they
 //    are read) and find the total of the cells before theirs by a decoupled look-back over the
 //    earlier workgroups' status words (aggregate, then inclusive prefix; workgroups are dispatched
 //    in index order and a workgroup only waits on lower ones, so every wait ends; the waits are
-//    also bounded), then write the final starts.  The last of them to finish places the step's far
-//    movers (a compact list filled where the moves were computed) behind the near arrivals of their
-//    destination cells, and resets the list, the status words and its own counter.
-//  * Workgroups [nsb, ...) clear the next step's move counters and run the overflow work of the
-//    pair kernel (its sums are needed only by fr_scatter).
+//    also bounded), then write the final starts, and clear the next step's move counters.  The
+//    last of them to finish places the step's far movers (a compact list filled where the moves
+//    were computed) behind the near arrivals of their destination cells, and resets the list, the
+//    status words and its own counter.
+// The pair kernel's overflow work runs after it (fk_pairs_ovf), before fr_scatter reads the sums.
 struct Arrivals {
   int *newcnt, *farcnt;        // arrival counts per cell (read, then cleared)
   int m, nsb;                  // cells, scan workgroups
   int *nstart;                 // [m + 1] new cell starts
   unsigned long long *status;  // [nsb] look-back words: kLbAgg | total, or kLbPre | inclusive prefix
   unsigned int *done;          // scan workgroups finished
-  int *farlist, *farlist_n;    // slots of this step's far movers and their number
+  int *farlist, *farlist_n;    // this step's far movers {slot, destination cell} and their number
   const int *mv;               // this step's move words
   const int *cnt9;             // this step's move counts per (cell, direction)
   const int *farslot;          // each far mover's slot among its destination's far arrivals
   int *bucket;                 // far arrivals by new position
   int *cnt9_next;              // the other move-count buffer, cleared for the next step
   long nclear;
-  const double *rstate;
   int ncf, cpr, n;
   int *error;                  // set when a look-back wait ran out (never, when dispatch is ordered)
 };
 
 constexpr unsigned long long kLbAgg = 1ull << 62, kLbPre = 2ull << 62, kLbVal = (1ull << 62) - 1;
 
-template <int S, bool SRK>
-__global__ __launch_bounds__(kFB) void fr_arrivals(DevParams P, FastArgs A, PairBufs B, Arrivals Q, StepArgs U) {
-  const int b = blockIdx.x;
-  if (b >= Q.nsb) {
-    const long stride = (long)(gridDim.x - Q.nsb) * kFB;
-    for (long e = (long)(b - Q.nsb) * kFB + threadIdx.x; e < Q.nclear; e += stride) Q.cnt9_next[e] = 0;
-    overflow_items<S, SRK, 3>(P, A, B, b - Q.nsb, gridDim.x - Q.nsb);
-    return;
+// one scan workgroup b < Q.nsb of the arrivals; sh: kFB ints of LDS, sprefix and slast LDS words
+__device__ __forceinline__ void arrivals_scan(const Arrivals &Q, const StepArgs &U, int b, int *sh, long long &sprefix,
+                                              int &slast) {
+  {
+    const long stride = (long)Q.nsb * kFB;
+    for (long e = (long)b * kFB + threadIdx.x; e < Q.nclear; e += stride) Q.cnt9_next[e] = 0;
   }
-  // graph-replayed steps: the counter read by this step's fr_scatter moves on here, one kernel
+  // graph-replayed steps: the counter read by this step's fr_scatter moves on here, one launch
   // after the previous step's fr_scatter read it
   if (U.step_inc && b == 0 && threadIdx.x == 0) *U.step_inc += 1ull;
-  __shared__ int sh[kFB];
-  __shared__ long long sprefix;
-  __shared__ int slast;
   const long base = (long)b * kFB * kScanItems + (long)threadIdx.x * kScanItems;
   int v[kScanItems];
   int s = 0;
@@ -866,31 +864,41 @@ __global__ __launch_bounds__(kFB) void fr_arrivals(DevParams P, FastArgs A, Pair
     sh[threadIdx.x] += t;
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {
+    // look-back by the first wave: a window of 64 predecessors read at once (one round trip per
+    // window instead of one per predecessor), summed up to the nearest one holding its prefix
+    const int lane = threadIdx.x;
     const unsigned long long total = (unsigned long long)sh[kFB - 1];
     long long excl = 0;
     if (b == 0) {
-      __hip_atomic_store(&Q.status[0], kLbPre | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) __hip_atomic_store(&Q.status[0], kLbPre | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      __hip_atomic_store(&Q.status[b], kLbAgg | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      for (int j = b - 1; j >= 0; --j) {
+      if (lane == 0) __hip_atomic_store(&Q.status[b], kLbAgg | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      for (int top = b - 1;; top -= 64) {
+        const int j = top - lane;
         unsigned long long w = 0;
-        for (long spin = 0; spin < (1L << 26); ++spin) {
-          w = __hip_atomic_load(&Q.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-          if (w) break;
-          __builtin_amdgcn_s_sleep(1);
+        if (j >= 0) {
+          for (long spin = 0; spin < (1L << 26); ++spin) {
+            w = __hip_atomic_load(&Q.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if (w) break;
+            __builtin_amdgcn_s_sleep(1);
+          }
+          if (!w) atomicOr(Q.error, 1);  // a wait that ran out: flagged (the starts are then wrong)
         }
-        if (!w) {  // a wait that ran out: flag it and go on (the starts are then wrong)
-          atomicOr(Q.error, 1);
-          break;
-        }
-        excl += (long long)(w & kLbVal);
-        if (w & kLbPre) break;
+        // lanes past block 0, holding a prefix, or whose wait ran out end the walk
+        const unsigned long long stops = __ballot(j < 0 || (w & kLbPre) || !w);
+        const int first = stops ? __ffsll((long long)stops) - 1 : 64;
+        long long v = (j >= 0 && lane <= first) ? (long long)(w & kLbVal) : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        excl += v;
+        if (stops) break;
       }
-      __hip_atomic_store(&Q.status[b], kLbPre | (unsigned long long)(excl + (long long)total), __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0)
+        __hip_atomic_store(&Q.status[b], kLbPre | (unsigned long long)(excl + (long long)total), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
-    sprefix = excl;
+    if (lane == 0) sprefix = excl;
   }
   __syncthreads();
   int run = (int)sprefix + sh[threadIdx.x] - s;
@@ -910,11 +918,7 @@ __global__ __launch_bounds__(kFB) void fr_arrivals(DevParams P, FastArgs A, Pair
   // destination cells, in the unordered slots their atomics gave them (fr_scatter orders them).
   const int nfar = __hip_atomic_load(Q.farlist_n, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
   for (int k = threadIdx.x; k < nfar; k += kFB) {
-    const int pf = Q.farlist[k];
-    const int mk = Q.mv[pf];
-    double sn, cn, x1, y1;
-    step_position<S, SRK>(P, Q.rstate + (long)pf * S, U, sn, cn, x1, y1);
-    const int cd = cell_base(mk >> 4, Q.cpr) + cell_of_position<S>(P, x1, y1, Q.ncf);
+    const int pf = Q.farlist[2 * k], cd = Q.farlist[2 * k + 1];
     const int q = Q.nstart[cd] + new_count_of(Q.cnt9, cd, Q.ncf, Q.cpr) + Q.farslot[pf];
     if (q < Q.n) Q.bucket[q] = pf;  // always, when the counts match the moves
   }
@@ -924,6 +928,32 @@ __global__ __launch_bounds__(kFB) void fr_arrivals(DevParams P, FastArgs A, Pair
     *Q.farlist_n = 0;
     *Q.done = 0u;
   }
+}
+
+// The pair kernel of a resident step with the step's arrivals: workgroups [0, nsb8) are the scan
+// workgroups of arrivals_scan (nsb of them; nsb8 = nsb rounded up to a multiple of 8, so that
+// workgroup b still runs on XCD b % 8 below), the others the row tiles of fk_pairs in its
+// XCD-aware order.  The scan's LDS words live in the tile stage, which a scan workgroup does not
+// use, so the kernel holds no more LDS than fk_pairs.
+template <int S, bool SRK>
+__global__ __launch_bounds__(kFB) void fr_pairs(DevParams P, FastArgs A, PairBufs B, Arrivals Q, StepArgs U) {
+  __shared__ PairLds<S, SRK, 3> L;
+  const int nsb8 = (Q.nsb + 7) & ~7;
+  if ((int)blockIdx.x < nsb8) {
+    if ((int)blockIdx.x < Q.nsb) {
+      static_assert(sizeof(L.cglob) >= kFB * sizeof(int), "scan scratch");
+      long long *words = reinterpret_cast<long long *>(L.sdat);
+      arrivals_scan(Q, U, (int)blockIdx.x, L.cglob, words[0], *reinterpret_cast<int *>(words + 1));
+    }
+    return;
+  }
+  const int b = (int)blockIdx.x - nsb8;
+  const int per = ((int)gridDim.x - nsb8) / 8;  // the tile part of the grid is a multiple of 8
+  const int tile = (b % 8) * per + b / 8;
+  if (tile >= A.ntiles) return;
+  const int4 td = A.tiles[tile];
+  if (td.z >= td.w) return;  // an unused row-tile slot
+  pairs_span<S, SRK, 3, true>(P, A, B, td.x, td.y, td.z, td.w, L);  // overflow: split for fk_pairs_ovf
 }
 
 // the step of every particle, written to its new slot with its id, natural frequency and
@@ -939,7 +969,7 @@ __global__ __launch_bounds__(256) void fr_scatter(
     float4 *__restrict__ snap, float *__restrict__ omega_f, int4 *__restrict__ rtiles, int wcap, int cmax,
     int *__restrict__ mvn, int *__restrict__ farcnt, int *__restrict__ farslot, int *__restrict__ cnt9n,
     int *__restrict__ newcnt, int *__restrict__ farlist, int *__restrict__ farlist_n, int *__restrict__ ovf_count) {
-  // the overflow work of this step was read by the arrivals kernel: the next pair kernel starts at zero
+  // the overflow work of this step was read by fk_pairs_ovf: the next pair kernel starts at zero
   if (blockIdx.x == 0 && threadIdx.x == 0) *ovf_count = 0;
   // the next pair kernel's row tiles (single systems) from the new starts, in the first
   // workgroups: their serial span search hides behind the other workgroups' memory traffic

@@ -18,10 +18,11 @@
 //   self terms (v0 cos phi, v0 sin phi, omega, -xi omega + sigma A / n) in FP64.
 //
 // Fused stochastic steps keep the state resident in cell order (see spss_fast_kernels.hpp):
-//   fk_pairs (MODE 3: phasor sums) -> fr_arrivals (new cell starts by a scan with a decoupled
-//   look-back, far arrivals, the pair kernel's overflow work) -> fr_scatter (FP64 Euler/SRK update
-//   written to the new slots, the next step's moves), three launches per step and no host
-//   synchronisation.
+//   fr_pairs (MODE 3 phasor sums; its first workgroups scan the arrival counts the previous step
+//   left, with a decoupled look-back, into the new cell starts and place the far arrivals) ->
+//   fk_pairs_ovf (the tiles that overflowed the LDS stage; usually none) -> fr_scatter (FP64
+//   Euler/SRK update written to the new slots, the next step's moves), three launches per step and
+//   no host synchronisation.
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -99,7 +100,7 @@ struct FastPath {
   // dt, stepper and v0 recorded (another step size or stepper needs fr_move again)
   int *mvb[2] = {nullptr, nullptr};
   int mvc = 0;
-  // the arrivals kernel: look-back status words per scan workgroup, its finished count, the
+  // the arrivals scan: look-back status words per scan workgroup, its finished count, the
   // far movers of the step (slots) and their number, a flag for a look-back wait that ran out
   unsigned long long *lb_status = nullptr;
   unsigned int *lb_done = nullptr;
@@ -279,7 +280,7 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->step_dev, 1));
   A(falloc(&f->lb_status, f->nsb));
   A(falloc(&f->lb_done, 1));
-  A(falloc(&f->farlist, n));
+  A(falloc(&f->farlist, 2L * n));  // {slot, destination cell} per far mover
   A(falloc(&f->farslot, n));
   A(falloc(&f->farlist_n, 1));
   A(falloc(&f->lb_error, 1));
@@ -434,20 +435,18 @@ static PairBufs rebuilt_bufs(const FastPath *f, const double *stage) {
   return B;
 }
 
-// the pair kernel, then the pass over the tiles whose cells overflowed its LDS stage (usually none)
+// the pair kernel of a rebuilt cell list (MODE 0-2), then the pass over the tiles whose cells
+// overflowed its LDS stage (usually none); the resident step has resident_pairs
 template <int S, bool SRK, int MODE>
 static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, const PairBufs &B) {
+  static_assert(MODE != 3, "resident steps launch fr_pairs");
   FastArgs a = args_of(f);
-  if (MODE == 3 && f->rtiles) {  // the resident order's row tiles, rebuilt every step
-    a.tiles = f->rtiles;
-    a.ntiles = f->ncf * f->rt_cmax;
-  } else if (MODE != 3 && f->btiles) {  // the rebuilt cell list's row tiles (rebuilt_row_tiles)
+  if (f->btiles) {  // the rebuilt cell list's row tiles (rebuilt_row_tiles)
     a.tiles = f->btiles;
     a.ntiles = f->ncf * f->rt_cmax;
   }
   fk_pairs<S, SRK, MODE><<<tile_grid(a.ntiles), kFB, 0, st>>>(P, a, B);
-  // the resident step (MODE 3) runs the overflow work inside its arrivals launch (fr_arrivals)
-  if (MODE != 3) fk_pairs_ovf<S, SRK, MODE><<<kOverflowBlocks, kFB, 0, st>>>(P, a, B);
+  fk_pairs_ovf<S, SRK, MODE><<<kOverflowBlocks, kFB, 0, st>>>(P, a, B);
 }
 
 int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const double *nat,
@@ -549,18 +548,9 @@ static void resident_enter(FastPath *f, hipStream_t st, const DevParams &P, cons
   f->resident_nat = nat != nullptr;
 }
 
-// one resident step after the phasor sums: moves, arrival counts, new cell starts, the step
-// applied and written to the new slots
-template <int S, bool SRK>
-static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const StepArgs &U, double *compl_out,
-                          const PairBufs &B) {
+// the arrivals of the current step (its moves are in place: ensure_moves)
+static Arrivals arrivals_of(const FastPath *f) {
   const int c = f->cur, o = 1 - c;
-  const int n = f->n;
-  const unsigned nb = nblocks(n);
-  const double *rnat = f->resident_nat ? f->rnat[c] : nullptr;
-  const Layout L = layout_of(f);
-  ensure_moves<S, SRK>(f, st, P, U);
-  int *const mv = f->mvb[f->mvc], *const mvn = f->mvb[1 - f->mvc];
   Arrivals Q;
   Q.newcnt = f->newcnt;
   Q.farcnt = f->farcnt;
@@ -571,18 +561,45 @@ static void resident_step(FastPath *f, hipStream_t st, const DevParams &P, const
   Q.done = f->lb_done;
   Q.farlist = f->farlist;
   Q.farlist_n = f->farlist_n;
-  Q.mv = mv;
+  Q.mv = f->mvb[f->mvc];
   Q.cnt9 = f->cnt9[c];
   Q.farslot = f->farslot;
   Q.bucket = f->tmp;
   Q.cnt9_next = f->cnt9[o];
   Q.nclear = (long)f->cells * 9;
-  Q.rstate = f->rstate[c];
   Q.ncf = f->ncf;
   Q.cpr = f->cpr;
-  Q.n = n;
+  Q.n = f->n;
   Q.error = f->lb_error;
-  fr_arrivals<S, SRK><<<f->nsb + kOverflowBlocks, kFB, 0, st>>>(P, args_of(f), B, Q, U);
+  return Q;
+}
+
+// the pair kernel of a resident step with the step's arrivals in its first workgroups (fr_pairs);
+// the work of tiles that overflowed the LDS stage follows in fk_pairs_ovf (timed apart)
+template <int S, bool SRK>
+static void resident_pairs(const FastPath *f, hipStream_t st, const DevParams &P, const StepArgs &U,
+                           const PairBufs &B) {
+  FastArgs a = args_of(f);
+  if (f->rtiles) {  // the resident order's row tiles, rebuilt every step
+    a.tiles = f->rtiles;
+    a.ntiles = f->ncf * f->rt_cmax;
+  }
+  const int nsb8 = (f->nsb + 7) & ~7;
+  if (f->timer) f->timer->begin(st);
+  fr_pairs<S, SRK><<<nsb8 + tile_grid(a.ntiles), kFB, 0, st>>>(P, a, B, arrivals_of(f), U);
+  if (f->timer) f->timer->end(st);
+  fk_pairs_ovf<S, SRK, 3><<<kOverflowBlocks, kFB, 0, st>>>(P, a, B);
+}
+
+// the step applied and written to the new slots (after resident_pairs)
+template <int S, bool SRK>
+static void resident_scatter(FastPath *f, hipStream_t st, const DevParams &P, const StepArgs &U, double *compl_out) {
+  const int c = f->cur, o = 1 - c;
+  const int n = f->n;
+  const unsigned nb = nblocks(n);
+  const double *rnat = f->resident_nat ? f->rnat[c] : nullptr;
+  const Layout L = layout_of(f);
+  int *const mv = f->mvb[f->mvc], *const mvn = f->mvb[1 - f->mvc];
   // (at least one workgroup per four rows of cells for the row tiles)
   const unsigned nbs = std::max(nb, (unsigned)(f->rtiles ? (f->ncf + 3) / 4 : 0));
   fr_scatter<S, SRK><<<nbs, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], mv, f->cnt9[c], f->tmp,
@@ -611,20 +628,17 @@ static int step_impl(FastPath *f, hipStream_t st, const DevParams &P, bool do_sr
   B.ssum = f->ssum;
   B.tsum = f->tsum;
   B.scnt = f->scnt;
-  if (f->timer) f->timer->begin(st);
-  if (do_srk)
-    launch_pairs<4, true, 3>(f, st, P, B);
-  else if (P.s == 4)
-    launch_pairs<4, false, 3>(f, st, P, B);
-  else
-    launch_pairs<3, false, 3>(f, st, P, B);
-  if (f->timer) f->timer->end(st);
-  if (do_srk)
-    resident_step<4, true>(f, st, P, U, compl_out, B);
-  else if (P.s == 4)
-    resident_step<4, false>(f, st, P, U, nullptr, B);
-  else
-    resident_step<3, false>(f, st, P, U, nullptr, B);
+  ensure_moves_of(f, st, P, do_srk, U);  // the arrivals in the pair launch scan them
+  if (do_srk) {
+    resident_pairs<4, true>(f, st, P, U, B);
+    resident_scatter<4, true>(f, st, P, U, compl_out);
+  } else if (P.s == 4) {
+    resident_pairs<4, false>(f, st, P, U, B);
+    resident_scatter<4, false>(f, st, P, U, nullptr);
+  } else {
+    resident_pairs<3, false>(f, st, P, U, B);
+    resident_scatter<3, false>(f, st, P, U, nullptr);
+  }
   FCHK(hipGetLastError());
   return SPSS_OK;
 }
@@ -655,7 +669,7 @@ int fast_step(FastPath *f, hipStream_t st, const DevParams &P, int srk, const do
 
 // ---------------------------------------------------------------------------
 // graph-replayed fused steps of a single system.  Each captured step advances the device step
-// counter in its fr_arrivals and reads it as its Philox step in its fr_scatter, so one graph
+// counter in its fr_pairs and reads it as its Philox step in its fr_scatter, so one graph
 // serves every replay.  Graphs of kGraphSteps steps (even: the double buffers end where they
 // started) carry the bulk of a run, graphs of one step the remainder; a graph is replayed only
 // from the buffer parity it was captured at.  Capture records launches without running them, so

@@ -287,6 +287,84 @@ __device__ __forceinline__ void walk_range(const DevParams &P, const VArgs &A, c
   }
 }
 
+// The list entries [0, L) of slot p, sixteen per pass, in the order and 4-slot grouping of
+// walk_range: each half of a pass issues its eight list loads, then its eight gathers of the
+// snapshot (and omega), before any arithmetic, so a wave keeps eight dependent load pairs in
+// flight instead of four; entries past L are the slot itself with weight zero.
+template <int S, bool SRK>
+__device__ __forceinline__ void walk_list(const DevParams &P, const VArgs &A, const int *__restrict__ list,
+                                          const float4 *__restrict__ snap, const float *__restrict__ omega,
+                                          const double *__restrict__ state, int p, int L, float xi, float yi,
+                                          const double *pi, double &sx, double &sy, double &tx, double &ty,
+                                          int &nnb) {
+  for (int t0 = 0; t0 < L; t0 += 16) {
+    float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an = 0.f;
+    float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int h = 0; h < 16; h += 8) {
+      int j[8];
+      float4 o[8];
+      float om[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) j[k] = t0 + h + k < L ? list[(long)(t0 + h + k) * A.n + p] : p;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        o[k] = snap[j[k]];
+        if (SRK) om[k] = omega[j[k]];
+      }
+#pragma unroll
+      for (int g = 0; g < 8; g += 4) {
+        float dd[4], m[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          dd[k] = dd_of(o[g + k], xi, yi, A.r2c);
+          m[k] = __builtin_amdgcn_fmed3f(-A.r2k * dd[k], 0.f, 1.f);
+        }
+        const float dmin = __builtin_fminf(__builtin_fminf(__builtin_fabsf(dd[0]), __builtin_fabsf(dd[1])),
+                                           __builtin_fminf(__builtin_fabsf(dd[2]), __builtin_fabsf(dd[3])));
+        if (dmin <= A.r2h) {  // rare: the reference's FP64 test, as in walk4
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (__builtin_fabsf(dd[k]) <= A.r2h) {
+              double dx, dy;
+              class_c_unsigned(P, pi[0], pi[1], state[(long)j[g + k] * S], state[(long)j[g + k] * S + 1], dx, dy);
+              m[k] = dx * dx + dy * dy <= P.rho2 ? 1.f : 0.f;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) m[k] = t0 + h + g + k < L ? m[k] : 0.f;
+        an += (m[0] + m[1]) + (m[2] + m[3]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          ax[k] = __builtin_fmaf(m[k], o[g + k].z, ax[k]);
+          ay[k] = __builtin_fmaf(m[k], o[g + k].w, ay[k]);
+          if (SRK) {
+            const float mo = m[k] * om[g + k];
+            bx[k] = __builtin_fmaf(mo, o[g + k].z, bx[k]);
+            by[k] = __builtin_fmaf(mo, o[g + k].w, by[k]);
+          }
+        }
+      }
+    }
+    sx += ((double)ax[0] + (double)ax[1]) + ((double)ax[2] + (double)ax[3]);
+    sy += ((double)ay[0] + (double)ay[1]) + ((double)ay[2] + (double)ay[3]);
+    nnb += (int)an;
+    if (SRK) {
+      tx += ((double)bx[0] + (double)bx[1]) + ((double)bx[2] + (double)bx[3]);
+      ty += ((double)by[0] + (double)by[1]) + ((double)by[2] + (double)by[3]);
+    }
+  }
+}
+
+// workgroups in an XCD-aware order: the hardware deals consecutive workgroups to the 8 XCDs in
+// turn, so workgroup b takes the particle block (b % 8) * per + b / 8 and every XCD walks one
+// contiguous eighth of the slots (neighbours' snapshots then stay in that XCD's L2); the grid is
+// a multiple of 8 and blocks past the slots only take part in the displacement maximum
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+  const int per = nblocks >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
 template <int S, bool SRK>
 __global__ __launch_bounds__(kVB) void vs_step(DevParams P, VArgs A, StepArgs U, const int *__restrict__ list,
                                                const int *__restrict__ len, const int *__restrict__ start,
@@ -297,7 +375,7 @@ __global__ __launch_bounds__(kVB) void vs_step(DevParams P, VArgs A, StepArgs U,
                                                float *__restrict__ nomega, double *__restrict__ compl_out) {
   __shared__ double wmax[kVB / 64];
   __shared__ int last;
-  const int p = blockIdx.x * kVB + threadIdx.x;
+  const int p = xcd_block((int)blockIdx.x, (int)gridDim.x) * kVB + threadIdx.x;
   double d2 = 0.0;
   if (p < A.n) {
     const double *pi = state + (long)p * S;
@@ -306,7 +384,7 @@ __global__ __launch_bounds__(kVB) void vs_step(DevParams P, VArgs A, StepArgs U,
     double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
     int nnb = 0;
     if (L >= 0) {
-      walk_range<S, SRK, false>(P, A, list, snap, omega, state, p, 0, L, me.x, me.y, pi, sx, sy, tx, ty, nnb);
+      walk_list<S, SRK>(P, A, list, snap, omega, state, p, L, me.x, me.y, pi, sx, sy, tx, ty, nnb);
     } else {  // (rare) the list outgrew the width: the 3 x 3 cells of the last rebuild hold every candidate
       const int c = cell[p];
       const int cx = c % A.ncv, cy = c / A.ncv;
@@ -623,8 +701,10 @@ static void launch_step(VerletFast *v, hipStream_t st, const DevParams &P, const
   vr_settle<S><<<nb, kVB, 0, st>>>(P, A, v->sstate, v->sid, v->snat, v->scell, v->vstate[c], v->vid,
                                    nat ? v->vnat : nullptr, v->vcell, v->vsnap[c], v->vomega[c]);
   vr_lists<S><<<nb, kVB, 0, st>>>(P, A, v->start, v->vcell, v->vstate[c], v->vsnap[c], v->list, v->len);
-  vs_step<S, SRK><<<nb, kVB, 0, st>>>(P, A, U, v->list, v->len, v->start, v->vcell, v->vstate[c], v->vid, nat, v->vsnap[c],
-                                      v->vomega[c], v->vstate[o], v->vsnap[o], v->vomega[o], SRK ? compl_out : nullptr);
+  const unsigned nb8 = (nb + 7u) & ~7u;  // a multiple of 8 for xcd_block
+  vs_step<S, SRK><<<nb8, kVB, 0, st>>>(P, A, U, v->list, v->len, v->start, v->vcell, v->vstate[c], v->vid, nat,
+                                       v->vsnap[c], v->vomega[c], v->vstate[o], v->vsnap[o], v->vomega[o],
+                                       SRK ? compl_out : nullptr);
 }
 
 static StepArgs vstep_args(double dt, double D_phi, double friction, int noise_mode, const double *noise,
