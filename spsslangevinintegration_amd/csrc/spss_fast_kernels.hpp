@@ -24,16 +24,16 @@
 #include "spss_fast.hpp"
 #include "spss_scan.hpp"
 #include "spss_step.hpp"
+#include "spss_tiles.hpp"
 
 namespace spss {
 
 namespace {
-constexpr int kFB = 256;        // threads per pair workgroup
+constexpr int kFB = kTileCap;   // threads per pair workgroup (one particle of a row tile each)
 constexpr int kStageMax = 1024; // staged entries per workgroup (particles + row padding)
 constexpr int kPad = 4;         // padding entries after each staged row (>= the unroll - 1)
 constexpr int kMaxRows = 7;     // 2k+1 with k <= 3
 constexpr int kMaxCols = 48;    // W + 2k
-constexpr int kRowTileCols = 1024;  // rows of at most this many cells get per-step tiles
 }  // namespace
 
 struct FastArgs {
@@ -178,16 +178,6 @@ __device__ __forceinline__ bool fp64_in_range(const DevParams &P, const double *
   class_c_unsigned(P, x_i, y_i, pj[0], pj[1], dx, dy);
   double dr2 = dx * dx + dy * dy;
   return dr2 <= P.rho2;
-}
-
-// inclusive prefix sum over the 64 lanes of a wave
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int t = __shfl_up(v, off, 64);
-    if (lane >= off) v += t;
-  }
-  return v;
 }
 
 // per particle after its pair sums: the RHS (MODE 0), the neighbour count (MODE 1) or the
@@ -758,43 +748,6 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int
                          farlist_n);
 }
 
-// The tiles of one row of cells for the next pair kernel, built by one wave from the row's cell
-// starts P[0..ncf] (LDS): greedy spans of whole cells, each as wide as possible with at most kFB
-// particles and at most wcap (<= 64) columns, span i in slot i of the row's cmax slots, empty
-// spans in the unused slots.  Full spans keep the workgroup's waves full where fixed-width
-// tiles leave the last wave of most tiles part-empty.  With cmax * wcap >= ncf every span stays
-// within wcap columns: a span is widened past the particle bound (a dense row) only as far as
-// the remaining slots need to cover the rest of the row.
-__device__ __forceinline__ void row_tiles(const int *P, int cbase, int row, int ncf, int wcap, int cmax,
-                                          int4 *__restrict__ slots) {
-  const int lane = threadIdx.x & 63;
-  int a = 0, i = 0;
-  while (a < ncf) {  // invariant: ncf - a <= (cmax - i) * wcap
-    const int e = a + 1 + lane;
-    // true for a leading run of lanes: P is non-decreasing
-    const bool fits = lane < wcap && e <= ncf && P[e] - P[a] <= kFB;
-    const int b = max(a + max(__popcll(__ballot(fits)), 1), ncf - (cmax - i - 1) * wcap);
-    if (lane == 0) slots[i] = make_int4(cbase, row, a, b);
-    a = b;
-    ++i;
-  }
-  for (int j = i + lane; j < cmax; j += 64) slots[j] = make_int4(cbase, row, 0, 0);
-}
-
-// rows [4 * blockIdx.x, 4 * blockIdx.x + 4) of a single system, one wave each; start(c) gives
-// the starts of the cell order the next pair kernel reads (c = 0 .. ncf * ncf).  Uniform per
-// workgroup, so the barrier is reached by all or none of its threads.
-template <class START>
-__device__ __forceinline__ void rows_to_tiles(START start, int ncf, int wcap, int cmax, int4 *__restrict__ tiles) {
-  __shared__ int P[4][kRowTileCols + 1];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + w;
-  if (row < ncf)
-    for (int c = lane; c <= ncf; c += 64) P[w][c] = start(row * ncf + c);
-  __syncthreads();
-  if (row < ncf) row_tiles(P[w], 0, row, ncf, wcap, cmax, tiles + (long)row * cmax);
-}
-
 // the row tiles of a cell order with final starts (entry into the resident order)
 __global__ __launch_bounds__(256) void fk_row_tiles(const int *__restrict__ start, int ncf, int wcap, int cmax,
                                                     int4 *__restrict__ tiles) {
@@ -933,8 +886,9 @@ __device__ __forceinline__ void arrivals_scan(const Arrivals &Q, const StepArgs 
 // The pair kernel of a resident step with the step's arrivals: workgroups [0, nsb8) are the scan
 // workgroups of arrivals_scan (nsb of them; nsb8 = nsb rounded up to a multiple of 8, so that
 // workgroup b still runs on XCD b % 8 below), the others the row tiles of fk_pairs in its
-// XCD-aware order.  The scan's LDS words live in the tile stage, which a scan workgroup does not
-// use, so the kernel holds no more LDS than fk_pairs.
+// XCD-aware order.  Dispatched first, the scan is done long before the tiles (placed last, it
+// ended after them: 58 against 54.5 us per launch at N = 1M).  The scan's LDS words live in the
+// tile stage, which a scan workgroup does not use, so the kernel holds no more LDS than fk_pairs.
 template <int S, bool SRK>
 __global__ __launch_bounds__(kFB) void fr_pairs(DevParams P, FastArgs A, PairBufs B, Arrivals Q, StepArgs U) {
   __shared__ PairLds<S, SRK, 3> L;

@@ -371,10 +371,13 @@ static int tile_grid(int ntiles) { return ((ntiles + 7) / 8) * 8; }
 
 static unsigned nblocks(long n) { return (unsigned)((n + 255) / 256); }
 
-// workgroups of the overflow pass (grid-stride over its items; 16 per CU): the dense rows of a
-// clustered state give thousands of items, and workgroups past the item count exit at once
-// (512 clustered replicas: 5.9e9 particle-steps/s with 4096, 5.7e9 with 1024, 3.8e9 with 256)
+// workgroups of the overflow pass (grid-stride over its items): the dense rows of a clustered
+// replica batch give thousands of items (512 clustered replicas: 5.9e9 particle-steps/s with 4096
+// workgroups, 5.7e9 with 1024, 3.8e9 with 256); a single system's tiles rarely overflow, and the
+// pass then costs its dispatch, which grows with the grid
 constexpr int kOverflowBlocks = 4096;
+constexpr int kOverflowBlocksSingle = 512;
+static int overflow_blocks(const FastPath *f) { return f->R > 1 ? kOverflowBlocks : kOverflowBlocksSingle; }
 
 static FastArgs args_of(const FastPath *f) {
   FastArgs a;
@@ -446,7 +449,7 @@ static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, 
     a.ntiles = f->ncf * f->rt_cmax;
   }
   fk_pairs<S, SRK, MODE><<<tile_grid(a.ntiles), kFB, 0, st>>>(P, a, B);
-  fk_pairs_ovf<S, SRK, MODE><<<kOverflowBlocks, kFB, 0, st>>>(P, a, B);
+  fk_pairs_ovf<S, SRK, MODE><<<overflow_blocks(f), kFB, 0, st>>>(P, a, B);
 }
 
 int fast_rhs(FastPath *f, hipStream_t st, const DevParams &P, const double *stage, const double *nat,
@@ -588,7 +591,7 @@ static void resident_pairs(const FastPath *f, hipStream_t st, const DevParams &P
   if (f->timer) f->timer->begin(st);
   fr_pairs<S, SRK><<<nsb8 + tile_grid(a.ntiles), kFB, 0, st>>>(P, a, B, arrivals_of(f), U);
   if (f->timer) f->timer->end(st);
-  fk_pairs_ovf<S, SRK, 3><<<kOverflowBlocks, kFB, 0, st>>>(P, a, B);
+  fk_pairs_ovf<S, SRK, 3><<<overflow_blocks(f), kFB, 0, st>>>(P, a, B);
 }
 
 // the step applied and written to the new slots (after resident_pairs)

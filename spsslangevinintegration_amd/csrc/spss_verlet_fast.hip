@@ -37,6 +37,7 @@
 #include "spss_launch.hpp"
 #include "spss_scan.hpp"
 #include "spss_step.hpp"
+#include "spss_tiles.hpp"
 #include "spss_verlet_fast.hpp"
 
 namespace spss {
@@ -167,14 +168,13 @@ __global__ __launch_bounds__(kVB) void vr_settle(DevParams P, VArgs A, const dou
 // ParticleSystemFirstOrderPtr.cpp:83-88, ascending slot inside a cell; j is listed when its
 // distance from the wrapped x_i to the stored x_j is within r_max (:694-739).  FP32 test outside a
 // guard band, the reference's FP64 ClassC formula inside it.
+// The list of slot q read from global memory (lists of a grid too wide for row tiles, and of tiles
+// whose stencil cells overflow the LDS stage).
 template <int S>
-__global__ __launch_bounds__(kVB) void vr_lists(DevParams P, VArgs A, const int *__restrict__ start,
-                                                const int *__restrict__ cell, const double *__restrict__ state,
-                                                const float4 *__restrict__ snap, int *__restrict__ list,
-                                                int *__restrict__ len) {
-  if (gate_off(A)) return;
-  const int q = blockIdx.x * kVB + threadIdx.x;
-  if (q >= A.n) return;
+__device__ __forceinline__ void list_of_slot(const DevParams &P, const VArgs &A, const int *__restrict__ start,
+                                             const int *__restrict__ cell, const double *__restrict__ state,
+                                             const float4 *__restrict__ snap, int *__restrict__ list,
+                                             int *__restrict__ len, int q) {
   const int c = cell[q];
   const int cx = c % A.ncv, cy = c / A.ncv;
   const float4 me = snap[q];
@@ -205,6 +205,16 @@ __global__ __launch_bounds__(kVB) void vr_lists(DevParams P, VArgs A, const int 
   // a list longer than the width is not stored: the step walks that particle's 3 x 3 cells instead
   len[q] = w <= A.K ? w : -1;
   if (w > A.K) atomicAdd(&A.ctl->overflow, 1ull);
+}
+
+template <int S>
+__global__ __launch_bounds__(kVB) void vr_lists(DevParams P, VArgs A, const int *__restrict__ start,
+                                                const int *__restrict__ cell, const double *__restrict__ state,
+                                                const float4 *__restrict__ snap, int *__restrict__ list,
+                                                int *__restrict__ len) {
+  if (gate_off(A)) return;
+  const int q = blockIdx.x * kVB + threadIdx.x;
+  if (q < A.n) list_of_slot<S>(P, A, start, cell, state, snap, list, len, q);
 }
 
 // ---------------------------------------------------------------------------
@@ -442,6 +452,363 @@ __global__ __launch_bounds__(kVB) void vs_step(DevParams P, VArgs A, StepArgs U,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Row tiles of the rebuild's cell order (grids of at most kRowTileCols columns).  One workgroup
+// per tile -- a span of whole cells of one row, as for the cell-list pair kernel -- stages the
+// snapshots (and omega) of the 3 x (W + 2) cells around its W cells into LDS.  A list entry is
+// then also kept as the candidate's LDS position in its tile's stage (16 bits), and a step reads
+// its candidates from LDS instead of gathering them from global memory: the same values in the
+// same order, so the sums equal the global walk's bit for bit.  A tile whose stencil cells
+// overflow the stage builds and walks its lists through global memory as before.
+
+constexpr int kVStage = 1024;  // staged snapshots per tile
+constexpr int kVCols = 48;     // staged columns W + 2
+
+template <bool SRK>
+struct VtLds {
+  float4 sdat[kVStage];
+  float somega[SRK ? kVStage : 1];
+  int cglob[3 * kVCols];     // first slot of staged cell (r, q), at r * (W + 2) + q
+  int ccnt[3 * kVCols];      // its particle count
+  int cpos[3 * kVCols + 1];  // its LDS start; [cells] = the staged total
+  double wmax[kVB / 64];
+  int last;
+};
+
+// the staging plan of tile (row, ca, cb): rows row-1 .. row+1, columns ca-1 .. cb (each wrapped
+// once); returns the staged total (the same for every thread)
+__device__ int vt_plan(const VArgs &A, const int *__restrict__ start, int row, int ca, int cb, int *cglob, int *ccnt,
+                       int *cpos) {
+  const int ncols = cb - ca + 2, ncells = 3 * ncols;
+  for (int e = threadIdx.x; e < ncells; e += kVB) {
+    const int r = e / ncols, q = e - r * ncols;
+    int rr = row + r - 1, cc = ca - 1 + q;
+    rr += rr < 0 ? A.ncv : (rr >= A.ncv ? -A.ncv : 0);
+    cc += cc < 0 ? A.ncv : (cc >= A.ncv ? -A.ncv : 0);
+    const int c = rr * A.ncv + cc;
+    const int g0 = start[c];
+    cglob[e] = g0;
+    ccnt[e] = start[c + 1] - g0;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {  // one wave: exclusive prefix of the counts, 64 cells per pass
+    const int lane = threadIdx.x;
+    int run = 0;
+    for (int e0 = 0; e0 < ncells; e0 += 64) {
+      const int e = e0 + lane;
+      const int v = e < ncells ? ccnt[e] : 0;
+      const int incl = wave_incl_scan(v, lane);
+      if (e < ncells) cpos[e] = run + incl - v;
+      run += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) cpos[ncells] = run;
+  }
+  __syncthreads();
+  return cpos[ncells];
+}
+
+// the staged cells' snapshots (and omega) into LDS, G lanes per cell
+template <bool SRK>
+__device__ void vt_stage(int ncells, const float4 *__restrict__ snap, const float *__restrict__ omega,
+                         VtLds<SRK> &L) {
+  const int G = ncells <= kVB / 8 ? 8 : (ncells <= kVB / 4 ? 4 : (ncells <= kVB / 2 ? 2 : 1));
+  const int sub = (int)threadIdx.x % G;
+  for (int c = (int)threadIdx.x / G; c < ncells; c += kVB / G) {
+    const int e0 = L.cpos[c], n0 = L.ccnt[c], g0 = L.cglob[c];
+    for (int j = sub; j < n0; j += G) {
+      L.sdat[e0 + j] = snap[g0 + j];
+      if (SRK) L.somega[e0 + j] = omega[g0 + j];
+    }
+  }
+  __syncthreads();
+}
+
+// global slot of staged entry t: the last staged cell starting at or before t (an empty cell
+// starts where the next one does, so the cell found holds t)
+__device__ __forceinline__ int vt_slot(const int *cglob, const int *cpos, int ncells, int t) {
+  int lo = 0, hi = ncells - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cpos[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return cglob[lo] + (t - cpos[lo]);
+}
+
+__global__ __launch_bounds__(256) void vr_tiles(VArgs A, const int *__restrict__ start, int wcap, int cmax,
+                                                int4 *__restrict__ tiles) {
+  if (gate_off(A)) return;
+  rows_to_tiles([&](int c) { return start[c]; }, A.ncv, wcap, cmax, tiles);
+}
+
+// The lists of a tile's particles, in list_of_slot's order (stencil cells in order, ascending slot
+// inside a cell) and with its decisions, from the staged snapshots; each entry is written twice:
+// as a slot (list) and as its LDS position in the tile's stage (list16).
+template <int S>
+__global__ __launch_bounds__(kVB) void vt_lists(DevParams P, VArgs A, const int4 *__restrict__ tiles, int ntiles,
+                                                const int *__restrict__ start, const int *__restrict__ cell,
+                                                const double *__restrict__ state, const float4 *__restrict__ snap,
+                                                int *__restrict__ list, unsigned short *__restrict__ list16,
+                                                int *__restrict__ len) {
+  if (gate_off(A)) return;
+  __shared__ VtLds<false> L;
+  const int tile = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  if (tile >= ntiles) return;
+  const int4 td = tiles[tile];
+  const int row = td.y, ca = td.z, cb = td.w;
+  if (ca >= cb) return;  // an unused tile slot
+  const int ncols = cb - ca + 2, ncells = 3 * ncols;
+  const int total = vt_plan(A, start, row, ca, cb, L.cglob, L.ccnt, L.cpos);
+  const int gtb = L.cglob[ncols + 1];
+  const int ntile = start[row * A.ncv + cb] - gtb;
+  if (total > kVStage) {
+    for (int u = threadIdx.x; u < ntile; u += kVB) list_of_slot<S>(P, A, start, cell, state, snap, list, len, gtb + u);
+    return;
+  }
+  vt_stage<false>(ncells, snap, nullptr, L);
+  const int ltb = L.cpos[ncols + 1];
+  for (int u = threadIdx.x; u < ntile; u += kVB) {
+    const int q = gtb + u, li = ltb + u;
+    const float4 me = L.sdat[li];
+    double xi, yi;
+    apply_pbc(P, state[(long)q * S], state[(long)q * S + 1], xi, yi);
+    const int qi = 1 + cell[q] % A.ncv - ca;  // staged column of the particle's cell
+    int w = 0;
+    for (int r = 0; r < 3; ++r)
+      for (int dq = -1; dq <= 1; ++dq) {
+        const int e = r * ncols + qi + dq;
+        const int t0 = L.cpos[e], t1 = t0 + L.ccnt[e], g0 = L.cglob[e];
+        for (int t = t0; t < t1; ++t) {
+          if (t == li) continue;
+          const float dd = dd_of(L.sdat[t], me.x, me.y, A.m2c);
+          bool in = dd < 0.f;
+          if (__builtin_fabsf(dd) <= A.m2h) {
+            const long g = g0 + (t - t0);
+            double dx, dy;
+            class_c_unsigned(P, xi, yi, state[g * S], state[g * S + 1], dx, dy);
+            in = dx * dx + dy * dy <= A.rmax2;
+          }
+          if (in) {
+            if (w < A.K) {
+              list[(long)w * A.n + q] = g0 + (t - t0);
+              list16[(long)w * A.n + q] = (unsigned short)t;
+            }
+            ++w;
+          }
+        }
+      }
+    len[q] = w <= A.K ? w : -1;
+    if (w > A.K) atomicAdd(&A.ctl->overflow, 1ull);
+  }
+}
+
+// walk_list with the candidates read from the tile's stage (entries past L: the particle itself,
+// li, with weight zero): the pass's sixteen list entries are loaded first (global memory), the
+// candidates four at a time from LDS, in walk_list's order and grouping
+template <int S, bool SRK>
+__device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A, const unsigned short *__restrict__ list16,
+                                              const VtLds<SRK> &Ls, int ncells, const double *__restrict__ state,
+                                              int p, int li, int L, float xi, float yi, const double *pi, double &sx,
+                                              double &sy, double &tx, double &ty, int &nnb) {
+  for (int t0 = 0; t0 < L; t0 += 16) {
+    float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an = 0.f;
+    float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
+    int j[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) j[k] = t0 + k < L ? (int)list16[(long)(t0 + k) * A.n + p] : li;
+#pragma unroll
+    for (int g = 0; g < 16; g += 4) {
+      float4 o[4];
+      float dd[4], m[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        o[k] = Ls.sdat[j[g + k]];
+        dd[k] = dd_of(o[k], xi, yi, A.r2c);
+        m[k] = __builtin_amdgcn_fmed3f(-A.r2k * dd[k], 0.f, 1.f);
+      }
+      const float dmin = __builtin_fminf(__builtin_fminf(__builtin_fabsf(dd[0]), __builtin_fabsf(dd[1])),
+                                         __builtin_fminf(__builtin_fabsf(dd[2]), __builtin_fabsf(dd[3])));
+      if (dmin <= A.r2h) {  // rare: the reference's FP64 test, as in walk4
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (__builtin_fabsf(dd[k]) <= A.r2h) {
+            const long g2 = vt_slot(Ls.cglob, Ls.cpos, ncells, j[g + k]);
+            double dx, dy;
+            class_c_unsigned(P, pi[0], pi[1], state[g2 * S], state[g2 * S + 1], dx, dy);
+            m[k] = dx * dx + dy * dy <= P.rho2 ? 1.f : 0.f;
+          }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) m[k] = t0 + g + k < L ? m[k] : 0.f;
+      an += (m[0] + m[1]) + (m[2] + m[3]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        ax[k] = __builtin_fmaf(m[k], o[k].z, ax[k]);
+        ay[k] = __builtin_fmaf(m[k], o[k].w, ay[k]);
+        if (SRK) {
+          const float mo = m[k] * Ls.somega[j[g + k]];
+          bx[k] = __builtin_fmaf(mo, o[k].z, bx[k]);
+          by[k] = __builtin_fmaf(mo, o[k].w, by[k]);
+        }
+      }
+    }
+    sx += ((double)ax[0] + (double)ax[1]) + ((double)ax[2] + (double)ax[3]);
+    sy += ((double)ay[0] + (double)ay[1]) + ((double)ay[2] + (double)ay[3]);
+    nnb += (int)an;
+    if (SRK) {
+      tx += ((double)bx[0] + (double)bx[1]) + ((double)bx[2] + (double)bx[3]);
+      ty += ((double)by[0] + (double)by[1]) + ((double)by[2] + (double)by[3]);
+    }
+  }
+}
+
+// The step of slot p (vs_step's per-particle work); candidates from the tile's stage when
+// `staged` (li: p's LDS position), else through global memory.  Returns |dr|^2.
+template <int S, bool SRK>
+__device__ __forceinline__ double vt_particle(const DevParams &P, const VArgs &A, const StepArgs &U, const Coef &K,
+                                              uint64_t step, const int *__restrict__ list,
+                                              const unsigned short *__restrict__ list16, const int *__restrict__ len,
+                                              const int *__restrict__ start, const int *__restrict__ cell,
+                                              const double *__restrict__ state, const int *__restrict__ id,
+                                              const double *__restrict__ nat, const float4 *__restrict__ snap,
+                                              const float *__restrict__ omega, double *__restrict__ nstate,
+                                              float4 *__restrict__ nsnap, float *__restrict__ nomega,
+                                              double *__restrict__ compl_out, const VtLds<SRK> *Ls, bool staged,
+                                              int ncells, int p, int li) {
+  const double *pi = state + (long)p * S;
+  const float4 me = snap[p];
+  const int Lp = len[p];
+  double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
+  int nnb = 0;
+  if (Lp >= 0 && staged) {
+    walk_list_lds<S, SRK>(P, A, list16, *Ls, ncells, state, p, li, Lp, me.x, me.y, pi, sx, sy, tx, ty, nnb);
+  } else if (Lp >= 0) {
+    walk_list<S, SRK>(P, A, list, snap, omega, state, p, Lp, me.x, me.y, pi, sx, sy, tx, ty, nnb);
+  } else {  // (rare) the list outgrew the width: the 3 x 3 cells of the last rebuild hold every candidate
+    const int c = cell[p];
+    const int cx = c % A.ncv, cy = c / A.ncv;
+    for (int dyc = -1; dyc <= 1; ++dyc)
+      for (int dxc = -1; dxc <= 1; ++dxc) {
+        const int cj = adjust_cell(cy + dyc, A.ncv) * A.ncv + adjust_cell(cx + dxc, A.ncv);
+        walk_range<S, SRK, true>(P, A, list, snap, omega, state, p, start[cj], start[cj + 1], me.x, me.y, pi, sx,
+                                 sy, tx, ty, nnb);
+      }
+  }
+  const int i = id[p];
+  double out[S], C, dp2;
+  step_particle<S, SRK>(P, K, pi, i, nat ? nat[p] : 0.0, make_double2(sx, sy), make_double2(tx, ty), nnb, U,
+                        step, out, C, &dp2);
+#pragma unroll
+  for (int k = 0; k < S; ++k) nstate[(long)p * S + k] = out[k];
+  nsnap[p] = vsnap_of(P, out);
+  nomega[p] = S == 4 ? (float)out[3] : 0.f;
+  if (SRK && compl_out) compl_out[i] = C;
+  return dp2;
+}
+
+__device__ __forceinline__ Coef vt_coef(const DevParams &P, const StepArgs &U) {
+  Coef K;
+  K.xi = P.xi;
+  K.sigma = P.sigma;
+  K.int_phi = U.int_phi;
+  K.sq2d = U.sq2d;
+  K.seed = U.seed;
+  return K;
+}
+
+// The particles of an oversized tile past the first kVB (a single cell holding more than a tile's
+// particles: dense clusters), through global memory; launched before vt_step, whose last
+// workgroup then sees their displacements too.
+template <int S, bool SRK>
+__global__ __launch_bounds__(kVB) void vt_rest(DevParams P, VArgs A, StepArgs U, const int4 *__restrict__ tiles,
+                                               int ntiles, const int *__restrict__ list, const int *__restrict__ len,
+                                               const int *__restrict__ start, const int *__restrict__ cell,
+                                               const double *__restrict__ state, const int *__restrict__ id,
+                                               const double *__restrict__ nat, const float4 *__restrict__ snap,
+                                               const float *__restrict__ omega, double *__restrict__ nstate,
+                                               float4 *__restrict__ nsnap, float *__restrict__ nomega,
+                                               double *__restrict__ compl_out) {
+  __shared__ double wmax[kVB / 64];
+  if ((int)blockIdx.x >= ntiles) return;
+  const int4 td = tiles[blockIdx.x];
+  if (td.z >= td.w) return;
+  const int g0 = start[td.y * A.ncv + td.z], ntile = start[td.y * A.ncv + td.w] - g0;
+  if (ntile <= kVB) return;
+  const Coef K = vt_coef(P, U);
+  const uint64_t step = U.step_dev ? (uint64_t)*U.step_dev : U.step;
+  double d2 = 0.0;
+  for (int u = kVB + threadIdx.x; u < ntile; u += kVB)
+    d2 = fmax(d2, vt_particle<S, SRK>(P, A, U, K, step, list, nullptr, len, start, cell, state, id, nat, snap, omega,
+                                      nstate, nsnap, nomega, compl_out, nullptr, false, 0, g0 + u, 0));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = wmax[0];
+    for (int w = 1; w < kVB / 64; ++w) bm = fmax(bm, wmax[w]);
+    atomicMax(&A.ctl->max2_bits, (unsigned long long)__double_as_longlong(bm));
+  }
+}
+
+// The step of a tile's first kVB particles, one per lane (vt_rest has the others); the step's
+// largest displacement and the next rebuild flag as in vs_step.
+template <int S, bool SRK>
+__global__ __launch_bounds__(kVB) void vt_step(DevParams P, VArgs A, StepArgs U, const int4 *__restrict__ tiles,
+                                               int ntiles, const int *__restrict__ list,
+                                               const unsigned short *__restrict__ list16, const int *__restrict__ len,
+                                               const int *__restrict__ start, const int *__restrict__ cell,
+                                               const double *__restrict__ state, const int *__restrict__ id,
+                                               const double *__restrict__ nat, const float4 *__restrict__ snap,
+                                               const float *__restrict__ omega, double *__restrict__ nstate,
+                                               float4 *__restrict__ nsnap, float *__restrict__ nomega,
+                                               double *__restrict__ compl_out) {
+  __shared__ VtLds<SRK> Ls;
+  const int tile = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int4 td = tile < ntiles ? tiles[tile] : make_int4(0, 0, 0, 0);
+  double d2 = 0.0;
+  if (td.z < td.w) {  // the same for the whole workgroup
+    const int row = td.y, ca = td.z, cb = td.w;
+    const int ncols = cb - ca + 2, ncells = 3 * ncols;
+    const bool staged = vt_plan(A, start, row, ca, cb, Ls.cglob, Ls.ccnt, Ls.cpos) <= kVStage;
+    if (staged) vt_stage<SRK>(ncells, snap, omega, Ls);
+    const int gtb = Ls.cglob[ncols + 1];
+    const int ntile = start[row * A.ncv + cb] - gtb;
+    const int u = threadIdx.x;
+    if (u < ntile)
+      d2 = vt_particle<S, SRK>(P, A, U, vt_coef(P, U), U.step_dev ? (uint64_t)*U.step_dev : U.step, list, list16, len,
+                               start, cell, state, id, nat, snap, omega, nstate, nsnap, nomega, compl_out, &Ls, staged,
+                               ncells, gtb + u, Ls.cpos[ncols + 1] + u);
+  }
+  // the step's largest squared displacement: wave, workgroup, then one atomic per workgroup
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
+  if ((threadIdx.x & 63) == 0) Ls.wmax[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = Ls.wmax[0];
+    for (int w = 1; w < kVB / 64; ++w) bm = fmax(bm, Ls.wmax[w]);
+    atomicMax(&A.ctl->max2_bits, (unsigned long long)__double_as_longlong(bm));
+    __threadfence();
+    Ls.last = atomicAdd(&A.ctl->done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (Ls.last && threadIdx.x == 0) {
+    // every other workgroup has added its maximum: CalculateMaxDisplacement's bookkeeping
+    // (ParticleSystemFirstOrderPtr.cpp:851-860) in the exact mode (one step of margin)
+    const double max2 = __longlong_as_double((long long)atomicAdd(&A.ctl->max2_bits, 0ull));
+    const double stepd = sqrt(max2);
+    double acc = A.ctl->acc + stepd;
+    const int flag = acc + stepd > A.half_skin ? 1 : 0;
+    if (flag) acc = 0.0;
+    A.ctl->acc = acc;
+    A.ctl->flag = flag;
+    A.ctl->max2_bits = 0ull;
+    A.ctl->done = 0u;
+    if (U.step_inc) *U.step_inc += 1ull;
+    __threadfence();
+  }
+}
+
 // the largest particle count of any 3 x 3 block of cells (the candidates of one list)
 __global__ __launch_bounds__(kVB) void vc_block_max(VArgs A, const int *__restrict__ count, int *__restrict__ out) {
   const int c = blockIdx.x * kVB + threadIdx.x;
@@ -495,6 +862,12 @@ struct VerletFast {
   int *vid = nullptr, *vcell = nullptr;
   double *vnat = nullptr;
   int *list = nullptr, *len = nullptr;
+  // row tiles of the rebuild's cell order (grids of at most kRowTileCols columns): ncv rows of
+  // vt_cmax slots, and the lists as LDS positions in the tiles' stages
+  bool tiled = false;
+  int vt_wcap = 0, vt_cmax = 0;
+  int4 *vtiles = nullptr;
+  unsigned short *list16 = nullptr;
   int *key = nullptr, *slotc = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr, *blocksum = nullptr;
   double *sstate = nullptr, *snat = nullptr;
   int *sid = nullptr, *scell = nullptr;
@@ -546,7 +919,7 @@ void vf_destroy(VerletFast *v) {
   void *b[] = {v->vstate[0], v->vstate[1], v->vsnap[0], v->vsnap[1], v->vomega[0], v->vomega[1], v->vid,
                v->vcell,     v->vnat,      v->list,     v->len,      v->key,       v->slotc,     v->count,
                v->start,     v->tmp,       v->blocksum, v->sstate,   v->snat,      v->sid,       v->scell,
-               v->ctl,       v->bmax};
+               v->ctl,       v->bmax,      v->vtiles,   v->list16};
   for (void *p : b) hipFree(p);
   delete v;
 }
@@ -595,6 +968,16 @@ int vf_create(VerletFast **out, hipStream_t st, const DevParams &P, double rho, 
   A(hipMalloc(&v->scell, sizeof(int) * nn));
   A(hipMalloc(&v->ctl, sizeof(VCtl)));
   A(hipMalloc(&v->bmax, sizeof(int)));
+  v->tiled = ncv <= kRowTileCols;
+  if (v->tiled) {
+    // slots per row: the greedy spans of a row 1.5x the mean at about 94% of a tile each, plus 2,
+    // and at least enough spans of vt_wcap columns to cover a row
+    v->vt_wcap = kVCols - 2;
+    const double row_mean = (double)n / (double)ncv;
+    v->vt_cmax = std::max((ncv + v->vt_wcap - 1) / v->vt_wcap,
+                          std::min(ncv, (int)std::ceil(1.5 * row_mean / (0.94 * kTileCap)) + 2));
+    A(hipMalloc(&v->vtiles, sizeof(int4) * (size_t)ncv * v->vt_cmax));
+  }
   if (e != hipSuccess) {
     vf_destroy(v);
     return vfail(SPSS_ERR_DEVICE, std::string("Verlet list buffers: ") + hipGetErrorString(e));
@@ -669,9 +1052,12 @@ int vf_enter(VerletFast *v, hipStream_t st, const DevParams &P, int srk, const d
   if (need > v->list_cap) {
     drop_graphs(v);  // captured launches hold the old list pointer
     hipFree(v->list);
+    hipFree(v->list16);
     v->list = nullptr;
+    v->list16 = nullptr;
     v->list_cap = 0;
     VCHK(hipMalloc(&v->list, sizeof(int) * need));
+    if (v->tiled) VCHK(hipMalloc(&v->list16, sizeof(unsigned short) * need));
     v->list_cap = need;
   }
   if (K != v->K) drop_graphs(v);  // the width is a launch argument of the captured steps
@@ -700,6 +1086,20 @@ static void launch_step(VerletFast *v, hipStream_t st, const DevParams &P, const
                                    v->snat, v->scell);
   vr_settle<S><<<nb, kVB, 0, st>>>(P, A, v->sstate, v->sid, v->snat, v->scell, v->vstate[c], v->vid,
                                    nat ? v->vnat : nullptr, v->vcell, v->vsnap[c], v->vomega[c]);
+  if (v->tiled) {
+    const int ntiles = v->ncv * v->vt_cmax;
+    const unsigned tg = (unsigned)((ntiles + 7) / 8 * 8);  // a multiple of 8 for xcd_block
+    vr_tiles<<<(v->ncv + 3) / 4, 256, 0, st>>>(A, v->start, v->vt_wcap, v->vt_cmax, v->vtiles);
+    vt_lists<S><<<tg, kVB, 0, st>>>(P, A, v->vtiles, ntiles, v->start, v->vcell, v->vstate[c], v->vsnap[c], v->list,
+                                    v->list16, v->len);
+    vt_rest<S, SRK><<<ntiles, kVB, 0, st>>>(P, A, U, v->vtiles, ntiles, v->list, v->len, v->start, v->vcell,
+                                            v->vstate[c], v->vid, nat, v->vsnap[c], v->vomega[c], v->vstate[o],
+                                            v->vsnap[o], v->vomega[o], SRK ? compl_out : nullptr);
+    vt_step<S, SRK><<<tg, kVB, 0, st>>>(P, A, U, v->vtiles, ntiles, v->list, v->list16, v->len, v->start, v->vcell,
+                                        v->vstate[c], v->vid, nat, v->vsnap[c], v->vomega[c], v->vstate[o],
+                                        v->vsnap[o], v->vomega[o], SRK ? compl_out : nullptr);
+    return;
+  }
   vr_lists<S><<<nb, kVB, 0, st>>>(P, A, v->start, v->vcell, v->vstate[c], v->vsnap[c], v->list, v->len);
   const unsigned nb8 = (nb + 7u) & ~7u;  // a multiple of 8 for xcd_block
   vs_step<S, SRK><<<nb8, kVB, 0, st>>>(P, A, U, v->list, v->len, v->start, v->vcell, v->vstate[c], v->vid, nat,

@@ -13,6 +13,8 @@
 * graph-replayed steps (spss_run) equal eager steps bit for bit, rebuilds decided on the device;
 * the Verlet and the cell-list FP32C paths take the same neighbours: their states agree to the
   FP32 summation order after 50 steps;
+* dense cells (a tile with more particles than a workgroup, tiles whose stencil cells overflow the
+  LDS stage, lists past the width) against the oracle over 12 SRK steps;
 * at N = 1,048,576, rho = 0.004: FP32C Verlet SRK against the FP64 path's exact-mode Verlet SRK fed
   the same Philox draws, |R_FP32C - R_FP64| <= 1e-4 over 5000 steps, the FP64 path pinned to the
   oracle for the first two steps.
@@ -180,6 +182,46 @@ def test_fp32c_verlet_lists_beyond_the_width_walk_their_cells(spss, monkeypatch)
           flush=True)
     assert d[:, 3].max() <= 1e-7 and d[:, 2].max() <= 1e-9
     assert res[0][1][0] == res[1][1][0]  # same rebuild flag
+
+
+def test_fp32c_verlet_dense_cells_vs_oracle(spss):
+    """three adjacent Verlet cells holding 400 particles each among a uniform background: the dense
+    cells' tile holds more particles than a workgroup (the rest are stepped apart), the tiles around
+    them overflow the LDS stage (their lists are built and walked through global memory) and the
+    dense particles' lists outgrow the width (they walk their cells); 12 Philox SRK steps against the
+    oracle fed the same draws, to the large configuration's tolerances"""
+    rho = 0.02
+    ncv = int(1.0 / (1.2 * rho))
+    h = 1.0 / ncv
+    rng = np.random.default_rng(77)
+    parts = [rng.random((12000, 2))]
+    for cx, cy in ((20, 20), (21, 20), (20, 21)):
+        parts.append(np.stack([(cx + 0.02 + 0.96 * rng.random(400)) * h, (cy + 0.02 + 0.96 * rng.random(400)) * h], 1))
+    xy = np.concatenate(parts)
+    n = len(xy)
+    kw, st, w = _config3(n, rho, seed=17)
+    st = st.reshape(-1, 4)
+    st[:, :2] = xy
+    st = st.reshape(-1)
+    seed, dt, steps = 0x5EED0007, 0.002, 12
+    print(f"[gpu] fp32c verlet dense cells n={n} rho={rho}", flush=True)
+    g = spss.ParticleSystem(precision=spss.PRECISION_FP32C, seed=seed, **kw)
+    g.set_natural_frequencies(w)
+    g.set_state(st)
+    g.run(spss.STEPPER_SRK, spss.METHOD_LINKED_LIST_VERLET, steps, dt, spss.NOISE_PHILOX)
+    flag, acc, overflow = g.verlet_status()
+    o = oracle.Oracle(**kw)
+    o.set_natural_frequencies(w)
+    so = st.copy()
+    for k in range(steps):
+        z0, z1 = _capi.philox_normals(seed, k, n)
+        o.srk_step(METHOD_LINKED_LIST_VERLET, so, dt, np.stack([z0, z1], axis=1).reshape(-1))
+    d = np.abs(g.get_state() - so).reshape(-1, 4)
+    print(f"  lists past the width {overflow}; max |gpu - oracle| pos {d[:, :2].max():.2e} "
+          f"phase {d[:, 2].max():.2e} omega {d[:, 3].max():.2e}", flush=True)
+    assert overflow > 0
+    assert d[:, :2].max() <= 1e-12 and d[:, 2].max() <= 1e-9 and d[:, 3].max() <= 1e-7
+    g.close()
 
 
 def test_fp32c_verlet_and_cell_list_take_the_same_neighbours(spss):
