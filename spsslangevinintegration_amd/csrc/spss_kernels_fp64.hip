@@ -92,16 +92,36 @@ __global__ void k_scatter(int n, const int *__restrict__ key, const int *__restr
   if (i < n) tmp[start[key[i]] + slot[i]] = i;
 }
 
-// stable position of i inside its cell = number of members with a smaller index
-__global__ void k_stable_rank(int n, const int *__restrict__ key, const int *__restrict__ start,
-                              const int *__restrict__ tmp, int *__restrict__ sorted) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int c = key[i];
-  const int b = start[c], e = start[c + 1];
-  int r = 0;
-  for (int t = b; t < e; ++t) r += (tmp[t] < i) ? 1 : 0;
-  sorted[b + r] = i;
+// Each cell's members in ascending particle index (the order in which the FP64 pair loop sums a
+// cell's members): one wave per cell.  Cells of at most 64 members (every cell at the densities
+// of the workloads) are sorted by a bitonic network across the wave's lanes, 21 compare-exchange
+// stages by lane shuffles; a denser cell ranks each member by counting the smaller indices of its
+// cell.
+__global__ __launch_bounds__(256) void k_stable_rank(int ncells, const int *__restrict__ start,
+                                                     const int *__restrict__ tmp, int *__restrict__ sorted) {
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= ncells) return;  // the whole wave
+  const int b = start[c], m = start[c + 1] - b;
+  if (m <= 64) {
+    int v = lane < m ? tmp[b + lane] : 0x7fffffff;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int o = __shfl_xor(v, j, 64);
+        // the lower lane of a pair keeps the smaller value in an ascending run of size k
+        v = (((lane & j) == 0) == ((lane & k) == 0)) ? min(v, o) : max(v, o);
+      }
+    if (lane < m) sorted[b + lane] = v;
+    return;
+  }
+  for (int t = lane; t < m; t += 64) {
+    const int i = tmp[b + t];
+    int r = 0;
+    for (int u = 0; u < m; ++u) r += tmp[b + u] < i ? 1 : 0;
+    sorted[b + r] = i;
+  }
 }
 
 // linked-list pair loop, ParticleSystemFirstOrderPtr.cpp:398-466 (x_i wrapped, x_j as stored) and
@@ -656,7 +676,7 @@ void launch_cell_build(hipStream_t st, const DevParams &P, const double *stage, 
   SPSS_DISPATCH_S(P.s, k_cell_keys<SS><<<dim3(nblk(P.n)), dim3(kBlock), 0, st>>>(P, stage, ncx, ncy, cb.key, cb.slot, cb.count));
   launch_scan(st, cb.count, cb.start, m, cb.blocksum);
   k_scatter<<<dim3(nblk(P.n)), dim3(kBlock), 0, st>>>(P.n, cb.key, cb.slot, cb.start, cb.tmp);
-  k_stable_rank<<<dim3(nblk(P.n)), dim3(kBlock), 0, st>>>(P.n, cb.key, cb.start, cb.tmp, cb.sorted);
+  k_stable_rank<<<dim3((m + 3) / 4), dim3(256), 0, st>>>(m, cb.start, cb.tmp, cb.sorted);
 }
 
 void launch_linked_list(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy,

@@ -12,18 +12,25 @@
 // side Hv = 1/int(1/r_max); between rebuilds they keep their slots.  Per slot: FP64 state
 // (double-buffered by step), id, natural frequency, FP32 snapshot {x, y wrapped into the box,
 // cos phi, sin phi} and FP32 omega (double-buffered with the state), and an ELL list of width K
-// stored slot-major (entry t of slot p at list[t * n + p], so a wave's loads of one entry are
-// contiguous) with its length.
+// stored slot-major (entry t of slot p at list[t * n + p]) with its length.  Grids of at most
+// kRowTileCols columns also have row tiles of the rebuild's cell order (spans of whole cells of
+// one row, as for the cell-list pair kernel) and a 16-bit copy of every list: the entries' LDS
+// positions in the tile's stage, tile-local (entry t of the tile's lane u at
+// (tile * K + t) * 256 + u, so that a pass reads contiguous memory, where the slot-major list puts
+// every entry of a particle on its own page).
 //
-// One step is a rebuild -- seven launches that return at once unless the device flag asks for
-// one: cell keys and histogram, the two-level scan, bucket fill, stable rank and gather into
-// scratch, back into the resident arrays with the new snapshot, list build -- and the step kernel:
-// each lane walks its list four entries at a time: FP32 minimum-image distance (x_j - x_i minus
-// its nearest integer), a guard band around rho^2 decided with the reference's FP64 ClassC formula
-// on the stored positions (so neighbour sets are exact), phasor sums in four FP32 partials flushed
-// into FP64 every 16 entries; then the Euler/SRK update of step_particle, the new snapshot and the
-// step's largest squared displacement.  The last workgroup to finish turns that maximum into the
-// next step's flag on the device.  No host synchronisation: steps replay from HIP graphs.
+// One step is a rebuild -- launches that return at once unless the device flag asks for one:
+// cell keys and histogram, the two-level scan, bucket fill, stable rank and gather into scratch,
+// back into the resident arrays with the new snapshot, the row tiles, the lists -- and the step:
+// one workgroup per row tile stages the snapshots (and omega) of the 3 x (W + 2) cells around its
+// W cells into LDS; each lane walks its list from LDS, sixteen entries per pass: FP32
+// minimum-image distance (x_j - x_i minus its nearest integer), a guard band around rho^2 decided
+// with the reference's FP64 ClassC formula on the stored positions (so neighbour sets are exact),
+// phasor sums in four FP32 partials flushed into FP64 every 16 entries; then the Euler/SRK update
+// of step_particle, the new snapshot and the step's largest squared displacement (maxima by
+// group of workgroups); a one-wave launch after the step (vc_flag) turns it into the next step's
+// flag on the device.  No host synchronisation: steps replay from HIP graphs.  Without tiles
+// (wider grids), vr_lists and vs_step do the same per particle through global memory.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,15 +56,16 @@ constexpr int kVGraphSteps = 10;  // steps of the bulk graph (even: the state bu
 }  // namespace
 
 // device control block
+constexpr int kGroups = 64, kLineU64 = 16;  // workgroup groups of the displacement maximum, a 128-byte line each
 struct VCtl {
-  int flag;                      // rebuild before the next step (should_update_lists_)
+  int flag;                     // rebuild before the next step (should_update_lists_)
   int pad0;
-  double acc;                    // accumulated_displacement_
-  unsigned long long max2_bits;  // the step's largest |dr|^2 as the bits of a non-negative double
-  unsigned int done;             // step workgroups finished
-  unsigned int pad1;
-  unsigned long long overflow;   // lists longer than the width K since entry (walked through their cells)
-  unsigned long long step;       // Philox step of graph-replayed steps
+  double acc;                   // accumulated_displacement_
+  unsigned long long overflow;  // lists longer than the width K since entry (walked through their cells)
+  unsigned long long step;      // Philox step of graph-replayed steps
+  // the step's largest |dr|^2 by group of workgroups (workgroup b in group b % kGroups), as the
+  // bits of a non-negative double, one word per cache line so that the groups' atomics run apart
+  unsigned long long gmax[kGroups * kLineU64];
 };
 
 struct VArgs {
@@ -375,6 +383,23 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
   return (b & 7) * per + (b >> 3);
 }
 
+// The step's largest squared displacement, for vc_flag: wave, workgroup, then one atomic per
+// workgroup on the maximum of its group (kGroups words on their own cache lines).  (A last-workgroup
+// count in the step kernel needs a release fence per workgroup, which writes the XCD's L2 back:
+// 0.45 ms per step at N = 1M with 7,000 workgroups.)
+__device__ __forceinline__ void step_max(const VArgs &A, double d2, double *wred) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
+  if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = wred[0];
+    for (int w = 1; w < kVB / 64; ++w) bm = fmax(bm, wred[w]);
+    if (bm > 0.0)
+      atomicMax(&A.ctl->gmax[(blockIdx.x & (kGroups - 1)) * kLineU64], (unsigned long long)__double_as_longlong(bm));
+  }
+}
+
 template <int S, bool SRK>
 __global__ __launch_bounds__(kVB) void vs_step(DevParams P, VArgs A, StepArgs U, const int *__restrict__ list,
                                                const int *__restrict__ len, const int *__restrict__ start,
@@ -384,7 +409,6 @@ __global__ __launch_bounds__(kVB) void vs_step(DevParams P, VArgs A, StepArgs U,
                                                double *__restrict__ nstate, float4 *__restrict__ nsnap,
                                                float *__restrict__ nomega, double *__restrict__ compl_out) {
   __shared__ double wmax[kVB / 64];
-  __shared__ int last;
   const int p = xcd_block((int)blockIdx.x, (int)gridDim.x) * kVB + threadIdx.x;
   double d2 = 0.0;
   if (p < A.n) {
@@ -422,34 +446,7 @@ __global__ __launch_bounds__(kVB) void vs_step(DevParams P, VArgs A, StepArgs U,
     nomega[p] = S == 4 ? (float)out[3] : 0.f;
     if (SRK && compl_out) compl_out[i] = C;
   }
-  // the step's largest squared displacement: wave, workgroup, then one atomic per workgroup
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
-  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d2;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double bm = wmax[0];
-    for (int w = 1; w < kVB / 64; ++w) bm = fmax(bm, wmax[w]);
-    atomicMax(&A.ctl->max2_bits, (unsigned long long)__double_as_longlong(bm));
-    __threadfence();
-    last = atomicAdd(&A.ctl->done, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    // every other workgroup has added its maximum: CalculateMaxDisplacement's bookkeeping
-    // (ParticleSystemFirstOrderPtr.cpp:851-860) in the exact mode (one step of margin)
-    const double max2 = __longlong_as_double((long long)atomicAdd(&A.ctl->max2_bits, 0ull));
-    const double stepd = sqrt(max2);
-    double acc = A.ctl->acc + stepd;
-    const int flag = acc + stepd > A.half_skin ? 1 : 0;
-    if (flag) acc = 0.0;
-    A.ctl->acc = acc;
-    A.ctl->flag = flag;
-    A.ctl->max2_bits = 0ull;
-    A.ctl->done = 0u;
-    if (U.step_inc) *U.step_inc += 1ull;
-    __threadfence();
-  }
+  step_max(A, d2, wmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -472,7 +469,6 @@ struct VtLds {
   int ccnt[3 * kVCols];      // its particle count
   int cpos[3 * kVCols + 1];  // its LDS start; [cells] = the staged total
   double wmax[kVB / 64];
-  int last;
 };
 
 // the staging plan of tile (row, ca, cb): rows row-1 .. row+1, columns ca-1 .. cb (each wrapped
@@ -542,7 +538,8 @@ __global__ __launch_bounds__(256) void vr_tiles(VArgs A, const int *__restrict__
 
 // The lists of a tile's particles, in list_of_slot's order (stencil cells in order, ascending slot
 // inside a cell) and with its decisions, from the staged snapshots; each entry is written twice:
-// as a slot (list) and as its LDS position in the tile's stage (list16).
+// as a slot (list, slot-major over all particles) and as its LDS position in the tile's stage
+// (list16, tile-local: entry t of the tile's lane u at (tile * K + t) * kVB + u).
 template <int S>
 __global__ __launch_bounds__(kVB) void vt_lists(DevParams P, VArgs A, const int4 *__restrict__ tiles, int ntiles,
                                                 const int *__restrict__ start, const int *__restrict__ cell,
@@ -568,6 +565,9 @@ __global__ __launch_bounds__(kVB) void vt_lists(DevParams P, VArgs A, const int4
   const int ltb = L.cpos[ncols + 1];
   for (int u = threadIdx.x; u < ntile; u += kVB) {
     const int q = gtb + u, li = ltb + u;
+    // the 16-bit copy, tile-local (entry t of lane u at (tile * K + t) * kVB + u): a pass of the
+    // step reads one contiguous block per entry instead of one page per entry
+    unsigned short *l16 = u < kVB ? list16 + (long)tile * A.K * kVB + u : nullptr;
     const float4 me = L.sdat[li];
     double xi, yi;
     apply_pbc(P, state[(long)q * S], state[(long)q * S + 1], xi, yi);
@@ -590,7 +590,7 @@ __global__ __launch_bounds__(kVB) void vt_lists(DevParams P, VArgs A, const int4
           if (in) {
             if (w < A.K) {
               list[(long)w * A.n + q] = g0 + (t - t0);
-              list16[(long)w * A.n + q] = (unsigned short)t;
+              if (l16) l16[(long)w * kVB] = (unsigned short)t;
             }
             ++w;
           }
@@ -602,10 +602,11 @@ __global__ __launch_bounds__(kVB) void vt_lists(DevParams P, VArgs A, const int4
 }
 
 // walk_list with the candidates read from the tile's stage (entries past L: the particle itself,
-// li, with weight zero): the pass's sixteen list entries are loaded first (global memory), the
-// candidates four at a time from LDS, in walk_list's order and grouping
+// li, with weight zero; l16: the lane's tile-local 16-bit list, entry t at l16[t * kVB]): the
+// pass's sixteen list entries are loaded first (global memory), the candidates four at a time from
+// LDS, in walk_list's order and grouping
 template <int S, bool SRK>
-__device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A, const unsigned short *__restrict__ list16,
+__device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A, const unsigned short *__restrict__ l16,
                                               const VtLds<SRK> &Ls, int ncells, const double *__restrict__ state,
                                               int p, int li, int L, float xi, float yi, const double *pi, double &sx,
                                               double &sy, double &tx, double &ty, int &nnb) {
@@ -614,7 +615,7 @@ __device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A
     float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
     int j[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) j[k] = t0 + k < L ? (int)list16[(long)(t0 + k) * A.n + p] : li;
+    for (int k = 0; k < 16; ++k) j[k] = t0 + k < L ? (int)l16[(long)(t0 + k) * kVB] : li;
 #pragma unroll
     for (int g = 0; g < 16; g += 4) {
       float4 o[4];
@@ -662,7 +663,8 @@ __device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A
 }
 
 // The step of slot p (vs_step's per-particle work); candidates from the tile's stage when
-// `staged` (li: p's LDS position), else through global memory.  Returns |dr|^2.
+// `staged` (li: p's LDS position, list16: p's tile-local 16-bit list), else through global
+// memory.  Returns |dr|^2.
 template <int S, bool SRK>
 __device__ __forceinline__ double vt_particle(const DevParams &P, const VArgs &A, const StepArgs &U, const Coef &K,
                                               uint64_t step, const int *__restrict__ list,
@@ -716,8 +718,8 @@ __device__ __forceinline__ Coef vt_coef(const DevParams &P, const StepArgs &U) {
 }
 
 // The particles of an oversized tile past the first kVB (a single cell holding more than a tile's
-// particles: dense clusters), through global memory; launched before vt_step, whose last
-// workgroup then sees their displacements too.
+// particles: dense clusters), through global memory; launched before vt_step; their displacements
+// join the group maxima too.
 template <int S, bool SRK>
 __global__ __launch_bounds__(kVB) void vt_rest(DevParams P, VArgs A, StepArgs U, const int4 *__restrict__ tiles,
                                                int ntiles, const int *__restrict__ list, const int *__restrict__ len,
@@ -739,19 +741,11 @@ __global__ __launch_bounds__(kVB) void vt_rest(DevParams P, VArgs A, StepArgs U,
   for (int u = kVB + threadIdx.x; u < ntile; u += kVB)
     d2 = fmax(d2, vt_particle<S, SRK>(P, A, U, K, step, list, nullptr, len, start, cell, state, id, nat, snap, omega,
                                       nstate, nsnap, nomega, compl_out, nullptr, false, 0, g0 + u, 0));
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
-  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d2;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double bm = wmax[0];
-    for (int w = 1; w < kVB / 64; ++w) bm = fmax(bm, wmax[w]);
-    atomicMax(&A.ctl->max2_bits, (unsigned long long)__double_as_longlong(bm));
-  }
+  step_max(A, d2, wmax);
 }
 
-// The step of a tile's first kVB particles, one per lane (vt_rest has the others); the step's
-// largest displacement and the next rebuild flag as in vs_step.
+// The step of a tile's first kVB particles, one per lane (vt_rest has the others), and the step's
+// largest displacement as in vs_step.
 template <int S, bool SRK>
 __global__ __launch_bounds__(kVB) void vt_step(DevParams P, VArgs A, StepArgs U, const int4 *__restrict__ tiles,
                                                int ntiles, const int *__restrict__ list,
@@ -775,38 +769,12 @@ __global__ __launch_bounds__(kVB) void vt_step(DevParams P, VArgs A, StepArgs U,
     const int ntile = start[row * A.ncv + cb] - gtb;
     const int u = threadIdx.x;
     if (u < ntile)
-      d2 = vt_particle<S, SRK>(P, A, U, vt_coef(P, U), U.step_dev ? (uint64_t)*U.step_dev : U.step, list, list16, len,
+      d2 = vt_particle<S, SRK>(P, A, U, vt_coef(P, U), U.step_dev ? (uint64_t)*U.step_dev : U.step, list,
+                               list16 + (long)tile * A.K * kVB + u, len,
                                start, cell, state, id, nat, snap, omega, nstate, nsnap, nomega, compl_out, &Ls, staged,
                                ncells, gtb + u, Ls.cpos[ncols + 1] + u);
   }
-  // the step's largest squared displacement: wave, workgroup, then one atomic per workgroup
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
-  if ((threadIdx.x & 63) == 0) Ls.wmax[threadIdx.x >> 6] = d2;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double bm = Ls.wmax[0];
-    for (int w = 1; w < kVB / 64; ++w) bm = fmax(bm, Ls.wmax[w]);
-    atomicMax(&A.ctl->max2_bits, (unsigned long long)__double_as_longlong(bm));
-    __threadfence();
-    Ls.last = atomicAdd(&A.ctl->done, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (Ls.last && threadIdx.x == 0) {
-    // every other workgroup has added its maximum: CalculateMaxDisplacement's bookkeeping
-    // (ParticleSystemFirstOrderPtr.cpp:851-860) in the exact mode (one step of margin)
-    const double max2 = __longlong_as_double((long long)atomicAdd(&A.ctl->max2_bits, 0ull));
-    const double stepd = sqrt(max2);
-    double acc = A.ctl->acc + stepd;
-    const int flag = acc + stepd > A.half_skin ? 1 : 0;
-    if (flag) acc = 0.0;
-    A.ctl->acc = acc;
-    A.ctl->flag = flag;
-    A.ctl->max2_bits = 0ull;
-    A.ctl->done = 0u;
-    if (U.step_inc) *U.step_inc += 1ull;
-    __threadfence();
-  }
+  step_max(A, d2, Ls.wmax);
 }
 
 // the largest particle count of any 3 x 3 block of cells (the candidates of one list)
@@ -824,9 +792,28 @@ __global__ void vc_enter(VCtl *c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     c->flag = 1;  // should_update_lists_ starts true (ParticleSystemFirstOrderPtr.cpp:99)
     c->acc = 0.0;
-    c->max2_bits = 0ull;
-    c->done = 0u;
     c->overflow = 0ull;
+  }
+  for (int g = threadIdx.x; g < kGroups; g += blockDim.x) c->gmax[g * kLineU64] = 0ull;
+}
+
+// After the step: CalculateMaxDisplacement's bookkeeping (ParticleSystemFirstOrderPtr.cpp:851-860)
+// in the exact mode (one step of margin) from the group maxima, which are cleared for the next
+// step; the graph's Philox step counter advances (the step kernel has read it).  One wave.
+__global__ void vc_flag(VCtl *c, double half_skin, unsigned long long *step_inc) {
+  const int lane = threadIdx.x;
+  double m = __longlong_as_double((long long)c->gmax[lane * kLineU64]);
+  c->gmax[lane * kLineU64] = 0ull;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (lane == 0) {
+    const double stepd = sqrt(m);
+    double acc = c->acc + stepd;
+    const int flag = acc + stepd > half_skin ? 1 : 0;
+    if (flag) acc = 0.0;
+    c->acc = acc;
+    c->flag = flag;
+    if (step_inc) *step_inc += 1ull;
   }
 }
 
@@ -873,7 +860,8 @@ struct VerletFast {
   int *sid = nullptr, *scell = nullptr;
   VCtl *ctl = nullptr;
   int *bmax = nullptr;
-  size_t list_cap = 0;  // ints allocated for the lists
+  size_t list_cap = 0;    // ints allocated for the lists
+  size_t list16_cap = 0;  // entries allocated for the tile-local 16-bit lists
   bool resident = false, resident_nat = false;
   int cur = 0;
   struct StepGraph {
@@ -1049,16 +1037,19 @@ int vf_enter(VerletFast *v, hipStream_t st, const DevParams &P, int srk, const d
   // an upper bound on the width from the environment (memory cap; lists beyond it walk their cells)
   if (const char *cap = std::getenv("SPSS_VERLET_WIDTH_CAP")) K = std::max(4, std::min(K, std::atoi(cap)));
   const size_t need = (size_t)v->n * (size_t)K;
-  if (need > v->list_cap) {
+  const size_t need16 = v->tiled ? (size_t)v->ncv * v->vt_cmax * (size_t)K * kVB : 0;  // tile-local
+  if (need > v->list_cap || need16 > v->list16_cap) {
     drop_graphs(v);  // captured launches hold the old list pointer
     hipFree(v->list);
     hipFree(v->list16);
     v->list = nullptr;
     v->list16 = nullptr;
     v->list_cap = 0;
+    v->list16_cap = 0;
     VCHK(hipMalloc(&v->list, sizeof(int) * need));
-    if (v->tiled) VCHK(hipMalloc(&v->list16, sizeof(unsigned short) * need));
+    if (v->tiled) VCHK(hipMalloc(&v->list16, sizeof(unsigned short) * need16));
     v->list_cap = need;
+    v->list16_cap = need16;
   }
   if (K != v->K) drop_graphs(v);  // the width is a launch argument of the captured steps
   v->K = K;
@@ -1098,13 +1089,14 @@ static void launch_step(VerletFast *v, hipStream_t st, const DevParams &P, const
     vt_step<S, SRK><<<tg, kVB, 0, st>>>(P, A, U, v->vtiles, ntiles, v->list, v->list16, v->len, v->start, v->vcell,
                                         v->vstate[c], v->vid, nat, v->vsnap[c], v->vomega[c], v->vstate[o],
                                         v->vsnap[o], v->vomega[o], SRK ? compl_out : nullptr);
-    return;
+  } else {
+    vr_lists<S><<<nb, kVB, 0, st>>>(P, A, v->start, v->vcell, v->vstate[c], v->vsnap[c], v->list, v->len);
+    const unsigned nb8 = (nb + 7u) & ~7u;  // a multiple of 8 for xcd_block
+    vs_step<S, SRK><<<nb8, kVB, 0, st>>>(P, A, U, v->list, v->len, v->start, v->vcell, v->vstate[c], v->vid, nat,
+                                         v->vsnap[c], v->vomega[c], v->vstate[o], v->vsnap[o], v->vomega[o],
+                                         SRK ? compl_out : nullptr);
   }
-  vr_lists<S><<<nb, kVB, 0, st>>>(P, A, v->start, v->vcell, v->vstate[c], v->vsnap[c], v->list, v->len);
-  const unsigned nb8 = (nb + 7u) & ~7u;  // a multiple of 8 for xcd_block
-  vs_step<S, SRK><<<nb8, kVB, 0, st>>>(P, A, U, v->list, v->len, v->start, v->vcell, v->vstate[c], v->vid, nat,
-                                       v->vsnap[c], v->vomega[c], v->vstate[o], v->vsnap[o], v->vomega[o],
-                                       SRK ? compl_out : nullptr);
+  vc_flag<<<1, kGroups, 0, st>>>(A.ctl, A.half_skin, U.step_inc);
 }
 
 static StepArgs vstep_args(double dt, double D_phi, double friction, int noise_mode, const double *noise,
