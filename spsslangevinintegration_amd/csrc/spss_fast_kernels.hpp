@@ -754,27 +754,28 @@ __global__ __launch_bounds__(256) void fk_row_tiles(const int *__restrict__ star
   rows_to_tiles([&](int c) { return start[c]; }, ncf, wcap, cmax, tiles);
 }
 
-// The arrivals of a resident step: the new cell starts and the far arrivals' slots, computed by
-// extra workgroups of the pair kernel (fr_pairs).  The move words and arrival counts they scan
-// were written by the previous step's fr_scatter (or fr_move on entry), so this work needs
-// nothing from the pair sums and runs beside the tiles instead of after them.
+// The arrivals of a resident step: the new cell starts, computed by extra workgroups of the pair
+// kernel (fr_pairs), and the far arrivals' slots (fr_far, the launch after it).  The move words and
+// arrival counts they read were written by the previous step's fr_scatter (or fr_move on entry), so
+// the scan needs nothing from the pair sums and runs beside the tiles instead of after them.
 //  * Workgroups [0, nsb) scan 2048 cells each (the arrival counts newcnt + farcnt, cleared as they
 //    are read) and find the total of the cells before theirs by a decoupled look-back over the
 //    earlier workgroups' status words (aggregate, then inclusive prefix; workgroups are dispatched
 //    in index order and a workgroup only waits on lower ones, so every wait ends; the waits are
 //    also bounded), then write the final starts, and clear the next step's move counters.  The
-//    last of them to finish places the step's far movers (a compact list filled where the moves
-//    were computed) behind the near arrivals of their destination cells, and resets the list, the
-//    status words and its own counter.
-// The pair kernel's overflow work runs after it (fk_pairs_ovf), before fr_scatter reads the sums.
+//    status words carry the values they publish, so relaxed atomics suffice: no fence, no acquire
+//    (on this GPU those write back or invalidate the XCD's L2 under the tiles running beside).
+//  * fr_far places the step's far movers (a compact list filled where the moves were computed)
+//    behind the near arrivals of their destination cells, clears the status words, and runs the
+//    pair kernel's overflow work (its sums are needed only by fr_scatter).
 struct Arrivals {
   int *newcnt, *farcnt;        // arrival counts per cell (read, then cleared)
   int m, nsb;                  // cells, scan workgroups
   int *nstart;                 // [m + 1] new cell starts
   unsigned long long *status;  // [nsb] look-back words: kLbAgg | total, or kLbPre | inclusive prefix
-  unsigned int *done;          // scan workgroups finished
-  int *farlist, *farlist_n;    // this step's far movers {slot, destination cell} and their number
-  const int *mv;               // this step's move words
+  const int *farlist;          // this step's far movers {slot, destination cell}
+  const int *farlist_n;        // their number
+  int *farlist_n_next;         // the next step's count, cleared here for this step's fr_scatter
   const int *cnt9;             // this step's move counts per (cell, direction)
   const int *farslot;          // each far mover's slot among its destination's far arrivals
   int *bucket;                 // far arrivals by new position
@@ -786,16 +787,19 @@ struct Arrivals {
 
 constexpr unsigned long long kLbAgg = 1ull << 62, kLbPre = 2ull << 62, kLbVal = (1ull << 62) - 1;
 
-// one scan workgroup b < Q.nsb of the arrivals; sh: kFB ints of LDS, sprefix and slast LDS words
-__device__ __forceinline__ void arrivals_scan(const Arrivals &Q, const StepArgs &U, int b, int *sh, long long &sprefix,
-                                              int &slast) {
+// one scan workgroup b < Q.nsb of the arrivals; sh: kFB ints of LDS, sprefix an LDS word
+__device__ __forceinline__ void arrivals_scan(const Arrivals &Q, const StepArgs &U, int b, int *sh,
+                                              long long &sprefix) {
   {
     const long stride = (long)Q.nsb * kFB;
     for (long e = (long)b * kFB + threadIdx.x; e < Q.nclear; e += stride) Q.cnt9_next[e] = 0;
   }
   // graph-replayed steps: the counter read by this step's fr_scatter moves on here, one launch
   // after the previous step's fr_scatter read it
-  if (U.step_inc && b == 0 && threadIdx.x == 0) *U.step_inc += 1ull;
+  if (b == 0 && threadIdx.x == 0) {
+    if (U.step_inc) *U.step_inc += 1ull;
+    *Q.farlist_n_next = 0;  // fr_scatter of this step appends the next step's far movers
+  }
   const long base = (long)b * kFB * kScanItems + (long)threadIdx.x * kScanItems;
   int v[kScanItems];
   int s = 0;
@@ -824,15 +828,15 @@ __device__ __forceinline__ void arrivals_scan(const Arrivals &Q, const StepArgs 
     const unsigned long long total = (unsigned long long)sh[kFB - 1];
     long long excl = 0;
     if (b == 0) {
-      if (lane == 0) __hip_atomic_store(&Q.status[0], kLbPre | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) __hip_atomic_store(&Q.status[0], kLbPre | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      if (lane == 0) __hip_atomic_store(&Q.status[b], kLbAgg | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) __hip_atomic_store(&Q.status[b], kLbAgg | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       for (int top = b - 1;; top -= 64) {
         const int j = top - lane;
         unsigned long long w = 0;
         if (j >= 0) {
           for (long spin = 0; spin < (1L << 26); ++spin) {
-            w = __hip_atomic_load(&Q.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            w = __hip_atomic_load(&Q.status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (w) break;
             __builtin_amdgcn_s_sleep(1);
           }
@@ -848,7 +852,7 @@ __device__ __forceinline__ void arrivals_scan(const Arrivals &Q, const StepArgs 
         if (stops) break;
       }
       if (lane == 0)
-        __hip_atomic_store(&Q.status[b], kLbPre | (unsigned long long)(excl + (long long)total), __ATOMIC_RELEASE,
+        __hip_atomic_store(&Q.status[b], kLbPre | (unsigned long long)(excl + (long long)total), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
     if (lane == 0) sprefix = excl;
@@ -861,26 +865,6 @@ __device__ __forceinline__ void arrivals_scan(const Arrivals &Q, const StepArgs 
     run += v[q];
   }
   if (b == Q.nsb - 1 && threadIdx.x == kFB - 1) Q.nstart[Q.m] = run;  // every cell's particles: n
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) slast = atomicAdd(Q.done, 1u) == (unsigned)(Q.nsb - 1);
-  __syncthreads();
-  if (!slast) return;
-  __threadfence();
-  // the last scan workgroup: every start is final.  Far movers behind the near arrivals of their
-  // destination cells, in the unordered slots their atomics gave them (fr_scatter orders them).
-  const int nfar = __hip_atomic_load(Q.farlist_n, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-  for (int k = threadIdx.x; k < nfar; k += kFB) {
-    const int pf = Q.farlist[2 * k], cd = Q.farlist[2 * k + 1];
-    const int q = Q.nstart[cd] + new_count_of(Q.cnt9, cd, Q.ncf, Q.cpr) + Q.farslot[pf];
-    if (q < Q.n) Q.bucket[q] = pf;  // always, when the counts match the moves
-  }
-  __syncthreads();
-  for (int j = threadIdx.x; j < Q.nsb; j += kFB) Q.status[j] = 0ull;
-  if (threadIdx.x == 0) {
-    *Q.farlist_n = 0;
-    *Q.done = 0u;
-  }
 }
 
 // The pair kernel of a resident step with the step's arrivals: workgroups [0, nsb8) are the scan
@@ -897,7 +881,7 @@ __global__ __launch_bounds__(kFB) void fr_pairs(DevParams P, FastArgs A, PairBuf
     if ((int)blockIdx.x < Q.nsb) {
       static_assert(sizeof(L.cglob) >= kFB * sizeof(int), "scan scratch");
       long long *words = reinterpret_cast<long long *>(L.sdat);
-      arrivals_scan(Q, U, (int)blockIdx.x, L.cglob, words[0], *reinterpret_cast<int *>(words + 1));
+      arrivals_scan(Q, U, (int)blockIdx.x, L.cglob, words[0]);
     }
     return;
   }
@@ -908,6 +892,22 @@ __global__ __launch_bounds__(kFB) void fr_pairs(DevParams P, FastArgs A, PairBuf
   const int4 td = A.tiles[tile];
   if (td.z >= td.w) return;  // an unused row-tile slot
   pairs_span<S, SRK, 3, true>(P, A, B, td.x, td.y, td.z, td.w, L);  // overflow: split for fk_pairs_ovf
+}
+
+// The launch after fr_pairs: the step's far movers behind the near arrivals of their destination
+// cells (the starts are final now), in the unordered slots their atomics gave them (fr_scatter
+// orders them); the look-back status words cleared; the pair kernel's overflow work.
+template <int S, bool SRK>
+__global__ __launch_bounds__(kFB) void fr_far(DevParams P, FastArgs A, PairBufs B, Arrivals Q) {
+  const int nfar = *Q.farlist_n;
+  for (int k = blockIdx.x * kFB + threadIdx.x; k < nfar; k += gridDim.x * kFB) {
+    const int pf = Q.farlist[2 * k], cd = Q.farlist[2 * k + 1];
+    const int q = Q.nstart[cd] + new_count_of(Q.cnt9, cd, Q.ncf, Q.cpr) + Q.farslot[pf];
+    if (q < Q.n) Q.bucket[q] = pf;  // always, when the counts match the moves
+  }
+  if (blockIdx.x == 0)
+    for (int j = threadIdx.x; j < Q.nsb; j += kFB) Q.status[j] = 0ull;
+  overflow_items<S, SRK, 3>(P, A, B, blockIdx.x, gridDim.x);
 }
 
 // the step of every particle, written to its new slot with its id, natural frequency and

@@ -19,10 +19,10 @@
 //
 // Fused stochastic steps keep the state resident in cell order (see spss_fast_kernels.hpp):
 //   fr_pairs (MODE 3 phasor sums; its first workgroups scan the arrival counts the previous step
-//   left, with a decoupled look-back, into the new cell starts and place the far arrivals) ->
-//   fk_pairs_ovf (the tiles that overflowed the LDS stage; usually none) -> fr_scatter (FP64
-//   Euler/SRK update written to the new slots, the next step's moves), three launches per step and
-//   no host synchronisation.
+//   left, with a decoupled look-back, into the new cell starts) -> fr_far (the far arrivals' slots
+//   and the tiles that overflowed the LDS stage; usually none) -> fr_scatter (FP64 Euler/SRK update
+//   written to the new slots, the next step's moves), three launches per step and no host
+//   synchronisation.
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -103,8 +103,9 @@ struct FastPath {
   // the arrivals scan: look-back status words per scan workgroup, its finished count, the
   // far movers of the step (slots) and their number, a flag for a look-back wait that ran out
   unsigned long long *lb_status = nullptr;
-  unsigned int *lb_done = nullptr;
-  int *farlist = nullptr, *farlist_n = nullptr, *lb_error = nullptr;
+  // far movers by step parity (cur): the step's list and count, the next step's filled by fr_scatter
+  int *farlist[2] = {nullptr, nullptr}, *farlist_n[2] = {nullptr, nullptr};
+  int *lb_error = nullptr;
   int *farslot = nullptr;  // each far mover's slot among its destination's far arrivals
   bool mv_ready = false;
   bool mv_srk = false;
@@ -279,10 +280,11 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->rep_chunk, nchunk));
   A(falloc(&f->step_dev, 1));
   A(falloc(&f->lb_status, f->nsb));
-  A(falloc(&f->lb_done, 1));
-  A(falloc(&f->farlist, 2L * n));  // {slot, destination cell} per far mover
+  for (int b = 0; b < 2; ++b) {
+    A(falloc(&f->farlist[b], 2L * n));  // {slot, destination cell} per far mover
+    A(falloc(&f->farlist_n[b], 1));
+  }
   A(falloc(&f->farslot, n));
-  A(falloc(&f->farlist_n, 1));
   A(falloc(&f->lb_error, 1));
   if (coef) {
     A(falloc(&f->coef, R));
@@ -311,8 +313,8 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   launch_fill_int(st, 1, f->ovf_count, 0);
   launch_fill_int(st, 2, reinterpret_cast<int *>(f->step_dev), 0);  // the 64-bit step counter
   launch_fill_int(st, 2L * f->nsb, reinterpret_cast<int *>(f->lb_status), 0);
-  launch_fill_int(st, 1, reinterpret_cast<int *>(f->lb_done), 0);
-  launch_fill_int(st, 1, f->farlist_n, 0);
+  launch_fill_int(st, 1, f->farlist_n[0], 0);
+  launch_fill_int(st, 1, f->farlist_n[1], 0);
   launch_fill_int(st, 1, f->lb_error, 0);
   if (coef) {
     launch_fill_double(st, 3L * R, f->racc, 0.0);
@@ -339,8 +341,8 @@ void fast_destroy(FastPath *f) {
                f->rnat[1],   f->rstart[0], f->rstart[1], f->rsnap,   f->romega,    f->newcnt,    f->mvb[0], f->mvb[1],
                f->cnt9[0],   f->cnt9[1],   f->farcnt,    f->ssum,      f->tsum,    f->scnt,      f->rep_first, f->rep_chunk,
                f->tiles,     f->coef,      f->step_dev,  f->zlast,   f->racc,      f->ovf_list,  f->ovf_count,
-               f->rtiles,    f->btiles,    f->lb_status, f->lb_done,  f->farlist,  f->farlist_n, f->lb_error,
-               f->farslot};
+               f->rtiles,    f->btiles,    f->lb_status, f->farlist[0], f->farlist[1], f->farlist_n[0],
+               f->farlist_n[1], f->lb_error, f->farslot};
   for (void *p : b) hipFree(p);
   delete f;
 }
@@ -506,7 +508,7 @@ static void drop_moves(FastPath *f, hipStream_t st) {
   launch_fill_int(st, f->cells, f->newcnt, 0);
   launch_fill_int(st, f->cells, f->farcnt, 0);
   launch_fill_int(st, (long)f->cells * 9, f->cnt9[f->cur], 0);
-  launch_fill_int(st, 1, f->farlist_n, 0);
+  launch_fill_int(st, 1, f->farlist_n[f->cur], 0);
   f->mv_ready = false;
 }
 
@@ -518,7 +520,7 @@ static void ensure_moves(FastPath *f, hipStream_t st, const DevParams &P, const 
   drop_moves(f, st);
   fr_move<S, SRK><<<nblocks(f->n), 256, 0, st>>>(P, layout_of(f), f->n, f->ncf, f->rstate[f->cur], U,
                                                  f->mvb[f->mvc], f->farcnt, f->farslot, f->cnt9[f->cur], f->newcnt,
-                                                 f->farlist, f->farlist_n);
+                                                 f->farlist[f->cur], f->farlist_n[f->cur]);
   f->mv_ready = true;
   f->mv_dt = U.dt;
   f->mv_srk = SRK;
@@ -561,10 +563,9 @@ static Arrivals arrivals_of(const FastPath *f) {
   Q.nsb = f->nsb;
   Q.nstart = f->rstart[o];
   Q.status = f->lb_status;
-  Q.done = f->lb_done;
-  Q.farlist = f->farlist;
-  Q.farlist_n = f->farlist_n;
-  Q.mv = f->mvb[f->mvc];
+  Q.farlist = f->farlist[c];
+  Q.farlist_n = f->farlist_n[c];
+  Q.farlist_n_next = f->farlist_n[o];
   Q.cnt9 = f->cnt9[c];
   Q.farslot = f->farslot;
   Q.bucket = f->tmp;
@@ -577,8 +578,9 @@ static Arrivals arrivals_of(const FastPath *f) {
   return Q;
 }
 
-// the pair kernel of a resident step with the step's arrivals in its first workgroups (fr_pairs);
-// the work of tiles that overflowed the LDS stage follows in fk_pairs_ovf (timed apart)
+// the pair kernel of a resident step with the step's arrival scan in its first workgroups
+// (fr_pairs), then the far arrivals and the work of tiles that overflowed the LDS stage (fr_far,
+// timed apart)
 template <int S, bool SRK>
 static void resident_pairs(const FastPath *f, hipStream_t st, const DevParams &P, const StepArgs &U,
                            const PairBufs &B) {
@@ -588,10 +590,11 @@ static void resident_pairs(const FastPath *f, hipStream_t st, const DevParams &P
     a.ntiles = f->ncf * f->rt_cmax;
   }
   const int nsb8 = (f->nsb + 7) & ~7;
+  const Arrivals Q = arrivals_of(f);
   if (f->timer) f->timer->begin(st);
-  fr_pairs<S, SRK><<<nsb8 + tile_grid(a.ntiles), kFB, 0, st>>>(P, a, B, arrivals_of(f), U);
+  fr_pairs<S, SRK><<<nsb8 + tile_grid(a.ntiles), kFB, 0, st>>>(P, a, B, Q, U);
   if (f->timer) f->timer->end(st);
-  fk_pairs_ovf<S, SRK, 3><<<overflow_blocks(f), kFB, 0, st>>>(P, a, B);
+  fr_far<S, SRK><<<overflow_blocks(f), kFB, 0, st>>>(P, a, B, Q);
 }
 
 // the step applied and written to the new slots (after resident_pairs)
@@ -609,7 +612,8 @@ static void resident_scatter(FastPath *f, hipStream_t st, const DevParams &P, co
                                         f->rstate[c], f->rid[c], rnat, f->ssum, f->tsum, f->scnt, U,
                                         SRK ? compl_out : nullptr, f->rstate[o], f->rid[o], f->rnat[o], f->rsnap,
                                         SRK ? f->romega : nullptr, f->rtiles, f->rt_wcap, f->rt_cmax, mvn, f->farcnt,
-                                        f->farslot, f->cnt9[o], f->newcnt, f->farlist, f->farlist_n, f->ovf_count);
+                                        f->farslot, f->cnt9[o], f->newcnt, f->farlist[o], f->farlist_n[o],
+                                        f->ovf_count);
   f->cur = o;
   f->mvc = 1 - f->mvc;  // the next step's moves (same dt, stepper and v0) are in place
 }
