@@ -93,33 +93,26 @@ __global__ void k_scatter(int n, const int *__restrict__ key, const int *__restr
 }
 
 // Each cell's members in ascending particle index (the order in which the FP64 pair loop sums a
-// cell's members): one wave per cell.  Cells of at most 64 members (every cell at the densities
-// of the workloads) are sorted by a bitonic network across the wave's lanes, 21 compare-exchange
-// stages by lane shuffles; a denser cell ranks each member by counting the smaller indices of its
-// cell.
-__global__ __launch_bounds__(256) void k_stable_rank(int ncells, const int *__restrict__ start,
-                                                     const int *__restrict__ tmp, int *__restrict__ sorted) {
+// cell's members).  Cells of at most 64 members (every cell at the densities of the workloads) are
+// sorted by one wave each (wave_sort64, grid-stride over cells); the members of denser cells are
+// then ranked one thread each by counting the smaller indices of their cell (grid-stride over the
+// particles, so a single cell holding most particles is spread over the whole grid).
+__global__ __launch_bounds__(256) void k_stable_rank(int n, int ncells, const int *__restrict__ key,
+                                                     const int *__restrict__ start, const int *__restrict__ tmp,
+                                                     int *__restrict__ sorted) {
   const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= ncells) return;  // the whole wave
-  const int b = start[c], m = start[c + 1] - b;
-  if (m <= 64) {
-    int v = lane < m ? tmp[b + lane] : 0x7fffffff;
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1)
-#pragma unroll
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        const int o = __shfl_xor(v, j, 64);
-        // the lower lane of a pair keeps the smaller value in an ascending run of size k
-        v = (((lane & j) == 0) == ((lane & k) == 0)) ? min(v, o) : max(v, o);
-      }
+  for (int c = blockIdx.x * 4 + (threadIdx.x >> 6); c < ncells; c += gridDim.x * 4) {
+    const int b = start[c], m = start[c + 1] - b;
+    if (m > 64) continue;  // the whole wave
+    const int v = wave_sort64(lane < m ? tmp[b + lane] : 0x7fffffff, lane);
     if (lane < m) sorted[b + lane] = v;
-    return;
   }
-  for (int t = lane; t < m; t += 64) {
-    const int i = tmp[b + t];
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int c = key[i];
+    const int b = start[c], e = start[c + 1];
+    if (e - b <= 64) continue;
     int r = 0;
-    for (int u = 0; u < m; ++u) r += tmp[b + u] < i ? 1 : 0;
+    for (int t = b; t < e; ++t) r += tmp[t] < i ? 1 : 0;
     sorted[b + r] = i;
   }
 }
@@ -676,7 +669,8 @@ void launch_cell_build(hipStream_t st, const DevParams &P, const double *stage, 
   SPSS_DISPATCH_S(P.s, k_cell_keys<SS><<<dim3(nblk(P.n)), dim3(kBlock), 0, st>>>(P, stage, ncx, ncy, cb.key, cb.slot, cb.count));
   launch_scan(st, cb.count, cb.start, m, cb.blocksum);
   k_scatter<<<dim3(nblk(P.n)), dim3(kBlock), 0, st>>>(P.n, cb.key, cb.slot, cb.start, cb.tmp);
-  k_stable_rank<<<dim3((m + 3) / 4), dim3(256), 0, st>>>(m, cb.start, cb.tmp, cb.sorted);
+  k_stable_rank<<<dim3(std::max((m + 3) / 4, (int)nblk(P.n))), dim3(256), 0, st>>>(P.n, m, cb.key, cb.start, cb.tmp,
+                                                                                 cb.sorted);
 }
 
 void launch_linked_list(hipStream_t st, const DevParams &P, const double *stage, int ncx, int ncy,

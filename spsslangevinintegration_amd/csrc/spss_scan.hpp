@@ -1,5 +1,5 @@
 // spss_scan.hpp -- two-level exclusive scan of int counts over many workgroups (cell starts of the
-// counting sorts, Verlet list offsets).  Device code shared by the FP64 and FP32C translation
+// counting sorts, Verlet list offsets), and a wave-wide sort of 64 values.  Device code shared by the FP64 and FP32C translation
 // units: device functions are inline and the kernels are templates, so each unit may include it.
 //
 //   scan_local<I>   one workgroup of 256 threads per 256*I counts: block-local exclusive starts
@@ -11,6 +11,20 @@
 #include <hip/hip_runtime.h>
 
 namespace spss {
+
+// ascending sort of one value per lane across a wave of 64 (unused lanes pass INT_MAX): a bitonic
+// network of 21 compare-exchange stages by lane shuffles
+__device__ __forceinline__ int wave_sort64(int v, int lane) {
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int o = __shfl_xor(v, j, 64);
+      // the lower lane of a pair keeps the smaller value in an ascending run of size k
+      v = (((lane & j) == 0) == ((lane & k) == 0)) ? min(v, o) : max(v, o);
+    }
+  return v;
+}
 
 constexpr int kScanItems = 8;  // counts per thread in the two-level scans (2048 per workgroup)
 

@@ -12,12 +12,11 @@
 // side Hv = 1/int(1/r_max); between rebuilds they keep their slots.  Per slot: FP64 state
 // (double-buffered by step), id, natural frequency, FP32 snapshot {x, y wrapped into the box,
 // cos phi, sin phi} and FP32 omega (double-buffered with the state), and an ELL list of width K
-// stored slot-major (entry t of slot p at list[t * n + p]) with its length.  Grids of at most
+// with its length, in blocks of 256 slots and chunks of four entries (lslot).  Grids of at most
 // kRowTileCols columns also have row tiles of the rebuild's cell order (spans of whole cells of
 // one row, as for the cell-list pair kernel) and a 16-bit copy of every list: the entries' LDS
-// positions in the tile's stage, tile-local (entry t of the tile's lane u at
-// (tile * K + t) * 256 + u, so that a pass reads contiguous memory, where the slot-major list puts
-// every entry of a particle on its own page).
+// positions in the tile's stage, tile-local in the same chunked form (l16chunk).  Either way a
+// wave's loads of one chunk are contiguous and a particle's entries are not on pages of their own.
 //
 // One step is a rebuild -- launches that return at once unless the device flag asks for one:
 // cell keys and histogram, the two-level scan, bucket fill, stable rank and gather into scratch,
@@ -53,6 +52,7 @@ namespace {
 thread_local std::string g_verr;
 constexpr int kVB = 256;         // threads per workgroup
 constexpr int kVGraphSteps = 10;  // steps of the bulk graph (even: the state buffers end where they started)
+constexpr int kGateGrid = 1024;   // workgroups of the launches that run on rebuilds (and vt_rest) only
 }  // namespace
 
 // device control block
@@ -63,6 +63,8 @@ struct VCtl {
   double acc;                   // accumulated_displacement_
   unsigned long long overflow;  // lists longer than the width K since entry (walked through their cells)
   unsigned long long step;      // Philox step of graph-replayed steps
+  int big_tiles;                // tiles of the last rebuild holding more particles than a workgroup
+  int pad1;
   // the step's largest |dr|^2 by group of workgroups (workgroup b in group b % kGroups), as the
   // bits of a non-negative double, one word per cache line so that the groups' atomics run apart
   unsigned long long gmax[kGroups * kLineU64];
@@ -102,6 +104,18 @@ __device__ __forceinline__ float dd_of(float4 o, float xi, float yi, float c2) {
   return __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, -c2));
 }
 
+// entry t of slot p in the ELL lists of width K (a multiple of 16): blocks of 256 slots, inside a
+// block chunks of 4 consecutive entries of one slot, slot-major (the chunks of entries 4c..4c+3 of
+// the block's 256 slots side by side).  A builder writes a slot's entries four at a time, the loads
+// of a chunk by a wave's lanes are one contiguous run, and a particle's entries lie 4 KB apart
+// (with one n-long array per entry they would each sit on a page of their own).
+__host__ __device__ __forceinline__ long lslot(long p, long t, int K) {
+  return ((((p >> 8) * (K >> 2) + (t >> 2)) * 256 + (p & 255)) << 2) + (t & 3);
+}
+// the 16-bit copy, tile-local in the same chunked form: chunk c of the tile's lane u at
+// (tile * K / 4 + c) * 256 + u, four 16-bit entries (uint2) each
+__device__ __forceinline__ long l16chunk(long tile, int u, int c, int K) { return (tile * (K >> 2) + c) * 256 + u; }
+
 // ---------------------------------------------------------------------------
 // rebuild (every launch returns at once when the flag is clear)
 
@@ -110,23 +124,26 @@ __global__ __launch_bounds__(kVB) void vr_keys(DevParams P, VArgs A, const doubl
                                                int *__restrict__ key, int *__restrict__ slotc,
                                                int *__restrict__ count) {
   if (gate_off(A)) return;
-  const int p = blockIdx.x * kVB + threadIdx.x;
-  if (p >= A.n) return;
-  double xw, yw;
-  apply_pbc(P, state[(long)p * S], state[(long)p * S + 1], xw, yw);
-  const int c = cell_coord(yw, A.ncv) * A.ncv + cell_coord(xw, A.ncv);
-  key[p] = c;
-  slotc[p] = atomicAdd(count + c, 1);
+  if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl->big_tiles = 0;  // counted again by vt_lists
+  for (int p = blockIdx.x * kVB + threadIdx.x; p < A.n; p += gridDim.x * kVB) {
+    double xw, yw;
+    apply_pbc(P, state[(long)p * S], state[(long)p * S + 1], xw, yw);
+    const int c = cell_coord(yw, A.ncv) * A.ncv + cell_coord(xw, A.ncv);
+    key[p] = c;
+    slotc[p] = atomicAdd(count + c, 1);
+  }
 }
 
 __global__ __launch_bounds__(kVB) void vr_scatter(VArgs A, const int *__restrict__ key, const int *__restrict__ slotc,
                                                   const int *__restrict__ start, int *__restrict__ tmp) {
   if (gate_off(A)) return;
-  const int p = blockIdx.x * kVB + threadIdx.x;
-  if (p < A.n) tmp[start[key[p]] + slotc[p]] = p;
+  for (int p = blockIdx.x * kVB + threadIdx.x; p < A.n; p += gridDim.x * kVB) tmp[start[key[p]] + slotc[p]] = p;
 }
 
-// stable rank inside the cell (by previous slot) and the gather into the new order, in scratch
+// each cell's particles in ascending previous slot, gathered into the new order in scratch: cells
+// of at most 64 particles one wave each (wave_sort64), then the particles of denser cells one
+// thread each, ranked by counting (grid-stride over the particles: a dense cluster is spread over
+// the whole grid)
 template <int S>
 __global__ __launch_bounds__(kVB) void vr_gather(VArgs A, const int *__restrict__ key, const int *__restrict__ start,
                                                  const int *__restrict__ tmp, const double *__restrict__ state,
@@ -134,18 +151,28 @@ __global__ __launch_bounds__(kVB) void vr_gather(VArgs A, const int *__restrict_
                                                  double *__restrict__ sstate, int *__restrict__ sid,
                                                  double *__restrict__ snat, int *__restrict__ scell) {
   if (gate_off(A)) return;
-  const int p = blockIdx.x * kVB + threadIdx.x;
-  if (p >= A.n) return;
-  const int c = key[p];
-  const int b = start[c], e = start[c + 1];
-  int r = 0;
-  for (int t = b; t < e; ++t) r += tmp[t] < p ? 1 : 0;
-  const int q = b + r;
+  const int lane = threadIdx.x & 63;
+  auto move = [&](int p, int q, int c) {
 #pragma unroll
-  for (int k = 0; k < S; ++k) sstate[(long)q * S + k] = state[(long)p * S + k];
-  sid[q] = id[p];
-  if (nat) snat[q] = nat[p];
-  scell[q] = c;
+    for (int k = 0; k < S; ++k) sstate[(long)q * S + k] = state[(long)p * S + k];
+    sid[q] = id[p];
+    if (nat) snat[q] = nat[p];
+    scell[q] = c;
+  };
+  for (int c = blockIdx.x * (kVB / 64) + (threadIdx.x >> 6); c < A.cells; c += gridDim.x * (kVB / 64)) {
+    const int b = start[c], m = start[c + 1] - b;
+    if (m > 64) continue;  // the whole wave
+    const int v = wave_sort64(lane < m ? tmp[b + lane] : 0x7fffffff, lane);
+    if (lane < m) move(v, b + lane, c);
+  }
+  for (int p = blockIdx.x * kVB + threadIdx.x; p < A.n; p += gridDim.x * kVB) {
+    const int c = key[p];
+    const int b = start[c], e = start[c + 1];
+    if (e - b <= 64) continue;
+    int r = 0;
+    for (int t = b; t < e; ++t) r += tmp[t] < p ? 1 : 0;
+    move(p, b + r, c);
+  }
 }
 
 // back into the resident arrays, with the snapshot of the new order
@@ -157,19 +184,19 @@ __global__ __launch_bounds__(kVB) void vr_settle(DevParams P, VArgs A, const dou
                                                  int *__restrict__ cell, float4 *__restrict__ snap,
                                                  float *__restrict__ omega) {
   if (gate_off(A)) return;
-  const int q = blockIdx.x * kVB + threadIdx.x;
-  if (q >= A.n) return;
-  double st[S];
+  for (int q = blockIdx.x * kVB + threadIdx.x; q < A.n; q += gridDim.x * kVB) {
+    double st[S];
 #pragma unroll
-  for (int k = 0; k < S; ++k) {
-    st[k] = sstate[(long)q * S + k];
-    state[(long)q * S + k] = st[k];
+    for (int k = 0; k < S; ++k) {
+      st[k] = sstate[(long)q * S + k];
+      state[(long)q * S + k] = st[k];
+    }
+    id[q] = sid[q];
+    if (nat) nat[q] = snat[q];
+    cell[q] = scell[q];
+    snap[q] = vsnap_of(P, st);
+    omega[q] = S == 4 ? (float)st[3] : 0.f;
   }
-  id[q] = sid[q];
-  if (nat) nat[q] = snat[q];
-  cell[q] = scell[q];
-  snap[q] = vsnap_of(P, st);
-  omega[q] = S == 4 ? (float)st[3] : 0.f;
 }
 
 // The lists: the candidates of the 3 x 3 cells around the particle's cell in the stencil order of
@@ -204,7 +231,7 @@ __device__ __forceinline__ void list_of_slot(const DevParams &P, const VArgs &A,
           in = dx * dx + dy * dy <= A.rmax2;
         }
         if (in) {
-          if (w < A.K) list[(long)w * A.n + q] = u;
+          if (w < A.K) list[lslot(q, w, A.K)] = u;
           ++w;
         }
       }
@@ -243,7 +270,7 @@ __device__ __forceinline__ void walk4(const DevParams &P, const VArgs &A, const 
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const bool in = !TAIL || t + k < end;
-    j[k] = CELLS ? (in ? t + k : p) : (in ? list[(long)(t + k) * A.n + p] : p);
+    j[k] = CELLS ? (in ? t + k : p) : (in ? list[lslot(p, t + k, A.K)] : p);
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -324,7 +351,7 @@ __device__ __forceinline__ void walk_list(const DevParams &P, const VArgs &A, co
       float4 o[8];
       float om[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) j[k] = t0 + h + k < L ? list[(long)(t0 + h + k) * A.n + p] : p;
+      for (int k = 0; k < 8; ++k) j[k] = t0 + h + k < L ? list[lslot(p, t0 + h + k, A.K)] : p;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         o[k] = snap[j[k]];
@@ -503,20 +530,40 @@ __device__ int vt_plan(const VArgs &A, const int *__restrict__ start, int row, i
   return cpos[ncells];
 }
 
-// the staged cells' snapshots (and omega) into LDS, G lanes per cell
+// the staged cells' snapshots (and omega) into LDS, G lanes per cell, each position taken as its
+// periodic image nearest the centre of its staged cell (the cells are those of the last rebuild,
+// which the particles may have left by up to half the skin, across the box's edge too) and re-based
+// to the stage's corner (col0, row0 in cells of side 1/ncv): within a stage the separations are
+// then plain differences, with no nearest-image correction per candidate.  Each staged coordinate
+// takes at most two more FP32 roundings (<= 2^-24 each, coordinates below 1), within what the
+// guard bands allow for.
 template <bool SRK>
-__device__ void vt_stage(int ncells, const float4 *__restrict__ snap, const float *__restrict__ omega,
-                         VtLds<SRK> &L) {
+__device__ void vt_stage(const VArgs &A, int ncols, int col0, int row0, const float4 *__restrict__ snap,
+                         const float *__restrict__ omega, VtLds<SRK> &L) {
+  const int ncells = 3 * ncols;
+  const float h = 1.f / (float)A.ncv;
+  const float x0 = (float)col0 * h, y0 = (float)row0 * h;
   const int G = ncells <= kVB / 8 ? 8 : (ncells <= kVB / 4 ? 4 : (ncells <= kVB / 2 ? 2 : 1));
   const int sub = (int)threadIdx.x % G;
   for (int c = (int)threadIdx.x / G; c < ncells; c += kVB / G) {
+    const int r = c / ncols, q = c - r * ncols;
+    // centre of the staged cell, unwrapped (its column and row may lie one cell past the edges)
+    const float xc = ((float)(col0 + q) + 0.5f) * h, yc = ((float)(row0 + r) + 0.5f) * h;
     const int e0 = L.cpos[c], n0 = L.ccnt[c], g0 = L.cglob[c];
     for (int j = sub; j < n0; j += G) {
-      L.sdat[e0 + j] = snap[g0 + j];
+      const float4 v = snap[g0 + j];
+      const float xu = v.x - __builtin_rintf(v.x - xc), yu = v.y - __builtin_rintf(v.y - yc);
+      L.sdat[e0 + j] = make_float4(xu - x0, yu - y0, v.z, v.w);
       if (SRK) L.somega[e0 + j] = omega[g0 + j];
     }
   }
   __syncthreads();
+}
+
+// d^2 - c2 of two staged entries (plain differences: see vt_stage)
+__device__ __forceinline__ float dd_staged(float4 o, float xi, float yi, float c2) {
+  const float dx = o.x - xi, dy = o.y - yi;
+  return __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, -c2));
 }
 
 // global slot of staged entry t: the last staged cell starting at or before t (an empty cell
@@ -539,74 +586,88 @@ __global__ __launch_bounds__(256) void vr_tiles(VArgs A, const int *__restrict__
 // The lists of a tile's particles, in list_of_slot's order (stencil cells in order, ascending slot
 // inside a cell) and with its decisions, from the staged snapshots; each entry is written twice:
 // as a slot (list, slot-major over all particles) and as its LDS position in the tile's stage
-// (list16, tile-local: entry t of the tile's lane u at (tile * K + t) * kVB + u).
+// (list16, tile-local chunks of four, l16chunk); both written a chunk at a time.
 template <int S>
 __global__ __launch_bounds__(kVB) void vt_lists(DevParams P, VArgs A, const int4 *__restrict__ tiles, int ntiles,
                                                 const int *__restrict__ start, const int *__restrict__ cell,
                                                 const double *__restrict__ state, const float4 *__restrict__ snap,
-                                                int *__restrict__ list, unsigned short *__restrict__ list16,
+                                                int *__restrict__ list, uint2 *__restrict__ list16,
                                                 int *__restrict__ len) {
   if (gate_off(A)) return;
   __shared__ VtLds<false> L;
-  const int tile = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  if (tile >= ntiles) return;
-  const int4 td = tiles[tile];
-  const int row = td.y, ca = td.z, cb = td.w;
-  if (ca >= cb) return;  // an unused tile slot
-  const int ncols = cb - ca + 2, ncells = 3 * ncols;
-  const int total = vt_plan(A, start, row, ca, cb, L.cglob, L.ccnt, L.cpos);
-  const int gtb = L.cglob[ncols + 1];
-  const int ntile = start[row * A.ncv + cb] - gtb;
-  if (total > kVStage) {
-    for (int u = threadIdx.x; u < ntile; u += kVB) list_of_slot<S>(P, A, start, cell, state, snap, list, len, gtb + u);
-    return;
-  }
-  vt_stage<false>(ncells, snap, nullptr, L);
-  const int ltb = L.cpos[ncols + 1];
-  for (int u = threadIdx.x; u < ntile; u += kVB) {
-    const int q = gtb + u, li = ltb + u;
-    // the 16-bit copy, tile-local (entry t of lane u at (tile * K + t) * kVB + u): a pass of the
-    // step reads one contiguous block per entry instead of one page per entry
-    unsigned short *l16 = u < kVB ? list16 + (long)tile * A.K * kVB + u : nullptr;
-    const float4 me = L.sdat[li];
-    double xi, yi;
-    apply_pbc(P, state[(long)q * S], state[(long)q * S + 1], xi, yi);
-    const int qi = 1 + cell[q] % A.ncv - ca;  // staged column of the particle's cell
-    int w = 0;
-    for (int r = 0; r < 3; ++r)
-      for (int dq = -1; dq <= 1; ++dq) {
-        const int e = r * ncols + qi + dq;
-        const int t0 = L.cpos[e], t1 = t0 + L.ccnt[e], g0 = L.cglob[e];
-        for (int t = t0; t < t1; ++t) {
-          if (t == li) continue;
-          const float dd = dd_of(L.sdat[t], me.x, me.y, A.m2c);
-          bool in = dd < 0.f;
-          if (__builtin_fabsf(dd) <= A.m2h) {
-            const long g = g0 + (t - t0);
-            double dx, dy;
-            class_c_unsigned(P, xi, yi, state[g * S], state[g * S + 1], dx, dy);
-            in = dx * dx + dy * dy <= A.rmax2;
-          }
-          if (in) {
-            if (w < A.K) {
-              list[(long)w * A.n + q] = g0 + (t - t0);
-              if (l16) l16[(long)w * kVB] = (unsigned short)t;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {  // (the same for the whole workgroup)
+    const int4 td = tiles[tile];
+    const int row = td.y, ca = td.z, cb = td.w;
+    if (ca >= cb) continue;  // an unused tile slot
+    const int ncols = cb - ca + 2, ncells = 3 * ncols;
+    __syncthreads();  // the previous tile's stage is consumed
+    const int total = vt_plan(A, start, row, ca, cb, L.cglob, L.ccnt, L.cpos);
+    const int gtb = L.cglob[ncols + 1];
+    const int ntile = start[row * A.ncv + cb] - gtb;
+    if (ntile > kVB && threadIdx.x == 0) atomicAdd(&A.ctl->big_tiles, 1);
+    if (total > kVStage) {
+      for (int u = threadIdx.x; u < ntile; u += kVB) list_of_slot<S>(P, A, start, cell, state, snap, list, len, gtb + u);
+      continue;
+    }
+    vt_stage<false>(A, ncols, ca - 1, row - 1, snap, nullptr, L);
+    const int ltb = L.cpos[ncols + 1];
+    for (int u = threadIdx.x; u < ntile; u += kVB) {
+      const int q = gtb + u, li = ltb + u;
+      // entries go out four at a time (a chunk of each list): c32 / c16 collect the current chunk
+      int4 c32 = make_int4(0, 0, 0, 0);
+      unsigned int c16lo = 0u, c16hi = 0u;
+      auto flush = [&](int w) {  // the chunk holding entries 4 * (w >> 2) ..
+        *reinterpret_cast<int4 *>(list + lslot(q, w & ~3, A.K)) = c32;
+        if (u < kVB) list16[l16chunk(tile, u, w >> 2, A.K)] = make_uint2(c16lo, c16hi);
+      };
+      const float4 me = L.sdat[li];
+      double xi, yi;
+      apply_pbc(P, state[(long)q * S], state[(long)q * S + 1], xi, yi);
+      const int qi = 1 + cell[q] % A.ncv - ca;  // staged column of the particle's cell
+      int w = 0;
+      for (int r = 0; r < 3; ++r)
+        for (int dq = -1; dq <= 1; ++dq) {
+          const int e = r * ncols + qi + dq;
+          const int t0 = L.cpos[e], t1 = t0 + L.ccnt[e], g0 = L.cglob[e];
+          for (int t = t0; t < t1; ++t) {
+            if (t == li) continue;
+            const float dd = dd_staged(L.sdat[t], me.x, me.y, A.m2c);
+            bool in = dd < 0.f;
+            if (__builtin_fabsf(dd) <= A.m2h) {
+              const long g = g0 + (t - t0);
+              double dx, dy;
+              class_c_unsigned(P, xi, yi, state[g * S], state[g * S + 1], dx, dy);
+              in = dx * dx + dy * dy <= A.rmax2;
             }
-            ++w;
+            if (in) {
+              if (w < A.K) {
+                const int g = g0 + (t - t0), k = w & 3;
+                c32.x = k == 0 ? g : c32.x;
+                c32.y = k == 1 ? g : c32.y;
+                c32.z = k == 2 ? g : c32.z;
+                c32.w = k == 3 ? g : c32.w;
+                const unsigned int h = (unsigned int)t << ((k & 1) * 16);
+                if (k < 2) c16lo = (k == 0 ? 0u : c16lo) | h;
+                else c16hi = (k == 2 ? 0u : c16hi) | h;
+                if (k == 3) flush(w);
+              }
+              ++w;
+            }
           }
         }
-      }
-    len[q] = w <= A.K ? w : -1;
-    if (w > A.K) atomicAdd(&A.ctl->overflow, 1ull);
+      if ((min(w, A.K) & 3) != 0) flush(min(w, A.K));  // the last, partial chunk
+      len[q] = w <= A.K ? w : -1;
+      if (w > A.K) atomicAdd(&A.ctl->overflow, 1ull);
+    }
   }
 }
 
 // walk_list with the candidates read from the tile's stage (entries past L: the particle itself,
-// li, with weight zero; l16: the lane's tile-local 16-bit list, entry t at l16[t * kVB]): the
-// pass's sixteen list entries are loaded first (global memory), the candidates four at a time from
-// LDS, in walk_list's order and grouping
+// li, with weight zero; l16: the lane's first chunk of the tile-local 16-bit list, chunk c at
+// l16[c * kVB]): the pass's sixteen list entries are loaded first (global memory, four chunks), the
+// candidates four at a time from LDS, in walk_list's order and grouping
 template <int S, bool SRK>
-__device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A, const unsigned short *__restrict__ l16,
+__device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A, const uint2 *__restrict__ l16,
                                               const VtLds<SRK> &Ls, int ncells, const double *__restrict__ state,
                                               int p, int li, int L, float xi, float yi, const double *pi, double &sx,
                                               double &sy, double &tx, double &ty, int &nnb) {
@@ -615,7 +676,12 @@ __device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A
     float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
     int j[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) j[k] = t0 + k < L ? (int)l16[(long)(t0 + k) * kVB] : li;
+    for (int c = 0; c < 4; ++c) {  // four chunks of four 16-bit entries (K is a multiple of 16)
+      const uint2 w = l16[(long)((t0 >> 2) + c) * kVB];
+      const int e[4] = {(int)(w.x & 0xffffu), (int)(w.x >> 16), (int)(w.y & 0xffffu), (int)(w.y >> 16)};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) j[4 * c + k] = t0 + 4 * c + k < L ? e[k] : li;
+    }
 #pragma unroll
     for (int g = 0; g < 16; g += 4) {
       float4 o[4];
@@ -623,7 +689,7 @@ __device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         o[k] = Ls.sdat[j[g + k]];
-        dd[k] = dd_of(o[k], xi, yi, A.r2c);
+        dd[k] = dd_staged(o[k], xi, yi, A.r2c);
         m[k] = __builtin_amdgcn_fmed3f(-A.r2k * dd[k], 0.f, 1.f);
       }
       const float dmin = __builtin_fminf(__builtin_fminf(__builtin_fabsf(dd[0]), __builtin_fabsf(dd[1])),
@@ -668,7 +734,7 @@ __device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A
 template <int S, bool SRK>
 __device__ __forceinline__ double vt_particle(const DevParams &P, const VArgs &A, const StepArgs &U, const Coef &K,
                                               uint64_t step, const int *__restrict__ list,
-                                              const unsigned short *__restrict__ list16, const int *__restrict__ len,
+                                              const uint2 *__restrict__ list16, const int *__restrict__ len,
                                               const int *__restrict__ start, const int *__restrict__ cell,
                                               const double *__restrict__ state, const int *__restrict__ id,
                                               const double *__restrict__ nat, const float4 *__restrict__ snap,
@@ -682,7 +748,8 @@ __device__ __forceinline__ double vt_particle(const DevParams &P, const VArgs &A
   double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
   int nnb = 0;
   if (Lp >= 0 && staged) {
-    walk_list_lds<S, SRK>(P, A, list16, *Ls, ncells, state, p, li, Lp, me.x, me.y, pi, sx, sy, tx, ty, nnb);
+    const float4 ms = Ls->sdat[li];  // the particle in the stage's coordinates
+    walk_list_lds<S, SRK>(P, A, list16, *Ls, ncells, state, p, li, Lp, ms.x, ms.y, pi, sx, sy, tx, ty, nnb);
   } else if (Lp >= 0) {
     walk_list<S, SRK>(P, A, list, snap, omega, state, p, Lp, me.x, me.y, pi, sx, sy, tx, ty, nnb);
   } else {  // (rare) the list outgrew the width: the 3 x 3 cells of the last rebuild hold every candidate
@@ -730,17 +797,18 @@ __global__ __launch_bounds__(kVB) void vt_rest(DevParams P, VArgs A, StepArgs U,
                                                float4 *__restrict__ nsnap, float *__restrict__ nomega,
                                                double *__restrict__ compl_out) {
   __shared__ double wmax[kVB / 64];
-  if ((int)blockIdx.x >= ntiles) return;
-  const int4 td = tiles[blockIdx.x];
-  if (td.z >= td.w) return;
-  const int g0 = start[td.y * A.ncv + td.z], ntile = start[td.y * A.ncv + td.w] - g0;
-  if (ntile <= kVB) return;
+  if (A.ctl->big_tiles == 0) return;  // the usual case: every tile fits its workgroup
   const Coef K = vt_coef(P, U);
   const uint64_t step = U.step_dev ? (uint64_t)*U.step_dev : U.step;
   double d2 = 0.0;
-  for (int u = kVB + threadIdx.x; u < ntile; u += kVB)
-    d2 = fmax(d2, vt_particle<S, SRK>(P, A, U, K, step, list, nullptr, len, start, cell, state, id, nat, snap, omega,
-                                      nstate, nsnap, nomega, compl_out, nullptr, false, 0, g0 + u, 0));
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int4 td = tiles[t];
+    if (td.z >= td.w) continue;
+    const int g0 = start[td.y * A.ncv + td.z], ntile = start[td.y * A.ncv + td.w] - g0;
+    for (int u = kVB + threadIdx.x; u < ntile; u += kVB)
+      d2 = fmax(d2, vt_particle<S, SRK>(P, A, U, K, step, list, nullptr, len, start, cell, state, id, nat, snap,
+                                        omega, nstate, nsnap, nomega, compl_out, nullptr, false, 0, g0 + u, 0));
+  }
   step_max(A, d2, wmax);
 }
 
@@ -749,7 +817,7 @@ __global__ __launch_bounds__(kVB) void vt_rest(DevParams P, VArgs A, StepArgs U,
 template <int S, bool SRK>
 __global__ __launch_bounds__(kVB) void vt_step(DevParams P, VArgs A, StepArgs U, const int4 *__restrict__ tiles,
                                                int ntiles, const int *__restrict__ list,
-                                               const unsigned short *__restrict__ list16, const int *__restrict__ len,
+                                               const uint2 *__restrict__ list16, const int *__restrict__ len,
                                                const int *__restrict__ start, const int *__restrict__ cell,
                                                const double *__restrict__ state, const int *__restrict__ id,
                                                const double *__restrict__ nat, const float4 *__restrict__ snap,
@@ -764,13 +832,13 @@ __global__ __launch_bounds__(kVB) void vt_step(DevParams P, VArgs A, StepArgs U,
     const int row = td.y, ca = td.z, cb = td.w;
     const int ncols = cb - ca + 2, ncells = 3 * ncols;
     const bool staged = vt_plan(A, start, row, ca, cb, Ls.cglob, Ls.ccnt, Ls.cpos) <= kVStage;
-    if (staged) vt_stage<SRK>(ncells, snap, omega, Ls);
+    if (staged) vt_stage<SRK>(A, ncols, ca - 1, row - 1, snap, omega, Ls);
     const int gtb = Ls.cglob[ncols + 1];
     const int ntile = start[row * A.ncv + cb] - gtb;
     const int u = threadIdx.x;
     if (u < ntile)
       d2 = vt_particle<S, SRK>(P, A, U, vt_coef(P, U), U.step_dev ? (uint64_t)*U.step_dev : U.step, list,
-                               list16 + (long)tile * A.K * kVB + u, len,
+                               list16 + l16chunk(tile, u, 0, A.K), len,
                                start, cell, state, id, nat, snap, omega, nstate, nsnap, nomega, compl_out, &Ls, staged,
                                ncells, gtb + u, Ls.cpos[ncols + 1] + u);
   }
@@ -854,7 +922,7 @@ struct VerletFast {
   bool tiled = false;
   int vt_wcap = 0, vt_cmax = 0;
   int4 *vtiles = nullptr;
-  unsigned short *list16 = nullptr;
+  uint2 *list16 = nullptr;  // chunks of four 16-bit entries
   int *key = nullptr, *slotc = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr, *blocksum = nullptr;
   double *sstate = nullptr, *snat = nullptr;
   int *sid = nullptr, *scell = nullptr;
@@ -1036,7 +1104,8 @@ int vf_enter(VerletFast *v, hipStream_t st, const DevParams &P, int srk, const d
   K = (K + 15) / 16 * 16;
   // an upper bound on the width from the environment (memory cap; lists beyond it walk their cells)
   if (const char *cap = std::getenv("SPSS_VERLET_WIDTH_CAP")) K = std::max(4, std::min(K, std::atoi(cap)));
-  const size_t need = (size_t)v->n * (size_t)K;
+  K = (K + 15) / 16 * 16;  // whole chunks of four entries, whole passes of sixteen
+  const size_t need = ((size_t)v->n + 255) / 256 * 256 * (size_t)K;  // lslot's blocks of 256 slots
   const size_t need16 = v->tiled ? (size_t)v->ncv * v->vt_cmax * (size_t)K * kVB : 0;  // tile-local
   if (need > v->list_cap || need16 > v->list16_cap) {
     drop_graphs(v);  // captured launches hold the old list pointer
@@ -1047,7 +1116,7 @@ int vf_enter(VerletFast *v, hipStream_t st, const DevParams &P, int srk, const d
     v->list_cap = 0;
     v->list16_cap = 0;
     VCHK(hipMalloc(&v->list, sizeof(int) * need));
-    if (v->tiled) VCHK(hipMalloc(&v->list16, sizeof(unsigned short) * need16));
+    if (v->tiled) VCHK(hipMalloc(&v->list16, sizeof(unsigned short) * need16));  // need16 entries
     v->list_cap = need;
     v->list16_cap = need16;
   }
@@ -1068,24 +1137,28 @@ static void launch_step(VerletFast *v, hipStream_t st, const DevParams &P, const
   const unsigned nb = vblocks(v->n);
   const VArgs &A = v->A;
   const double *nat = v->resident_nat ? v->vnat : nullptr;
-  vr_keys<S><<<nb, kVB, 0, st>>>(P, A, v->vstate[c], v->key, v->slotc, v->count);
+  // the rebuild launches return at once on most steps: a fixed grid of at most kGateGrid
+  // workgroups (grid-stride loops) keeps what they cost then down to a dispatch
+  const unsigned gg = std::min(nb, (unsigned)kGateGrid);
+  vr_keys<S><<<gg, kVB, 0, st>>>(P, A, v->vstate[c], v->key, v->slotc, v->count);
   scan_local<<<v->nsb, 256, 0, st>>>(v->count, nullptr, v->cells, v->start, v->blocksum, ctl_flag(A.ctl));
   scan_add<<<v->nsb, 256, 0, st>>>(v->count, nullptr, v->cells, v->start, v->blocksum, v->nsb, nullptr,
                                    ctl_flag(A.ctl));
-  vr_scatter<<<nb, kVB, 0, st>>>(A, v->key, v->slotc, v->start, v->tmp);
-  vr_gather<S><<<nb, kVB, 0, st>>>(A, v->key, v->start, v->tmp, v->vstate[c], v->vid, nat, v->sstate, v->sid,
-                                   v->snat, v->scell);
-  vr_settle<S><<<nb, kVB, 0, st>>>(P, A, v->sstate, v->sid, v->snat, v->scell, v->vstate[c], v->vid,
+  vr_scatter<<<gg, kVB, 0, st>>>(A, v->key, v->slotc, v->start, v->tmp);
+  vr_gather<S><<<gg, kVB, 0, st>>>(A, v->key, v->start, v->tmp, v->vstate[c], v->vid, nat, v->sstate, v->sid, v->snat,
+                                   v->scell);
+  vr_settle<S><<<gg, kVB, 0, st>>>(P, A, v->sstate, v->sid, v->snat, v->scell, v->vstate[c], v->vid,
                                    nat ? v->vnat : nullptr, v->vcell, v->vsnap[c], v->vomega[c]);
   if (v->tiled) {
     const int ntiles = v->ncv * v->vt_cmax;
     const unsigned tg = (unsigned)((ntiles + 7) / 8 * 8);  // a multiple of 8 for xcd_block
+    const unsigned gt = std::min(tg, (unsigned)kGateGrid);
     vr_tiles<<<(v->ncv + 3) / 4, 256, 0, st>>>(A, v->start, v->vt_wcap, v->vt_cmax, v->vtiles);
-    vt_lists<S><<<tg, kVB, 0, st>>>(P, A, v->vtiles, ntiles, v->start, v->vcell, v->vstate[c], v->vsnap[c], v->list,
+    vt_lists<S><<<gt, kVB, 0, st>>>(P, A, v->vtiles, ntiles, v->start, v->vcell, v->vstate[c], v->vsnap[c], v->list,
                                     v->list16, v->len);
-    vt_rest<S, SRK><<<ntiles, kVB, 0, st>>>(P, A, U, v->vtiles, ntiles, v->list, v->len, v->start, v->vcell,
-                                            v->vstate[c], v->vid, nat, v->vsnap[c], v->vomega[c], v->vstate[o],
-                                            v->vsnap[o], v->vomega[o], SRK ? compl_out : nullptr);
+    vt_rest<S, SRK><<<gt, kVB, 0, st>>>(P, A, U, v->vtiles, ntiles, v->list, v->len, v->start, v->vcell, v->vstate[c],
+                                        v->vid, nat, v->vsnap[c], v->vomega[c], v->vstate[o], v->vsnap[o],
+                                        v->vomega[o], SRK ? compl_out : nullptr);
     vt_step<S, SRK><<<tg, kVB, 0, st>>>(P, A, U, v->vtiles, ntiles, v->list, v->list16, v->len, v->start, v->vcell,
                                         v->vstate[c], v->vid, nat, v->vsnap[c], v->vomega[c], v->vstate[o],
                                         v->vsnap[o], v->vomega[o], SRK ? compl_out : nullptr);
@@ -1281,7 +1354,7 @@ int vf_lists(VerletFast *v, hipStream_t st, std::vector<int> *counts, std::vecto
   counts->assign(n, 0);
   flat->clear();
   if (!v->resident) return SPSS_OK;
-  std::vector<int> id(n), len(n), list((size_t)n * v->K);
+  std::vector<int> id(n), len(n), list(((size_t)n + 255) / 256 * 256 * (size_t)v->K);
   VCHK(hipMemcpyAsync(id.data(), v->vid, sizeof(int) * n, hipMemcpyDeviceToHost, st));
   VCHK(hipMemcpyAsync(len.data(), v->len, sizeof(int) * n, hipMemcpyDeviceToHost, st));
   VCHK(hipMemcpyAsync(list.data(), v->list, sizeof(int) * list.size(), hipMemcpyDeviceToHost, st));
@@ -1295,7 +1368,7 @@ int vf_lists(VerletFast *v, hipStream_t st, std::vector<int> *counts, std::vecto
   for (int i = 0; i < n; ++i) {
     const int p = slot_of[i];
     (*counts)[i] = len[p];
-    for (int t = 0; t < len[p]; ++t) flat->push_back(id[list[(size_t)t * n + p]]);
+    for (int t = 0; t < len[p]; ++t) flat->push_back(id[list[lslot(p, t, v->K)]]);
   }
   return SPSS_OK;
 }
