@@ -45,13 +45,13 @@ def pair_kernel_bytes(n, cells):
 
 
 # HBM bytes per launch of that kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled
-# by the copy-kernel calibration, by cell subdivision (profiles/fk_pairs_euler_N1M_hbm_traffic.txt)
-PAIR_KERNEL_TRAFFIC = {1: 3.9915e7}
-# HBM bytes of the three kernels of one resident step, the same way (filled in from the profiles)
-STEP_TRAFFIC = {}
+# by the copy-kernel calibration, by cell subdivision (profiles/fr_pairs_euler_N1M_hbm_traffic.txt)
+PAIR_KERNEL_TRAFFIC = {1: 4.4389e7}
+# HBM bytes of the three kernels of one resident step (fr_pairs, fr_far, fr_scatter), the same way
+STEP_TRAFFIC = {1: 1.9573e8}
 # VALU instructions issued per CU-cycle by that kernel (SQ_INSTS_VALU / (256 CUs x cycles),
-# profiles/fk_pairs_euler_N1M_valu.txt): at or above the nominal one per CU-cycle, i.e. VALU-bound
-PAIR_KERNEL_VALU_PER_CU_CYCLE = 1.01
+# profiles/fr_pairs_euler_N1M_valu.txt): at the nominal one per CU-cycle, i.e. VALU-bound
+PAIR_KERNEL_VALU_PER_CU_CYCLE = 1.00
 # algorithmic HBM bytes per particle-step of the cell-list step (s = 4 state, one RHS): FP64 state
 # read + write 16*s, FP32 snapshot x, y, phi written and read 8*3, cell key and sorted index written
 # and read 16, cell start/count table 8 per cell -- the compulsory traffic of the reference
@@ -63,8 +63,8 @@ def step_bytes(n, cells, s=4):
 # bytes the implementation's resident step moves per particle (its own design, for comparison):
 # pair kernel 36 (snapshot read, sums and count written), scatter 132 (move word, id, FP64 state,
 # natural frequency, sums read; state, id, natural frequency, snapshot, next move word written),
-# and per cell 68 (scan: counts read, local start written; far pass: local start read, start
-# written, counts and the 9 move counters cleared)
+# and per cell 68 (the arrivals scan: arrival counts read and cleared, starts written, the 9 move
+# counters of the next step cleared; the scatter: starts and move counts read)
 IMPL_STEP_BYTES = 36.0 + 132.0 + 68.0 * 62500.0 / 1048576.0
 
 
@@ -493,13 +493,13 @@ def main():
         "timed_steps": {"graph_replayed": replayed, "eager": eager, "graphs_captured": captured},
         "roofline": {"bound": "hbm", "achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": step_achieved / HBM_PEAK, "traffic": STEP_TRAFFIC.get(subdiv),
-                     "kernel": "resident cell-list step (fk_pairs, fr_arrivals, fr_scatter)",
+                     "kernel": "resident cell-list step (fr_pairs, fr_far, fr_scatter)",
                      "algorithmic_bytes_per_particle_step": step_bytes(n, 250 * 250),
                      "pair_kernel": {"achieved": pair_achieved / 1e9, "frac": pair_achieved / HBM_PEAK,
                                      "us": pair_ms * 1e3, "bytes_per_particle": pair_kernel_bytes(n, cells),
                                      "traffic": PAIR_KERNEL_TRAFFIC.get(subdiv)}},
         # the pair kernel is bound by VALU issue, not HBM (no MFMA work on this path)
-        "valu": {"kernel": "fk_pairs", "valu_instr_per_cu_cycle": PAIR_KERNEL_VALU_PER_CU_CYCLE},
+        "valu": {"kernel": "fr_pairs", "valu_instr_per_cu_cycle": PAIR_KERNEL_VALU_PER_CU_CYCLE},
         "cpu_baseline": cpu,
         "pair_kernel_us": pair_ms * 1e3,
         "hbm_copy_gbps": copy_gbps,
