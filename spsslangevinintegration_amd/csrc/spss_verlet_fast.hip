@@ -485,13 +485,14 @@ __global__ __launch_bounds__(kVB) void vs_step(DevParams P, VArgs A, StepArgs U,
 // same order, so the sums equal the global walk's bit for bit.  A tile whose stencil cells
 // overflow the stage builds and walks its lists through global memory as before.
 
-constexpr int kVStage = 1024;  // staged snapshots per tile
+constexpr int kVStage = 1024;     // staged snapshots per tile
+constexpr int kVFar = kVStage;    // one more stage slot, far from everything: the lists' padding
 constexpr int kVCols = 48;     // staged columns W + 2
 
 template <bool SRK>
 struct VtLds {
-  float4 sdat[kVStage];
-  float somega[SRK ? kVStage : 1];
+  float4 sdat[kVStage + 1];  // [kVFar]: x, y far away, phasor 0
+  float somega[SRK ? kVStage + 1 : 1];
   int cglob[3 * kVCols];     // first slot of staged cell (r, q), at r * (W + 2) + q
   int ccnt[3 * kVCols];      // its particle count
   int cpos[3 * kVCols + 1];  // its LDS start; [cells] = the staged total
@@ -556,6 +557,10 @@ __device__ void vt_stage(const VArgs &A, int ncols, int col0, int row0, const fl
       L.sdat[e0 + j] = make_float4(xu - x0, yu - y0, v.z, v.w);
       if (SRK) L.somega[e0 + j] = omega[g0 + j];
     }
+  }
+  if (threadIdx.x == 0) {  // d^2 of the far slot is ~1e36: out of range, never in a guard band
+    L.sdat[kVFar] = make_float4(1e18f, 1e18f, 0.f, 0.f);
+    if (SRK) L.somega[kVFar] = 0.f;
   }
   __syncthreads();
 }
@@ -655,17 +660,26 @@ __global__ __launch_bounds__(kVB) void vt_lists(DevParams P, VArgs A, const int4
             }
           }
         }
-      if ((min(w, A.K) & 3) != 0) flush(min(w, A.K));  // the last, partial chunk
+      if (w < A.K) {  // the 16-bit list padded with the far slot to a whole pass of 16 (K is a multiple of 16)
+        for (int t = w; t < ((w + 15) & ~15); ++t) {
+          const int k = t & 3;
+          const unsigned int h = (unsigned int)kVFar << ((k & 1) * 16);
+          if (k < 2) c16lo = (k == 0 ? 0u : c16lo) | h;
+          else c16hi = (k == 2 ? 0u : c16hi) | h;
+          if (k == 3) flush(t);
+        }
+      }
       len[q] = w <= A.K ? w : -1;
       if (w > A.K) atomicAdd(&A.ctl->overflow, 1ull);
     }
   }
 }
 
-// walk_list with the candidates read from the tile's stage (entries past L: the particle itself,
-// li, with weight zero; l16: the lane's first chunk of the tile-local 16-bit list, chunk c at
-// l16[c * kVB]): the pass's sixteen list entries are loaded first (global memory, four chunks), the
-// candidates four at a time from LDS, in walk_list's order and grouping
+// walk_list with the candidates read from the tile's stage (l16: the lane's first chunk of the
+// tile-local 16-bit list, chunk c at l16[c * kVB]; the entries past L up to the pass's end are the
+// far slot, weight 0, so no entry needs a mask): the pass's sixteen list entries are loaded first
+// (global memory, four chunks), the candidates four at a time from LDS, in walk_list's order and
+// grouping
 template <int S, bool SRK>
 __device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A, const uint2 *__restrict__ l16,
                                               const VtLds<SRK> &Ls, int ncells, const double *__restrict__ state,
@@ -680,7 +694,7 @@ __device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A
       const uint2 w = l16[(long)((t0 >> 2) + c) * kVB];
       const int e[4] = {(int)(w.x & 0xffffu), (int)(w.x >> 16), (int)(w.y & 0xffffu), (int)(w.y >> 16)};
 #pragma unroll
-      for (int k = 0; k < 4; ++k) j[4 * c + k] = t0 + 4 * c + k < L ? e[k] : li;
+      for (int k = 0; k < 4; ++k) j[4 * c + k] = e[k];  // past L: the far slot (weight 0)
     }
 #pragma unroll
     for (int g = 0; g < 16; g += 4) {
@@ -704,8 +718,6 @@ __device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A
             m[k] = dx * dx + dy * dy <= P.rho2 ? 1.f : 0.f;
           }
       }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) m[k] = t0 + g + k < L ? m[k] : 0.f;
       an += (m[0] + m[1]) + (m[2] + m[3]);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
