@@ -685,48 +685,70 @@ __device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A
                                               const VtLds<SRK> &Ls, int ncells, const double *__restrict__ state,
                                               int p, int li, int L, float xi, float yi, const double *pi, double &sx,
                                               double &sy, double &tx, double &ty, int &nnb) {
+  uint2 nxt[4];  // the next pass's four chunks, loaded while the current pass computes
+  if (L > 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) nxt[c] = l16[(long)c * kVB];
+  }
   for (int t0 = 0; t0 < L; t0 += 16) {
     float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an = 0.f;
     float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
+    uint2 cur[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) cur[c] = nxt[c];
+    if (t0 + 16 < L) {  // within the list's whole passes (at most K entries)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) nxt[c] = l16[(long)(((t0 + 16) >> 2) + c) * kVB];
+    }
     int j[16];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {  // four chunks of four 16-bit entries (K is a multiple of 16)
-      const uint2 w = l16[(long)((t0 >> 2) + c) * kVB];
+      const uint2 w = cur[c];
       const int e[4] = {(int)(w.x & 0xffffu), (int)(w.x >> 16), (int)(w.y & 0xffffu), (int)(w.y >> 16)};
 #pragma unroll
       for (int k = 0; k < 4; ++k) j[4 * c + k] = e[k];  // past L: the far slot (weight 0)
     }
 #pragma unroll
-    for (int g = 0; g < 16; g += 4) {
-      float4 o[4];
-      float dd[4], m[4];
+    for (int g8 = 0; g8 < 16; g8 += 8) {  // two groups' candidates read from LDS together
+      float4 o8[8];
+      float om8[8];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        o[k] = Ls.sdat[j[g + k]];
-        dd[k] = dd_staged(o[k], xi, yi, A.r2c);
-        m[k] = __builtin_amdgcn_fmed3f(-A.r2k * dd[k], 0.f, 1.f);
+      for (int k = 0; k < 8; ++k) {
+        o8[k] = Ls.sdat[j[g8 + k]];
+        om8[k] = SRK ? Ls.somega[j[g8 + k]] : 0.f;
       }
-      const float dmin = __builtin_fminf(__builtin_fminf(__builtin_fabsf(dd[0]), __builtin_fabsf(dd[1])),
-                                         __builtin_fminf(__builtin_fabsf(dd[2]), __builtin_fabsf(dd[3])));
-      if (dmin <= A.r2h) {  // rare: the reference's FP64 test, as in walk4
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (__builtin_fabsf(dd[k]) <= A.r2h) {
-            const long g2 = vt_slot(Ls.cglob, Ls.cpos, ncells, j[g + k]);
-            double dx, dy;
-            class_c_unsigned(P, pi[0], pi[1], state[g2 * S], state[g2 * S + 1], dx, dy);
-            m[k] = dx * dx + dy * dy <= P.rho2 ? 1.f : 0.f;
+      for (int h = 0; h < 8; h += 4) {
+        const int g = g8 + h;
+        const float4 *o = o8 + h;
+        float dd[4], m[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          dd[k] = dd_staged(o[k], xi, yi, A.r2c);
+          m[k] = __builtin_amdgcn_fmed3f(-A.r2k * dd[k], 0.f, 1.f);
+        }
+        const float dmin = __builtin_fminf(__builtin_fminf(__builtin_fabsf(dd[0]), __builtin_fabsf(dd[1])),
+                                           __builtin_fminf(__builtin_fabsf(dd[2]), __builtin_fabsf(dd[3])));
+        if (dmin <= A.r2h) {  // rare: the reference's FP64 test, as in walk4
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (__builtin_fabsf(dd[k]) <= A.r2h) {
+              const long g2 = vt_slot(Ls.cglob, Ls.cpos, ncells, j[g + k]);
+              double dx, dy;
+              class_c_unsigned(P, pi[0], pi[1], state[g2 * S], state[g2 * S + 1], dx, dy);
+              m[k] = dx * dx + dy * dy <= P.rho2 ? 1.f : 0.f;
+            }
+        }
+        an += (m[0] + m[1]) + (m[2] + m[3]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          ax[k] = __builtin_fmaf(m[k], o[k].z, ax[k]);
+          ay[k] = __builtin_fmaf(m[k], o[k].w, ay[k]);
+          if (SRK) {
+            const float mo = m[k] * om8[h + k];
+            bx[k] = __builtin_fmaf(mo, o[k].z, bx[k]);
+            by[k] = __builtin_fmaf(mo, o[k].w, by[k]);
           }
-      }
-      an += (m[0] + m[1]) + (m[2] + m[3]);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        ax[k] = __builtin_fmaf(m[k], o[k].z, ax[k]);
-        ay[k] = __builtin_fmaf(m[k], o[k].w, ay[k]);
-        if (SRK) {
-          const float mo = m[k] * Ls.somega[j[g + k]];
-          bx[k] = __builtin_fmaf(mo, o[k].z, bx[k]);
-          by[k] = __builtin_fmaf(mo, o[k].w, by[k]);
         }
       }
     }
