@@ -125,12 +125,32 @@ __global__ __launch_bounds__(kVB) void vr_keys(DevParams P, VArgs A, const doubl
                                                int *__restrict__ count) {
   if (gate_off(A)) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl->big_tiles = 0;  // counted again by vt_lists
-  for (int p = blockIdx.x * kVB + threadIdx.x; p < A.n; p += gridDim.x * kVB) {
-    double xw, yw;
-    apply_pbc(P, state[(long)p * S], state[(long)p * S + 1], xw, yw);
-    const int c = cell_coord(yw, A.ncv) * A.ncv + cell_coord(xw, A.ncv);
-    key[p] = c;
-    slotc[p] = atomicAdd(count + c, 1);
+  // the particles are in the last rebuild's cell order and have moved less than half the skin
+  // since, so a wave holds a few runs of equal keys: one atomic per run, the run's lanes take
+  // consecutive places after it (the order inside a cell is settled by vr_gather)
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int nwave = (A.n + 63) & ~63;  // every lane of a wave takes the same trips
+  for (int p = blockIdx.x * kVB + threadIdx.x; p < nwave; p += gridDim.x * kVB) {
+    const bool valid = p < A.n;
+    int c = -1;
+    if (valid) {
+      double xw, yw;
+      apply_pbc(P, state[(long)p * S], state[(long)p * S + 1], xw, yw);
+      c = cell_coord(yw, A.ncv) * A.ncv + cell_coord(xw, A.ncv);
+      key[p] = c;
+    }
+    const int pc = __shfl_up(c, 1, 64);
+    const unsigned long long heads = __ballot(lane == 0 || pc != c);
+    const int seg = 63 - __clzll((long long)(heads & (below | (1ull << lane))));  // this run's first lane
+    int base = 0;
+    if (valid && seg == lane) {
+      const unsigned long long later = lane == 63 ? 0ull : heads >> (lane + 1);
+      const int len = later ? __ffsll((long long)later) : 64 - lane;
+      base = atomicAdd(count + c, len);
+    }
+    base = __shfl(base, seg, 64);
+    if (valid) slotc[p] = base + (lane - seg);
   }
 }
 
@@ -182,8 +202,12 @@ __global__ __launch_bounds__(kVB) void vr_settle(DevParams P, VArgs A, const dou
                                                  const int *__restrict__ scell, double *__restrict__ state,
                                                  int *__restrict__ id, double *__restrict__ nat,
                                                  int *__restrict__ cell, float4 *__restrict__ snap,
-                                                 float *__restrict__ omega) {
+                                                 float *__restrict__ omega, const int *__restrict__ start,
+                                                 int4 *__restrict__ tiles, int wcap, int cmax) {
   if (gate_off(A)) return;
+  // the row tiles of the new order (tiled grids), one wave per row in the first workgroups: they
+  // need only the final cell starts
+  if (tiles && (int)blockIdx.x * 4 < A.ncv) rows_to_tiles([&](int c) { return start[c]; }, A.ncv, wcap, cmax, tiles);
   for (int q = blockIdx.x * kVB + threadIdx.x; q < A.n; q += gridDim.x * kVB) {
     double st[S];
 #pragma unroll
@@ -582,12 +606,6 @@ __device__ __forceinline__ int vt_slot(const int *cglob, const int *cpos, int nc
   return cglob[lo] + (t - cpos[lo]);
 }
 
-__global__ __launch_bounds__(256) void vr_tiles(VArgs A, const int *__restrict__ start, int wcap, int cmax,
-                                                int4 *__restrict__ tiles) {
-  if (gate_off(A)) return;
-  rows_to_tiles([&](int c) { return start[c]; }, A.ncv, wcap, cmax, tiles);
-}
-
 // The lists of a tile's particles, in list_of_slot's order (stencil cells in order, ascending slot
 // inside a cell) and with its decisions, from the staged snapshots; each entry is written twice:
 // as a slot (list, slot-major over all particles) and as its LDS position in the tile's stage
@@ -820,7 +838,8 @@ __device__ __forceinline__ Coef vt_coef(const DevParams &P, const StepArgs &U) {
 
 // The particles of an oversized tile past the first kVB (a single cell holding more than a tile's
 // particles: dense clusters), through global memory; launched before vt_step; their displacements
-// join the group maxima too.
+// join the group maxima too.  (A launch of its own: as extra workgroups of vt_step, its particle
+// loop would raise vt_step's registers.)
 template <int S, bool SRK>
 __global__ __launch_bounds__(kVB) void vt_rest(DevParams P, VArgs A, StepArgs U, const int4 *__restrict__ tiles,
                                                int ntiles, const int *__restrict__ list, const int *__restrict__ len,
@@ -1181,13 +1200,16 @@ static void launch_step(VerletFast *v, hipStream_t st, const DevParams &P, const
   vr_scatter<<<gg, kVB, 0, st>>>(A, v->key, v->slotc, v->start, v->tmp);
   vr_gather<S><<<gg, kVB, 0, st>>>(A, v->key, v->start, v->tmp, v->vstate[c], v->vid, nat, v->sstate, v->sid, v->snat,
                                    v->scell);
-  vr_settle<S><<<gg, kVB, 0, st>>>(P, A, v->sstate, v->sid, v->snat, v->scell, v->vstate[c], v->vid,
-                                   nat ? v->vnat : nullptr, v->vcell, v->vsnap[c], v->vomega[c]);
+  // tiled grids: the row tiles of the new order are built by vr_settle's first workgroups, so its
+  // grid covers a wave per row as well
+  const unsigned sg = v->tiled ? std::max(gg, (unsigned)((v->ncv + 3) / 4)) : gg;
+  vr_settle<S><<<sg, kVB, 0, st>>>(P, A, v->sstate, v->sid, v->snat, v->scell, v->vstate[c], v->vid,
+                                   nat ? v->vnat : nullptr, v->vcell, v->vsnap[c], v->vomega[c], v->start,
+                                   v->tiled ? v->vtiles : nullptr, v->vt_wcap, v->vt_cmax);
   if (v->tiled) {
     const int ntiles = v->ncv * v->vt_cmax;
     const unsigned tg = (unsigned)((ntiles + 7) / 8 * 8);  // a multiple of 8 for xcd_block
     const unsigned gt = std::min(tg, (unsigned)kGateGrid);
-    vr_tiles<<<(v->ncv + 3) / 4, 256, 0, st>>>(A, v->start, v->vt_wcap, v->vt_cmax, v->vtiles);
     vt_lists<S><<<gt, kVB, 0, st>>>(P, A, v->vtiles, ntiles, v->start, v->vcell, v->vstate[c], v->vsnap[c], v->list,
                                     v->list16, v->len);
     vt_rest<S, SRK><<<gt, kVB, 0, st>>>(P, A, U, v->vtiles, ntiles, v->list, v->len, v->start, v->vcell, v->vstate[c],
