@@ -702,12 +702,10 @@ template <int S, bool SRK>
 __device__ __forceinline__ void walk_list_lds(const DevParams &P, const VArgs &A, const uint2 *__restrict__ l16,
                                               const VtLds<SRK> &Ls, int ncells, const double *__restrict__ state,
                                               int p, int li, int L, float xi, float yi, const double *pi, double &sx,
-                                              double &sy, double &tx, double &ty, int &nnb) {
+                                              double &sy, double &tx, double &ty, int &nnb, const uint2 *first) {
   uint2 nxt[4];  // the next pass's four chunks, loaded while the current pass computes
-  if (L > 0) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) nxt[c] = l16[(long)c * kVB];
-  }
+  for (int c = 0; c < 4; ++c) nxt[c] = first[c];  // the first pass's, loaded with the list's length
   for (int t0 = 0; t0 < L; t0 += 16) {
     float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an = 0.f;
     float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
@@ -795,13 +793,18 @@ __device__ __forceinline__ double vt_particle(const DevParams &P, const VArgs &A
                                               double *__restrict__ compl_out, const VtLds<SRK> *Ls, bool staged,
                                               int ncells, int p, int li) {
   const double *pi = state + (long)p * S;
+  uint2 first[4];  // the first pass of the 16-bit list (allocated for every lane of a tile)
+  if (staged) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) first[c] = list16[(long)c * kVB];
+  }
   const float4 me = snap[p];
   const int Lp = len[p];
   double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
   int nnb = 0;
   if (Lp >= 0 && staged) {
     const float4 ms = Ls->sdat[li];  // the particle in the stage's coordinates
-    walk_list_lds<S, SRK>(P, A, list16, *Ls, ncells, state, p, li, Lp, ms.x, ms.y, pi, sx, sy, tx, ty, nnb);
+    walk_list_lds<S, SRK>(P, A, list16, *Ls, ncells, state, p, li, Lp, ms.x, ms.y, pi, sx, sy, tx, ty, nnb, first);
   } else if (Lp >= 0) {
     walk_list<S, SRK>(P, A, list, snap, omega, state, p, Lp, me.x, me.y, pi, sx, sy, tx, ty, nnb);
   } else {  // (rare) the list outgrew the width: the 3 x 3 cells of the last rebuild hold every candidate
