@@ -48,7 +48,7 @@ def pair_kernel_bytes(n, cells):
 # by the copy-kernel calibration, by cell subdivision (profiles/fr_pairs_euler_N1M_hbm_traffic.txt)
 PAIR_KERNEL_TRAFFIC = {1: 4.4389e7}
 # HBM bytes of the three kernels of one resident step (fr_pairs, fr_far, fr_scatter), the same way
-STEP_TRAFFIC = {1: 1.9573e8}
+STEP_TRAFFIC = {1: 1.9577e8}
 # VALU instructions issued per CU-cycle by that kernel (SQ_INSTS_VALU / (256 CUs x cycles),
 # profiles/fr_pairs_euler_N1M_valu.txt): at the nominal one per CU-cycle, i.e. VALU-bound
 PAIR_KERNEL_VALU_PER_CU_CYCLE = 1.00
