@@ -22,8 +22,10 @@
 // cell keys and histogram, the two-level scan, bucket fill, stable rank and gather into scratch,
 // back into the resident arrays with the new snapshot, the row tiles, the lists -- and the step:
 // one workgroup per row tile stages the snapshots (and omega) of the 3 x (W + 2) cells around its
-// W cells into LDS; each lane walks its list from LDS, sixteen entries per pass: FP32
-// minimum-image distance (x_j - x_i minus its nearest integer), a guard band around rho^2 decided
+// W cells into LDS; each lane walks its list from LDS, sixteen entries per pass (the next pass's
+// entries loaded meanwhile): FP32 distance (the stage holds each position as its periodic image
+// nearest its staged cell, so separations are plain differences; the walks through global memory
+// take x_j - x_i minus its nearest integer), a guard band around rho^2 decided
 // with the reference's FP64 ClassC formula on the stored positions (so neighbour sets are exact),
 // phasor sums in four FP32 partials flushed into FP64 every 16 entries; then the Euler/SRK update
 // of step_particle, the new snapshot and the step's largest squared displacement (maxima by
