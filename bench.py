@@ -143,40 +143,59 @@ def ensemble_config(R=512, seed=0x5EED0004):
     return groups
 
 
+def split_by_size(groups, n_split):
+    """Each model's replicas as two batches, N_r < n_split and N_r >= n_split (n_split 0: one batch
+    per model), so that each batch's cell size is tuned for the densities it holds"""
+    out = {}
+    for model, g in groups.items():
+        for dense in ((False, True) if n_split > 0 else (None,)):
+            k = np.arange(len(g["ns"])) if dense is None else np.nonzero((g["ns"] >= n_split) == dense)[0]
+            if k.size == 0:
+                continue
+            key = f"model{model}" + ("" if dense is None else ("_large" if dense else "_small"))
+            out[key] = dict(model=model, idx=g["idx"][k], ns=g["ns"][k], xi=g["xi"][k], sigma=g["sigma"][k],
+                            D=g["D"][k], seeds=g["seeds"][k], states=[g["states"][i] for i in k],
+                            nats=[g["nats"][i] for i in k])
+    return out
+
+
 def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0.03, every=10, graph=True,
-                     tune_it=True):
+                     tune_it=True, n_split=24576):
     """512-replica ensemble: batched run (autotuned cell size and tiles, graph replays) against the
     plain loop (the same kernels at the same cell size through one single-system handle per
     replica, one after another).  graph=False, tune_it=False: eager launches, default tiles (a
-    kernel trace of the batch's kernels)."""
-    groups = ensemble_config()
+    kernel trace of the batch's kernels).  n_split > 0: each model's replicas in two batches by
+    size (split_by_size)."""
+    groups = split_by_size(ensemble_config(), n_split)
     total_n = sum(int(g["ns"].sum()) for g in groups.values())
     batches, tune = {}, {}
-    for model, g in groups.items():
+    for key, g in groups.items():
+        model = g["model"]
         b = spss.ReplicaBatch(model, g["ns"], g["xi"], g["sigma"], g["D"], g["seeds"], v0=v0, rho=rho)
         b.set_natural_frequencies(np.concatenate(g["nats"]))
         b.set_state(np.concatenate(g["states"]))
         if tune_it:
             ctable, k = b.autotune_cells(dt, probe_steps=10)
             table, chosen = b.autotune(dt, probe_steps=10)
-            tune[model] = {"cell_subdivision_ms_per_step": [round(float(t), 4) for t in ctable],
-                           "cell_subdivision": k, "ms_per_step": [round(float(t), 4) for t in table],
-                           "chosen": int(chosen)}
-        batches[model] = b
+            tune[key] = {"cell_subdivision_ms_per_step": [round(float(t), 4) for t in ctable],
+                         "cell_subdivision": k, "ms_per_step": [round(float(t), 4) for t in table],
+                         "chosen": int(chosen)}
+        batches[key] = b
     # sample for the plain-loop comparison: plain_steps steps of every replica, both ways
     t0 = time.perf_counter()
     for b in batches.values():
         b.run(plain_steps, dt, observe_every=every, observe_from=0, graph=graph)
     t_batch_sample = time.perf_counter() - t0
-    zb = {m: b.order_stats() for m, b in batches.items()}
+    zb = {key: b.order_stats() for key, b in batches.items()}
     t_plain, max_err = 0.0, 0.0
-    for model, g in groups.items():
+    for key, g in groups.items():
         if plain_steps <= 0:
             break
+        model = g["model"]
         for k in range(len(g["ns"])):
             h = spss.ParticleSystem(model=model, n=int(g["ns"][k]), v0=v0, xi=g["xi"][k], sigma=g["sigma"][k],
                                     rho=rho, D_phi=g["D"][k], natfreq_local=1, precision=spss.PRECISION_FP32C,
-                                    seed=int(g["seeds"][k]), cell_subdivision=batches[model].cell_subdivision)
+                                    seed=int(g["seeds"][k]), cell_subdivision=batches[key].cell_subdivision)
             h.set_natural_frequencies(g["nats"][k])
             h.set_state(g["states"][k])
             h.synchronize()
@@ -187,22 +206,22 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
                 rs.append(np.hypot(*h.order_parameter(2)))
             h.synchronize()
             t_plain += time.perf_counter() - t0
-            max_err = max(max_err, abs(float(np.mean(rs)) - float(zb[model][0][k])))
+            max_err = max(max_err, abs(float(np.mean(rs)) - float(zb[key][0][k])))
             h.close()
     # the full batched run: time averages over the second half
-    for model, b in batches.items():
+    for key, b in batches.items():
         b.reset(0)
-        b.set_state(np.concatenate(groups[model]["states"]))
+        b.set_state(np.concatenate(groups[key]["states"]))
     t0 = time.perf_counter()
     for b in batches.values():
         b.run(steps, dt, observe_every=every, observe_from=steps // 2, graph=graph)
     t_batch = time.perf_counter() - t0
     R_mean = np.zeros(sum(len(g["ns"]) for g in groups.values()))
     R_var = np.zeros_like(R_mean)
-    for model, b in batches.items():
+    for key, b in batches.items():
         mean, var, cnt, z, step = b.order_stats()
-        R_mean[groups[model]["idx"]] = mean
-        R_var[groups[model]["idx"]] = var
+        R_mean[groups[key]["idx"]] = mean
+        R_var[groups[key]["idx"]] = var
         b.close()
     return {
         "replicas": int(R_mean.size), "particles": total_n, "steps": steps, "dt": dt,
@@ -212,6 +231,7 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
         "sample_plain_particle_steps_per_s": total_n * plain_steps / t_plain if t_plain else None,
         "speedup_vs_plain_loop": t_plain / t_batch_sample if t_plain else None,
         "graph_replays": graph,
+        "batches": {key: {"replicas": int(len(g["ns"])), "particles": int(g["ns"].sum())} for key, g in groups.items()},
         "order_param_max_abs_err_vs_plain": max_err,
         "mean_R_over_replicas": float(R_mean.mean()),
         "autotune": tune,
@@ -401,6 +421,9 @@ def main():
     ap.add_argument("--only-batch", action="store_true",
                     help="run the batched ensemble alone (with --batch-eager: eager launches, default tiles)")
     ap.add_argument("--batch-eager", action="store_true")
+    ap.add_argument("--batch-split", type=int, default=24576,
+                    help="replicas of at least this many particles in batches of their own, whose autotuned cell "
+                         "size can differ (0: one batch per model)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the sharding by slabs of cell rows is "
@@ -412,7 +435,7 @@ def main():
 
     if args.only_batch:
         res = batched_ensemble(spss, steps=args.batch_steps, plain_steps=args.plain_steps,
-                               graph=not args.batch_eager, tune_it=not args.batch_eager)
+                               graph=not args.batch_eager, tune_it=not args.batch_eager, n_split=args.batch_split)
         print(json.dumps({"batched": res}), flush=True)
         return
 
@@ -455,7 +478,8 @@ def main():
     batched = None
     if not args.no_batch:
         print("[bench] batched ensemble", file=sys.stderr, flush=True)
-        batched = batched_ensemble(spss, steps=args.batch_steps, plain_steps=args.plain_steps)
+        batched = batched_ensemble(spss, steps=args.batch_steps, plain_steps=args.plain_steps,
+                                   n_split=args.batch_split)
         print(f"[bench] batched {batched['speedup_vs_plain_loop']:.1f}x plain loop", file=sys.stderr, flush=True)
     screen = None
     if not args.no_screen:
