@@ -159,8 +159,23 @@ def split_by_size(groups, n_split):
     return out
 
 
+def run_batches(batches, nsteps, dt, every, observe_from, graph, concurrent):
+    """every batch advanced nsteps; concurrent: each from a host thread of its own, so the batches'
+    streams run side by side on the GPU (the library's error state is per thread, each batch has its
+    own stream)"""
+    if not concurrent or len(batches) < 2:
+        for b in batches.values():
+            b.run(nsteps, dt, observe_every=every, observe_from=observe_from, graph=graph)
+        return
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=len(batches)) as ex:
+        futs = [ex.submit(b.run, nsteps, dt, every, observe_from, graph) for b in batches.values()]
+        for f in futs:
+            f.result()
+
+
 def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0.03, every=10, graph=True,
-                     tune_it=True, n_split=24576):
+                     tune_it=True, n_split=24576, concurrent=True):
     """512-replica ensemble: batched run (autotuned cell size and tiles, graph replays) against the
     plain loop (the same kernels at the same cell size through one single-system handle per
     replica, one after another).  graph=False, tune_it=False: eager launches, default tiles (a
@@ -183,8 +198,7 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
         batches[key] = b
     # sample for the plain-loop comparison: plain_steps steps of every replica, both ways
     t0 = time.perf_counter()
-    for b in batches.values():
-        b.run(plain_steps, dt, observe_every=every, observe_from=0, graph=graph)
+    run_batches(batches, plain_steps, dt, every, 0, graph, concurrent)
     t_batch_sample = time.perf_counter() - t0
     zb = {key: b.order_stats() for key, b in batches.items()}
     t_plain, max_err = 0.0, 0.0
@@ -213,8 +227,7 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
         b.reset(0)
         b.set_state(np.concatenate(groups[key]["states"]))
     t0 = time.perf_counter()
-    for b in batches.values():
-        b.run(steps, dt, observe_every=every, observe_from=steps // 2, graph=graph)
+    run_batches(batches, steps, dt, every, steps // 2, graph, concurrent)
     t_batch = time.perf_counter() - t0
     R_mean = np.zeros(sum(len(g["ns"]) for g in groups.values()))
     R_var = np.zeros_like(R_mean)
@@ -231,6 +244,7 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
         "sample_plain_particle_steps_per_s": total_n * plain_steps / t_plain if t_plain else None,
         "speedup_vs_plain_loop": t_plain / t_batch_sample if t_plain else None,
         "graph_replays": graph,
+        "batches_concurrent": concurrent,
         "batches": {key: {"replicas": int(len(g["ns"])), "particles": int(g["ns"].sum())} for key, g in groups.items()},
         "order_param_max_abs_err_vs_plain": max_err,
         "mean_R_over_replicas": float(R_mean.mean()),
@@ -421,6 +435,9 @@ def main():
     ap.add_argument("--only-batch", action="store_true",
                     help="run the batched ensemble alone (with --batch-eager: eager launches, default tiles)")
     ap.add_argument("--batch-eager", action="store_true")
+    ap.add_argument("--batch-sequential", action="store_true",
+                    help="run the ensemble's batches one after another (default: side by side, a host thread and "
+                         "stream each)")
     ap.add_argument("--batch-split", type=int, default=24576,
                     help="replicas of at least this many particles in batches of their own, whose autotuned cell "
                          "size can differ (0: one batch per model)")
@@ -435,7 +452,8 @@ def main():
 
     if args.only_batch:
         res = batched_ensemble(spss, steps=args.batch_steps, plain_steps=args.plain_steps,
-                               graph=not args.batch_eager, tune_it=not args.batch_eager, n_split=args.batch_split)
+                               graph=not args.batch_eager, tune_it=not args.batch_eager, n_split=args.batch_split,
+                               concurrent=not args.batch_sequential)
         print(json.dumps({"batched": res}), flush=True)
         return
 
@@ -479,7 +497,7 @@ def main():
     if not args.no_batch:
         print("[bench] batched ensemble", file=sys.stderr, flush=True)
         batched = batched_ensemble(spss, steps=args.batch_steps, plain_steps=args.plain_steps,
-                                   n_split=args.batch_split)
+                                   n_split=args.batch_split, concurrent=not args.batch_sequential)
         print(f"[bench] batched {batched['speedup_vs_plain_loop']:.1f}x plain loop", file=sys.stderr, flush=True)
     screen = None
     if not args.no_screen:
