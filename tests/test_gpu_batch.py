@@ -157,3 +157,40 @@ def test_batch_cell_autotune_keeps_replicas_equal_to_single_systems(spss):
         m = model * int(ns[r])
         np.testing.assert_array_equal(got[off:off + m], want[r])
         off += m
+
+
+def test_batches_run_side_by_side_equal_sequential_runs(spss):
+    """Batches driven from host threads of their own (each batch has its own stream; the library's
+    error state is per thread) give the same states and order-parameter readings, bit for bit, as
+    the same batches run one after the other: what the ensemble benchmark relies on."""
+    from concurrent.futures import ThreadPoolExecutor
+    dt, v0, rho, K = 0.005, 0.01, 0.03, 30
+    print("[gpu] batches side by side", flush=True)
+    specs = [(4, _replicas(4, 4, 41)), (3, _replicas(3, 3, 42)), (4, _replicas(4, 2, 43))]
+
+    def make():
+        out = []
+        for model, (ns, xi, sig, Dp, seeds, states, nats) in specs:
+            b = spss.ReplicaBatch(model, ns, xi, sig, Dp, seeds, v0=v0, rho=rho)
+            b.set_natural_frequencies(np.concatenate(nats))
+            b.set_state(np.concatenate(states))
+            out.append(b)
+        return out
+
+    results = []
+    for concurrent in (False, True):
+        bs = make()
+        if concurrent:
+            with ThreadPoolExecutor(max_workers=len(bs)) as ex:
+                for f in [ex.submit(b.run, K, dt, 3, 9, True) for b in bs]:
+                    f.result()
+        else:
+            for b in bs:
+                b.run(K, dt, observe_every=3, observe_from=9, graph=True)
+        results.append([(b.get_state(), b.order_stats()[:3]) for b in bs])
+        for b in bs:
+            b.close()
+    for (s0, z0), (s1, z1) in zip(results[0], results[1]):
+        np.testing.assert_array_equal(s0, s1)
+        for a, c in zip(z0, z1):
+            np.testing.assert_array_equal(a, c)
