@@ -45,10 +45,11 @@ def pair_kernel_bytes(n, cells):
 
 
 # HBM bytes per launch of that kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled
-# by the copy-kernel calibration, by cell subdivision (profiles/fr_pairs_euler_N1M_hbm_traffic.txt)
-PAIR_KERNEL_TRAFFIC = {1: 4.4389e7}
+# by the copy-kernel calibration, by cell layout (subdivision, x split)
+# (profiles/fr_pairs_euler_N1M_hbm_traffic.txt)
+PAIR_KERNEL_TRAFFIC = {(1, 1): 4.4389e7, (1, 2): 4.7781e7, (1, 4): 5.4909e7}
 # HBM bytes of the three kernels of one resident step (fr_pairs, fr_far, fr_scatter), the same way
-STEP_TRAFFIC = {1: 1.9577e8}
+STEP_TRAFFIC = {(1, 1): 1.9577e8, (1, 2): 2.0656e8, (1, 4): 2.2848e8}
 # VALU instructions issued per CU-cycle by that kernel (SQ_INSTS_VALU / (256 CUs x cycles),
 # profiles/fr_pairs_euler_N1M_valu.txt): at the nominal one per CU-cycle, i.e. VALU-bound
 PAIR_KERNEL_VALU_PER_CU_CYCLE = 1.00
@@ -65,7 +66,8 @@ def step_bytes(n, cells, s=4):
 # natural frequency, sums read; state, id, natural frequency, snapshot, next move word written),
 # and per cell 68 (the arrivals scan: arrival counts read and cleared, starts written, the 9 move
 # counters of the next step cleared; the scatter: starts and move counts read)
-IMPL_STEP_BYTES = 36.0 + 132.0 + 68.0 * 62500.0 / 1048576.0
+def impl_step_bytes(n, cells):
+    return 36.0 + 132.0 + 68.0 * cells / n
 
 
 def headline_system(spss, n=1 << 20, precision=None, subdiv=1, seed=0x5EED0003, verlet=False):
@@ -192,8 +194,8 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
         if tune_it:
             ctable, k = b.autotune_cells(dt, probe_steps=10)
             table, chosen = b.autotune(dt, probe_steps=10)
-            tune[key] = {"cell_subdivision_ms_per_step": [round(float(t), 4) for t in ctable],
-                         "cell_subdivision": k, "ms_per_step": [round(float(t), 4) for t in table],
+            tune[key] = {"cell_layout_ms_per_step": [round(float(t), 4) for t in ctable],
+                         "cell_layout": list(b.cell_layout), "ms_per_step": [round(float(t), 4) for t in table],
                          "chosen": int(chosen)}
         batches[key] = b
     # sample for the plain-loop comparison: plain_steps steps of every replica, both ways
@@ -210,6 +212,8 @@ def batched_ensemble(spss, steps=3000, plain_steps=100, dt=0.005, v0=0.01, rho=0
             h = spss.ParticleSystem(model=model, n=int(g["ns"][k]), v0=v0, xi=g["xi"][k], sigma=g["sigma"][k],
                                     rho=rho, D_phi=g["D"][k], natfreq_local=1, precision=spss.PRECISION_FP32C,
                                     seed=int(g["seeds"][k]), cell_subdivision=batches[key].cell_subdivision)
+            if batches[key].cell_layout[1] != 1:
+                h.set_cell_layout(*batches[key].cell_layout)
             h.set_natural_frequencies(g["nats"][k])
             h.set_state(g["states"][k])
             h.synchronize()
@@ -422,7 +426,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--subdiv", type=int, default=0, help="cell subdivision 1-3; 0: the autotuner picks it")
+    ap.add_argument("--subdiv", type=int, default=0,
+                    help="cell subdivision 1-3; 0: the autotuner picks the cell layout")
+    ap.add_argument("--xsplit", type=int, default=1, help="with --subdiv: columns per square cell (1, 2, 4)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20, help="timed steps of the all-core CPU baseline")
     ap.add_argument("--cpu-single-steps", type=int, default=3, help="timed steps of the one-core CPU baseline")
@@ -462,12 +468,15 @@ def main():
     g, kw, st0, natf = headline_system(spss, subdiv=max(args.subdiv, 1))
     n = kw["n"]
     tune = None
-    if args.subdiv == 0:  # cell size picked by the autotuner (outside the timed region)
+    if args.subdiv == 0:  # cell layout picked by the autotuner (outside the timed region)
         table, k = g.autotune_cells(spss.STEPPER_EULER, 0.002, probe_steps=20)
-        tune = {"ms_per_step_by_subdivision": [round(float(t), 4) for t in table], "chosen": int(k)}
+        tune = {"layouts": [list(c) for c in spss.CELL_LAYOUTS],
+                "ms_per_step_by_layout": [round(float(t), 4) for t in table], "chosen": list(g.cell_layout)}
         print(f"[bench] cell autotuner: {tune}", file=sys.stderr, flush=True)
-    subdiv = g.cell_subdivision
-    cells = (250 * subdiv) ** 2
+    elif args.xsplit != 1:
+        g.set_cell_layout(args.subdiv, args.xsplit)
+    subdiv, xsplit = g.cell_layout
+    cells = (250 * subdiv) ** 2 * xsplit
     print(f"[bench] headline n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
     wall, eager, replayed, captured = time_steps(g, spss, spss.STEPPER_EULER, args.steps, args.warmup)
     pair_ms = time_pair_kernel(g, spss, spss.STEPPER_EULER, max(args.steps // 5, 20))
@@ -479,6 +488,7 @@ def main():
     step_achieved = step_bytes(n, 250 * 250) * value
     # secondary: the SRK stepper (complementary force, 2 draws per particle)
     g2, _, _, _ = headline_system(spss, subdiv=subdiv)
+    g2.set_cell_layout(subdiv, xsplit)
     srk_steps = max(args.steps // 5, 10)
     wall2, _, _, _ = time_steps(g2, spss, spss.STEPPER_SRK, srk_steps, 10)
     g2.close()
@@ -530,22 +540,22 @@ def main():
         "config": {"workload": "N=1,048,576 uniform in the unit box, rho=0.004 cell list refreshed every step, "
                                "stochastic Euler dt=0.002, m=0.5 sigma=1.5 D=0.02, Lorentzian(0,0.05) omega, "
                                "FP32 distance tests, phasor sums in FP32 partials combined in FP64, FP64 state",
-                   "n": n, "rho": 0.004, "cell_subdivision": subdiv, "cell_autotune": tune,
+                   "n": n, "rho": 0.004, "cell_subdivision": subdiv, "cell_xsplit": xsplit, "cell_autotune": tune,
                    "parallelism": "single GPU"},
         "timed_steps": {"graph_replayed": replayed, "eager": eager, "graphs_captured": captured},
         "roofline": {"bound": "hbm", "achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": step_achieved / HBM_PEAK, "traffic": STEP_TRAFFIC.get(subdiv),
+                     "frac": step_achieved / HBM_PEAK, "traffic": STEP_TRAFFIC.get((subdiv, xsplit)),
                      "kernel": "resident cell-list step (fr_pairs, fr_far, fr_scatter)",
                      "algorithmic_bytes_per_particle_step": step_bytes(n, 250 * 250),
                      "pair_kernel": {"achieved": pair_achieved / 1e9, "frac": pair_achieved / HBM_PEAK,
                                      "us": pair_ms * 1e3, "bytes_per_particle": pair_kernel_bytes(n, cells),
-                                     "traffic": PAIR_KERNEL_TRAFFIC.get(subdiv)}},
+                                     "traffic": PAIR_KERNEL_TRAFFIC.get((subdiv, xsplit))}},
         # the pair kernel is bound by VALU issue, not HBM (no MFMA work on this path)
         "valu": {"kernel": "fr_pairs", "valu_instr_per_cu_cycle": PAIR_KERNEL_VALU_PER_CU_CYCLE},
         "cpu_baseline": cpu,
         "pair_kernel_us": pair_ms * 1e3,
         "hbm_copy_gbps": copy_gbps,
-        "impl_step_bytes_frac": IMPL_STEP_BYTES * value / HBM_PEAK,
+        "impl_step_bytes_frac": impl_step_bytes(n, cells) * value / HBM_PEAK,
         "pair_interactions_per_s": pairs / n * value if pairs else None,
         "srk_particle_steps_per_s": n * srk_steps / wall2,
         "verlet_srk": verlet,

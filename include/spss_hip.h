@@ -159,14 +159,27 @@ int spss_get_step(spss_system *sys, long *step);
 int spss_set_step(spss_system *sys, long step);
 
 /* cell-size autotuner of the FP32C linked-list path (no reference counterpart): times
- * `probe_steps` stochastic Euler (stepper = SPSS_STEPPER_EULER) or Runge-Kutta steps with cells
- * of side 1/(k*int(1/rho)) for k = 1, 2, 3 and keeps the fastest; ms per step into
- * table_ms[k-1] (-1 for a grid that is too fine; at most table_cap entries), the chosen k into
- * *chosen.  State, step counter and complementary force are restored.  Neighbour sets do not
- * depend on k; the coupling sums change within the FP32C tolerance (candidate order). */
+ * `probe_steps` stochastic Euler (stepper = SPSS_STEPPER_EULER) or Runge-Kutta steps for each
+ * cell layout (k, xsplit) of SPSS_CELL_LAYOUTS (see spss_set_cell_layout) and keeps the fastest;
+ * ms per step into table_ms[i] for layout i (-1 for a grid that is too fine; at most table_cap
+ * entries), the chosen k into *chosen (spss_get_cell_layout gives both).  State, step counter and
+ * complementary force are restored.  Neighbour sets do not depend on the layout; the coupling
+ * sums change within the FP32C tolerance (candidate order). */
 int spss_autotune_cells(spss_system *sys, int stepper, double dt, int probe_steps, double *table_ms, int table_cap,
                         int *chosen);
 int spss_get_cell_subdivision(spss_system *sys, int *k);
+/* The FP32C cell grid (no reference counterpart; the reference's grid is k = 1, xsplit = 1):
+ * cells 1/(k*xsplit*int(1/rho)) wide and 1/(k*int(1/rho)) high, k in 1..3, xsplit 1, 2 or 4.  A
+ * particle's candidates are the 2k+1 rows of 2k*xsplit+1 cells around its own: every pair within
+ * rho still, fewer candidates beyond it at a narrower xsplit.  The layouts the autotuners time, in
+ * table order: */
+#define SPSS_CELL_LAYOUTS {{1, 1}, {2, 1}, {3, 1}, {1, 2}, {1, 4}}
+#define SPSS_CELL_LAYOUT_COUNT 5
+/* rebuilds the FP32C cell path with that grid (the state is kept; the next fused step re-enters
+ * the cell order); neighbour sets do not depend on it, the coupling sums change within the FP32C
+ * tolerance (candidate order) */
+int spss_set_cell_layout(spss_system *sys, int k, int xsplit);
+int spss_get_cell_layout(spss_system *sys, int *k, int *xsplit);
 /* all-pairs autotuner (no reference counterpart): times `probe_evals` RHS evaluations of the current
  * state by EvaluateInteractionsWithAllPairs (ParticleSystemFirstOrderPtr.cpp:225-316) for each of
  * the kernel's workgroup / j-tile shapes {64,64}, {64,256}, {128,128}, {128,512}, {256,256},
@@ -264,14 +277,18 @@ int spss_batch_set_tile_widths(spss_batch *b, const int *W);
  * particles per workgroup), keeps the fastest; ms per step into table_ms[<= table_cap], the
  * chosen index into *chosen.  State and step counter are restored, the averages cleared. */
 int spss_batch_autotune(spss_batch *b, double dt, int probe_steps, double *table_ms, int table_cap, int *chosen);
-/* cell-size autotuner: times `probe_steps` steps at cell subdivisions 1, 2, 3 (default tile widths),
- * keeps the fastest; ms per step into table_ms[<= table_cap] (-1 where a subdivision does not
- * build), the chosen subdivision into *chosen.  State and step counter are restored, the averages
- * cleared.  Later results depend on the cell size to the FP32C accuracy; a replica stays
- * bit-identical to a single FP32C system of the same subdivision. */
+/* cell-size autotuner: times `probe_steps` steps for each cell layout of SPSS_CELL_LAYOUTS (default
+ * tile widths), keeps the fastest; ms per step into table_ms[i] for layout i (at most table_cap
+ * entries; -1 where a layout does not build), the chosen subdivision into *chosen
+ * (spss_batch_get_cell_layout gives both).  State and step counter are restored, the averages
+ * cleared.  Later results depend on the cell layout to the FP32C accuracy; a replica stays
+ * bit-identical to a single FP32C system of the same layout. */
 int spss_batch_autotune_cells(spss_batch *b, double dt, int probe_steps, double *table_ms, int table_cap,
                               int *chosen);
 int spss_batch_get_cell_subdivision(spss_batch *b, int *k);
+/* the batch's cell grid as spss_set_cell_layout (state and step counter kept, averages cleared) */
+int spss_batch_set_cell_layout(spss_batch *b, int k, int xsplit);
+int spss_batch_get_cell_layout(spss_batch *b, int *k, int *xsplit);
 
 /* ---- the network of the learned sweep screen (no reference counterpart).  A fully connected
  * FP32 network of widths dims[0] -> ... -> dims[ndims-1] (at most 8 layers, widths <= 4096):

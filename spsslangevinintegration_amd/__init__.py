@@ -6,7 +6,7 @@ of include/spss_hip.h).  This package is thin ctypes plumbing for tests and
 benchmarks; importing it fails if the library has not been built.
 """
 from . import _capi  # noqa: F401
-from ._capi import (DOPRI5_ADAPTIVE, DOPRI5_CONST, METHOD_ALL_PAIRS, METHOD_AUTO,  # noqa: F401
+from ._capi import (CELL_LAYOUTS, DOPRI5_ADAPTIVE, DOPRI5_CONST, METHOD_ALL_PAIRS, METHOD_AUTO,  # noqa: F401
                     METHOD_GLOBAL_FIELD, METHOD_LINKED_LIST,
                     METHOD_LINKED_LIST_VERLET, METHOD_VERLET, NOISE_DEVICE, NOISE_HOST, NOISE_NONE,
                     NOISE_PHILOX, PRECISION_FP32C, PRECISION_FP64, STEPPER_EULER, STEPPER_RK4, STEPPER_SRK,

@@ -18,6 +18,8 @@ VERLET_COMPAT, VERLET_EXACT = 0, 1
 STEPPER_RK4, STEPPER_EULER, STEPPER_SRK = 0, 1, 2
 NOISE_NONE, NOISE_HOST, NOISE_DEVICE, NOISE_PHILOX = 0, 1, 2, 3
 DOPRI5_CONST, DOPRI5_ADAPTIVE = 0, 1
+# FP32C cell layouts (subdivision k, x split) in the autotuners' table order (SPSS_CELL_LAYOUTS)
+CELL_LAYOUTS = ((1, 1), (2, 1), (3, 1), (1, 2), (1, 4))
 # int observer(void *user, double t, const double *state, int n, int s)
 OBSERVER_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_int, C.c_int)
 
@@ -34,11 +36,13 @@ EXPORTS = [
     "spss_averaging_reset", "spss_averaging_accumulate", "spss_averaging_get",
     "spss_kernel_timer_start", "spss_kernel_timer_stop", "spss_device_copy_bandwidth",
     "spss_philox4x32_10", "spss_philox_normals", "spss_autotune_cells", "spss_get_cell_subdivision",
+    "spss_set_cell_layout", "spss_get_cell_layout",
     "spss_autotune_all_pairs", "spss_get_all_pairs_config", "spss_set_all_pairs_config",
     "spss_batch_create", "spss_batch_destroy", "spss_batch_set_state", "spss_batch_get_state",
     "spss_batch_set_natural_frequencies", "spss_batch_reset", "spss_batch_run", "spss_batch_order_stats",
     "spss_batch_get_tile_widths", "spss_batch_set_tile_widths", "spss_batch_autotune",
     "spss_batch_autotune_cells", "spss_batch_get_cell_subdivision",
+    "spss_batch_set_cell_layout", "spss_batch_get_cell_layout",
     "spss_mlp_create", "spss_mlp_destroy", "spss_mlp_param_count", "spss_mlp_set_params", "spss_mlp_get_params",
     "spss_mlp_predict", "spss_mlp_gradients", "spss_mlp_set_data", "spss_mlp_train", "spss_mlp_reset_optimizer",
     "spss_mlp_adam_steps",
@@ -128,6 +132,8 @@ def load(path=None):
         "spss_philox_normals": (i, [C.c_uint64, C.c_uint64, C.c_uint32, l, l, vp, vp]),
         "spss_autotune_cells": (i, [vp, i, d, i, vp, i, C.POINTER(i)]),
         "spss_get_cell_subdivision": (i, [vp, C.POINTER(i)]),
+        "spss_set_cell_layout": (i, [vp, i, i]),
+        "spss_get_cell_layout": (i, [vp, C.POINTER(i), C.POINTER(i)]),
         "spss_autotune_all_pairs": (i, [vp, i, vp, i, C.POINTER(i)]),
         "spss_get_all_pairs_config": (i, [vp, C.POINTER(i), C.POINTER(i)]),
         "spss_set_all_pairs_config": (i, [vp, i]),
@@ -144,6 +150,8 @@ def load(path=None):
         "spss_batch_autotune": (i, [vp, d, i, vp, i, C.POINTER(i)]),
         "spss_batch_autotune_cells": (i, [vp, d, i, vp, i, C.POINTER(i)]),
         "spss_batch_get_cell_subdivision": (i, [vp, C.POINTER(i)]),
+        "spss_batch_set_cell_layout": (i, [vp, i, i]),
+        "spss_batch_get_cell_layout": (i, [vp, C.POINTER(i), C.POINTER(i)]),
         "spss_mlp_create": (i, [C.POINTER(vp), vp, i, i, C.c_uint64]),
         "spss_mlp_destroy": (i, [vp]),
         "spss_mlp_param_count": (i, [vp, C.POINTER(l)]),

@@ -84,13 +84,25 @@ class ReplicaBatch:
         A.check(self.lib.spss_batch_set_tile_widths(self.h, w.ctypes.data))
 
     def autotune_cells(self, dt, probe_steps=20):
-        """times cell subdivisions 1, 2, 3 and keeps the fastest; returns (ms per step for each,
-        -1 where one does not build; the chosen subdivision)"""
-        table = np.zeros(3)
+        """times every cell layout of CELL_LAYOUTS and keeps the fastest; returns (ms per step for
+        each, -1 where one does not build; the chosen subdivision; cell_layout gives the chosen
+        (subdivision, x split))"""
+        table = np.zeros(len(A.CELL_LAYOUTS))
         chosen = C.c_int()
-        A.check(self.lib.spss_batch_autotune_cells(self.h, float(dt), int(probe_steps), A.ptr(table), 3,
+        A.check(self.lib.spss_batch_autotune_cells(self.h, float(dt), int(probe_steps), A.ptr(table), table.size,
                                                    C.byref(chosen)))
         return table, chosen.value
+
+    def set_cell_layout(self, k, xsplit):
+        """cells 1/(k*xsplit*int(1/rho)) wide and 1/(k*int(1/rho)) high (state and step kept)"""
+        A.check(self.lib.spss_batch_set_cell_layout(self.h, int(k), int(xsplit)))
+
+    @property
+    def cell_layout(self):
+        """(subdivision, x split) of the cell grid"""
+        k, x = C.c_int(), C.c_int()
+        A.check(self.lib.spss_batch_get_cell_layout(self.h, C.byref(k), C.byref(x)))
+        return k.value, x.value
 
     @property
     def cell_subdivision(self):

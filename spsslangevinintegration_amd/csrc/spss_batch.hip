@@ -32,6 +32,7 @@ struct spss_batch {
   std::vector<int> rep_n;
   std::vector<Coef> coef;  // per-replica coefficients (to rebuild the cell path at another cell size)
   int nc = 1, k = 1;       // int(1/rho) and the cell subdivision of the current cell path
+  int xsplit = 1;          // its columns per square cell
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
@@ -222,7 +223,7 @@ int spss_batch_create(const spss_batch_params *bp, int nrep, const spss_replica 
   b->coef = coef;
   b->nc = nc;
   b->k = k;
-  int rc = fast_create_replicas(&b->fast, b->st, P, bp->rho, nc, k, nrep, b->rep_n.data(), coef.data());
+  int rc = fast_create_replicas(&b->fast, b->st, P, bp->rho, nc, k, 1, nrep, b->rep_n.data(), coef.data());
   if (rc != SPSS_OK) {
     spss_batch_destroy(b);
     return fast_fail(rc);
@@ -332,25 +333,26 @@ int spss_batch_set_tile_widths(spss_batch *b, const int *W) {
   return rc == SPSS_OK ? SPSS_OK : fast_fail(rc);
 }
 
-// the cell path rebuilt at cell subdivision k (state, natural frequencies and step counter are
-// the caller's to restore)
-static int rebuild_cells(spss_batch *b, int k) {
+// the cell path rebuilt at cell subdivision k and x split xs (state, natural frequencies and step
+// counter are the caller's to restore)
+static int rebuild_cells(spss_batch *b, int k, int xs) {
   drop_graph(b);
   FastPath *f = nullptr;
-  int rc = fast_create_replicas(&f, b->st, b->P, b->bp.rho, b->nc, k, b->R, b->rep_n.data(), b->coef.data());
+  int rc = fast_create_replicas(&f, b->st, b->P, b->bp.rho, b->nc, k, xs, b->R, b->rep_n.data(), b->coef.data());
   if (rc != SPSS_OK) return rc;
   fast_destroy(b->fast);
   b->fast = f;
   b->k = k;
+  b->xsplit = xs;
   return SPSS_OK;
 }
 
-// Cell size of the batch: times probe_steps graph-replayed steps at cell subdivisions 1, 2 and 3
-// (cells of side 1/(k int(1/rho)), each at its default tile widths), keeps the fastest and
-// restores the state and the step counter; the time averages start again.  The cell size
-// changes the FP32 candidate order, so the results of later steps depend on the choice (to
-// the FP32C accuracy); a replica stays bit-identical to a single FP32C system of the same cell
-// subdivision.
+// Cell size of the batch: times probe_steps graph-replayed steps for each cell layout of
+// kCellLayout (cells 1/(k xs int(1/rho)) wide and 1/(k int(1/rho)) high, each at its default
+// tile widths), keeps the fastest and restores the state and the step counter; the time averages
+// start again.  The cell layout changes the FP32 candidate order, so the results of later steps
+// depend on the choice (to the FP32C accuracy); a replica stays bit-identical to a single FP32C
+// system of the same cell layout.
 int spss_batch_autotune_cells(spss_batch *b, double dt, int probe_steps, double *table_ms, int table_cap,
                               int *chosen) {
   if (!b || probe_steps < 2 || !(dt > 0.0)) return set_error(SPSS_ERR_INVALID, "bad argument");
@@ -361,10 +363,11 @@ int spss_batch_autotune_cells(spss_batch *b, double dt, int probe_steps, double 
   int rc = fast_batch_stats(b->fast, b->st, nullptr, nullptr, &step0);
   if (rc != SPSS_OK) return fast_fail(rc);
   double best = 1e300;
-  int best_k = b->k;
-  for (int k = 1; k <= 3; ++k) {
+  int best_k = b->k, best_x = b->xsplit;
+  for (int c = 0; c < kCellLayouts; ++c) {
+    const int k = kCellLayout[c][0], xs = kCellLayout[c][1];
     double ms_step = -1.0;
-    if (rebuild_cells(b, k) == SPSS_OK) {
+    if (rebuild_cells(b, k, xs) == SPSS_OK) {
       BTRY(spss_batch_set_state(b, saved.data()));
       BTRY(spss_batch_run(b, 2 + probe_steps % 2, dt, 0, 0, 1));  // enter, warm up
       EventPair ev;
@@ -378,11 +381,12 @@ int spss_batch_autotune_cells(spss_batch *b, double dt, int probe_steps, double 
       if (ms_step < best) {
         best = ms_step;
         best_k = k;
+        best_x = xs;
       }
     }
-    if (table_ms && k - 1 < table_cap) table_ms[k - 1] = ms_step;  // -1: this subdivision does not build
+    if (table_ms && c < table_cap) table_ms[c] = ms_step;  // -1: this layout does not build
   }
-  rc = rebuild_cells(b, best_k);
+  rc = rebuild_cells(b, best_k, best_x);
   if (rc != SPSS_OK) return fast_fail(rc);
   BTRY(spss_batch_set_state(b, saved.data()));
   rc = fast_batch_reset(b->fast, b->st, step0);
@@ -394,6 +398,29 @@ int spss_batch_autotune_cells(spss_batch *b, double dt, int probe_steps, double 
 int spss_batch_get_cell_subdivision(spss_batch *b, int *k) {
   if (!b || !k) return set_error(SPSS_ERR_INVALID, "NULL argument");
   *k = b->k;
+  return SPSS_OK;
+}
+
+int spss_batch_set_cell_layout(spss_batch *b, int k, int xsplit) {
+  if (!b) return set_error(SPSS_ERR_INVALID, "NULL handle");
+  BTRY(sync_master(b));
+  std::vector<double> saved((size_t)b->s * b->n);
+  BCHK(hipMemcpyAsync(saved.data(), b->state, sizeof(double) * saved.size(), hipMemcpyDeviceToHost, b->st));
+  BCHK(hipStreamSynchronize(b->st));
+  uint64_t step0 = 0;
+  int rc = fast_batch_stats(b->fast, b->st, nullptr, nullptr, &step0);
+  if (rc != SPSS_OK) return fast_fail(rc);
+  rc = rebuild_cells(b, k, xsplit);
+  if (rc != SPSS_OK) return fast_fail(rc);
+  BTRY(spss_batch_set_state(b, saved.data()));
+  rc = fast_batch_reset(b->fast, b->st, step0);
+  return rc == SPSS_OK ? SPSS_OK : fast_fail(rc);
+}
+
+int spss_batch_get_cell_layout(spss_batch *b, int *k, int *xsplit) {
+  if (!b || !k || !xsplit) return set_error(SPSS_ERR_INVALID, "NULL argument");
+  *k = b->k;
+  *xsplit = b->xsplit;
   return SPSS_OK;
 }
 

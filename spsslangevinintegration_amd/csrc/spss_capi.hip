@@ -89,6 +89,7 @@ struct spss_system {
   bool master_stale = false;
   int resident_kind = 0;
   int ap_config = kAllPairsDefault;  // all-pairs workgroup and j-tile sizes (spss_autotune_all_pairs)
+  int xsplit = 1;                    // FP32C cell grid: columns per square cell (spss_set_cell_layout)
   KernelTimer *timer = nullptr;  // around the pair kernel of the RHS
   // Dormand-Prince 5(4) buffers (allocated on first use): two states and derivatives, k2..k6,
   // a stage state and a pinned host copy for observers
@@ -553,7 +554,7 @@ int spss_create(const spss_params *p, spss_system **out) {
       spss_destroy(h);
       return fail(SPSS_ERR_UNSUPPORTED, "FP32C precision is implemented for models 3 and 4");
     }
-    rc = fast_create(&h->fast, h->stream, h->P, h->p.rho, h->ncx, h->p.cell_subdivision);
+    rc = fast_create(&h->fast, h->stream, h->P, h->p.rho, h->ncx, h->p.cell_subdivision, h->xsplit);
     if (rc != SPSS_OK) {
       spss_destroy(h);
       return fail(rc, fast_last_error());
@@ -844,13 +845,15 @@ int spss_autotune_cells(spss_system *h, int stepper, double dt, int probe_steps,
     h->P.higher_order = pho0;
     return SPSS_OK;
   };
-  int best = h->p.cell_subdivision;
+  const int k0 = h->p.cell_subdivision, x0 = h->xsplit;
+  int best = k0, best_x = x0;
   double best_ms = 1e300;
   int rc = SPSS_OK;
-  for (int k = 1; k <= 3 && rc == SPSS_OK; ++k) {
+  for (int c = 0; c < kCellLayouts && rc == SPSS_OK; ++c) {
+    const int k = kCellLayout[c][0], xs = kCellLayout[c][1];
     FastPath *f = nullptr;
-    if (fast_create(&f, h->stream, h->P, h->p.rho, h->ncx, k) != SPSS_OK) {  // grid too fine: not a candidate
-      if (k - 1 < table_cap && table_ms) table_ms[k - 1] = -1.0;
+    if (fast_create(&f, h->stream, h->P, h->p.rho, h->ncx, k, xs) != SPSS_OK) {  // grid too fine: not a candidate
+      if (c < table_cap && table_ms) table_ms[c] = -1.0;
       continue;
     }
     if ((rc = restore()) != SPSS_OK) {
@@ -860,6 +863,7 @@ int spss_autotune_cells(spss_system *h, int stepper, double dt, int probe_steps,
     fast_destroy(h->fast);
     h->fast = f;
     h->p.cell_subdivision = k;
+    h->xsplit = xs;
     const int warm = 2;
     double ms = 0.0;
     for (int s = 0; s < warm + probe_steps && rc == SPSS_OK; ++s) {
@@ -875,19 +879,22 @@ int spss_autotune_cells(spss_system *h, int stepper, double dt, int probe_steps,
     if (rc != SPSS_OK) break;
     ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
     ms /= probe_steps;
-    if (k - 1 < table_cap && table_ms) table_ms[k - 1] = ms;
-    if (ms < best_ms) best_ms = ms, best = k;
+    if (c < table_cap && table_ms) table_ms[c] = ms;
+    if (ms < best_ms) best_ms = ms, best = k, best_x = xs;
   }
   // keep the fastest grid (or the original one after a failure) with the original state
-  if (rc == SPSS_OK && best != h->p.cell_subdivision) {
+  if (rc != SPSS_OK) best = k0, best_x = x0;
+  if (best != h->p.cell_subdivision || best_x != h->xsplit) {
     FastPath *f = nullptr;
-    rc = fast_create(&f, h->stream, h->P, h->p.rho, h->ncx, best);
-    if (rc == SPSS_OK) {
+    const int rc1 = fast_create(&f, h->stream, h->P, h->p.rho, h->ncx, best, best_x);
+    if (rc1 == SPSS_OK) {
       fast_destroy(h->fast);
       h->fast = f;
+      if (h->timer) fast_set_timer(h->fast, h->timer);
       h->p.cell_subdivision = best;
-    } else {
-      rc = fail(rc, fast_last_error());
+      h->xsplit = best_x;
+    } else if (rc == SPSS_OK) {
+      rc = fail(rc1, fast_last_error());
     }
   }
   const int rc2 = restore();
@@ -960,6 +967,29 @@ int spss_get_all_pairs_config(spss_system *h, int *block, int *jtile) {
 int spss_get_cell_subdivision(spss_system *h, int *k) {
   if (!h || !k) return fail(SPSS_ERR_INVALID, "NULL argument");
   *k = h->p.cell_subdivision;
+  return SPSS_OK;
+}
+
+int spss_set_cell_layout(spss_system *h, int k, int xsplit) {
+  if (!h) return fail(SPSS_ERR_INVALID, "NULL handle");
+  if (h->p.precision != SPSS_PRECISION_FP32C || !h->fast)
+    return fail(SPSS_ERR_UNSUPPORTED, "the cell layout belongs to the FP32C linked-list path");
+  TRY(master_write(h, true));  // the state in original order; the next fused step re-enters
+  FastPath *f = nullptr;
+  const int rc = fast_create(&f, h->stream, h->P, h->p.rho, h->ncx, k, xsplit);
+  if (rc != SPSS_OK) return fail(rc, fast_last_error());
+  fast_destroy(h->fast);
+  h->fast = f;
+  if (h->timer) fast_set_timer(h->fast, h->timer);
+  h->p.cell_subdivision = k;
+  h->xsplit = xsplit;
+  return SPSS_OK;
+}
+
+int spss_get_cell_layout(spss_system *h, int *k, int *xsplit) {
+  if (!h || !k || !xsplit) return fail(SPSS_ERR_INVALID, "NULL argument");
+  *k = h->p.cell_subdivision;
+  *xsplit = h->xsplit;
   return SPSS_OK;
 }
 

@@ -5,6 +5,7 @@
 
 #include <vector>
 
+#include "../../include/spss_hip.h"
 #include "spss_device.hpp"
 #include "spss_launch.hpp"
 
@@ -12,7 +13,12 @@ namespace spss {
 
 struct FastPath;
 
-int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv);
+// cells 1/(subdiv*xsplit*nc) wide and 1/(subdiv*nc) high (xsplit 1, 2 or 4: narrower stencil columns)
+int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv, int xsplit);
+int fast_cell_xsplit(const FastPath *f);
+// the cell layouts the autotuners time: {subdivision, x split}
+constexpr int kCellLayouts = SPSS_CELL_LAYOUT_COUNT;
+constexpr int kCellLayout[kCellLayouts][2] = SPSS_CELL_LAYOUTS;
 
 // per-replica coefficients of the stochastic step of a batch (a single system takes them from
 // its params): phase friction xi, coupling sigma, sqrt(2 D) for Euler and SRK, Philox key
@@ -23,8 +29,8 @@ struct Coef {
 // a batch of R independent replicas on one cell grid, particles stored replica after replica
 // (rep_n[r] each, summing to P.n); coef (R entries) makes it a batch stepped by
 // fast_batch_step with per-replica xi, sigma, noise and seed
-int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv, int R,
-                         const int *rep_n, const Coef *coef);
+int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv,
+                         int xsplit, int R, const int *rep_n, const Coef *coef);
 // tile width (cells per pair workgroup) of every replica: read and set (the autotuner's knob)
 int fast_tile_widths(FastPath *f, std::vector<int> *W);
 int fast_set_tile_widths(FastPath *f, const std::vector<int> &W);

@@ -32,18 +32,21 @@ namespace {
 constexpr int kFB = kTileCap;   // threads per pair workgroup (one particle of a row tile each)
 constexpr int kStageMax = 1024; // staged entries per workgroup (particles + row padding)
 constexpr int kPad = 4;         // padding entries after each staged row (>= the unroll - 1)
-constexpr int kMaxRows = 7;     // 2k+1 with k <= 3
-constexpr int kMaxCols = 48;    // W + 2k
+constexpr int kMaxCells = 336;  // staged cells (2ky+1) x (W+2kx) of one workgroup
+constexpr int kMaxSpan = 25;    // stencil columns 2kx+1 (kx = k * xsplit <= 12)
 }  // namespace
 
+// The cell grid: ncx x ncy cells of side Hx x Hy per replica (cell c = cy * ncx + cx).  A particle's
+// stencil is the (2ky+1) rows x (2kx+1) columns around its cell, kx * Hx = ky * Hy = 1/int(1/rho):
+// square cells (kx = ky = k), or cells split into kx/ky columns in x (a narrower stencil in x).
 struct FastArgs {
-  int n, ncf, k, ntiles, cpr;  // cpr: cells per replica (ncf * ncf)
+  int n, ncx, ncy, kx, ky, ntiles, cpr;  // cpr: cells per replica (ncx * ncy)
   const int4 *tiles;           // per tile: first cell of its replica, row, first column, end column
   int4 *ovf_list;              // work of tiles whose staged cells overflow the LDS stage, for fk_pairs_ovf:
                                // {first cell of the replica, row, ca, cb} a column span that fits, or
                                // {.., .., column, -1 - j} particles j*kFB.. of a cell that does not
   int *ovf_count;              // their number, zeroed by scan_add or fr_scatter
-  double H;
+  double Hx, Hy;
   // guard band of the FP32 range test: |d^2 - r2c| <= r2h is decided in FP64; r2k is a power of
   // two with r2k * r2h > 4, so clamp(-r2k * (d^2 - r2c), 0, 1) is exactly 1 below the band, 0 above
   float r2c, r2h, r2k;
@@ -98,21 +101,21 @@ constexpr int kFar = 9;
 __device__ __forceinline__ int move_order(int d) { return d == 4 ? 0 : (d < 4 ? d + 1 : d); }
 
 // source cell of an arrival into cell c by move d (within c's replica)
-__device__ __forceinline__ int move_source(int c, int d, int ncf, int cpr) {
+__device__ __forceinline__ int move_source(int c, int d, int ncx, int ncy, int cpr) {
   const int base = cell_base(c, cpr), l = c - base;
-  const int cx = l % ncf, cy = l / ncf;
-  const int sx = (cx - (d % 3 - 1) + ncf) % ncf;
-  const int sy = (cy - (d / 3 - 1) + ncf) % ncf;
-  return base + sy * ncf + sx;
+  const int cx = l % ncx, cy = l / ncx;
+  const int sx = (cx - (d % 3 - 1) + ncx) % ncx;
+  const int sy = (cy - (d / 3 - 1) + ncy) % ncy;
+  return base + sy * ncx + sx;
 }
 
-// a cell coordinate at most one cell outside [0, ncf) wrapped into it
-__device__ __forceinline__ int wrap_cell(int c, int ncf) { return c < 0 ? c + ncf : (c >= ncf ? c - ncf : c); }
+// a cell coordinate at most one cell outside [0, nc) wrapped into it
+__device__ __forceinline__ int wrap_cell(int c, int nc) { return c < 0 ? c + nc : (c >= nc ? c - nc : c); }
 
-__device__ __forceinline__ int new_count_of(const int *cnt9, int c, int ncf, int cpr) {
+__device__ __forceinline__ int new_count_of(const int *cnt9, int c, int ncx, int ncy, int cpr) {
   int s = 0;
 #pragma unroll
-  for (int d = 0; d < 9; ++d) s += cnt9[(long)move_source(c, d, ncf, cpr) * 9 + d];
+  for (int d = 0; d < 9; ++d) s += cnt9[(long)move_source(c, d, ncx, ncy, cpr) * 9 + d];
   return s;
 }
 
@@ -120,13 +123,13 @@ __device__ __forceinline__ int new_count_of(const int *cnt9, int c, int ncf, int
 // rebuilt cell list
 
 template <int S>
-__global__ void fk_keys(DevParams P, Layout L, const double *__restrict__ stage, int ncf, int *__restrict__ key,
-                        int *__restrict__ slot, int *__restrict__ count) {
+__global__ void fk_keys(DevParams P, Layout L, const double *__restrict__ stage, int ncx, int ncy,
+                        int *__restrict__ key, int *__restrict__ slot, int *__restrict__ count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   double xw, yw;
   apply_pbc(P, stage[(long)i * S], stage[(long)i * S + 1], xw, yw);
-  const int c = replica_of(L, i) * L.cpr + cell_coord(yw, ncf) * ncf + cell_coord(xw, ncf);
+  const int c = replica_of(L, i) * L.cpr + cell_coord(yw, ncy) * ncx + cell_coord(xw, ncx);
   key[i] = c;
   slot[i] = atomicAdd(count + c, 1);
 }
@@ -140,17 +143,18 @@ __global__ void fk_scatter(int n, const int *__restrict__ key, const int *__rest
 // snapshot entry of a particle: offsets inside cell (cx, cy) and the unit phasor
 // (cos phi, sin phi), each computed in FP64 and rounded once to FP32
 template <int S>
-__device__ __forceinline__ float4 snap_entry(const DevParams &P, const double *st, int cx, int cy, double H) {
+__device__ __forceinline__ float4 snap_entry(const DevParams &P, const double *st, int cx, int cy, double Hx,
+                                             double Hy) {
   double xw, yw;
   apply_pbc(P, st[0], st[1], xw, yw);
   double sn, cn;
   sincos(st[2], &sn, &cn);
-  return make_float4((float)(xw - (double)cx * H), (float)(yw - (double)cy * H), (float)cn, (float)sn);
+  return make_float4((float)(xw - (double)cx * Hx), (float)(yw - (double)cy * Hy), (float)cn, (float)sn);
 }
 
 // stable rank + snapshot in sorted order
 template <int S>
-__global__ void fk_rank(DevParams P, const double *__restrict__ stage, int ncf, int cpr, double H,
+__global__ void fk_rank(DevParams P, const double *__restrict__ stage, int ncx, int cpr, double Hx, double Hy,
                         const int *__restrict__ key, const int *__restrict__ start, const int *__restrict__ tmp,
                         int *__restrict__ sorted, float4 *__restrict__ snap, float *__restrict__ omega_f) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,7 +167,7 @@ __global__ void fk_rank(DevParams P, const double *__restrict__ stage, int ncf, 
   sorted[p] = i;
   const double *st = stage + (long)i * S;
   const int l = c - cell_base(c, cpr);
-  snap[p] = snap_entry<S>(P, st, l % ncf, l / ncf, H);
+  snap[p] = snap_entry<S>(P, st, l % ncx, l / ncx, Hx, Hy);
   if (omega_f) omega_f[p] = (S == 4) ? (float)st[3] : 0.f;
 }
 
@@ -231,9 +235,9 @@ struct PairLds {
   float4 sdat[kStageMax];                          // staged x, y in its cell, cos phi, sin phi
   float somega[SRK ? kStageMax : 1];
   int sg[(MODE == 1 || MODE == 2) ? kStageMax : 1];  // global sorted position (pair export)
-  int cglob[kMaxRows * kMaxCols];                  // global sorted start of staged cell (r, q) at r*ncols+q
-  int ccnt[kMaxRows * kMaxCols];                   // its particle count
-  int cpos[kMaxRows * kMaxCols + 1];               // its LDS start; [cells] = staged total with padding
+  int cglob[kMaxCells];                            // global sorted start of staged cell (r, q) at r*ncols+q
+  int ccnt[kMaxCells];                             // its particle count
+  int cpos[kMaxCells + 1];                         // its LDS start; [cells] = staged total with padding
 };
 
 // FP64 decision of a guard-band candidate (global sorted position g) for particle pi
@@ -323,10 +327,10 @@ __device__ __forceinline__ void flush_row(const float *ax, const float *ay, floa
 // The work of a span of cells whose staged cells overflow the LDS stage, appended to the overflow
 // list: the widest column spans that fit, and for a column that does not fit by itself its
 // particles in chunks of kFB.  ccnt holds the particles of every staged cell (row-major over the
-// 2k+1 rows and the W+2k staged columns); colsum is LDS scratch for W+2k column sums.
+// 2ky+1 rows and the W+2kx staged columns); colsum is LDS scratch for W+2kx column sums.
 __device__ __forceinline__ void split_span(const FastArgs &A, int cbase, int row, int ca, int cb, const int *ccnt,
                                            int *colsum) {
-  const int k = A.k, nrows = 2 * k + 1, ncols = cb - ca + 2 * k;
+  const int kx = A.kx, ky = A.ky, nrows = 2 * ky + 1, ncols = cb - ca + 2 * kx;
   __syncthreads();  // every thread has read the staged total that colsum overwrites
   for (int q = threadIdx.x; q < ncols; q += kFB) {
     int sum = 0;
@@ -337,13 +341,13 @@ __device__ __forceinline__ void split_span(const FastArgs &A, int cbase, int row
   if (threadIdx.x != 0) return;
   for (int a = ca; a < cb;) {
     int tot = nrows * kPad;
-    for (int q = a - ca; q <= a - ca + 2 * k; ++q) tot += colsum[q];
+    for (int q = a - ca; q <= a - ca + 2 * kx; ++q) tot += colsum[q];
     int b = a + 1;
     if (tot <= kStageMax) {
-      while (b < cb && tot + colsum[b - ca + 2 * k] <= kStageMax) tot += colsum[b++ - ca + 2 * k];
+      while (b < cb && tot + colsum[b - ca + 2 * kx] <= kStageMax) tot += colsum[b++ - ca + 2 * kx];
       A.ovf_list[atomicAdd(A.ovf_count, 1)] = make_int4(cbase, row, a, b);
     } else {
-      const int nch = (ccnt[k * ncols + (a - ca) + k] + kFB - 1) / kFB;
+      const int nch = (ccnt[ky * ncols + (a - ca) + kx] + kFB - 1) / kFB;
       const int at = atomicAdd(A.ovf_count, nch);
       for (int j = 0; j < nch; ++j) A.ovf_list[at + j] = make_int4(cbase, row, a, -1 - j);
     }
@@ -354,10 +358,10 @@ __device__ __forceinline__ void split_span(const FastArgs &A, int cbase, int row
 // MODE 0: RHS; 1: count neighbours; 2: write neighbour lists; 3: phasor sums of the resident
 // state for the stochastic step
 //
-// One workgroup per tile of W cells of one row.  The (2k+1) x (W+2k) cells around the tile
+// One workgroup per tile of W cells of one row.  The (2ky+1) x (W+2kx) cells around the tile
 // are staged into LDS (x re-based to the staged column 0 in FP64, rounded once; y as the
 // offset in its cell), each staged row followed by kPad entries far outside the box.  A
-// lane owns one particle of the tile and walks the (2k+1) contiguous LDS ranges of its
+// lane owns one particle of the tile and walks the (2ky+1) contiguous LDS ranges of its
 // stencil four candidates at a time without per-candidate bounds: reads past the end of a
 // range land in farther cells of the same staged row (out of range, see below) or in the
 // padding, so the loop body has no branch on the common path.  A tile whose cells do not fit
@@ -367,7 +371,7 @@ __device__ __forceinline__ void split_span(const FastArgs &A, int cbase, int row
 // Range test: an FP32 squared distance below the guard band r2c +- r2h is in range, above it
 // out of range; the band (rare) is decided by the reference's FP64 ClassC formula, and only
 // for candidates inside the lane's stencil.  A candidate beyond the stencil is at least
-// k cells = 1/int(1/rho) >= rho away in x, so its FP32 distance is never below the band.
+// kx cells = 1/int(1/rho) >= rho away in x, so its FP32 distance is never below the band.
 //
 // Coupling sums go through phasors:
 //   sum_j sin(phi_j - phi_i - alpha) = Im(e^{-i(phi_i + alpha)} S_i),  S_i = sum_j e^{i phi_j}
@@ -376,7 +380,8 @@ __device__ __forceinline__ void split_span(const FastArgs &A, int cbase, int row
 // with the phasor components rounded once from FP64 to FP32, accumulated in four FP32
 // partial sums per stencil row and combined in FP64.  The particle itself is among its
 // candidates exactly once (distinct stencil columns since int(1/rho) >= 3) and is removed
-// afterwards.  A particle's candidates, their order and grouping depend on its own stencil
+// afterwards (distinct stencil columns and rows: ncx >= 2kx+1, ncy >= 2ky+1).  A particle's
+// candidates, their order and grouping depend on its own stencil
 // cells only (not on the tile), so every tiling and the overflow pass give identical sums.
 template <int S, bool SRK, int MODE, bool SPLIT>
 __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A, const PairBufs &B, int cbase,
@@ -387,22 +392,22 @@ __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A
   float4 *sdat = L.sdat;
   int *cglob = L.cglob, *ccnt = L.ccnt, *cpos = L.cpos;
   const int W = cb - ca;
-  const int k = A.k;
-  const int nrows = 2 * k + 1;
-  const int ncols = W + 2 * k;
-  const int ncf = A.ncf;
-  const double H = A.H;
+  const int kx = A.kx, ky = A.ky;
+  const int nrows = 2 * ky + 1;
+  const int ncols = W + 2 * kx;
+  const int ncx = A.ncx, ncy = A.ncy;
+  const double Hx = A.Hx, Hy = A.Hy;
   const int *start = B.start;
-  const int ncells = nrows * ncols;
+  const int ncells = nrows * ncols;  // <= kMaxCells (the host caps W)
 
   // ---- staging plan: global extent of every staged cell, then the LDS layout
   for (int e = threadIdx.x; e < ncells; e += kFB) {
     const int r = e / ncols, q = e % ncols;
-    // at most k < ncf cells outside [0, ncf): one wrap each
-    int rrow = row + r - k, col = ca - k + q;
-    rrow += rrow < 0 ? ncf : (rrow >= ncf ? -ncf : 0);
-    col += col < 0 ? ncf : (col >= ncf ? -ncf : 0);
-    const int c = cbase + rrow * ncf + col;
+    // at most ky < ncy rows and kx < ncx columns outside the grid: one wrap each
+    int rrow = row + r - ky, col = ca - kx + q;
+    rrow += rrow < 0 ? ncy : (rrow >= ncy ? -ncy : 0);
+    col += col < 0 ? ncx : (col >= ncx ? -ncx : 0);
+    const int c = cbase + rrow * ncx + col;
     const int g0 = start[c];
     cglob[e] = g0;
     ccnt[e] = start[c + 1] - g0;
@@ -439,7 +444,7 @@ __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A
     for (int c = (int)threadIdx.x / G; c < ncells; c += kFB / G) {
       const int q = c % ncols;
       const int e0 = cpos[c], n0 = ccnt[c], g0 = cglob[c];
-      const double xq = (double)q * H;
+      const double xq = (double)q * Hx;
       for (int j = sub; j < n0; j += G) {
         const int g = g0 + j;
         const float4 v = B.snap[g];
@@ -458,10 +463,10 @@ __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A
   }
   __syncthreads();
 
-  // particles of the tile: staged row k, columns k .. k+W-1 (contiguous in LDS and globally)
-  const int gtb = cglob[k * ncols + k];
-  const int ntile = start[cbase + row * ncf + cb] - start[cbase + row * ncf + ca];
-  const int ltb = cpos[k * ncols + k];
+  // particles of the tile: staged row ky, columns kx .. kx+W-1 (contiguous in LDS and globally)
+  const int gtb = cglob[ky * ncols + kx];
+  const int ntile = start[cbase + row * ncx + cb] - start[cbase + row * ncx + ca];
+  const int ltb = cpos[ky * ncols + kx];
   const float r2c = A.r2c, r2h = A.r2h, r2k = A.r2k;
 
   for (int u = threadIdx.x; u < ntile; u += kFB) {
@@ -470,12 +475,12 @@ __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A
     const int i = RES ? B.rid[gp] : B.sorted[gp];
     const double *pi = RES ? B.rstate + (long)gp * S : B.stage + (long)i * S;
     // own staged column: last tile cell whose global start is <= gp
-    int qi = k;
+    int qi = kx;
     {
-      int lo = k, hi = k + W - 1;
+      int lo = kx, hi = kx + W - 1;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (cglob[k * ncols + mid] <= gp) lo = mid; else hi = mid - 1;
+        if (cglob[ky * ncols + mid] <= gp) lo = mid; else hi = mid - 1;
       }
       qi = lo;
     }
@@ -487,8 +492,8 @@ __device__ __forceinline__ bool pairs_span(const DevParams &P, const FastArgs &A
     const int wbase = (MODE == 2) ? B.pstart[i] : 0;
 
     for (int r = 0; r < nrows; ++r) {
-      const float syi = (float)((double)yi - (double)(r - k) * H);
-      const int e0 = r * ncols + qi - k, e1 = r * ncols + qi + k;
+      const float syi = (float)((double)yi - (double)(r - ky) * Hy);
+      const int e0 = r * ncols + qi - kx, e1 = r * ncols + qi + kx;
       const int lo = cpos[e0];
       const int hi = cpos[e1] + ccnt[e1];  // end of the last stencil cell (before any padding)
       // global position of staged entry t inside this row's stencil (band decisions, lists)
@@ -538,7 +543,7 @@ struct OvfLds {
   float4 sdat[kStageMax];
   float somega[SRK ? kStageMax : 1];
   int sg[kStageMax];
-  int cglob[kMaxRows], ccnt[kMaxRows];
+  int cglob[kMaxSpan], ccnt[kMaxSpan];
 };
 
 // A cell whose own stencil does not fit the LDS stage (a dense cluster): its particles
@@ -551,11 +556,11 @@ __device__ __forceinline__ void pairs_dense_cell(const DevParams &P, const FastA
   constexpr bool RES = MODE >= 3;
   constexpr bool SUMS = MODE == 0 || MODE >= 3;
   constexpr int kChunk = kStageMax - kPad;
-  const int k = A.k, nrows = 2 * k + 1, ncf = A.ncf;
-  const double H = A.H;
+  const int kx = A.kx, ky = A.ky, nrows = 2 * ky + 1, nspan = 2 * kx + 1, ncx = A.ncx, ncy = A.ncy;
+  const double Hx = A.Hx, Hy = A.Hy;
   const float r2c = A.r2c, r2h = A.r2h, r2k = A.r2k;
   const int *start = B.start;
-  const int cown = cbase + row * ncf + q;
+  const int cown = cbase + row * ncx + q;
   const int gp1 = start[cown + 1];
   {
     const int pc = start[cown] + chunk * kFB;
@@ -564,8 +569,8 @@ __device__ __forceinline__ void pairs_dense_cell(const DevParams &P, const FastA
     const int i = act ? (RES ? B.rid[gp] : B.sorted[gp]) : 0;
     const double *pi = act ? (RES ? B.rstate + (long)gp * S : B.stage + (long)i * S) : nullptr;
     const float4 me = act ? B.snap[gp] : make_float4(0.f, 0.f, 0.f, 0.f);
-    // x re-based to the window's column 0 (the particle's own column is k)
-    const float xi_t = (float)((double)k * H + (double)me.x);
+    // x re-based to the window's column 0 (the particle's own column is kx)
+    const float xi_t = (float)((double)kx * Hx + (double)me.x);
     const float yi = me.y, ci = me.z, si = me.w;
     double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
     int nnb = 0;
@@ -573,18 +578,18 @@ __device__ __forceinline__ void pairs_dense_cell(const DevParams &P, const FastA
     const int wbase = (MODE == 2 && act) ? B.pstart[i] : 0;
     for (int r = 0; r < nrows; ++r) {
       __syncthreads();  // the previous row's stage is consumed
-      if (threadIdx.x < nrows) {
+      if (threadIdx.x < nspan) {
         const int dc = threadIdx.x;
-        const int rrow = ((row + r - k) % ncf + ncf) % ncf;
-        const int col = ((q - k + dc) % ncf + ncf) % ncf;
-        const int c = cbase + rrow * ncf + col;
+        const int rrow = ((row + r - ky) % ncy + ncy) % ncy;
+        const int col = ((q - kx + dc) % ncx + ncx) % ncx;
+        const int c = cbase + rrow * ncx + col;
         L.cglob[dc] = start[c];
         L.ccnt[dc] = start[c + 1] - start[c];
       }
       __syncthreads();
       int len = 0;
-      for (int dc = 0; dc < nrows; ++dc) len += L.ccnt[dc];
-      const float syi = (float)((double)yi - (double)(r - k) * H);
+      for (int dc = 0; dc < nspan; ++dc) len += L.ccnt[dc];
+      const float syi = (float)((double)yi - (double)(r - ky) * Hy);
       float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, an = 0.f;
       float bx[4] = {0.f, 0.f, 0.f, 0.f}, by[4] = {0.f, 0.f, 0.f, 0.f};
       for (int c0 = 0; c0 < len; c0 += kChunk) {
@@ -597,7 +602,7 @@ __device__ __forceinline__ void pairs_dense_cell(const DevParams &P, const FastA
             while (t >= base + L.ccnt[dc]) base += L.ccnt[dc++];
             const int g = L.cglob[dc] + (t - base);
             const float4 v = B.snap[g];
-            L.sdat[e] = make_float4((float)((double)dc * H + (double)v.x), v.y, v.z, v.w);
+            L.sdat[e] = make_float4((float)((double)dc * Hx + (double)v.x), v.y, v.z, v.w);
             if (SRK) L.somega[e] = B.omega_f[g];
             L.sg[e] = g;
           } else {
@@ -609,10 +614,10 @@ __device__ __forceinline__ void pairs_dense_cell(const DevParams &P, const FastA
         __syncthreads();
         if (act) {
           int st = -1;  // the particle itself, in its own row
-          if (r == k) {
-            const int s0 = L.cglob[k] <= gp && gp < L.cglob[k] + L.ccnt[k] ? gp - L.cglob[k] : -1;
+          if (r == ky) {
+            const int s0 = L.cglob[kx] <= gp && gp < L.cglob[kx] + L.ccnt[kx] ? gp - L.cglob[kx] : -1;
             int base = 0;
-            for (int dc = 0; dc < k; ++dc) base += L.ccnt[dc];
+            for (int dc = 0; dc < kx; ++dc) base += L.ccnt[dc];
             if (s0 >= 0 && base + s0 >= c0 && base + s0 < c0 + clen) st = base + s0 - c0;
           }
           auto gof = [&](int t) -> int { return L.sg[t]; };
@@ -667,10 +672,10 @@ __global__ __launch_bounds__(kFB) void fk_pairs_ovf(DevParams P, FastArgs A, Pai
 }
 
 template <int S>
-__device__ __forceinline__ int cell_of_position(const DevParams &P, double x, double y, int ncf) {
+__device__ __forceinline__ int cell_of_position(const DevParams &P, double x, double y, int ncx, int ncy) {
   double xw, yw;
   apply_pbc(P, x, y, xw, yw);
-  return cell_coord(yw, ncf) * ncf + cell_coord(xw, ncf);
+  return cell_coord(yw, ncy) * ncx + cell_coord(xw, ncx);
 }
 
 // ---------------------------------------------------------------------------
@@ -684,7 +689,7 @@ __device__ __forceinline__ int cell_of_position(const DevParams &P, double x, do
 // the end): the run search reads its neighbours.
 // (sn, cn: sin and cos of the phase pi[2]; xw, yw: pi[0], pi[1] wrapped into the box)
 template <int S, bool SRK>
-__device__ __forceinline__ void move_and_count(const DevParams &P, const Layout &L, int ncf, const double *pi,
+__device__ __forceinline__ void move_and_count(const DevParams &P, const Layout &L, int ncx, int ncy, const double *pi,
                                                double sn, double cn, double xw, double yw, const StepArgs &U, int p,
                                                bool valid, int *__restrict__ mv, int *__restrict__ farcnt,
                                                int *__restrict__ farslot, int *__restrict__ cnt9,
@@ -694,26 +699,26 @@ __device__ __forceinline__ void move_and_count(const DevParams &P, const Layout 
   if (valid) {
     double x1, y1;
     position_from<S, SRK>(P, pi, U, sn, cn, x1, y1);
-    const int ox = cell_coord(xw, ncf), oy = cell_coord(yw, ncf);
+    const int ox = cell_coord(xw, ncx), oy = cell_coord(yw, ncy);
     apply_pbc(P, x1, y1, xw, yw);
-    const int nx = cell_coord(xw, ncf), ny = cell_coord(yw, ncf);
+    const int nx = cell_coord(xw, ncx), ny = cell_coord(yw, ncy);
     const int base = replica_of(L, p) * L.cpr;
     int dxc = nx - ox, dyc = ny - oy;
-    if (dxc == ncf - 1) dxc = -1;
-    if (dxc == 1 - ncf) dxc = 1;
-    if (dyc == ncf - 1) dyc = -1;
-    if (dyc == 1 - ncf) dyc = 1;
+    if (dxc == ncx - 1) dxc = -1;
+    if (dxc == 1 - ncx) dxc = 1;
+    if (dyc == ncy - 1) dyc = -1;
+    if (dyc == 1 - ncy) dyc = 1;
     if (dxc >= -1 && dxc <= 1 && dyc >= -1 && dyc <= 1) {
       d = (dyc + 1) * 3 + (dxc + 1);
     } else {
       d = kFar;
-      farslot[p] = atomicAdd(farcnt + base + ny * ncf + nx, 1);
+      farslot[p] = atomicAdd(farcnt + base + ny * ncx + nx, 1);
       const int k = atomicAdd(farlist_n, 1);  // rare: the arrivals scan places these
       farlist[2 * k] = p;
-      farlist[2 * k + 1] = base + ny * ncf + nx;
+      farlist[2 * k + 1] = base + ny * ncx + nx;
     }
-    cell = base + oy * ncf + ox;
-    dest = base + ny * ncf + nx;
+    cell = base + oy * ncx + ox;
+    dest = base + ny * ncx + nx;
     mv[p] = (cell << 4) | d;
   }
   const int lane = threadIdx.x & 63;
@@ -731,7 +736,7 @@ __device__ __forceinline__ void move_and_count(const DevParams &P, const Layout 
 // the moves of a resident order whose moves no fr_scatter has computed yet (entry into the
 // resident order, or a step with another dt or stepper); cnt9, farcnt and newcnt start at zero
 template <int S, bool SRK>
-__global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int ncf,
+__global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int ncx, int ncy,
                                                const double *__restrict__ rstate, StepArgs U, int *__restrict__ mv,
                                                int *__restrict__ farcnt, int *__restrict__ farslot,
                                                int *__restrict__ cnt9, int *__restrict__ newcnt,
@@ -744,14 +749,14 @@ __global__ __launch_bounds__(256) void fr_move(DevParams P, Layout L, int n, int
     sincos(pi[2], &sn, &cn);
     apply_pbc(P, pi[0], pi[1], xw, yw);
   }
-  move_and_count<S, SRK>(P, L, ncf, pi, sn, cn, xw, yw, U, p, valid, mv, farcnt, farslot, cnt9, newcnt, farlist,
-                         farlist_n);
+  move_and_count<S, SRK>(P, L, ncx, ncy, pi, sn, cn, xw, yw, U, p, valid, mv, farcnt, farslot, cnt9, newcnt,
+                         farlist, farlist_n);
 }
 
 // the row tiles of a cell order with final starts (entry into the resident order)
-__global__ __launch_bounds__(256) void fk_row_tiles(const int *__restrict__ start, int ncf, int wcap, int cmax,
-                                                    int4 *__restrict__ tiles) {
-  rows_to_tiles([&](int c) { return start[c]; }, ncf, wcap, cmax, tiles);
+__global__ __launch_bounds__(256) void fk_row_tiles(const int *__restrict__ start, int ncx, int ncy, int wcap,
+                                                    int cmax, int4 *__restrict__ tiles) {
+  rows_to_tiles([&](int c) { return start[c]; }, ncx, ncy, wcap, cmax, tiles);
 }
 
 // The arrivals of a resident step: the new cell starts, computed by extra workgroups of the pair
@@ -781,7 +786,7 @@ struct Arrivals {
   int *bucket;                 // far arrivals by new position
   int *cnt9_next;              // the other move-count buffer, cleared for the next step
   long nclear;
-  int ncf, cpr, n;
+  int ncx, ncy, cpr, n;
   int *error;                  // set when a look-back wait ran out (never, when dispatch is ordered)
 };
 
@@ -902,7 +907,7 @@ __global__ __launch_bounds__(kFB) void fr_far(DevParams P, FastArgs A, PairBufs 
   const int nfar = *Q.farlist_n;
   for (int k = blockIdx.x * kFB + threadIdx.x; k < nfar; k += gridDim.x * kFB) {
     const int pf = Q.farlist[2 * k], cd = Q.farlist[2 * k + 1];
-    const int q = Q.nstart[cd] + new_count_of(Q.cnt9, cd, Q.ncf, Q.cpr) + Q.farslot[pf];
+    const int q = Q.nstart[cd] + new_count_of(Q.cnt9, cd, Q.ncx, Q.ncy, Q.cpr) + Q.farslot[pf];
     if (q < Q.n) Q.bucket[q] = pf;  // always, when the counts match the moves
   }
   if (blockIdx.x == 0)
@@ -915,7 +920,8 @@ __global__ __launch_bounds__(kFB) void fr_far(DevParams P, FastArgs A, PairBufs 
 // farcnt and newcnt as fr_move would (the new position depends on phase and omega only)
 template <int S, bool SRK>
 __global__ __launch_bounds__(256) void fr_scatter(
-    DevParams P, Layout L, int n, int ncf, double H, const int *__restrict__ start, const int *__restrict__ nstart,
+    DevParams P, Layout L, int n, int ncx, int ncy, double Hx, double Hy, const int *__restrict__ start,
+    const int *__restrict__ nstart,
     const int *__restrict__ mv, const int *__restrict__ cnt9, const int *__restrict__ bucket,
     const double *__restrict__ rstate, const int *__restrict__ rid, const double *__restrict__ rnat,
     const double2 *__restrict__ ssum, const double2 *__restrict__ tsum, const int *__restrict__ scnt, StepArgs U,
@@ -927,7 +933,8 @@ __global__ __launch_bounds__(256) void fr_scatter(
   if (blockIdx.x == 0 && threadIdx.x == 0) *ovf_count = 0;
   // the next pair kernel's row tiles (single systems) from the new starts, in the first
   // workgroups: their serial span search hides behind the other workgroups' memory traffic
-  if (rtiles && (int)blockIdx.x * 4 < ncf) rows_to_tiles([&](int c) { return nstart[c]; }, ncf, wcap, cmax, rtiles);
+  if (rtiles && (int)blockIdx.x * 4 < ncy)
+    rows_to_tiles([&](int c) { return nstart[c]; }, ncx, ncy, wcap, cmax, rtiles);
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = p < n;
   const int lane = threadIdx.x & 63;
@@ -985,25 +992,25 @@ __global__ __launch_bounds__(256) void fr_scatter(
     if (SRK && compl_out) compl_out[id] = C;
     int cd;
     if (d == kFar) {
-      cd = base + cell_of_position<S>(P, out[0], out[1], ncf);
-      const int b = nstart[cd] + new_count_of(cnt9, cd, ncf, cpr);
+      cd = base + cell_of_position<S>(P, out[0], out[1], ncx, ncy);
+      const int b = nstart[cd] + new_count_of(cnt9, cd, ncx, ncy, cpr);
       const int e = nstart[cd + 1];
       int rank = 0;
       for (int q = b; q < e; ++q) rank += (bucket[q] < p) ? 1 : 0;
       pos = b + rank;
     } else {
       // destination cell, then the arrivals ordered before move d: the counts of the moves e
-      // with move_order(e) < move_order(d) out of their source cells (ncf >= 3: one wrap each)
+      // with move_order(e) < move_order(d) out of their source cells (ncx, ncy >= 3: one wrap each)
       const int ls = cs - base;
-      const int cx = ls % ncf, cy = ls / ncf;
-      const int tx = wrap_cell(cx + d % 3 - 1, ncf), ty = wrap_cell(cy + d / 3 - 1, ncf);
-      cd = base + ty * ncf + tx;
+      const int cx = ls % ncx, cy = ls / ncx;
+      const int tx = wrap_cell(cx + d % 3 - 1, ncx), ty = wrap_cell(cy + d / 3 - 1, ncy);
+      cd = base + ty * ncx + tx;
       int b0 = 0;
       const int od = move_order(d);
 #pragma unroll
       for (int e = 0; e < 9; ++e) {
-        const int sx = wrap_cell(tx - (e % 3 - 1), ncf), sy = wrap_cell(ty - (e / 3 - 1), ncf);
-        const int v = cnt9[(long)(base + sy * ncf + sx) * 9 + e];
+        const int sx = wrap_cell(tx - (e % 3 - 1), ncx), sy = wrap_cell(ty - (e / 3 - 1), ncy);
+        const int v = cnt9[(long)(base + sy * ncx + sx) * 9 + e];
         b0 += move_order(e) < od ? v : 0;
       }
       pos = nstart[cd] + b0 + rank;
@@ -1020,14 +1027,14 @@ __global__ __launch_bounds__(256) void fr_scatter(
       sincos(out[2], &sn2, &cn2);
       apply_pbc(P, out[0], out[1], xw2, yw2);
       const int ld = cd - base;
-      snap[pos] = make_float4((float)(xw2 - (double)(ld % ncf) * H), (float)(yw2 - (double)(ld / ncf) * H),
+      snap[pos] = make_float4((float)(xw2 - (double)(ld % ncx) * Hx), (float)(yw2 - (double)(ld / ncx) * Hy),
                               (float)cn2, (float)sn2);
       if (omega_f) omega_f[pos] = (S == 4) ? (float)out[3] : 0.f;
     }
   }
   if (mvn)
-    move_and_count<S, SRK>(P, L, ncf, out, sn2, cn2, xw2, yw2, U, pos, valid, mvn, farcnt, farslot, cnt9n, newcnt,
-                           farlist, farlist_n);
+    move_and_count<S, SRK>(P, L, ncx, ncy, out, sn2, cn2, xw2, yw2, U, pos, valid, mvn, farcnt, farslot, cnt9n,
+                           newcnt, farlist, farlist_n);
 }
 
 // advance the device step counter of graph-replayed runs

@@ -1,15 +1,16 @@
 // spss_kernels_fast.hip -- FP32 cell-list path (SPSS_PRECISION_FP32C), models 3 and 4: host side.
 //
 // Data path of one RHS evaluation (stage state in FP64 AoS, original order):
-//   fk_keys      cell of every particle on a grid of side 1/(k*nc), histogram by atomics
+//   fk_keys      cell of every particle on a grid of cells 1/(k*xsplit*nc) wide and 1/(k*nc) high,
+//                histogram by atomics
 //   scan         exclusive scan of the cell counts (two kernels)
 //   fk_scatter   unstable bucket fill
 //   fk_rank      stable position inside the cell (ascending particle index) -> sorted order,
 //                and the snapshot: FP32 offsets of the particle inside its cell and its
 //                unit phasor (cos phi, sin phi); omega for the SRK force
-//   fk_pairs     one workgroup per tile of W cells of one row: the (2k+1) rows x (W+2k)
-//                cells around it are staged into LDS with x re-based to the tile (FP64
-//                rounding, once per staged particle); each lane walks the (2k+1) contiguous
+//   fk_pairs     one workgroup per tile of W cells of one row: the (2ky+1) rows x (W+2kx)
+//                cells around it (kx = k*xsplit, ky = k) are staged into LDS with x re-based to
+//                the tile (FP64 rounding, once per staged particle); each lane walks the (2ky+1) contiguous
 //                row ranges of its particle in FP32 and accumulates the phasors of its
 //                neighbours in FP64 (coupling = Im(e^{-i(phi+alpha)} sum e^{i phi_j})).
 //                Pairs whose FP32 distance lies in a
@@ -43,8 +44,10 @@ thread_local std::string g_fast_err;
 struct FastPath {
   int n = 0, s = 4;
   double rho = 0.0, rho2 = 0.0;
-  int nc = 1, k = 1, ncf = 1, cpr = 1, cells = 1, nsb = 1;
-  double H = 1.0;
+  // grid: ncx x ncy cells of Hx x Hy per replica, stencil (2ky+1) x (2kx+1) cells; k the subdivision,
+  // xsplit the columns of a square cell
+  int nc = 1, k = 1, xsplit = 1, ncx = 1, ncy = 1, kx = 1, ky = 1, cpr = 1, cells = 1, nsb = 1;
+  double Hx = 1.0, Hy = 1.0;
   float r2c = 0.f, r2h = 0.f, r2k = 0.f;
   // replicas (one for a single system) and the tiles of the pair kernel
   int R = 1;
@@ -135,30 +138,37 @@ static hipError_t falloc(T **p, size_t count) {
   return hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
 }
 
+// widest tile: its staged (2ky+1) x (W+2kx) cells fit the pair kernel's cell tables, and at most
+// 48 * xsplit staged columns (48 square cells: about the width of a dense row's full tile)
+static int max_tile_width(const FastPath *f) {
+  return std::min(kMaxCells / (2 * f->ky + 1), 48 * f->xsplit) - 2 * f->kx;
+}
+
 // default tile width of a replica: about 0.8 particles per thread at its mean density, with
-// the staged (2k+1) x (W+2k) cells and their row padding within 85% of the LDS stage
-static int default_tile_width(double per_cell, int k, int ncf) {
-  const int nrows = 2 * k + 1;
+// the staged (2ky+1) x (W+2kx) cells and their row padding within 85% of the LDS stage
+static int default_tile_width(const FastPath *f, double per_cell) {
+  const int nrows = 2 * f->ky + 1;
   // about 80% of the workgroup's lanes: a tile just above kFB particles would need a second pass
   // in which one wave walks the whole stencil for a handful of particles
   int W = (int)(0.8 * kFB / std::max(per_cell, 1e-9));
-  const int wstage = (int)((0.85 * kStageMax - nrows * kPad) / std::max(per_cell * nrows, 1e-9)) - 2 * k;
+  const int wstage = (int)((0.85 * kStageMax - nrows * kPad) / std::max(per_cell * nrows, 1e-9)) - 2 * f->kx;
   W = std::min(W, std::max(wstage, 1));
-  return std::max(1, std::min(W, std::min(kMaxCols - 2 * k, ncf)));
+  return std::max(1, std::min(W, std::min(max_tile_width(f), f->ncx)));
 }
 
 // tile table for per-replica tile widths, and the guard band of the widest span any launch can
-// stage: the FP32 distance error bound (coordinates <= (W+2k+1) cells from the staged origin,
-// rounded once; offsets < one cell) times a safety factor of 8.  A single system's resident
-// steps stage per-step row tiles of up to rt_wcap columns, so their width counts too.
+// stage: the FP32 distance error bound (x coordinates <= (W+2kx+1) cells from the staged origin,
+// y <= 2ky+2 cells, rounded once; offsets < one cell) times a safety factor of 8.  A single
+// system's resident steps stage per-step row tiles of up to rt_wcap columns, so their width
+// counts too.
 static int set_tiles(FastPath *f, const std::vector<int> &W) {
   std::vector<int4> t;
   int wmax = f->rtiles ? f->rt_wcap : 1;
   for (int r = 0; r < f->R; ++r) {
-    const int w = std::max(1, std::min(W[r], std::min(kMaxCols - 2 * f->k, f->ncf)));
+    const int w = std::max(1, std::min(W[r], std::min(max_tile_width(f), f->ncx)));
     wmax = std::max(wmax, w);
-    for (int row = 0; row < f->ncf; ++row)
-      for (int ca = 0; ca < f->ncf; ca += w) t.push_back(make_int4(r * f->cpr, row, ca, std::min(ca + w, f->ncf)));
+    for (int row = 0; row < f->ncy; ++row)
+      for (int ca = 0; ca < f->ncx; ca += w) t.push_back(make_int4(r * f->cpr, row, ca, std::min(ca + w, f->ncx)));
   }
   if ((int)t.size() > f->tiles_cap) {
     hipFree(f->tiles);
@@ -170,9 +180,9 @@ static int set_tiles(FastPath *f, const std::vector<int> &W) {
   f->ntiles = (int)t.size();
   f->W_h = W;
   ++f->gen;
-  const double span = (double)(wmax + 2 * f->k + 1) * f->H;
+  const double span = std::max((double)(wmax + 2 * f->kx + 1) * f->Hx, (double)(2 * f->ky + 2) * f->Hy);
   const double eps = std::ldexp(1.0, -24);
-  const double dxerr = 2.0 * span * eps + 2.0 * f->H * eps;
+  const double dxerr = 2.0 * span * eps + 2.0 * std::max(f->Hx, f->Hy) * eps;
   double g = 8.0 * (2.0 * dxerr / f->rho + 4.0 * eps);
   g = std::max(g, 64.0 * eps);
   f->r2c = (float)f->rho2;
@@ -181,11 +191,15 @@ static int set_tiles(FastPath *f, const std::vector<int> &W) {
   return SPSS_OK;
 }
 
-int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv, int R,
-                         const int *rep_n, const Coef *coef) {
+int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv,
+                         int xsplit, int R, const int *rep_n, const Coef *coef) {
   *out = nullptr;
   if (subdiv < 1 || subdiv > 3) {
     g_fast_err = "cell_subdivision must be 1, 2 or 3";
+    return SPSS_ERR_INVALID;
+  }
+  if (xsplit != 1 && xsplit != 2 && xsplit != 4) {
+    g_fast_err = "the cell x split must be 1, 2 or 4";
     return SPSS_ERR_INVALID;
   }
   if (P.xs != 1.0 || P.ys != 1.0) {
@@ -197,7 +211,7 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
                  "more than once (use the FP64 precision there)";
     return SPSS_ERR_UNSUPPORTED;
   }
-  const long cpr = (long)nc * subdiv * nc * subdiv;
+  const long cpr = (long)nc * subdiv * xsplit * nc * subdiv;
   if (R < 1 || cpr * R >= (1L << 27)) {
     g_fast_err = "the FP32C cell list supports at most 2^27 cells over all replicas";
     return SPSS_ERR_UNSUPPORTED;
@@ -209,8 +223,13 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   f->rho2 = rho * rho;
   f->nc = nc;
   f->k = subdiv;
-  f->ncf = nc * subdiv;
-  f->H = 1.0 / (double)f->ncf;
+  f->xsplit = xsplit;
+  f->ncy = nc * subdiv;
+  f->ncx = f->ncy * xsplit;
+  f->ky = subdiv;
+  f->kx = subdiv * xsplit;
+  f->Hx = 1.0 / (double)f->ncx;
+  f->Hy = 1.0 / (double)f->ncy;
   f->cpr = (int)cpr;
   f->R = R;
   f->cells = (int)(cpr * R);
@@ -244,15 +263,15 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   A(falloc(&f->tmp, n));
   A(falloc(&f->sorted, n));
   A(falloc(&f->blocksum, f->nsb + 1));
-  if (R == 1 && f->ncf <= kRowTileCols) {
+  if (R == 1 && f->ncx <= kRowTileCols) {
     // slots per row: the greedy spans of a row 1.5x the mean, at about 94% of kFB each, plus 2,
     // and at least enough spans of rt_wcap columns to cover a row
-    const double row_mean = (double)P.n / (double)f->ncf;
-    f->rt_wcap = std::min(64, kMaxCols - 2 * f->k);
-    f->rt_cmax = std::max((f->ncf + f->rt_wcap - 1) / f->rt_wcap,
-                          std::min(f->ncf, (int)std::ceil(1.5 * row_mean / (0.94 * kFB)) + 2));
-    A(falloc(&f->rtiles, (size_t)f->ncf * f->rt_cmax));
-    A(falloc(&f->btiles, (size_t)f->ncf * f->rt_cmax));
+    const double row_mean = (double)P.n / (double)f->ncy;
+    f->rt_wcap = std::min(64, max_tile_width(f));
+    f->rt_cmax = std::max((f->ncx + f->rt_wcap - 1) / f->rt_wcap,
+                          std::min(f->ncx, (int)std::ceil(1.5 * row_mean / (0.94 * kFB)) + 2));
+    A(falloc(&f->rtiles, (size_t)f->ncy * f->rt_cmax));
+    A(falloc(&f->btiles, (size_t)f->ncy * f->rt_cmax));
   }
   A(falloc(&f->snap, n));
   A(falloc(&f->omega_f, n));
@@ -299,7 +318,7 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
     return ffail(std::string("hipMalloc/hipMemcpy: ") + hipGetErrorString(e));
   }
   std::vector<int> W(R);
-  for (int r = 0; r < R; ++r) W[r] = default_tile_width((double)rep_n[r] / (double)cpr, f->k, f->ncf);
+  for (int r = 0; r < R; ++r) W[r] = default_tile_width(f, (double)rep_n[r] / (double)cpr);
   int rc = set_tiles(f, W);
   if (rc != SPSS_OK) {
     fast_destroy(f);
@@ -324,10 +343,12 @@ int fast_create_replicas(FastPath **out, hipStream_t st, const DevParams &P, dou
   return SPSS_OK;
 }
 
-int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv) {
+int fast_create(FastPath **out, hipStream_t st, const DevParams &P, double rho, int nc, int subdiv, int xsplit) {
   const int n = P.n;
-  return fast_create_replicas(out, st, P, rho, nc, subdiv, 1, &n, nullptr);
+  return fast_create_replicas(out, st, P, rho, nc, subdiv, xsplit, 1, &n, nullptr);
 }
+
+int fast_cell_xsplit(const FastPath *f) { return f ? f->xsplit : 1; }
 
 void fast_destroy(FastPath *f) {
   if (!f) return;
@@ -384,14 +405,17 @@ static int overflow_blocks(const FastPath *f) { return f->R > 1 ? kOverflowBlock
 static FastArgs args_of(const FastPath *f) {
   FastArgs a;
   a.n = f->n;
-  a.ncf = f->ncf;
-  a.k = f->k;
+  a.ncx = f->ncx;
+  a.ncy = f->ncy;
+  a.kx = f->kx;
+  a.ky = f->ky;
   a.ntiles = f->ntiles;
   a.cpr = f->cpr;
   a.tiles = f->tiles;
   a.ovf_list = f->ovf_list;
   a.ovf_count = f->ovf_count;
-  a.H = f->H;
+  a.Hx = f->Hx;
+  a.Hy = f->Hy;
   a.r2c = f->r2c;
   a.r2h = f->r2h;
   a.r2k = f->r2k;
@@ -413,21 +437,24 @@ static void build(FastPath *f, hipStream_t st, const DevParams &P, const double 
   const unsigned nb = nblocks(n);
   const Layout L = layout_of(f);
   if (P.s == 4)
-    fk_keys<4><<<nb, 256, 0, st>>>(P, L, stage, f->ncf, f->key, f->slot, f->count);
+    fk_keys<4><<<nb, 256, 0, st>>>(P, L, stage, f->ncx, f->ncy, f->key, f->slot, f->count);
   else
-    fk_keys<3><<<nb, 256, 0, st>>>(P, L, stage, f->ncf, f->key, f->slot, f->count);
+    fk_keys<3><<<nb, 256, 0, st>>>(P, L, stage, f->ncx, f->ncy, f->key, f->slot, f->count);
   scan_local<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum);
   scan_add<<<f->nsb, 256, 0, st>>>(f->count, nullptr, f->cells, start, f->blocksum, f->nsb, f->ovf_count);
   fk_scatter<<<nb, 256, 0, st>>>(n, f->key, f->slot, start, f->tmp);
   if (P.s == 4)
-    fk_rank<4><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->cpr, f->H, f->key, start, f->tmp, f->sorted, snap, omega);
+    fk_rank<4><<<nb, 256, 0, st>>>(P, stage, f->ncx, f->cpr, f->Hx, f->Hy, f->key, start, f->tmp, f->sorted, snap,
+                                   omega);
   else
-    fk_rank<3><<<nb, 256, 0, st>>>(P, stage, f->ncf, f->cpr, f->H, f->key, start, f->tmp, f->sorted, snap, nullptr);
+    fk_rank<3><<<nb, 256, 0, st>>>(P, stage, f->ncx, f->cpr, f->Hx, f->Hy, f->key, start, f->tmp, f->sorted, snap,
+                                   nullptr);
 }
 
 // row tiles of the rebuilt cell list (single systems), from the starts build() left in f->start
 static void rebuilt_row_tiles(FastPath *f, hipStream_t st) {
-  if (f->btiles) fk_row_tiles<<<(f->ncf + 3) / 4, 256, 0, st>>>(f->start, f->ncf, f->rt_wcap, f->rt_cmax, f->btiles);
+  if (f->btiles)
+    fk_row_tiles<<<(f->ncy + 3) / 4, 256, 0, st>>>(f->start, f->ncx, f->ncy, f->rt_wcap, f->rt_cmax, f->btiles);
 }
 
 static PairBufs rebuilt_bufs(const FastPath *f, const double *stage) {
@@ -448,7 +475,7 @@ static void launch_pairs(const FastPath *f, hipStream_t st, const DevParams &P, 
   FastArgs a = args_of(f);
   if (f->btiles) {  // the rebuilt cell list's row tiles (rebuilt_row_tiles)
     a.tiles = f->btiles;
-    a.ntiles = f->ncf * f->rt_cmax;
+    a.ntiles = f->ncy * f->rt_cmax;
   }
   fk_pairs<S, SRK, MODE><<<tile_grid(a.ntiles), kFB, 0, st>>>(P, a, B);
   fk_pairs_ovf<S, SRK, MODE><<<overflow_blocks(f), kFB, 0, st>>>(P, a, B);
@@ -518,7 +545,7 @@ template <int S, bool SRK>
 static void ensure_moves(FastPath *f, hipStream_t st, const DevParams &P, const StepArgs &U) {
   if (f->mv_ready && f->mv_dt == U.dt && f->mv_srk == SRK && f->mv_v0 == P.v0) return;
   drop_moves(f, st);
-  fr_move<S, SRK><<<nblocks(f->n), 256, 0, st>>>(P, layout_of(f), f->n, f->ncf, f->rstate[f->cur], U,
+  fr_move<S, SRK><<<nblocks(f->n), 256, 0, st>>>(P, layout_of(f), f->n, f->ncx, f->ncy, f->rstate[f->cur], U,
                                                  f->mvb[f->mvc], f->farcnt, f->farslot, f->cnt9[f->cur], f->newcnt,
                                                  f->farlist[f->cur], f->farlist_n[f->cur]);
   f->mv_ready = true;
@@ -547,7 +574,7 @@ static void resident_enter(FastPath *f, hipStream_t st, const DevParams &P, cons
   else
     fr_gather<3><<<nb, 256, 0, st>>>(f->n, state, f->sorted, nat, f->rstate[c], f->rid[c], f->rnat[c]);
   if (f->rtiles)
-    fk_row_tiles<<<(f->ncf + 3) / 4, 256, 0, st>>>(f->rstart[c], f->ncf, f->rt_wcap, f->rt_cmax, f->rtiles);
+    fk_row_tiles<<<(f->ncy + 3) / 4, 256, 0, st>>>(f->rstart[c], f->ncx, f->ncy, f->rt_wcap, f->rt_cmax, f->rtiles);
   drop_moves(f, st);
   f->resident = true;
   f->resident_nat = nat != nullptr;
@@ -571,7 +598,8 @@ static Arrivals arrivals_of(const FastPath *f) {
   Q.bucket = f->tmp;
   Q.cnt9_next = f->cnt9[o];
   Q.nclear = (long)f->cells * 9;
-  Q.ncf = f->ncf;
+  Q.ncx = f->ncx;
+  Q.ncy = f->ncy;
   Q.cpr = f->cpr;
   Q.n = f->n;
   Q.error = f->lb_error;
@@ -587,7 +615,7 @@ static void resident_pairs(const FastPath *f, hipStream_t st, const DevParams &P
   FastArgs a = args_of(f);
   if (f->rtiles) {  // the resident order's row tiles, rebuilt every step
     a.tiles = f->rtiles;
-    a.ntiles = f->ncf * f->rt_cmax;
+    a.ntiles = f->ncy * f->rt_cmax;
   }
   const int nsb8 = (f->nsb + 7) & ~7;
   const Arrivals Q = arrivals_of(f);
@@ -607,8 +635,9 @@ static void resident_scatter(FastPath *f, hipStream_t st, const DevParams &P, co
   const Layout L = layout_of(f);
   int *const mv = f->mvb[f->mvc], *const mvn = f->mvb[1 - f->mvc];
   // (at least one workgroup per four rows of cells for the row tiles)
-  const unsigned nbs = std::max(nb, (unsigned)(f->rtiles ? (f->ncf + 3) / 4 : 0));
-  fr_scatter<S, SRK><<<nbs, 256, 0, st>>>(P, L, n, f->ncf, f->H, f->rstart[c], f->rstart[o], mv, f->cnt9[c], f->tmp,
+  const unsigned nbs = std::max(nb, (unsigned)(f->rtiles ? (f->ncy + 3) / 4 : 0));
+  fr_scatter<S, SRK><<<nbs, 256, 0, st>>>(P, L, n, f->ncx, f->ncy, f->Hx, f->Hy, f->rstart[c], f->rstart[o], mv,
+                                        f->cnt9[c], f->tmp,
                                         f->rstate[c], f->rid[c], rnat, f->ssum, f->tsum, f->scnt, U,
                                         SRK ? compl_out : nullptr, f->rstate[o], f->rid[o], f->rnat[o], f->rsnap,
                                         SRK ? f->romega : nullptr, f->rtiles, f->rt_wcap, f->rt_cmax, mvn, f->farcnt,

@@ -43,18 +43,25 @@ __device__ __forceinline__ void row_tiles(const int *P, int cbase, int row, int 
   for (int j = i + lane; j < cmax; j += 64) slots[j] = make_int4(cbase, row, 0, 0);
 }
 
-// rows [4 * blockIdx.x, 4 * blockIdx.x + 4) of a single system, one wave each; start(c) gives
-// the starts of the cell order the tiles are for (c = 0 .. ncf * ncf).  Uniform per
-// workgroup, so the barrier is reached by all or none of its threads.
+// rows [4 * blockIdx.x, 4 * blockIdx.x + 4) of a single system of ncy rows of ncx cells, one
+// wave each; start(c) gives the starts of the cell order the tiles are for (c = 0 .. ncx * ncy).
+// Uniform per workgroup, so the barrier is reached by all or none of its threads.
 template <class START>
-__device__ __forceinline__ void rows_to_tiles(START start, int ncf, int wcap, int cmax, int4 *__restrict__ tiles) {
+__device__ __forceinline__ void rows_to_tiles(START start, int ncx, int ncy, int wcap, int cmax,
+                                              int4 *__restrict__ tiles) {
   __shared__ int P[4][kRowTileCols + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + w;
-  if (row < ncf)
-    for (int c = lane; c <= ncf; c += 64) P[w][c] = start(row * ncf + c);
+  if (row < ncy)
+    for (int c = lane; c <= ncx; c += 64) P[w][c] = start(row * ncx + c);
   __syncthreads();
-  if (row < ncf) row_tiles(P[w], 0, row, ncf, wcap, cmax, tiles + (long)row * cmax);
+  if (row < ncy) row_tiles(P[w], 0, row, ncx, wcap, cmax, tiles + (long)row * cmax);
+}
+
+// square grids (ncf x ncf)
+template <class START>
+__device__ __forceinline__ void rows_to_tiles(START start, int ncf, int wcap, int cmax, int4 *__restrict__ tiles) {
+  rows_to_tiles(start, ncf, ncf, wcap, cmax, tiles);
 }
 
 }  // namespace spss

@@ -196,13 +196,25 @@ class ParticleSystem:
 
     # ---- cell-size autotuner (FP32C)
     def autotune_cells(self, stepper, dt, probe_steps=10):
-        """times the FP32C step at cell subdivisions 1, 2, 3 and keeps the fastest; returns
-        (ms per step for each, -1 where the grid is too fine; the chosen subdivision)"""
-        table = np.zeros(3)
+        """times the FP32C step for every cell layout of CELL_LAYOUTS and keeps the fastest; returns
+        (ms per step for each, -1 where the grid is too fine; the chosen subdivision; cell_layout
+        gives the chosen (subdivision, x split))"""
+        table = np.zeros(len(A.CELL_LAYOUTS))
         k = C.c_int()
-        A.check(self.lib.spss_autotune_cells(self.h, int(stepper), float(dt), int(probe_steps), A.ptr(table), 3,
-                                             C.byref(k)))
+        A.check(self.lib.spss_autotune_cells(self.h, int(stepper), float(dt), int(probe_steps), A.ptr(table),
+                                             table.size, C.byref(k)))
         return table, k.value
+
+    def set_cell_layout(self, k, xsplit):
+        """FP32C cells 1/(k*xsplit*int(1/rho)) wide and 1/(k*int(1/rho)) high (the state is kept)"""
+        A.check(self.lib.spss_set_cell_layout(self.h, int(k), int(xsplit)))
+
+    @property
+    def cell_layout(self):
+        """(subdivision, x split) of the FP32C cell grid"""
+        k, x = C.c_int(), C.c_int()
+        A.check(self.lib.spss_get_cell_layout(self.h, C.byref(k), C.byref(x)))
+        return k.value, x.value
 
     def autotune_all_pairs(self, probe_evals=3):
         """times the all-pairs kernel's workgroup / j-tile shapes on the current state and keeps the

@@ -1,8 +1,9 @@
 """Cell-size autotuner of the FP32C path (spss_autotune_cells).
 
 The probes must leave no trace: state, step counter and complementary force are restored, and
-the system then steps exactly (bit for bit) like a system created with the chosen subdivision.
-Every subdivision gives the same neighbour sets (pair lists equal to the FP64 ones).
+the system then steps exactly (bit for bit) like a system created with the chosen cell layout
+(subdivision and x split).  Every layout gives the same neighbour sets (pair lists equal to the
+FP64 ones).
 """
 import numpy as np
 import pytest
@@ -39,14 +40,16 @@ def test_autotune_restores_and_matches_a_fresh_system(spss, stepper_name):
     before = g.get_state().copy()
     step = g.step
     table, k = g.autotune_cells(stepper, dt, probe_steps=5)
-    print(f"{stepper_name}: ms/step at k=1,2,3: {np.round(table, 4)}; chosen {k}")
-    assert k in (1, 2, 3) and g.cell_subdivision == k
+    kc, xs = g.cell_layout
+    print(f"{stepper_name}: ms/step for layouts {spss.CELL_LAYOUTS}: {np.round(table, 4)}; chosen {(kc, xs)}")
+    assert k == kc and g.cell_subdivision == k
     assert np.all(table > 0)
-    assert k == 1 + int(np.argmin(table))
+    assert (kc, xs) == spss.CELL_LAYOUTS[int(np.argmin(table))]
     np.testing.assert_array_equal(g.get_state(), before)
     assert g.step == step
-    # from here on it steps like a system built with subdivision k from the same state
+    # from here on it steps like a system built with that layout from the same state
     h = _make(spss, n, rho, k)
+    h.set_cell_layout(k, xs)
     h.set_state(before)
     h.step = step
     for x in (g, h):
@@ -60,8 +63,9 @@ def test_neighbour_sets_do_not_depend_on_the_subdivision(spss):
     st = np.zeros((n, 4))
     st[:, 0], st[:, 1] = rng.random(n), rng.random(n)
     pairs = []
-    for k in (1, 2, 3):
+    for k, xs in spss.CELL_LAYOUTS:
         g = _make(spss, n, rho, k)
+        g.set_cell_layout(k, xs)
         g.set_state(st.reshape(-1))
         pairs.append(g.neighbour_pairs(spss.METHOD_LINKED_LIST, st.reshape(-1)))
     assert pairs[0].shape[0] > 10 * n

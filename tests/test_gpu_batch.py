@@ -43,12 +43,14 @@ def _replicas(model, R, seed):
     return ns, xi, sig, Dp, seeds, states, nats
 
 
-def _singles(spss, model, ns, xi, sig, Dp, seeds, states, nats, K, dt, v0, rho, k=1):
+def _singles(spss, model, ns, xi, sig, Dp, seeds, states, nats, K, dt, v0, rho, k=1, xs=1):
     out = []
     for r in range(len(ns)):
         g = spss.ParticleSystem(model=model, n=int(ns[r]), v0=v0, xi=xi[r], sigma=sig[r], rho=rho, D_phi=Dp[r],
                                 natfreq_local=1, precision=spss.PRECISION_FP32C, seed=int(seeds[r]),
                                 cell_subdivision=k)
+        if xs != 1:
+            g.set_cell_layout(k, xs)
         g.set_natural_frequencies(nats[r])
         g.set_state(states[r])
         g.run(spss.STEPPER_EULER, spss.METHOD_LINKED_LIST, K, dt, spss.NOISE_PHILOX)
@@ -134,9 +136,9 @@ def test_batch_tile_widths_and_autotune_do_not_change_results(spss):
 
 
 def test_batch_cell_autotune_keeps_replicas_equal_to_single_systems(spss):
-    """the cell-size autotuner rebuilds the batch at subdivisions 1, 2, 3, restores state and step
+    """the cell-size autotuner rebuilds the batch at every cell layout, restores state and step
     counter, and leaves a batch whose replicas are bit-identical to single FP32C systems of the
-    chosen subdivision"""
+    chosen layout"""
     model, R, K, dt, v0, rho = 4, 4, 12, 0.005, 0.01, 0.03
     print("[gpu] batch cell autotune", flush=True)
     ns, xi, sig, Dp, seeds, states, nats = _replicas(model, R, 77)
@@ -144,14 +146,16 @@ def test_batch_cell_autotune_keeps_replicas_equal_to_single_systems(spss):
     b.set_natural_frequencies(np.concatenate(nats))
     b.set_state(np.concatenate(states))
     table, k = b.autotune_cells(dt, probe_steps=4)
-    print(f"  ms/step by subdivision {np.round(table, 4)} chosen {k}", flush=True)
-    assert k in (1, 2, 3) and b.cell_subdivision == k and table[k - 1] == table[table > 0].min()
+    kc, xs = b.cell_layout
+    print(f"  ms/step by layout {spss.CELL_LAYOUTS}: {np.round(table, 4)} chosen {(kc, xs)}", flush=True)
+    assert k == kc and b.cell_subdivision == k
+    assert table[spss.CELL_LAYOUTS.index((kc, xs))] == table[table > 0].min()
     assert b.order_stats()[4] == 0  # step counter restored
     np.testing.assert_array_equal(b.get_state(), np.concatenate(states))
     b.run(K, dt, observe_every=0, graph=True)
     got = b.get_state()
     b.close()
-    want = _singles(spss, model, ns, xi, sig, Dp, seeds, states, nats, K, dt, v0, rho, k=k)
+    want = _singles(spss, model, ns, xi, sig, Dp, seeds, states, nats, K, dt, v0, rho, k=kc, xs=xs)
     off = 0
     for r in range(R):
         m = model * int(ns[r])
