@@ -51,8 +51,8 @@ PAIR_KERNEL_TRAFFIC = {(1, 1): 4.4389e7, (1, 2): 4.7781e7, (1, 4): 5.4909e7}
 # HBM bytes of the three kernels of one resident step (fr_pairs, fr_far, fr_scatter), the same way
 STEP_TRAFFIC = {(1, 1): 1.9577e8, (1, 2): 2.0656e8, (1, 4): 2.2848e8}
 # VALU instructions issued per CU-cycle by that kernel (SQ_INSTS_VALU / (256 CUs x cycles),
-# profiles/fr_pairs_euler_N1M_valu.txt): at the nominal one per CU-cycle, i.e. VALU-bound
-PAIR_KERNEL_VALU_PER_CU_CYCLE = 1.00
+# profiles/fr_pairs_euler_N1M_valu.txt), by cell layout: the nominal rate is one per CU-cycle
+PAIR_KERNEL_VALU_PER_CU_CYCLE = {(1, 1): 0.99, (1, 2): 0.91, (1, 4): 0.86}
 # algorithmic HBM bytes per particle-step of the cell-list step (s = 4 state, one RHS): FP64 state
 # read + write 16*s, FP32 snapshot x, y, phi written and read 8*3, cell key and sorted index written
 # and read 16, cell start/count table 8 per cell -- the compulsory traffic of the reference
@@ -551,7 +551,7 @@ def main():
                                      "us": pair_ms * 1e3, "bytes_per_particle": pair_kernel_bytes(n, cells),
                                      "traffic": PAIR_KERNEL_TRAFFIC.get((subdiv, xsplit))}},
         # the pair kernel is bound by VALU issue, not HBM (no MFMA work on this path)
-        "valu": {"kernel": "fr_pairs", "valu_instr_per_cu_cycle": PAIR_KERNEL_VALU_PER_CU_CYCLE},
+        "valu": {"kernel": "fr_pairs", "valu_instr_per_cu_cycle": PAIR_KERNEL_VALU_PER_CU_CYCLE.get((subdiv, xsplit))},
         "cpu_baseline": cpu,
         "pair_kernel_us": pair_ms * 1e3,
         "hbm_copy_gbps": copy_gbps,
