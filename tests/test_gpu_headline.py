@@ -13,6 +13,9 @@ frequencies omega_i / m with omega_i ~ Lorentzian(0, 0.05) clipped at +-10, v0 =
   (pointwise, 1e-12), then FP32C and FP64 fed the same Philox draws for 5000 Euler steps (the
   large configuration's horizon) and 1000 SRK steps: |R_FP32C - R_FP64| <= 1e-4 at every
   checkpoint (the accuracy target of the FP32-compensated path).
+
+The FP32C checks run on the reference's square cells and on the cells split 4 ways in x that the
+cell autotuner picks for this workload (spss_set_cell_layout(1, 4)).
 """
 import time
 
@@ -35,6 +38,7 @@ KW = dict(model=4, n=N, v0=0.005, xi=1.0 / M, sigma=SIGMA / M, rho=RHO, alpha=0.
 TOL_R = 1e-4
 TOL_COUPLING = 1e-6
 HOST_THREADS = 16
+LAYOUTS = [(1, 1), (1, 4)]  # (subdivision, x split) of the FP32C cell grid
 
 
 @pytest.fixture(scope="module")
@@ -54,9 +58,11 @@ def init():
     return st.reshape(-1), w
 
 
-def _system(spss, init, precision):
+def _system(spss, init, precision, layout=(1, 1)):
     st, w = init
     g = spss.ParticleSystem(seed=SEED, precision=precision, **KW)
+    if layout != (1, 1):
+        g.set_cell_layout(*layout)
     g.set_natural_frequencies(w)
     g.set_state(st)
     return g
@@ -71,14 +77,15 @@ def _oracle(init, higher=False):
     return o
 
 
-def test_headline_pairs_bitexact(spss, init):
-    g = _system(spss, init, spss.PRECISION_FP32C)
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_headline_pairs_bitexact(spss, init, layout):
+    g = _system(spss, init, spss.PRECISION_FP32C, layout)
     o = oracle.Oracle(**KW)
     states = [("initial", init[0])]
     g.run(spss.STEPPER_EULER, spss.METHOD_LINKED_LIST, 200, DT, spss.NOISE_PHILOX)
     states.append(("after 200 steps", g.get_state()))
     for name, st in states:
-        print(f"[gpu] headline pairs N={N} rho={RHO} ({name})", flush=True)
+        print(f"[gpu] headline pairs N={N} rho={RHO} layout={layout} ({name})", flush=True)
         t0 = time.time()
         got = g.neighbour_pairs(spss.METHOD_LINKED_LIST, st)
         t1 = time.time()
@@ -91,13 +98,14 @@ def test_headline_pairs_bitexact(spss, init):
     g.close()
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("higher", [False, True])
-def test_headline_rhs_vs_oracle(spss, init, higher):
-    print(f"[gpu] headline RHS N={N} higher_order={higher}", flush=True)
+def test_headline_rhs_vs_oracle(spss, init, higher, layout):
+    print(f"[gpu] headline RHS N={N} higher_order={higher} layout={layout}", flush=True)
     st = init[0].copy().reshape(-1, 4)
     st[:, 3] = np.random.default_rng(5).normal(0.0, 0.5, N)  # omega != 0 so the SRK force is not trivial
     st = st.reshape(-1)
-    g = _system(spss, init, spss.PRECISION_FP32C)
+    g = _system(spss, init, spss.PRECISION_FP32C, layout)
     g.set_higher_order(higher)
     o = _oracle(init, higher)
     kg = np.zeros(4 * N)
@@ -138,11 +146,12 @@ def test_headline_fp64_path_vs_oracle_two_steps(spss, init, stepper):
     g.close()
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("stepper,steps", [("euler", 5000), ("srk", 1000)])
-def test_headline_fp32c_order_parameter_vs_fp64(spss, init, stepper, steps):
-    print(f"[gpu] headline FP32C vs FP64 {stepper}, {steps} steps", flush=True)
+def test_headline_fp32c_order_parameter_vs_fp64(spss, init, stepper, steps, layout):
+    print(f"[gpu] headline FP32C vs FP64 {stepper}, {steps} steps, layout {layout}", flush=True)
     sid = spss.STEPPER_SRK if stepper == "srk" else spss.STEPPER_EULER
-    g32 = _system(spss, init, spss.PRECISION_FP32C)
+    g32 = _system(spss, init, spss.PRECISION_FP32C, layout)
     g64 = _system(spss, init, spss.PRECISION_FP64)
     chunk = 500
     worst = 0.0
