@@ -9,6 +9,8 @@
   the replica's seed, the step and the particle's index inside its replica).  After 12 steps, model 4:
   positions within 1e-12, phases within 1e-9, omega within 1e-7 (the FP32C pair-sum accuracy);
   model 3, whose phase takes the coupling directly: positions within 1e-11, phases within 1e-7.
+  On square cells and on cells split 4 ways in x (the layout the autotuner picks for the large
+  replicas).
 * Full size: the 512 replicas of the ensemble configuration (10.1M particles) for 20 graph-replayed
   steps; every replica equals a single FP32C system stepped alone by spss_run, bit for bit.
 """
@@ -51,15 +53,18 @@ def _group(model, ns, ms, seed):
     return xi, sig, Dp, seeds, states, nats
 
 
+@pytest.mark.parametrize("layout", [(1, 1), (1, 4)])
 @pytest.mark.parametrize("model,ns,ms", [
     (4, [2048, 65536, 9001, 30000], [0.5, 1.0, 1.0, 0.5]),
     (3, [2048, 65536, 12345], [0.0, 0.0, 0.0]),
 ])
-def test_batch_replicas_vs_oracle(spss, model, ns, ms):
+def test_batch_replicas_vs_oracle(spss, model, ns, ms, layout):
     steps = 12
-    print(f"[gpu] batch vs oracle model={model} ns={ns}", flush=True)
+    print(f"[gpu] batch vs oracle model={model} ns={ns} layout={layout}", flush=True)
     xi, sig, Dp, seeds, states, nats = _group(model, ns, ms, 40 + model)
     b = spss.ReplicaBatch(model, ns, xi, sig, Dp, seeds, v0=V0, rho=RHO)
+    if layout != (1, 1):
+        b.set_cell_layout(*layout)
     b.set_natural_frequencies(np.concatenate(nats))
     b.set_state(np.concatenate(states))
     b.run(steps, DT, observe_every=0, graph=True)
