@@ -20,22 +20,22 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
   return v;
 }
 
-// The tiles of one row of cells, built by one wave from the row's cell starts P[0..ncf] (LDS):
+// The tiles of one row of cells, built by one wave from the row's cell starts P[0..ncx] (LDS):
 // greedy spans of whole cells, each as wide as possible with at most kTileCap particles and at
 // most wcap (<= 64) columns, span i in slot i of the row's cmax slots, empty spans in the unused
 // slots.  Full spans keep the workgroup's waves full where fixed-width
-// tiles leave the last wave of most tiles part-empty.  With cmax * wcap >= ncf every span stays
+// tiles leave the last wave of most tiles part-empty.  With cmax * wcap >= ncx every span stays
 // within wcap columns: a span is widened past the particle bound (a dense row) only as far as
 // the remaining slots need to cover the rest of the row.
-__device__ __forceinline__ void row_tiles(const int *P, int cbase, int row, int ncf, int wcap, int cmax,
+__device__ __forceinline__ void row_tiles(const int *P, int cbase, int row, int ncx, int wcap, int cmax,
                                           int4 *__restrict__ slots) {
   const int lane = threadIdx.x & 63;
   int a = 0, i = 0;
-  while (a < ncf) {  // invariant: ncf - a <= (cmax - i) * wcap
+  while (a < ncx) {  // invariant: ncx - a <= (cmax - i) * wcap
     const int e = a + 1 + lane;
     // true for a leading run of lanes: P is non-decreasing
-    const bool fits = lane < wcap && e <= ncf && P[e] - P[a] <= kTileCap;
-    const int b = max(a + max(__popcll(__ballot(fits)), 1), ncf - (cmax - i - 1) * wcap);
+    const bool fits = lane < wcap && e <= ncx && P[e] - P[a] <= kTileCap;
+    const int b = max(a + max(__popcll(__ballot(fits)), 1), ncx - (cmax - i - 1) * wcap);
     if (lane == 0) slots[i] = make_int4(cbase, row, a, b);
     a = b;
     ++i;
