@@ -1007,11 +1007,13 @@ __global__ __launch_bounds__(256) void fr_scatter(
       cd = base + ty * ncx + tx;
       int b0 = 0;
       const int od = move_order(d);
+      if (od > 0) {  // stayers (od = 0) come first: no counts to add (whole waves of them skip the loads)
 #pragma unroll
-      for (int e = 0; e < 9; ++e) {
-        const int sx = wrap_cell(tx - (e % 3 - 1), ncx), sy = wrap_cell(ty - (e / 3 - 1), ncy);
-        const int v = cnt9[(long)(base + sy * ncx + sx) * 9 + e];
-        b0 += move_order(e) < od ? v : 0;
+        for (int e = 0; e < 9; ++e) {
+          const int sx = wrap_cell(tx - (e % 3 - 1), ncx), sy = wrap_cell(ty - (e / 3 - 1), ncy);
+          const int v = cnt9[(long)(base + sy * ncx + sx) * 9 + e];
+          b0 += move_order(e) < od ? v : 0;
+        }
       }
       pos = nstart[cd] + b0 + rank;
     }
