@@ -59,9 +59,11 @@ struct Layout {
   const int *rep_first;  // [R + 1] first particle of each replica
   const int *rep_chunk;
   int cpr;
+  int nrep;  // R (a single system: 1, every particle and cell in replica 0)
 };
 
 __device__ __forceinline__ int replica_of(const Layout &L, int p) {
+  if (L.nrep == 1) return 0;
   int r = L.rep_chunk[p >> kChunkShift];
   while (L.rep_first[r + 1] <= p) ++r;
   return r;
@@ -971,7 +973,7 @@ __global__ __launch_bounds__(256) void fr_scatter(
   double sn2 = 0.0, cn2 = 1.0, xw2 = 0.0, yw2 = 0.0;
   if (valid) {
     const int cpr = L.cpr;
-    const int r = cs / cpr, base = r * cpr;
+    const int r = L.nrep == 1 ? 0 : cs / cpr, base = r * cpr;
     Coef K;
     if (U.coef) {
       K = U.coef[r];

@@ -427,6 +427,7 @@ static Layout layout_of(const FastPath *f) {
   L.rep_first = f->rep_first;
   L.rep_chunk = f->rep_chunk;
   L.cpr = f->cpr;
+  L.nrep = f->R;
   return L;
 }
 
