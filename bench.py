@@ -49,7 +49,7 @@ def pair_kernel_bytes(n, cells):
 # (profiles/fr_pairs_euler_N1M_hbm_traffic.txt)
 PAIR_KERNEL_TRAFFIC = {(1, 1): 4.4389e7, (1, 2): 4.7781e7, (1, 4): 5.4909e7}
 # HBM bytes of the three kernels of one resident step (fr_pairs, fr_far, fr_scatter), the same way
-STEP_TRAFFIC = {(1, 1): 1.9577e8, (1, 2): 2.0656e8, (1, 4): 2.2848e8}
+STEP_TRAFFIC = {(1, 1): 1.9577e8, (1, 2): 2.0656e8, (1, 4): 2.1588e8}
 # VALU instructions issued per CU-cycle by that kernel (SQ_INSTS_VALU / (256 CUs x cycles),
 # profiles/fr_pairs_euler_N1M_valu.txt), by cell layout: the nominal rate is one per CU-cycle
 PAIR_KERNEL_VALU_PER_CU_CYCLE = {(1, 1): 0.99, (1, 2): 0.91, (1, 4): 0.86}
