@@ -41,3 +41,14 @@ def test_abi_version_and_error_string_without_gpu():
     h = ctypes.c_void_p()
     assert lib.spss_create(ctypes.byref(p), ctypes.byref(h)) == _capi.SPSS_OK + 1
     assert b"n must be positive" in lib.spss_last_error()
+
+
+def test_cell_layout_table_matches_the_header():
+    """the Python table of FP32C cell layouts is the header's SPSS_CELL_LAYOUTS, in the same order"""
+    from spsslangevinintegration_amd import _capi
+    text = open(os.path.join(ROOT, "include", "spss_hip.h")).read()
+    table = re.search(r"#define SPSS_CELL_LAYOUTS\s+\{(.*)\}\s*$", text, flags=re.M).group(1)
+    pairs = tuple((int(a), int(b)) for a, b in re.findall(r"\{\s*(\d+)\s*,\s*(\d+)\s*\}", table))
+    count = int(re.search(r"#define SPSS_CELL_LAYOUT_COUNT\s+(\d+)", text).group(1))
+    assert pairs == _capi.CELL_LAYOUTS and len(pairs) == count
+    assert all(k in (1, 2, 3) and x in (1, 2, 4) for k, x in pairs)
